@@ -1,0 +1,13 @@
+"""MI355X-native axisymmetric viscoelastic FDTD (drop-in for the reference's
+internal/fdtd behind the C ABI in include/geofdtd.h).
+
+    fdtd      the reference's fdtd package API over libgeofdtd.so (ctypes)
+    csrc/     HIP kernels for gfx950 and the C ABI
+
+The directory name is not a Python identifier; load it with
+importlib (see tests/conftest.py) under any alias.
+"""
+
+from . import fdtd  # noqa: F401
+
+__all__ = ["fdtd"]

@@ -1,0 +1,243 @@
+// fdtd_kernels.hip — gfx950 kernels of the axisymmetric viscoelastic FDTD.
+//
+//   k_build_tables   per-row and per-column coefficients, FP64 math rounded
+//                    once to FP32 (model.go:157-209, pml.go:29-63)
+//   k_step           one fused time step of every tile of every model
+//                    (fdtd.go:149-307; receivers fdtd.go:330-342; the
+//                    MaxSpeed divergence probe fdtd.go:351-360 folded in)
+//   k_surface_velocity, k_max_speed, k_fill_*, k_set_base   small helpers
+//
+// The stencil is HBM-bound (about one flop per byte), so there is no MFMA
+// here: the design goal is that each state array crosses HBM exactly once per
+// step.  See stencil.h for the tile march.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "coef.h"
+#include "kernels.h"
+#include "stencil.h"
+
+namespace gfk {
+
+// ------------------------------------------------------------------ helpers
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_f32(float *p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_set_base(int32_t *p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+// --------------------------------------------------------- coefficient build
+
+__global__ void k_build_tables(TableParams P, const LayerParams *__restrict__ L, float *__restrict__ rowc,
+                               float *__restrict__ colc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.nz) build_row(P, L, i, rowc);
+  if (i < P.pitch) build_col(P, i, colc);
+}
+
+// ---------------------------------------------------------------- time step
+
+__device__ __forceinline__ float shfl_up1(float x) { return __shfl_up(x, 1, kLanes); }
+__device__ __forceinline__ float shfl_dn1(float x) { return __shfl_down(x, 1, kLanes); }
+
+// One wavefront = one tile; four tiles per 256-thread workgroup.  Tiles are
+// numbered model by model (GridDesc::tile_base), row-major within a model.
+template <int NM, bool SC>
+__global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, int32_t nmodels, int32_t n_off,
+                                              const int32_t *__restrict__ nbase, int32_t probe) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  // Wave-uniform by construction; readfirstlane lets the compiler keep the
+  // descriptor reads scalar.
+  const int tile = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+
+  int m = 0;
+  if (nmodels > 1) {
+    int lo = 0, hi = nmodels - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (gds[mid].tile_base <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    m = lo;
+  }
+  const GridDesc &g = gds[m];
+  const int n = n_off + (nbase ? nbase[0] : 0);
+  if (n >= g.steps) return;
+  // MaxSpeed every 256 steps (run.go:118-124), folded into the step.
+  const bool do_max = probe && (n & 255) == 0;
+  const int lt = tile - g.tile_base;
+  if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
+
+  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
+  const int i0 = tr * kOwn, j0 = tzi * g.th;
+  const int j1 = min(j0 + g.th, g.nz);
+  const int c = i0 - 1 + lane;
+  const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
+  const int rb = n & 1, wb = rb ^ 1;
+  const ColConst cc = load_col(g, c);
+
+  const float *A_vr = g.f[rb][kVr], *A_vz = g.f[rb][kVz], *A_trr = g.f[rb][kTrr];
+  const float *A_tqq = g.f[rb][kTqq], *A_tzz = g.f[rb][kTzz], *A_trz = g.f[rb][kTrz];
+  float *B_vr = g.f[wb][kVr], *B_vz = g.f[wb][kVz], *B_trr = g.f[wb][kTrr];
+  float *B_tqq = g.f[wb][kTqq], *B_tzz = g.f[wb][kTzz], *B_trz = g.f[wb][kTrz];
+
+  // Carried state, named by row relative to the velocity row t being built.
+  float trr_p = 0.f, tqq_p = 0.f;          // old trr, tqq at t-1
+  float trz_p = ld(A_trz, g, j0 - 2, c);   // old trz at t-1
+  float tzz_p = 0.f;                       // old tzz at t-1
+  float tzz_c = ld_tzz(A_tzz, g, n, j0 - 1, c);  // old tzz at t
+  float vrn_p = 0.f, vzn_p = 0.f, vzn_pp = 0.f;  // new vr(t-1), vz(t-1), vz(t-2)
+  uint32_t mbits = 0;
+
+  for (int t = j0 - 1; t <= j1; t++) {
+    const float trr = ld(A_trr, g, t, c), tqq = ld(A_tqq, g, t, c), trz = ld(A_trz, g, t, c);
+    const float tzz_dn = ld_tzz(A_tzz, g, n, t + 1, c);
+    const float vr_old = ld(A_vr, g, t, c), vz_old = ld(A_vz, g, t, c);
+    const float trr1 = shfl_dn1(trr), tqq1 = shfl_dn1(tqq), trzm = shfl_up1(trz);
+    const bool row_own = t >= j0 && t < j1;
+    float vrn, vzn;
+    velocity(g, rb, t, c, row_own && own, cc, trr, trr1, tqq, tqq1, trz, trzm, trz_p, tzz_c, tzz_dn, vr_old, vz_old,
+             vrn, vzn);
+    if (row_own && own) {
+      const size_t k = (size_t)t * g.pitch + c;
+      B_vr[k] = vrn;
+      B_vz[k] = vzn;
+      if (do_max) mbits = max(mbits, max(abs_bits(vrn), abs_bits(vzn)));
+    }
+    const float vrn_m = shfl_up1(vrn);
+    if (t == 0 && g.nrec > 0) {
+      // Receivers sit on the surface row; the tile holding row 0 records them.
+      const float sv = surface_velocity(g, c, cc, vrn, vrn_m, vzn);
+      const int64_t slot = (int64_t)n - g.rec_base;
+      if (slot >= 0 && slot < g.trace_cap) {
+        for (int r = 0; r < g.nrec; r++) {
+          const int col = g.rec_col[r];
+          if (own && col == c) g.traces[(size_t)r * g.trace_cap + slot] = sv;
+        }
+      }
+    }
+    const int s = t - 1;
+    if (s >= j0) {
+      const float vrn_pm = shfl_up1(vrn_p);   // new vr(s, c-1)
+      const float vzn_p1 = shfl_dn1(vzn_p);   // new vz(s, c+1)
+      if (own) {
+        float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
+        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz);
+        const size_t k = (size_t)s * g.pitch + c;
+        B_trr[k] = nrr;
+        B_tqq[k] = nqq;
+        B_tzz[k] = nzz;
+        B_trz[k] = nrz;
+      }
+    }
+    trr_p = trr;
+    tqq_p = tqq;
+    trz_p = trz;
+    tzz_p = tzz_c;
+    tzz_c = tzz_dn;
+    vzn_pp = vzn_p;
+    vzn_p = vzn;
+    vrn_p = vrn;
+  }
+
+  if (do_max) {
+    for (int off = 32; off >= 1; off >>= 1) mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, off, kLanes));
+    if (lane == 0) atomicMax(&g.maxbits[n >> 8], mbits);
+  }
+}
+
+// Sim.SurfaceVelocity for one column of the buffer that holds step `cur`.
+__global__ void k_surface_velocity(const GridDesc *__restrict__ gd, int32_t buf, int32_t col, float *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const GridDesc &g = *gd;
+  const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
+  const ColConst cc = load_col(g, col);
+  const float vrm = col > 0 ? vr[col - 1] : 0.f;
+  out[0] = surface_velocity(g, col, cc, vr[col], vrm, vz[col]);
+  out[1] = vr[col];
+}
+
+// Sim.MaxSpeed over both velocity fields of one buffer.
+__global__ void k_max_speed(const GridDesc *__restrict__ gd, int32_t buf, uint32_t *slot) {
+  const GridDesc &g = *gd;
+  const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
+  const int64_t cells = (int64_t)g.nz * g.pitch;
+  uint32_t m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, max(abs_bits(vr[i]), abs_bits(vz[i])));
+  for (int off = 32; off >= 1; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, kLanes));
+  if ((threadIdx.x & (kLanes - 1)) == 0) atomicMax(slot, m);
+}
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill_f32, dim3(blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers, float *rowc, float *colc,
+                               hipStream_t st) {
+  int n = P.nz > P.pitch ? P.nz : P.pitch;
+  hipLaunchKernelGGL(k_build_tables, dim3((n + 255) / 256), dim3(256), 0, st, P, d_layers, rowc, colc);
+  return hipGetLastError();
+}
+
+template <int NM, bool SC>
+static hipError_t launch_step_t(const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
+                                const int32_t *nbase, int32_t probe, hipStream_t st) {
+  unsigned blocks = (unsigned)((tiles + 3) / 4);
+  hipLaunchKernelGGL((k_step<NM, SC>), dim3(blocks), dim3(256), 0, st, d, nmodels, n_off, nbase, probe);
+  return hipGetLastError();
+}
+
+hipError_t launch_step(int nmech, bool scalar, const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
+                       const int32_t *nbase, int32_t probe, hipStream_t st) {
+  if (tiles <= 0) return hipSuccess;
+  if (nmech == 0) return launch_step_t<0, true>(d, nmodels, tiles, n_off, nbase, probe, st);
+  if (scalar) return launch_step_t<1, true>(d, nmodels, tiles, n_off, nbase, probe, st);
+  switch (nmech) {
+    case 1: return launch_step_t<1, false>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_t<2, false>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 3: return launch_step_t<3, false>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_t<4, false>(d, nmodels, tiles, n_off, nbase, probe, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_surface_velocity(const GridDesc *d, int32_t buf, int32_t col, float *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_surface_velocity, dim3(1), dim3(64), 0, st, d, buf, col, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_speed(const GridDesc *d, int32_t buf, uint32_t *slot, int64_t cells, hipStream_t st) {
+  int blocks = (int)std::min<int64_t>((cells + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_max_speed, dim3(blocks), dim3(256), 0, st, d, buf, slot);
+  return hipGetLastError();
+}
+
+}  // namespace gfk

@@ -1,0 +1,295 @@
+// run.cpp — fdtd.Run (internal/fdtd/run.go:88-127) for one shot or many.
+//
+// Many shots advance together: every step is one launch over the tiles of
+// every model still running, so uneven grids share the GPU instead of
+// queueing behind one another.  Models are ordered by step count, longest
+// first, so the models still running are always a prefix of the tile list
+// and the launch shrinks as short runs retire.  Blocks of kGraphSteps steps
+// are replayed from captured HIP graphs.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/geofdtd.h"
+#include "kernels.h"
+#include "model.h"
+#include "sim.h"
+
+namespace gfs {
+int api_set_err(int code, const std::string &msg);
+}
+
+#define HIPCHK(expr)                                       \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return gfs::hip_fail(e_, #expr); \
+  } while (0)
+
+namespace {
+
+using gfs::Sim;
+
+struct Group {
+  int nmech = 0;
+  bool scalar = true;
+  std::vector<int> order;        // shot indices, longest run first
+  std::vector<int64_t> prefix;   // tiles of the first k models
+  gfk::GridDesc *d_descs = nullptr;
+  int32_t *d_nbase = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_models = -1;  // active-model count the graph was captured for
+};
+
+void free_group(Group &G) {
+  if (G.exec) hipGraphExecDestroy(G.exec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  if (G.d_descs) hipFree(G.d_descs);
+  if (G.d_nbase) hipFree(G.d_nbase);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  G.d_descs = nullptr;
+  G.d_nbase = nullptr;
+}
+
+// Models of the group still running at step n.
+int active_models(const Group &G, const std::vector<int32_t> &steps, int64_t n) {
+  int k = 0;
+  while (k < (int)G.order.size() && steps[G.order[k]] > n) k++;
+  return k;
+}
+
+int capture(Group &G, int models, hipStream_t st) {
+  if (G.exec) hipGraphExecDestroy(G.exec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < gfs::kGraphSteps; k++) {
+    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.d_descs, models, G.prefix[models], k, G.d_nbase, 1, st);
+    if (e != hipSuccess) {
+      hipGraph_t tmp = nullptr;
+      hipStreamEndCapture(st, &tmp);
+      if (tmp) hipGraphDestroy(tmp);
+      return gfs::hip_fail(e, "graph capture");
+    }
+  }
+  HIPCHK(hipStreamEndCapture(st, &G.graph));
+  HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+  G.graph_models = models;
+  return GF_OK;
+}
+
+}  // namespace
+
+extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
+  if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
+  std::vector<std::unique_ptr<Sim>> sims(n_shots);
+  std::vector<int32_t> st(n_shots, GF_OK), steps(n_shots, 0);
+  std::vector<std::string> msg(n_shots);
+  std::vector<std::vector<int32_t>> cols(n_shots);
+  int first_err = GF_OK;
+  std::string first_msg;
+  auto record = [&](int i, int code, const std::string &m) {
+    st[i] = code;
+    msg[i] = m;
+    if (!first_err) {
+      first_err = code;
+      first_msg = m;
+    }
+  };
+
+  // ---- per-shot set-up, in the reference's order of checks ----
+  for (int i = 0; i < n_shots; i++) {
+    const gf_shot &sh = shots[i];
+    if (!sh.model || (sh.n_ranges > 0 && !sh.ranges) || (sh.n_force > 0 && !sh.force) ||
+        (sh.n_ranges > 0 && !results[i].traces)) {
+      record(i, GF_ERR_ARGUMENT, "geofdtd: null argument in shot");
+      continue;
+    }
+    sims[i].reset(new Sim());
+    Sim &s = *sims[i];
+    int rc = s.init(*sh.model, std::max(sh.steps, 0));
+    if (rc) {
+      record(i, rc, gf_last_error());
+      sims[i].reset();
+      continue;
+    }
+    if (sh.steps <= 0) {
+      record(i, GF_ERR_INVALID, "fdtd: a shot needs a positive number of steps");
+      sims[i].reset();
+      continue;
+    }
+    if ((rc = s.set_force(sh.force, sh.n_force))) {
+      record(i, rc, gf_last_error());
+      sims[i].reset();
+      continue;
+    }
+    bool bad = false;
+    for (int r = 0; r < sh.n_ranges && !bad; r++) {
+      int c = (int)round(sh.ranges[r] / s.R.h);
+      c = std::max(0, std::min(c, s.R.nr - 2));
+      const double snapped = (double)c * s.R.h;
+      if (snapped >= sh.model->max_range_m) {
+        record(i, GF_ERR_INVALID,
+               "fdtd: receiver at " + gfm::go_g(sh.ranges[r]) + " m is outside the interior (" +
+                   gfm::go_g(sh.model->max_range_m) + " m)");
+        bad = true;
+      }
+      cols[i].push_back(c);
+      if (results[i].snapped) results[i].snapped[r] = snapped;
+    }
+    if (bad) {
+      sims[i].reset();
+      continue;
+    }
+    if ((rc = s.set_receivers(cols[i].data(), (int32_t)cols[i].size(), sh.steps, 0)) ||
+        (rc = s.reset_max_slots(sh.steps))) {
+      record(i, rc, gf_last_error());
+      sims[i].reset();
+      continue;
+    }
+    steps[i] = sh.steps;
+    results[i].dt = s.R.dt;
+    results[i].t0 = 0.5 * s.R.dt;
+    results[i].nr = s.R.nr;
+    results[i].nz = s.R.nz;
+    results[i].peak = 0;
+  }
+
+  // ---- group by kernel variant; one launch per group per step ----
+  std::vector<Group> groups;
+  for (int i = 0; i < n_shots; i++) {
+    if (!sims[i]) continue;
+    const Sim &s = *sims[i];
+    const int nm = s.g.lossy ? s.g.nmech : 0;
+    const bool sc = s.g.scalar != 0;
+    auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &G) { return G.nmech == nm && G.scalar == sc; });
+    if (it == groups.end()) {
+      groups.emplace_back();
+      groups.back().nmech = nm;
+      groups.back().scalar = sc;
+      it = groups.end() - 1;
+    }
+    it->order.push_back(i);
+  }
+  hipStream_t stream = nullptr;
+  int rc = GF_OK;
+  int64_t max_steps = 0;
+  for (int i = 0; i < n_shots; i++) max_steps = std::max<int64_t>(max_steps, sims[i] ? steps[i] : 0);
+  if (!groups.empty()) {
+    stream = sims[groups[0].order[0]]->stream;
+    for (Group &G : groups) {
+      std::stable_sort(G.order.begin(), G.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+      std::vector<gfk::GridDesc> descs;
+      G.prefix.assign(1, 0);
+      for (int i : G.order) {
+        gfk::GridDesc d = sims[i]->g;
+        d.tile_base = (int32_t)G.prefix.back();
+        descs.push_back(d);
+        G.prefix.push_back(G.prefix.back() + (int64_t)d.tiles_r * d.tiles_z);
+      }
+      if (G.prefix.back() > (int64_t)INT32_MAX) {
+        rc = gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: batch has too many tiles");
+        break;
+      }
+      hipError_t e = hipMalloc(&G.d_descs, sizeof(gfk::GridDesc) * descs.size());
+      if (e == hipSuccess) e = hipMalloc(&G.d_nbase, 64);
+      if (e == hipSuccess) e = hipMemcpy(G.d_descs, descs.data(), sizeof(gfk::GridDesc) * descs.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        rc = gfs::hip_fail(e, "batch descriptors");
+        break;
+      }
+    }
+    for (auto &sp : sims)
+      if (sp && !rc) {
+        hipError_t e = hipStreamSynchronize(sp->stream);
+        if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
+      }
+
+    // ---- the time loop ----
+    for (int64_t n = 0; n < max_steps && !rc;) {
+      const bool graph_block = n + gfs::kGraphSteps <= max_steps;
+      for (Group &G : groups) {
+        const int act = active_models(G, steps, n);
+        if (act == 0) continue;
+        if (graph_block) {
+          // Re-capture when the running set has shrunk by a tenth of its tiles.
+          if (!G.exec || G.graph_models < act ||
+              (G.prefix[act] < G.prefix[G.graph_models] - G.prefix[G.graph_models] / 10)) {
+            if ((rc = capture(G, act, stream))) break;
+          }
+          hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)n, stream);
+          if (e == hipSuccess) e = hipGraphLaunch(G.exec, stream);
+          if (e != hipSuccess) {
+            rc = gfs::hip_fail(e, "graph launch");
+            break;
+          }
+        } else {
+          hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.d_descs, act, G.prefix[act], (int32_t)n, nullptr, 1, stream);
+          if (e != hipSuccess) {
+            rc = gfs::hip_fail(e, "step launch");
+            break;
+          }
+        }
+      }
+      n += graph_block ? gfs::kGraphSteps : 1;
+    }
+    if (!rc) {
+      hipError_t e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
+    }
+  }
+  for (Group &G : groups) free_group(G);
+  if (rc) return rc;
+
+  // ---- results: divergence probes, then traces ----
+  for (int i = 0; i < n_shots; i++) {
+    if (!sims[i]) continue;
+    Sim &s = *sims[i];
+    const int64_t slots = (steps[i] - 1) / 256 + 1;
+    std::vector<uint32_t> bits(slots);
+    if (hipMemcpy(bits.data(), s.d_max, sizeof(uint32_t) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+      record(i, GF_ERR_DEVICE, "geofdtd: reading divergence probes failed");
+      continue;
+    }
+    double peak = 0;
+    bool diverged = false;
+    for (int64_t k = 0; k < slots; k++) {
+      const double p = gfs::bits_to_speed(bits[k]);
+      peak = std::isnan(p) || std::isnan(peak) ? NAN : std::max(peak, p);
+      if (std::isnan(p) || std::isinf(p)) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "fdtd: diverged at step %lld of %d", (long long)(k * 256), steps[i]);
+        record(i, GF_ERR_DIVERGED, buf);
+        diverged = true;
+        break;
+      }
+    }
+    results[i].peak = peak;
+    if (diverged) continue;
+    const size_t nrec = cols[i].size();
+    if (nrec > 0) {
+      std::vector<float> tmp((size_t)steps[i] * nrec);
+      if (hipMemcpy(tmp.data(), s.d_traces, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        record(i, GF_ERR_DEVICE, "geofdtd: reading traces failed");
+        continue;
+      }
+      for (size_t k = 0; k < tmp.size(); k++) results[i].traces[k] = tmp[k];
+    }
+  }
+  for (int i = 0; i < n_shots; i++) {
+    if (statuses) statuses[i] = st[i];
+    results[i].status = st[i];
+    snprintf(results[i].error, sizeof results[i].error, "%s", msg[i].c_str());
+  }
+  if (first_err) return gfs::api_set_err(first_err, first_msg);
+  return gfs::api_set_err(GF_OK, "");
+}

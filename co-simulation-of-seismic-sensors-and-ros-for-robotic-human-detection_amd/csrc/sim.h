@@ -1,0 +1,82 @@
+// sim.h — the object behind a gf_sim handle (one model on one GPU).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "model.h"
+
+namespace gfs {
+
+// Step kernels per captured graph (the time-step block a graph replays).
+constexpr int kGraphSteps = 100;
+
+int hip_fail(hipError_t e, const char *where);
+int check_device(std::string &why);
+int choose_tile_rows(int nr, int nz);
+
+// Ordered |v| bits back to a speed; NaN bits stay NaN (math.Max semantics).
+inline double bits_to_speed(uint32_t b) {
+  float f;
+  memcpy(&f, &b, sizeof f);
+  return (double)f;
+}
+
+// Algorithmic HBM bytes of one fused step: every wavefield and memory
+// variable read and written once in float32, plus 4 psi read and written per
+// cell of each C-PML strip.
+inline double step_bytes(const gfm::Resolved &R) {
+  const double cells = (double)R.nr * R.nz;
+  const int nm = R.lossy ? R.nmech : 0;
+  const int r0 = R.first_rint < R.first_rhalf ? R.first_rint : R.first_rhalf;
+  const int z0 = R.first_zint < R.first_zhalf ? R.first_zint : R.first_zhalf;
+  const double strip = (double)R.nz * (R.nr - (r0 < R.nr ? r0 : R.nr)) + (double)R.nr * (R.nz - (z0 < R.nz ? z0 : R.nz));
+  return 4.0 * 2.0 * (6 + 4 * nm) * cells + 32.0 * strip;
+}
+
+struct Sim {
+  gfm::Resolved R;
+  gfk::GridDesc g;  // host copy of the device descriptor
+  gfk::GridDesc *d_desc = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  float *rowc = nullptr, *colc = nullptr;
+  float *d_src = nullptr;
+  int32_t *d_rec = nullptr;
+  float *d_traces = nullptr;
+  int64_t rec_cap_cols = 0, trace_floats = 0;
+  uint32_t *d_max = nullptr;
+  int64_t max_slots = 0;
+  void *d_scratch = nullptr;  // [0] graph base step, then small outputs
+  void *h_scratch = nullptr;  // pinned mirror of small outputs
+  int32_t *nbase_dev = nullptr;
+  std::vector<double> force;
+  int64_t step = 0;
+  // timing of the step kernels since the last gf_sim_sync_timing
+  bool timing_open = false;
+  int64_t launches = 0;
+  // graph of kGraphSteps steps
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_probe = false;
+
+  ~Sim();
+  int init(const gf_model &m, int32_t steps_limit);
+  int upload_desc();
+  int set_force(const double *force, int64_t n);
+  int set_receivers(const int32_t *cols, int32_t ncols, int64_t cap, int64_t rec_base);
+  int reset_max_slots(int64_t steps);
+  int launch_one(int64_t n, bool probe);
+  int advance(int64_t n, bool use_graph, bool probe);
+  int ensure_graph(bool probe);
+};
+
+}  // namespace gfs

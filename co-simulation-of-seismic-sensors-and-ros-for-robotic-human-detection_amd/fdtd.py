@@ -1,0 +1,505 @@
+"""Python face of libgeofdtd: the reference's fdtd package API over the C ABI.
+
+The names and argument meanings follow internal/fdtd of the reference
+(Ryan-Hancock/Co-Simulation-of-Seismic-Sensors-and-ROS-for-Robotic-Human-Detection):
+
+    Model, Layer, SLS        fdtd.Model (model.go:37-76), layer.Layer, visco.SLS
+    New(model) -> Sim        fdtd.New (fdtd.go:45-97)
+    Sim.Dt/Cells/Time/Drive/Column/Step/SurfaceVelocity/RadialVelocity/MaxSpeed
+                             fdtd.go:118-360
+    Run(Shot) -> Result      fdtd.Run (run.go:88-127)
+    Ricker, RickerSpectrum   run.go:19-46
+    Fit                      visco.Fit (visco.go:55-68)
+
+Errors the Go code returns as `error` values raise FdtdError with the same
+message.  Every call goes through libgeofdtd.so and the GPU; there is no CPU
+fallback, and a missing library or device raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgeofdtd.so")
+
+GF_OK, GF_ERR_INVALID, GF_ERR_DIVERGED, GF_ERR_DEVICE, GF_ERR_ARGUMENT = 0, 1, 2, 3, 4
+
+
+class FdtdError(RuntimeError):
+    """An error value from the solver; .status is the C ABI status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+# ---------------------------------------------------------------- ABI structs
+
+
+class GfLayer(C.Structure):
+    _fields_ = [
+        ("thickness_m", C.c_double),
+        ("vp_mps", C.c_double),
+        ("vs_mps", C.c_double),
+        ("density_kgm3", C.c_double),
+        ("qs", C.c_double),
+        ("qp", C.c_double),
+    ]
+
+
+class GfSls(C.Structure):
+    _fields_ = [("tau_sigma", C.c_double), ("tau_eps", C.c_double)]
+
+
+class GfModel(C.Structure):
+    _fields_ = [
+        ("stack", C.POINTER(GfLayer)),
+        ("n_layers", C.c_int32),
+        ("relax", GfSls),
+        ("ref_freq_hz", C.c_double),
+        ("max_range_m", C.c_double),
+        ("depth_m", C.c_double),
+        ("spacing_m", C.c_double),
+        ("pml_cells", C.c_int32),
+        ("pml_reflection", C.c_double),
+        ("dominant_freq_hz", C.c_double),
+        ("courant", C.c_double),
+        ("gsls_mechanisms", C.c_int32),
+        ("gsls_fmin_hz", C.c_double),
+        ("gsls_fmax_hz", C.c_double),
+    ]
+
+
+class GfShot(C.Structure):
+    _fields_ = [
+        ("model", C.POINTER(GfModel)),
+        ("force", C.POINTER(C.c_double)),
+        ("n_force", C.c_int64),
+        ("ranges", C.POINTER(C.c_double)),
+        ("n_ranges", C.c_int32),
+        ("steps", C.c_int32),
+    ]
+
+
+class GfResult(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("t0", C.c_double),
+        ("nr", C.c_int32),
+        ("nz", C.c_int32),
+        ("peak", C.c_double),
+        ("traces", C.POINTER(C.c_double)),
+        ("snapped", C.POINTER(C.c_double)),
+        ("status", C.c_int32),
+        ("error", C.c_char * 244),
+    ]
+
+
+# Every symbol include/geofdtd.h declares, with its ctypes signature.
+_D, _I, _I32, _I64 = C.c_double, C.c_int, C.c_int32, C.c_int64
+_PD, _PI32, _PI64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+SIGNATURES = {
+    "gf_last_error": (C.c_char_p, []),
+    "gf_abi_version": (_I, []),
+    "gf_visco_fit": (_I, [_D, _D, C.POINTER(GfSls)]),
+    "gf_spacing_for": (_I, [C.POINTER(GfModel), _D, _D, _PD]),
+    "gf_ricker": (_I, [_D, _D, _D, _I32, _PD]),
+    "gf_ricker_spectrum": (_I, [_D, _D, _D, _PD, _PD]),
+    "gf_sim_create": (_I, [C.POINTER(GfModel), C.POINTER(C.c_void_p)]),
+    "gf_sim_destroy": (None, [C.c_void_p]),
+    "gf_sim_info": (_I, [C.c_void_p, _PD, _PI32, _PI32]),
+    "gf_sim_time": (_I, [C.c_void_p, _PD]),
+    "gf_sim_drive": (_I, [C.c_void_p, _PD, _I64]),
+    "gf_sim_column": (_I, [C.c_void_p, _D, _PI32, _PD]),
+    "gf_sim_step": (_I, [C.c_void_p]),
+    "gf_sim_step_n": (_I, [C.c_void_p, _I32, _PI32, _I32, _PD]),
+    "gf_sim_surface_velocity": (_I, [C.c_void_p, _I32, _PD]),
+    "gf_sim_radial_velocity": (_I, [C.c_void_p, _I32, _PD]),
+    "gf_sim_max_speed": (_I, [C.c_void_p, _PD]),
+    "gf_run": (_I, [C.POINTER(GfShot), C.POINTER(GfResult)]),
+    "gf_run_batch": (_I, [C.POINTER(GfShot), _I32, C.POINTER(GfResult), _PI32]),
+    "gf_sim_set_receivers": (_I, [C.c_void_p, _PI32, _I32, _I32]),
+    "gf_sim_advance": (_I, [C.c_void_p, _I32, _I32]),
+    "gf_sim_sync_timing": (_I, [C.c_void_p, _PD, _PI64]),
+    "gf_sim_read_traces": (_I, [C.c_void_p, _PD, _PI32]),
+    "gf_sim_step_bytes": (_I, [C.c_void_p, _PD]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libgeofdtd.so (built by __graft_entry__.build()).  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FdtdError(GF_ERR_DEVICE, f"libgeofdtd.so is not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != GF_OK:
+        msg = load_library().gf_last_error().decode()
+        raise FdtdError(rc, msg)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+# ------------------------------------------------------------ Go-shaped types
+
+
+@dataclass
+class Layer:
+    """layer.Layer (internal/layer/layer.go:24-36); Qp is an extension (0 = Qs)."""
+
+    Thickness: float = 0.0
+    Vp: float = 0.0
+    Vs: float = 0.0
+    Density: float = 0.0
+    Qs: float = 0.0
+    Qp: float = 0.0
+
+
+@dataclass
+class SLS:
+    """visco.SLS (internal/visco/visco.go:42-47)."""
+
+    TauSigma: float = 0.0
+    TauEps: float = 0.0
+
+
+@dataclass
+class Model:
+    """fdtd.Model (internal/fdtd/model.go:37-76).  Zero values take the
+    reference's defaults.  GslsMechanisms > 0 selects the per-layer,
+    multi-mechanism attenuation extension."""
+
+    Stack: List[Layer] = field(default_factory=list)
+    Relax: SLS = field(default_factory=SLS)
+    RefFreq: float = 0.0
+    MaxRange: float = 0.0
+    Depth: float = 0.0
+    Spacing: float = 0.0
+    PMLCells: int = 0
+    PMLReflection: float = 0.0
+    DominantFreq: float = 0.0
+    Courant: float = 0.0
+    GslsMechanisms: int = 0
+    GslsFmin: float = 0.0
+    GslsFmax: float = 0.0
+
+    def to_c(self):
+        """(GfModel, keep-alive array of layers)."""
+        layers = (GfLayer * max(1, len(self.Stack)))()
+        for i, l in enumerate(self.Stack):
+            layers[i] = GfLayer(l.Thickness, l.Vp, l.Vs, l.Density, l.Qs, l.Qp)
+        m = GfModel(
+            C.cast(layers, C.POINTER(GfLayer)),
+            len(self.Stack),
+            GfSls(self.Relax.TauSigma, self.Relax.TauEps),
+            self.RefFreq,
+            self.MaxRange,
+            self.Depth,
+            self.Spacing,
+            self.PMLCells,
+            self.PMLReflection,
+            self.DominantFreq,
+            self.Courant,
+            self.GslsMechanisms,
+            self.GslsFmin,
+            self.GslsFmax,
+        )
+        return m, layers
+
+    def SpacingFor(self, max_freq: float, cells_per_wavelength: float) -> float:
+        """Model.SpacingFor (model.go:121-136)."""
+        lib = load_library()
+        m, keep = self.to_c()
+        out = C.c_double()
+        _check(lib.gf_spacing_for(C.byref(m), max_freq, cells_per_wavelength, C.byref(out)))
+        return out.value
+
+
+def Fit(q: float, f0: float) -> SLS:
+    """visco.Fit (visco.go:55-68)."""
+    lib = load_library()
+    s = GfSls()
+    _check(lib.gf_visco_fit(q, f0, C.byref(s)))
+    return SLS(s.tau_sigma, s.tau_eps)
+
+
+def Ricker(peak: float, f0: float, dt: float, n: int) -> np.ndarray:
+    """fdtd.Ricker (run.go:19-31)."""
+    lib = load_library()
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    _check(lib.gf_ricker(peak, f0, dt, n, _dptr(out)))
+    return out[:n]
+
+
+def RickerSpectrum(peak: float, f0: float, f: float) -> complex:
+    """fdtd.RickerSpectrum (run.go:41-46)."""
+    lib = load_library()
+    re, im = C.c_double(), C.c_double()
+    _check(lib.gf_ricker_spectrum(peak, f0, f, C.byref(re), C.byref(im)))
+    return complex(re.value, im.value)
+
+
+class Sim:
+    """*fdtd.Sim (fdtd.go:20-42) on the GPU.  Use New(model)."""
+
+    def __init__(self, handle: C.c_void_p, model: Model):
+        self._h = handle
+        self._model = model
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load_library().gf_sim_destroy(h)
+            self._h = None
+
+    def close(self) -> None:
+        self.__del__()
+
+    def Dt(self) -> float:
+        dt = C.c_double()
+        _check(load_library().gf_sim_info(self._h, C.byref(dt), None, None))
+        return dt.value
+
+    def Cells(self):
+        nr, nz = C.c_int32(), C.c_int32()
+        _check(load_library().gf_sim_info(self._h, None, C.byref(nr), C.byref(nz)))
+        return nr.value, nz.value
+
+    def Time(self) -> float:
+        t = C.c_double()
+        _check(load_library().gf_sim_time(self._h, C.byref(t)))
+        return t.value
+
+    def Drive(self, force: Sequence[float]) -> None:
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        _check(load_library().gf_sim_drive(self._h, _dptr(f), len(f)))
+
+    def Column(self, r: float):
+        c, snapped = C.c_int32(), C.c_double()
+        _check(load_library().gf_sim_column(self._h, r, C.byref(c), C.byref(snapped)))
+        return c.value, snapped.value
+
+    def Step(self) -> None:
+        _check(load_library().gf_sim_step(self._h))
+
+    def StepN(self, n: int, cols: Sequence[int] = ()) -> np.ndarray:
+        """n Steps, recording SurfaceVelocity at cols after each: [len(cols), n]."""
+        cols_a = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.zeros((len(cols_a), max(n, 0)), dtype=np.float64)
+        _check(
+            load_library().gf_sim_step_n(
+                self._h, n, cols_a.ctypes.data_as(_PI32), len(cols_a), _dptr(out) if len(cols_a) else None
+            )
+        )
+        return out
+
+    def SurfaceVelocity(self, i: int) -> float:
+        v = C.c_double()
+        _check(load_library().gf_sim_surface_velocity(self._h, i, C.byref(v)))
+        return v.value
+
+    def RadialVelocity(self, i: int) -> float:
+        v = C.c_double()
+        _check(load_library().gf_sim_radial_velocity(self._h, i, C.byref(v)))
+        return v.value
+
+    def MaxSpeed(self) -> float:
+        v = C.c_double()
+        _check(load_library().gf_sim_max_speed(self._h, C.byref(v)))
+        return v.value
+
+    # device-resident benchmarking
+    def SetReceivers(self, cols: Sequence[int], capacity_steps: int) -> None:
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        _check(load_library().gf_sim_set_receivers(self._h, c.ctypes.data_as(_PI32), len(c), capacity_steps))
+
+    def Advance(self, n: int, use_graph: bool = True) -> None:
+        _check(load_library().gf_sim_advance(self._h, n, 1 if use_graph else 0))
+
+    def SyncTiming(self):
+        ms, launches = C.c_double(), C.c_int64()
+        _check(load_library().gf_sim_sync_timing(self._h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def ReadTraces(self, ncols: int) -> np.ndarray:
+        n = C.c_int32()
+        _check(load_library().gf_sim_read_traces(self._h, None, C.byref(n)))
+        out = np.zeros((ncols, n.value), dtype=np.float64)
+        _check(load_library().gf_sim_read_traces(self._h, _dptr(out), C.byref(n)))
+        return out
+
+    def StepBytes(self) -> float:
+        b = C.c_double()
+        _check(load_library().gf_sim_step_bytes(self._h, C.byref(b)))
+        return b.value
+
+
+def New(m: Model) -> Sim:
+    """fdtd.New (fdtd.go:45-97)."""
+    lib = load_library()
+    cm, keep = m.to_c()
+    h = C.c_void_p()
+    _check(lib.gf_sim_create(C.byref(cm), C.byref(h)))
+    return Sim(h, m)
+
+
+@dataclass
+class Shot:
+    """fdtd.Shot (run.go:49-61)."""
+
+    Model: Model
+    Force: Sequence[float]
+    Ranges: Sequence[float]
+    Steps: int
+    Progress: Optional[Callable[[int, int], None]] = None
+
+
+@dataclass
+class Trace:
+    """fdtd.Trace (run.go:63-70)."""
+
+    Range: float
+    Vertical: np.ndarray
+
+
+@dataclass
+class Result:
+    """fdtd.Result (run.go:72-86)."""
+
+    Dt: float
+    T0: float
+    Traces: List[Trace]
+    Cells: tuple
+    Peak: float
+
+
+def _shot_c(sh: Shot):
+    cm, keep_l = sh.Model.to_c()
+    force = np.ascontiguousarray(sh.Force, dtype=np.float64)
+    ranges = np.ascontiguousarray(sh.Ranges, dtype=np.float64)
+    c = GfShot(
+        C.pointer(cm),
+        _dptr(force) if len(force) else None,
+        len(force),
+        _dptr(ranges) if len(ranges) else None,
+        len(ranges),
+        sh.Steps,
+    )
+    return c, (cm, keep_l, force, ranges)
+
+
+def RunBatch(shots: Sequence[Shot]) -> List[object]:
+    """Many shots advanced together on the GPU; each entry is a Result or the
+    FdtdError that shot's fdtd.Run would have returned."""
+    lib = load_library()
+    n = len(shots)
+    cshots = (GfShot * n)()
+    cres = (GfResult * n)()
+    keep, outs = [], []
+    for i, sh in enumerate(shots):
+        c, k = _shot_c(sh)
+        cshots[i] = c
+        keep.append(k)
+        tr = np.zeros((max(len(sh.Ranges), 1), max(sh.Steps, 1)), dtype=np.float64)
+        sn = np.zeros(max(len(sh.Ranges), 1), dtype=np.float64)
+        outs.append((tr, sn))
+        cres[i].traces = _dptr(tr)
+        cres[i].snapped = _dptr(sn)
+    statuses = np.zeros(n, dtype=np.int32)
+    rc = lib.gf_run_batch(cshots, n, cres, statuses.ctypes.data_as(_PI32))
+    if rc != GF_OK and not statuses.any():
+        _check(rc)
+    results: List[object] = []
+    for i, sh in enumerate(shots):
+        if statuses[i] != GF_OK:
+            results.append(FdtdError(int(statuses[i]), cres[i].error.decode()))
+            continue
+        tr, sn = outs[i]
+        r = cres[i]
+        traces = [Trace(float(sn[k]), tr[k, : sh.Steps].copy()) for k in range(len(sh.Ranges))]
+        results.append(Result(r.dt, r.t0, traces, (r.nr, r.nz), r.peak))
+    return results
+
+
+def Run(sh: Shot) -> Result:
+    """fdtd.Run (run.go:88-127).  Progress, when set, is called every 500
+    steps as the reference does; the steps run in blocks between calls."""
+    if sh.Progress is None:
+        r = RunBatch([sh])[0]
+        if isinstance(r, FdtdError):
+            raise r
+        return r
+    return _run_with_progress(sh)
+
+
+def _run_with_progress(sh: Shot) -> Result:
+    """Run's loop (run.go:104-126) driven from the host: steps run on the GPU
+    in blocks that end where the reference reports progress (every 500 steps)
+    or probes for divergence (every 256)."""
+    s = New(sh.Model)
+    try:
+        if sh.Steps <= 0:
+            raise FdtdError(GF_ERR_INVALID, "fdtd: a shot needs a positive number of steps")
+        s.Drive(sh.Force)
+        cols, snapped = [], []
+        for r in sh.Ranges:
+            c, sn = s.Column(r)
+            if sn >= sh.Model.MaxRange:
+                raise FdtdError(
+                    GF_ERR_INVALID,
+                    f"fdtd: receiver at {_go_g(r)} m is outside the interior ({_go_g(sh.Model.MaxRange)} m)",
+                )
+            cols.append(c)
+            snapped.append(sn)
+        out = np.zeros((len(cols), sh.Steps), dtype=np.float64)
+        peak = 0.0
+        n = 0
+        while n < sh.Steps:
+            # next step index that triggers an event
+            nxt = n
+            while nxt < sh.Steps - 1 and nxt % 500 != 0 and nxt % 256 != 0:
+                nxt += 1
+            block = nxt - n + 1
+            out[:, n : nxt + 1] = s.StepN(block, cols)
+            if nxt % 500 == 0:
+                sh.Progress(nxt, sh.Steps)
+            if nxt % 256 == 0:
+                p = s.MaxSpeed()
+                peak = p if (np.isnan(p) or p > peak) else peak
+                if np.isnan(p) or np.isinf(p):
+                    raise FdtdError(GF_ERR_DIVERGED, f"fdtd: diverged at step {nxt} of {sh.Steps}")
+            n = nxt + 1
+        dt = s.Dt()
+        traces = [Trace(snapped[k], out[k]) for k in range(len(cols))]
+        return Result(dt, 0.5 * dt, traces, s.Cells(), peak)
+    finally:
+        s.close()
+
+
+def _go_g(x: float) -> str:
+    """Go's %g: shortest round-trip digits."""
+    for p in range(1, 18):
+        t = "%.*g" % (p, x)
+        if float(t) == x:
+            return t
+    return repr(x)

@@ -1,0 +1,194 @@
+/*
+ * geofdtd.h — C ABI of libgeofdtd, the MI355X (gfx950) axisymmetric
+ * viscoelastic FDTD solver.
+ *
+ * The entry points mirror the Go package internal/fdtd of the reference
+ * (Ryan-Hancock/Co-Simulation-of-Seismic-Sensors-and-ROS-for-Robotic-Human-Detection)
+ * one for one, so that a cgo shim can keep that package's API unchanged:
+ * each declaration below names the Go symbol it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status (GF_OK == 0).  On failure,
+ *     gf_last_error() returns a thread-local message with the same text the
+ *     Go code would have put in its error value.
+ *   - All buffers are caller-owned.  The library copies what it needs at call
+ *     time and keeps no caller pointer after returning (the cgo pointer rule).
+ *   - Handles are opaque and single-threaded, like *fdtd.Sim; distinct handles
+ *     may be used from distinct threads.  Calls block the calling thread.
+ *   - Results leave the library as float64, like the reference's
+ *     []units.Velocity; the GPU computes in float32 with float32 field state.
+ */
+#ifndef GEOFDTD_H
+#define GEOFDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF_ABI_VERSION 1
+
+enum gf_status {
+  GF_OK = 0,
+  GF_ERR_INVALID = 1,  /* a model, shot or stack the reference rejects     */
+  GF_ERR_DIVERGED = 2, /* "fdtd: diverged at step %d of %d" (run.go:118-124) */
+  GF_ERR_DEVICE = 3,   /* HIP runtime failure, or no gfx950 device          */
+  GF_ERR_ARGUMENT = 4  /* null pointer, negative size, bad handle           */
+};
+
+/* layer.Layer (internal/layer/layer.go:24-36).  The last entry of a stack is
+ * the half-space and its thickness is ignored.  qp is an extension used only
+ * by the GSLS attenuation model (gf_model.gsls_mechanisms > 0); zero there
+ * means qp == qs.  qs == 0 means lossless, as in the reference. */
+typedef struct gf_layer {
+  double thickness_m;
+  double vp_mps;
+  double vs_mps;
+  double density_kgm3;
+  double qs;
+  double qp;
+} gf_layer;
+
+/* visco.SLS (internal/visco/visco.go:42-47). */
+typedef struct gf_sls {
+  double tau_sigma;
+  double tau_eps;
+} gf_sls;
+
+/* fdtd.Model (internal/fdtd/model.go:37-76).  Zero values select the same
+ * defaults as the reference: ref_freq 30 Hz, pml_cells 20, pml_reflection
+ * 1e-4, courant 0.5, dominant_freq "middle of the resolved band".
+ *
+ * Extension (not in the reference; zero reproduces the reference exactly):
+ *   gsls_mechanisms  0: one scalar SLS `relax` over the whole model, Qp = Qs
+ *                       (the reference physics).
+ *                    N in [1, 8]: per-layer attenuation from gf_layer.qs/qp,
+ *                       N relaxation mechanisms fitted to constant Q over
+ *                       [gsls_fmin_hz, gsls_fmax_hz] (default 5-150 Hz),
+ *                       separate P and S relaxation.  `relax` is ignored. */
+typedef struct gf_model {
+  const gf_layer *stack;
+  int32_t n_layers;
+  gf_sls relax;
+  double ref_freq_hz;
+  double max_range_m;
+  double depth_m;
+  double spacing_m;
+  int32_t pml_cells;
+  double pml_reflection;
+  double dominant_freq_hz;
+  double courant;
+  int32_t gsls_mechanisms;
+  double gsls_fmin_hz;
+  double gsls_fmax_hz;
+} gf_model;
+
+/* fdtd.Shot (internal/fdtd/run.go:49-61), minus Progress: a caller that wants
+ * progress drives gf_sim_step_n in blocks and reports between them. */
+typedef struct gf_shot {
+  const gf_model *model;
+  const double *force; /* one sample per step; samples past n_force are 0 */
+  int64_t n_force;
+  const double *ranges; /* receiver ranges, m */
+  int32_t n_ranges;
+  int32_t steps;
+} gf_shot;
+
+/* fdtd.Result (internal/fdtd/run.go:72-86).  traces and snapped are
+ * caller-owned: traces is [n_ranges][steps] row-major, snapped [n_ranges].
+ * status and error carry this shot's outcome (the Go error text), so that a
+ * batch can report each shot's error on its own. */
+typedef struct gf_result {
+  double dt;
+  double t0;
+  int32_t nr;
+  int32_t nz;
+  double peak;
+  double *traces;
+  double *snapped;
+  int32_t status;
+  char error[244];
+} gf_result;
+
+/* Last error message of the calling thread ("" after a success). */
+const char *gf_last_error(void);
+int gf_abi_version(void);
+
+/* visco.Fit (internal/visco/visco.go:55-68). */
+int gf_visco_fit(double q, double f0_hz, gf_sls *out);
+
+/* fdtd.Model.SpacingFor (internal/fdtd/model.go:121-136). */
+int gf_spacing_for(const gf_model *m, double max_freq_hz,
+                   double cells_per_wavelength, double *out_spacing_m);
+
+/* fdtd.Ricker and fdtd.RickerSpectrum (internal/fdtd/run.go:19-46). */
+int gf_ricker(double peak_n, double f0_hz, double dt, int32_t n, double *out);
+int gf_ricker_spectrum(double peak_n, double f0_hz, double f_hz, double *re,
+                       double *im);
+
+/* ---- *fdtd.Sim ------------------------------------------------------- */
+typedef struct gf_sim gf_sim;
+
+/* fdtd.New (internal/fdtd/fdtd.go:45-97).  Allocates the grid in HBM and
+ * builds the coefficient tables in FP64 on the device. */
+int gf_sim_create(const gf_model *m, gf_sim **out);
+void gf_sim_destroy(gf_sim *s);
+
+/* Sim.Dt and Sim.Cells (fdtd.go:118-121). */
+int gf_sim_info(const gf_sim *s, double *dt, int32_t *nr, int32_t *nz);
+/* Sim.Time (fdtd.go:125). */
+int gf_sim_time(const gf_sim *s, double *out);
+/* Sim.Drive (fdtd.go:129).  Copies the series (the Go version retains it). */
+int gf_sim_drive(gf_sim *s, const double *force, int64_t n);
+/* Sim.Column (fdtd.go:142-146). */
+int gf_sim_column(const gf_sim *s, double r_m, int32_t *col, double *snapped_m);
+/* Sim.Step (fdtd.go:149-156). */
+int gf_sim_step(gf_sim *s);
+/* n calls of Sim.Step, recording Sim.SurfaceVelocity at each of cols after
+ * every step into out[c * n + k] (out may be NULL when ncols == 0). */
+int gf_sim_step_n(gf_sim *s, int32_t n, const int32_t *cols, int32_t ncols,
+                  double *out);
+/* Sim.SurfaceVelocity, Sim.RadialVelocity, Sim.MaxSpeed (fdtd.go:330-360). */
+int gf_sim_surface_velocity(gf_sim *s, int32_t col, double *out);
+int gf_sim_radial_velocity(gf_sim *s, int32_t col, double *out);
+int gf_sim_max_speed(gf_sim *s, double *out);
+
+/* ---- fdtd.Run ------------------------------------------------------- */
+
+/* fdtd.Run (internal/fdtd/run.go:88-127): one shot, start to finish. */
+int gf_run(const gf_shot *shot, gf_result *res);
+
+/* Many independent shots advanced together on one GPU (no reference
+ * counterpart as an API; it replaces running fdtd.Run once per shot in a
+ * worker pool, cmd/geobank/main.go:128-167).  Each result equals what gf_run
+ * returns for that shot.  A shot that diverges reports GF_ERR_DIVERGED in
+ * statuses[i] and does not stop the others; the call returns GF_OK if the
+ * batch ran, and the first per-shot error otherwise. */
+int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results,
+                 int32_t *statuses);
+
+/* ---- device-resident benchmarking -------------------------------------
+ * The timed region of a benchmark must start with inputs already in HBM, so
+ * these split gf_run into set-up, advance and read-back.  Receivers record
+ * into a device buffer of capacity_steps samples each. */
+int gf_sim_set_receivers(gf_sim *s, const int32_t *cols, int32_t ncols,
+                         int32_t capacity_steps);
+/* Advance n steps on the device without host synchronisation per step.
+ * use_graph != 0 replays the steps from captured HIP graphs. */
+int gf_sim_advance(gf_sim *s, int32_t n, int32_t use_graph);
+/* Wait for the device, then report the elapsed time of every step kernel
+ * launched since the last call, measured with HIP events on the solver's
+ * stream, and how many step kernels that covers. */
+int gf_sim_sync_timing(gf_sim *s, double *kernel_ms, int64_t *kernel_launches);
+/* Copy the recorded receiver samples out: out[c * steps_recorded + k]. */
+int gf_sim_read_traces(gf_sim *s, double *out, int32_t *steps_recorded);
+/* Algorithmic HBM bytes one step moves (each state array read and written
+ * once, plus the C-PML strips): the roofline numerator. */
+int gf_sim_step_bytes(const gf_sim *s, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEOFDTD_H */
