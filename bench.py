@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark: Green's-function traces per second of the MI355X FDTD solver.
+
+One benchmark "step" is one complete shot of the workload: every time step of
+the FDTD run over one soil model, recording every receiver.  With the default
+config-2 workload that is 80,000 time steps of a 2048 x 1024 grid and 100
+surface traces.  Inputs (model tables, source series, receiver columns) are
+resident in HBM before the timed region; each timed shot resets the fields on
+the device and runs to the end, leaving its traces in HBM.
+
+Printed (one JSON line on stdout):
+  value       traces/s over the timed shots (wall clock, bracketed by device syncs)
+  roofline    the step kernel's algorithmic HBM bytes per launch over its
+              average launch time (HIP events on the solver's stream), against
+              MI355X's 8 TB/s; `traffic` is the rocprofv3-measured HBM bytes
+              per launch when a committed profile summary provides it
+  cpu_baseline the FP64 CPU restatement of the reference scheme on the same
+              workload, a bounded sample of time steps on every worker thread,
+              scaled to whole shots
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "co-simulation-of-seismic-sensors-and-ros-for-robotic-human-detection_amd")
+METRIC = "GF traces/s: batched FDTD vs surrogate at stated L2 err; stencil HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_package():
+    name = "cosim_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(
+        name, os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR]
+    )
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(name: str):
+    import cosim_amd.workloads as W
+
+    kind, variant = name.split("-")
+    if kind == "c1":
+        return W.c1(variant)
+    if kind == "c2":
+        return W.c2(variant)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def describe(w, nr, nz, dt) -> dict:
+    return {
+        "workload": f"{w.name}: {nr}x{nz} cells, h={w.model.Spacing:g} m, {w.steps} steps of dt={dt:.3g} s, "
+                    f"{len(w.ranges)} receivers",
+        "grid": [nr, nz],
+        "time_steps_per_shot": w.steps,
+        "receivers": len(w.ranges),
+        "physics": "one scalar SLS (reference form)" if w.model.GslsMechanisms == 0
+        else f"GSLS, {w.model.GslsMechanisms} mechanisms, per-layer Qs/Qp",
+        "source": "GRF footstep" if w.name.startswith("c2") else "Ricker",
+    }
+
+
+def traffic_from_profiles(workload_name: str):
+    """HBM bytes per step launch measured by rocprofv3 --pmc, if committed."""
+    path = os.path.join(ROOT, "profiles", f"k_step_{workload_name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(w, gpu_traces, sample_steps: int, threads: int) -> tuple:
+    """Time the FP64 oracle on `threads` concurrent copies of the shot for
+    `sample_steps` time steps; scale to whole shots.  Also compares the GPU's
+    traces over that window with the oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    O.lib()
+    steps = [sample_steps] * threads
+    t0 = time.perf_counter()
+    outs = O.run_many([w.model] * threads, [w.force] * threads, w.ranges, steps, threads)
+    elapsed = time.perf_counter() - t0
+    shot_seconds = elapsed * w.steps / sample_steps  # each thread ran one shot's prefix
+    traces_per_s = threads * len(w.ranges) / shot_seconds
+    nr, nz = w.model.Grid()[1:]
+    cell_steps_per_s_core = nr * nz * sample_steps / elapsed
+    ref = outs[0]
+    errs = []
+    for k in range(len(w.ranges)):
+        b = ref[k]
+        nb = float(np.linalg.norm(b))
+        if nb > 0:
+            errs.append(float(np.linalg.norm(gpu_traces[k, :sample_steps] - b) / nb))
+    base = {
+        "value": traces_per_s,
+        "unit": "traces/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{threads} concurrent shots x {sample_steps} of {w.steps} time steps "
+                  f"({elapsed:.1f} s), scaled by time steps; one shot per thread",
+        "cell_steps_per_s_per_core": cell_steps_per_s_core,
+    }
+    acc = {
+        "rel_l2_max": max(errs) if errs else None,
+        "rel_l2_median": float(np.median(errs)) if errs else None,
+        "traces_compared": len(errs),
+        "window_steps": sample_steps,
+        "against": "FP64 CPU restatement of the reference scheme, same shot",
+    }
+    return base, acc
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--workload", default="c2-R", help="c2-R (default), c2-X, c1-R, c1-X")
+    ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
+    ap.add_argument("--cpu-sample-steps", type=int, default=240)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu_count)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="shorten every shot to this many time steps (profiling runs; recorded in config)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, run 5 plain device copies of MIB MiB (the HBM counter "
+                         "calibration for rocprofv3 --pmc runs) and exit")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return 0
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); every shot is independent, so "
+              "N GPUs would each run their own shots with no exchange — see DESIGN.md", file=sys.stderr)
+        return 2
+
+    pkg = load_package()
+    fd = pkg.fdtd
+    import numpy as np
+
+    w = workload(args.workload)
+    if args.time_steps:
+        w.steps = args.time_steps
+    dt, nr, nz = w.model.Grid()
+    log(f"workload {w.name}: {nr}x{nz}, {w.steps} steps, {len(w.ranges)} receivers")
+    sim = fd.New(w.model)
+    sim.Drive(w.force)
+    cols = [sim.Column(r)[0] for r in w.ranges]
+    sim.SetReceivers(cols, w.steps)
+    use_graph = not args.no_graph
+    step_bytes = sim.StepBytes()
+
+    for i in range(args.warmup):
+        sim.Reset()
+        sim.Advance(w.steps, use_graph)
+        sim.SyncTiming()
+        log(f"warmup shot {i + 1}/{args.warmup} done")
+
+    sim.SyncTiming()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sim.Reset()
+        sim.Advance(w.steps, use_graph)
+    kernel_ms, launches = sim.SyncTiming()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} shots in {wall:.3f} s; {launches} step launches in {kernel_ms:.1f} ms of device time")
+
+    traces = sim.ReadTraces(len(cols))
+    if not np.all(np.isfinite(traces)):
+        raise SystemExit("non-finite traces")
+    total_traces = len(w.ranges) * args.steps
+    value = total_traces / wall
+    avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = step_bytes / avg_launch_s / 1e9
+    roof = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic_from_profiles(w.name),
+        "kernel": "k_step (fused velocity+stress+memory-variable step)",
+        "algorithmic_bytes_per_launch": step_bytes,
+        "avg_launch_us": avg_launch_s * 1e6,
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "traces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": describe(w, nr, nz, dt),
+        "roofline": roof,
+        "cells_per_s": nr * nz * w.steps * args.steps / wall,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {threads} threads x {args.cpu_sample_steps} steps")
+        base, acc = cpu_baseline(w, traces, args.cpu_sample_steps, threads)
+        out["cpu_baseline"] = base
+        out["accuracy"] = acc
+        out["speedup_vs_cpu"] = value / base["value"]
+    else:
+        out["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -104,6 +104,14 @@ Sim::~Sim() {
   if (stream) hipStreamDestroy(stream);
 }
 
+// Zero the state part of the arena (fields, psi strips, memory variables) with
+// a fill kernel on the solver's stream; tables and buffers stay.
+int Sim::reset_state() {
+  HIPCHK(gfk::launch_fill_f32((float *)arena, (int64_t)(state_bytes / sizeof(float)), 0.f, stream));
+  step = 0;
+  return GF_OK;
+}
+
 int Sim::upload_desc() {
   HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipMemcpy(d_desc, &g, sizeof g, hipMemcpyHostToDevice));
@@ -144,6 +152,7 @@ int Sim::init(const gf_model &m, int32_t steps_limit) {
     total += align_up(p.floats * sizeof(float), 256);
   }
   arena_bytes = total;
+  state_bytes = parts[p_rowc].off;  // fields, strips and memory come first
   HIPCHK(hipMalloc(&arena, total));
   HIPCHK(hipMemsetAsync(arena, 0, total, stream));
   char *base = (char *)arena;
@@ -360,6 +369,25 @@ int gf_ricker_spectrum(double peak, double f0, double f, double *re, double *im)
   if (!re || !im) return set_err(GF_ERR_ARGUMENT, "geofdtd: null output");
   gfm::ricker_spectrum(peak, f0, f, *re, *im);
   return ok();
+}
+
+int gf_model_grid(const gf_model *m, double *dt, int32_t *nr, int32_t *nz) {
+  if (!m) return set_err(GF_ERR_ARGUMENT, "geofdtd: null model");
+  gfm::Resolved R;
+  std::string err;
+  int rc = gfm::resolve(*m, R, err);
+  if (rc) return set_err(rc, err);
+  if (dt) *dt = R.dt;
+  if (nr) *nr = R.nr;
+  if (nz) *nz = R.nz;
+  return ok();
+}
+
+int gf_sim_reset(gf_sim *sp) {
+  Sim *s = reinterpret_cast<Sim *>(sp);
+  if (!s) return set_err(GF_ERR_ARGUMENT, "geofdtd: null handle");
+  int rc = s->reset_state();
+  return rc ? rc : ok();
 }
 
 int gf_sim_create(const gf_model *m, gf_sim **out) {

@@ -48,6 +48,7 @@ struct Sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *arena = nullptr;
   size_t arena_bytes = 0;
+  size_t state_bytes = 0;  // leading part of the arena that holds state
   float *rowc = nullptr, *colc = nullptr;
   float *d_src = nullptr;
   int32_t *d_rec = nullptr;
@@ -71,6 +72,7 @@ struct Sim {
   ~Sim();
   int init(const gf_model &m, int32_t steps_limit);
   int upload_desc();
+  int reset_state();
   int set_force(const double *force, int64_t n);
   int set_receivers(const int32_t *cols, int32_t ncols, int64_t cap, int64_t rec_base);
   int reset_max_slots(int64_t steps);

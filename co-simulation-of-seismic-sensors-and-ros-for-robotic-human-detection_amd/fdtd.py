@@ -111,6 +111,8 @@ SIGNATURES = {
     "gf_spacing_for": (_I, [C.POINTER(GfModel), _D, _D, _PD]),
     "gf_ricker": (_I, [_D, _D, _D, _I32, _PD]),
     "gf_ricker_spectrum": (_I, [_D, _D, _D, _PD, _PD]),
+    "gf_model_grid": (_I, [C.POINTER(GfModel), _PD, _PI32, _PI32]),
+    "gf_sim_reset": (_I, [C.c_void_p]),
     "gf_sim_create": (_I, [C.POINTER(GfModel), C.POINTER(C.c_void_p)]),
     "gf_sim_destroy": (None, [C.c_void_p]),
     "gf_sim_info": (_I, [C.c_void_p, _PD, _PI32, _PI32]),
@@ -226,6 +228,14 @@ class Model:
         )
         return m, layers
 
+    def Grid(self):
+        """(dt, nr, nz) that New would use, computed on the host only."""
+        lib = load_library()
+        m, keep = self.to_c()
+        dt, nr, nz = C.c_double(), C.c_int32(), C.c_int32()
+        _check(lib.gf_model_grid(C.byref(m), C.byref(dt), C.byref(nr), C.byref(nz)))
+        return dt.value, nr.value, nz.value
+
     def SpacingFor(self, max_freq: float, cells_per_wavelength: float) -> float:
         """Model.SpacingFor (model.go:121-136)."""
         lib = load_library()
@@ -327,6 +337,10 @@ class Sim:
         v = C.c_double()
         _check(load_library().gf_sim_max_speed(self._h, C.byref(v)))
         return v.value
+
+    def Reset(self) -> None:
+        """Back to the state New leaves (step 0, zero fields); keeps set-up."""
+        _check(load_library().gf_sim_reset(self._h))
 
     # device-resident benchmarking
     def SetReceivers(self, cols: Sequence[int], capacity_steps: int) -> None:

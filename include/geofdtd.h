@@ -128,6 +128,10 @@ int gf_ricker(double peak_n, double f0_hz, double dt, int32_t n, double *out);
 int gf_ricker_spectrum(double peak_n, double f0_hz, double f_hz, double *re,
                        double *im);
 
+/* The grid fdtd.New would build for a model (fdtd.go:45-64: validation,
+ * nr, nz, dt) without allocating anything or touching a device. */
+int gf_model_grid(const gf_model *m, double *dt, int32_t *nr, int32_t *nz);
+
 /* ---- *fdtd.Sim ------------------------------------------------------- */
 typedef struct gf_sim gf_sim;
 
@@ -135,6 +139,12 @@ typedef struct gf_sim gf_sim;
  * builds the coefficient tables in FP64 on the device. */
 int gf_sim_create(const gf_model *m, gf_sim **out);
 void gf_sim_destroy(gf_sim *s);
+
+/* Return a simulation to the state New leaves it in (all fields, memory
+ * variables and C-PML psi zero, step 0), keeping its allocation, tables,
+ * source and receivers.  No reference counterpart: it lets a caller run many
+ * shots of one model without re-allocating, as a benchmark does. */
+int gf_sim_reset(gf_sim *s);
 
 /* Sim.Dt and Sim.Cells (fdtd.go:118-121). */
 int gf_sim_info(const gf_sim *s, double *dt, int32_t *nr, int32_t *nz);
