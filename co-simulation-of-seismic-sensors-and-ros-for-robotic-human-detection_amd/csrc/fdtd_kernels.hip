@@ -47,8 +47,25 @@ __global__ void k_build_tables(TableParams P, const LayerParams *__restrict__ L,
 
 // ---------------------------------------------------------------- time step
 
-__device__ __forceinline__ float shfl_up1(float x) { return __shfl_up(x, 1, kLanes); }
-__device__ __forceinline__ float shfl_dn1(float x) { return __shfl_down(x, 1, kLanes); }
+// Neighbour lanes through DPP wave shifts: a VALU modifier, no LDS crossbar
+// round trip.  wave_shr:1 hands lane i the value of lane i-1, wave_shl:1 the
+// value of lane i+1; the lane that has no source gets 0 (its value is never
+// used: lane 0 needs no left neighbour, lane 63 no right one).
+__device__ __forceinline__ float lane_prev(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin.  Renumber them so that each
+// XCD gets one contiguous run of tiles: a tile's halo rows and columns then
+// belong to tiles on the same XCD and are shared through that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int x = b & 7, j = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + j;
+}
 
 // One wavefront = one tile; four tiles per 256-thread workgroup.  Tiles are
 // numbered model by model (GridDesc::tile_base), row-major within a model.
@@ -58,7 +75,8 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   const int lane = threadIdx.x & (kLanes - 1);
   // Wave-uniform by construction; readfirstlane lets the compiler keep the
   // descriptor reads scalar.
-  const int tile = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int tile = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
 
   int m = 0;
   if (nmodels > 1) {
@@ -85,36 +103,39 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
   const int rb = n & 1, wb = rb ^ 1;
   const ColConst cc = load_col(g, c);
-
-  const float *A_vr = g.f[rb][kVr], *A_vz = g.f[rb][kVz], *A_trr = g.f[rb][kTrr];
-  const float *A_tqq = g.f[rb][kTqq], *A_tzz = g.f[rb][kTzz], *A_trz = g.f[rb][kTrz];
-  float *B_vr = g.f[wb][kVr], *B_vz = g.f[wb][kVz], *B_trr = g.f[wb][kTrr];
-  float *B_tqq = g.f[wb][kTqq], *B_tzz = g.f[wb][kTzz], *B_trz = g.f[wb][kTrz];
+  const float *const *A = g.f[rb];
+  float *const *B = g.f[wb];
 
   // Carried state, named by row relative to the velocity row t being built.
   float trr_p = 0.f, tqq_p = 0.f;          // old trr, tqq at t-1
-  float trz_p = ld(A_trz, g, j0 - 2, c);   // old trz at t-1
+  float trz_p = ld(A[kTrz], g, j0 - 2, c); // old trz at t-1
   float tzz_p = 0.f;                       // old tzz at t-1
-  float tzz_c = ld_tzz(A_tzz, g, n, j0 - 1, c);  // old tzz at t
-  float vrn_p = 0.f, vzn_p = 0.f, vzn_pp = 0.f;  // new vr(t-1), vz(t-1), vz(t-2)
+  float tzz_c = ld_tzz(A[kTzz], g, n, j0 - 1, c);  // old tzz at t
+  float vrn_p = 0.f, vzn_p = 0.f, vzn_pp = 0.f;   // new vr(t-1), vz(t-1), vz(t-2)
   uint32_t mbits = 0;
+  MemVars<NM> mem_p;                       // memory variables of row t-1
+  RowLoad cur = load_row(A, g, n, j0 - 1, c);
 
   for (int t = j0 - 1; t <= j1; t++) {
-    const float trr = ld(A_trr, g, t, c), tqq = ld(A_tqq, g, t, c), trz = ld(A_trz, g, t, c);
-    const float tzz_dn = ld_tzz(A_tzz, g, n, t + 1, c);
-    const float vr_old = ld(A_vr, g, t, c), vz_old = ld(A_vz, g, t, c);
-    const float trr1 = shfl_dn1(trr), tqq1 = shfl_dn1(tqq), trzm = shfl_up1(trz);
+    // Issue next row's loads and this row's memory variables before any
+    // arithmetic, so their latency hides behind it.
+    RowLoad nxt;
+    if (t < j1) nxt = load_row(A, g, n, t + 1, c);
+    MemVars<NM> mem_c;
+    if (own && t >= j0 && t < j1) load_mem<NM>(g, t, c, mem_c);
+
+    const float trr1 = lane_next(cur.trr), tqq1 = lane_next(cur.tqq), trzm = lane_prev(cur.trz);
     const bool row_own = t >= j0 && t < j1;
     float vrn, vzn;
-    velocity(g, rb, t, c, row_own && own, cc, trr, trr1, tqq, tqq1, trz, trzm, trz_p, tzz_c, tzz_dn, vr_old, vz_old,
-             vrn, vzn);
+    velocity(g, rb, t, c, row_own && own, cc, cur.trr, trr1, cur.tqq, tqq1, cur.trz, trzm, trz_p, tzz_c,
+             cur.tzz_dn, cur.vr, cur.vz, vrn, vzn);
     if (row_own && own) {
       const size_t k = (size_t)t * g.pitch + c;
-      B_vr[k] = vrn;
-      B_vz[k] = vzn;
+      B[kVr][k] = vrn;
+      B[kVz][k] = vzn;
       if (do_max) mbits = max(mbits, max(abs_bits(vrn), abs_bits(vzn)));
     }
-    const float vrn_m = shfl_up1(vrn);
+    const float vrn_m = lane_prev(vrn);
     if (t == 0 && g.nrec > 0) {
       // Receivers sit on the surface row; the tile holding row 0 records them.
       const float sv = surface_velocity(g, c, cc, vrn, vrn_m, vzn);
@@ -128,26 +149,29 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
     }
     const int s = t - 1;
     if (s >= j0) {
-      const float vrn_pm = shfl_up1(vrn_p);   // new vr(s, c-1)
-      const float vzn_p1 = shfl_dn1(vzn_p);   // new vz(s, c+1)
+      const float vrn_pm = lane_prev(vrn_p);   // new vr(s, c-1)
+      const float vzn_p1 = lane_next(vzn_p);   // new vz(s, c+1)
       if (own) {
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
-        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz);
+        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_p);
         const size_t k = (size_t)s * g.pitch + c;
-        B_trr[k] = nrr;
-        B_tqq[k] = nqq;
-        B_tzz[k] = nzz;
-        B_trz[k] = nrz;
+        B[kTrr][k] = nrr;
+        B[kTqq][k] = nqq;
+        B[kTzz][k] = nzz;
+        B[kTrz][k] = nrz;
+        store_mem<NM>(g, s, c, mem_p);
       }
     }
-    trr_p = trr;
-    tqq_p = tqq;
-    trz_p = trz;
+    trr_p = cur.trr;
+    tqq_p = cur.tqq;
+    trz_p = cur.trz;
     tzz_p = tzz_c;
-    tzz_c = tzz_dn;
+    tzz_c = cur.tzz_dn;
     vzn_pp = vzn_p;
     vzn_p = vzn;
     vrn_p = vrn;
+    mem_p = mem_c;
+    cur = nxt;
   }
 
   if (do_max) {

@@ -133,6 +133,23 @@ GF_HD float ld_tzz(const float *base, const GridDesc &g, int n, int t, int c) {
   return ld(base, g, t, c);
 }
 
+// The six old field values of one cell that the velocity update of row t
+// reads (tzz one row down, for dtzz).
+struct RowLoad {
+  float trr, tqq, trz, tzz_dn, vr, vz;
+};
+
+GF_HD RowLoad load_row(const float *const *A, const GridDesc &g, int n, int t, int c) {
+  RowLoad L;
+  L.trr = ld(A[kTrr], g, t, c);
+  L.tqq = ld(A[kTqq], g, t, c);
+  L.trz = ld(A[kTrz], g, t, c);
+  L.tzz_dn = ld_tzz(A[kTzz], g, n, t + 1, c);
+  L.vr = ld(A[kVr], g, t, c);
+  L.vz = ld(A[kVz], g, t, c);
+  return L;
+}
+
 // New vr and vz at (t, c): Sim.velocities (fdtd.go:158-217) for one cell.
 // Inputs are the old stresses around the cell (tzz_dn = tzz(t+1, c),
 // trz_up = trz(t-1, c), *1 = column c+1, trzm = trz(t, c-1)).  Velocity psi
@@ -195,14 +212,43 @@ GF_HD float relax_cn(float decay, float &r, float drive_term) {
   return avg;
 }
 
+// Memory variables of one cell, [component rr, qq, zz, rz][mechanism].  The
+// kernel loads them a row ahead of use so their latency overlaps arithmetic.
+template <int NM>
+struct MemVars {
+  float r[4][NM > 0 ? NM : 1];
+};
+
+template <int NM>
+GF_HD void load_mem(const GridDesc &g, int s, int c, MemVars<NM> &m) {
+  if (NM == 0) return;
+  const size_t cell = (size_t)s * g.pitch + (size_t)c, ms = (size_t)g.mem_stride;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int q = 0; q < NM; q++) m.r[k][q] = g.mem[(size_t)(k * NM + q) * ms + cell];
+}
+
+template <int NM>
+GF_HD void store_mem(const GridDesc &g, int s, int c, const MemVars<NM> &m) {
+  if (NM == 0) return;
+  const size_t cell = (size_t)s * g.pitch + (size_t)c, ms = (size_t)g.mem_stride;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int q = 0; q < NM; q++) g.mem[(size_t)(k * NM + q) * ms + cell] = m.r[k][q];
+}
+
 // New stresses at (s, c): Sim.stresses (fdtd.go:219-307) for one cell.
 // vr/vz are this step's new velocities: vrm = vr(s, c-1), vr_dn = vr(s+1, c),
 // vz1 = vz(s, c+1), vz_up = vz(s-1, c).  trr..trz carry the old values in and
-// the new ones out.  NM = 0 is elastic; SC selects the reference's scalar
-// relaxation (one drive for P and S alike), otherwise separate P and S drives.
+// the new ones out, mem the memory variables.  NM = 0 is elastic; SC selects
+// the reference's scalar relaxation (NM = 1, one drive for P and S alike),
+// otherwise separate P and S drives per mechanism.
 template <int NM, bool SC>
 GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, float vr, float vrm, float vr_dn,
-                  float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz) {
+                  float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz,
+                  MemVars<NM> &mem) {
   const float invh = g.invh, dt = g.dt;
   const size_t nz = (size_t)g.nz;
   const float *rc = g.rowc;
@@ -238,16 +284,13 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
   }
   const float err = l2m * dvr + lam * (vrr + dvz);
   const float eqq = lam * (dvr + dvz) + l2m * vrr;
-  const size_t cell = (size_t)s * g.pitch + (size_t)c;
-  const size_t ms = (size_t)g.mem_stride;
-  float *mem = g.mem;
 
   if (NM == 0) {
     trr += dt * err;
     tqq += dt * eqq;
   } else if (SC) {
-    trr += dt * (err + relax_cn(g.mem_decay, mem[0 * ms + cell], g.mem_drive * err));
-    tqq += dt * (eqq + relax_cn(g.mem_decay, mem[1 * ms + cell], g.mem_drive * eqq));
+    trr += dt * (err + relax_cn(g.mem_decay, mem.r[0][0], g.mem_drive * err));
+    tqq += dt * (eqq + relax_cn(g.mem_decay, mem.r[1][0], g.mem_drive * eqq));
   } else {
     const float mu2 = 2.f * rc[kRowMu * nz + s];
     const float pb = l2m * (dvr + vrr + dvz);
@@ -255,8 +298,8 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
 #pragma unroll
     for (int q = 0; q < NM; q++) {
       const float dP = rc[(kRowN + q) * nz + s], dS = rc[(kRowN + NM + q) * nz + s];
-      sr += relax_cn(g.decay[q], mem[(0 * NM + q) * ms + cell], dP * pb - dS * (mu2 * (vrr + dvz)));
-      sq += relax_cn(g.decay[q], mem[(1 * NM + q) * ms + cell], dP * pb - dS * (mu2 * (dvr + dvz)));
+      sr += relax_cn(g.decay[q], mem.r[0][q], dP * pb - dS * (mu2 * (vrr + dvz)));
+      sq += relax_cn(g.decay[q], mem.r[1][q], dP * pb - dS * (mu2 * (dvr + dvz)));
     }
     trr += dt * (err + sr);
     tqq += dt * (eqq + sq);
@@ -267,7 +310,7 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     if (NM == 0) {
       tzz += dt * ezz;
     } else if (SC) {
-      tzz += dt * (ezz + relax_cn(g.mem_decay, mem[2 * ms + cell], g.mem_drive * ezz));
+      tzz += dt * (ezz + relax_cn(g.mem_decay, mem.r[2][0], g.mem_drive * ezz));
     } else {
       const float mu2 = 2.f * rc[kRowMu * nz + s];
       const float pb = l2m * (dvr + vrr + dvz);
@@ -275,7 +318,7 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
 #pragma unroll
       for (int q = 0; q < NM; q++) {
         const float dP = rc[(kRowN + q) * nz + s], dS = rc[(kRowN + NM + q) * nz + s];
-        sz += relax_cn(g.decay[q], mem[(2 * NM + q) * ms + cell], dP * pb - dS * (mu2 * (dvr + vrr)));
+        sz += relax_cn(g.decay[q], mem.r[2][q], dP * pb - dS * (mu2 * (dvr + vrr)));
       }
       tzz += dt * (ezz + sz);
     }
@@ -302,12 +345,11 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     if (NM == 0) {
       trz += dt * e;
     } else if (SC) {
-      trz += dt * (e + relax_cn(g.mem_decay, mem[3 * ms + cell], g.mem_drive * e));
+      trz += dt * (e + relax_cn(g.mem_decay, mem.r[3][0], g.mem_drive * e));
     } else {
       float sz = 0.f;
 #pragma unroll
-      for (int q = 0; q < NM; q++)
-        sz += relax_cn(g.decay[q], mem[(3 * NM + q) * ms + cell], rc[(kRowN + NM + q) * nz + s] * e);
+      for (int q = 0; q < NM; q++) sz += relax_cn(g.decay[q], mem.r[3][q], rc[(kRowN + NM + q) * nz + s] * e);
       trz += dt * (e + sz);
     }
   }

@@ -2,9 +2,9 @@
 //
 // TEST INFRASTRUCTURE.  It runs the per-lane phases of stencil.h and the
 // coefficient builders of coef.h exactly as k_step and k_build_tables do,
-// with the kernel's cross-lane shuffles replaced by array reads (HIP's
-// __shfl_up/__shfl_down return the caller's own value when the source lane
-// is outside the wavefront, and so do these).  Every device array is a
+// with the kernel's cross-lane DPP shifts replaced by array reads (the lane
+// with no source reads its own value here and 0 on the device; neither value
+// is used).  Every device array is a
 // separate allocation of exactly the size the device arena gives it, so an
 // AddressSanitizer build reports any out-of-bounds access the kernel's
 // indexing would make.  It is never part of the product path.
@@ -106,8 +106,11 @@ void tile_step(const GridDesc &g, int n, int lt, bool probe) {
       for (int L = 0; L < kLanes; L++) {
         if (!own[L]) continue;
         float nrr = trr_p[L], nqq = tqq_p[L], nzz = tzz_p[L], nrz = trz_p[L];
+        MemVars<NM> mem;
+        load_mem<NM>(g, s, c[L], mem);
         stress<NM, SC>(g, n, s, c[L], cc[L], vrn_p[L], up(vrn_p, L), vrn[L], vzn_p[L], dn(vzn_p, L), vzn_pp[L],
-                       nrr, nqq, nzz, nrz);
+                       nrr, nqq, nzz, nrz, mem);
+        store_mem<NM>(g, s, c[L], mem);
         const size_t k = (size_t)s * g.pitch + c[L];
         B_trr[k] = nrr;
         B_tqq[k] = nqq;
