@@ -102,12 +102,14 @@ def cpu_baseline(w, gpu_traces, sample_steps: int, threads: int) -> tuple:
     nr, nz = w.model.Grid()[1:]
     cell_steps_per_s_core = nr * nz * sample_steps / elapsed
     ref = outs[0]
+    # Only receivers the wavefield has reached inside the window: the others
+    # hold nothing but the stencil's exponentially small precursor.
+    peaks = np.max(np.abs(ref), axis=1)
     errs = []
     for k in range(len(w.ranges)):
         b = ref[k]
-        nb = float(np.linalg.norm(b))
-        if nb > 0:
-            errs.append(float(np.linalg.norm(gpu_traces[k, :sample_steps] - b) / nb))
+        if peaks[k] > 1e-3 * peaks.max():
+            errs.append(float(np.linalg.norm(gpu_traces[k, :sample_steps] - b) / np.linalg.norm(b)))
     base = {
         "value": traces_per_s,
         "unit": "traces/s",
@@ -134,7 +136,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--workload", default="c2-R", help="c2-R (default), c2-X, c1-R, c1-X")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
-    ap.add_argument("--cpu-sample-steps", type=int, default=240)
+    ap.add_argument("--cpu-sample-steps", type=int, default=400)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-steps", type=int, default=0,
@@ -176,6 +178,8 @@ def main() -> int:
     sim.SetReceivers(cols, w.steps)
     use_graph = not args.no_graph
     step_bytes = sim.StepBytes()
+    tile_rows = sim.Autotune(10) if not os.environ.get("GEOFDTD_TILE_ROWS") else int(os.environ["GEOFDTD_TILE_ROWS"])
+    log(f"tile rows {tile_rows}")
 
     for i in range(args.warmup):
         sim.Reset()
@@ -207,6 +211,7 @@ def main() -> int:
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic_from_profiles(w.name),
         "kernel": "k_step (fused velocity+stress+memory-variable step)",
+        "tile_rows": tile_rows,
         "algorithmic_bytes_per_launch": step_bytes,
         "avg_launch_us": avg_launch_s * 1e6,
     }

@@ -20,6 +20,7 @@
 #include "layout.h"
 #include "model.h"
 #include "sim.h"
+#include "tune.h"
 
 using gfk::GridDesc;
 
@@ -71,19 +72,24 @@ int check_device(std::string &why) {
   return GF_OK;
 }
 
-// Tile height: enough rows per wave to amortise the two halo rows, few enough
-// that the grid fills 256 CUs.  GEOFDTD_TILE_ROWS overrides.
-int choose_tile_rows(int nr, int nz) {
+// Tile height when nothing better is known: the tallest of the unrolled
+// variants (8, 4, 2, 1 rows) that still puts about a dozen waves on each of
+// the 1024 SIMDs, since the step is bound by how many loads are in flight.
+// GEOFDTD_TILE_ROWS overrides.
+int choose_tile_rows_cells(int64_t tiles_r_times_nz) {
   if (const char *e = getenv("GEOFDTD_TILE_ROWS")) {
     int v = atoi(e);
     if (v >= 1 && v <= 4096) return v;
   }
+  const int64_t want_waves = 1024 * 12;
+  for (int th = 8; th > 1; th /= 2)
+    if (tiles_r_times_nz / th >= want_waves) return th;
+  return 1;
+}
+
+int choose_tile_rows(int nr, int nz) {
   const int tiles_r = (nr + gfk::kOwn - 1) / gfk::kOwn;
-  const int64_t target_waves = 256 * 16;
-  int th = (int)std::max<int64_t>(1, (int64_t)tiles_r * nz / target_waves);
-  int p = 4;
-  while (p * 2 <= th && p < 64) p *= 2;
-  return p;
+  return choose_tile_rows_cells((int64_t)tiles_r * nz);
 }
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -102,6 +108,44 @@ Sim::~Sim() {
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
   if (stream) hipStreamDestroy(stream);
+}
+
+int Sim::set_tile_rows(int th) {
+  if (th < 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: tile rows must be positive");
+  g.th = th;
+  g.tiles_z = (g.nz + th - 1) / th;
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  return upload_desc();
+}
+
+// Time each candidate tile height on this model, keep the fastest, and put
+// the model back to step 0 with zero state.
+int Sim::autotune(int trial_steps, int *chosen) {
+  const int64_t s0 = step;
+  gft::Key key{g.lossy ? g.nmech : 0, g.scalar, 1, (int64_t)log2((double)R.nr * R.nz)};
+  int err = GF_OK;
+  auto prepare = [&](int th) {
+    err = set_tile_rows(th);
+    return err == GF_OK;
+  };
+  auto launch = [&](int th, int n) {
+    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, th, d_desc, 1, (int64_t)g.tiles_r * g.tiles_z,
+                            (int32_t)(s0 + n), nullptr, 0, stream);
+  };
+  int best = gft::cached(key);
+  if (!best) best = gft::tune(key, stream, trial_steps, prepare, launch, nullptr);
+  if (!best) return err ? err : set_err(GF_ERR_DEVICE, "geofdtd: autotune found no working tile height");
+  int rc = set_tile_rows(best);
+  if (rc) return rc;
+  if ((rc = reset_state())) return rc;
+  if (d_traces && trace_floats) HIPCHK(gfk::launch_fill_f32(d_traces, trace_floats, 0.f, stream));
+  if (d_max && max_slots) HIPCHK(gfk::launch_fill_u32(d_max, max_slots, 0u, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (chosen) *chosen = best;
+  return GF_OK;
 }
 
 // Zero the state part of the arena (fields, psi strips, memory variables) with
@@ -269,7 +313,7 @@ int Sim::reset_max_slots(int64_t steps) {
 }
 
 int Sim::launch_one(int64_t n, bool probe) {
-  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, d_desc, 1, (int64_t)g.tiles_r * g.tiles_z,
+  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, 1, (int64_t)g.tiles_r * g.tiles_z,
                           (int32_t)n, nullptr, probe ? 1 : 0, stream));
   return GF_OK;
 }
@@ -317,7 +361,7 @@ int Sim::ensure_graph(bool probe) {
   HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, d_desc, 1,
+    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, 1,
                                     (int64_t)g.tiles_r * g.tiles_z, k, nbase_dev, probe ? 1 : 0, stream);
     if (e != hipSuccess) {
       hipGraph_t tmp;
@@ -387,6 +431,23 @@ int gf_sim_reset(gf_sim *sp) {
   Sim *s = reinterpret_cast<Sim *>(sp);
   if (!s) return set_err(GF_ERR_ARGUMENT, "geofdtd: null handle");
   int rc = s->reset_state();
+  return rc ? rc : ok();
+}
+
+int gf_sim_autotune(gf_sim *sp, int32_t trial_steps, int32_t *tile_rows) {
+  Sim *s = reinterpret_cast<Sim *>(sp);
+  if (!s || trial_steps < 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: bad autotune arguments");
+  int chosen = 0;
+  int rc = s->autotune(trial_steps, &chosen);
+  if (rc) return rc;
+  if (tile_rows) *tile_rows = chosen;
+  return ok();
+}
+
+int gf_sim_set_tile_rows(gf_sim *sp, int32_t tile_rows) {
+  Sim *s = reinterpret_cast<Sim *>(sp);
+  if (!s) return set_err(GF_ERR_ARGUMENT, "geofdtd: null handle");
+  int rc = s->set_tile_rows(tile_rows);
   return rc ? rc : ok();
 }
 

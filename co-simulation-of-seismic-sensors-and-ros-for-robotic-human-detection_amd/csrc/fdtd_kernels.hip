@@ -69,7 +69,13 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 // One wavefront = one tile; four tiles per 256-thread workgroup.  Tiles are
 // numbered model by model (GridDesc::tile_base), row-major within a model.
-template <int NM, bool SC>
+//
+// TH > 0: the tile is TH rows tall (every model of the launch has g.th == TH)
+// and every load of the tile -- TH + 2 rows of the six fields and TH rows of
+// memory variables -- is issued before any arithmetic, so a wave pays one
+// memory latency per step instead of one per row.  TH == 0: the tile height
+// is g.th at run time and loads run one row ahead of the arithmetic.
+template <int NM, bool SC, int TH>
 __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, int32_t nmodels, int32_t n_off,
                                               const int32_t *__restrict__ nbase, int32_t probe) {
   const int lane = threadIdx.x & (kLanes - 1);
@@ -96,9 +102,10 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   const int lt = tile - g.tile_base;
   if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
 
+  const int th = TH > 0 ? TH : g.th;
   const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
-  const int i0 = tr * kOwn, j0 = tzi * g.th;
-  const int j1 = min(j0 + g.th, g.nz);
+  const int i0 = tr * kOwn, j0 = tzi * th;
+  const int j1 = min(j0 + th, g.nz);
   const int c = i0 - 1 + lane;
   const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
   const int rb = n & 1, wb = rb ^ 1;
@@ -113,17 +120,10 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   float tzz_c = ld_tzz(A[kTzz], g, n, j0 - 1, c);  // old tzz at t
   float vrn_p = 0.f, vzn_p = 0.f, vzn_pp = 0.f;   // new vr(t-1), vz(t-1), vz(t-2)
   uint32_t mbits = 0;
-  MemVars<NM> mem_p;                       // memory variables of row t-1
-  RowLoad cur = load_row(A, g, n, j0 - 1, c);
 
-  for (int t = j0 - 1; t <= j1; t++) {
-    // Issue next row's loads and this row's memory variables before any
-    // arithmetic, so their latency hides behind it.
-    RowLoad nxt;
-    if (t < j1) nxt = load_row(A, g, n, t + 1, c);
-    MemVars<NM> mem_c;
-    if (own && t >= j0 && t < j1) load_mem<NM>(g, t, c, mem_c);
-
+  // One row of the march: new velocities of row t from `cur`, then new
+  // stresses of row s = t-1 with that row's memory variables `mem_s`.
+  auto row = [&](int t, const RowLoad &cur, MemVars<NM> &mem_s) {
     const float trr1 = lane_next(cur.trr), tqq1 = lane_next(cur.tqq), trzm = lane_prev(cur.trz);
     const bool row_own = t >= j0 && t < j1;
     float vrn, vzn;
@@ -149,17 +149,17 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
     }
     const int s = t - 1;
     if (s >= j0) {
-      const float vrn_pm = lane_prev(vrn_p);   // new vr(s, c-1)
-      const float vzn_p1 = lane_next(vzn_p);   // new vz(s, c+1)
+      const float vrn_pm = lane_prev(vrn_p);  // new vr(s, c-1)
+      const float vzn_p1 = lane_next(vzn_p);  // new vz(s, c+1)
       if (own) {
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
-        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_p);
+        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
         const size_t k = (size_t)s * g.pitch + c;
         B[kTrr][k] = nrr;
         B[kTqq][k] = nqq;
         B[kTzz][k] = nzz;
         B[kTrz][k] = nrz;
-        store_mem<NM>(g, s, c, mem_p);
+        store_mem<NM>(g, s, c, mem_s);
       }
     }
     trr_p = cur.trr;
@@ -170,8 +170,41 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
     vzn_pp = vzn_p;
     vzn_p = vzn;
     vrn_p = vrn;
-    mem_p = mem_c;
-    cur = nxt;
+  };
+
+  if constexpr (TH > 0) {
+    RowLoad L[TH + 2];
+    MemVars<NM> M[TH + 1];
+#pragma unroll
+    for (int k = 0; k < TH + 2; k++) {
+      const int t = j0 - 1 + k;
+      if (t <= j1) L[k] = load_row(A, g, n, t, c);
+    }
+#pragma unroll
+    for (int k = 0; k < TH; k++) {
+      const int t = j0 + k;
+      if (own && t < j1) load_mem<NM>(g, t, c, M[k + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < TH + 2; k++) {
+      const int t = j0 - 1 + k;
+      if (t > j1) break;
+      row(t, L[k], M[k > 0 ? k - 1 : 0]);
+    }
+  } else {
+    MemVars<NM> mem_p;  // memory variables of row t-1
+    RowLoad cur = load_row(A, g, n, j0 - 1, c);
+    for (int t = j0 - 1; t <= j1; t++) {
+      // Next row's loads and this row's memory variables go out before any
+      // arithmetic, so their latency hides behind it.
+      RowLoad nxt;
+      if (t < j1) nxt = load_row(A, g, n, t + 1, c);
+      MemVars<NM> mem_c;
+      if (own && t >= j0 && t < j1) load_mem<NM>(g, t, c, mem_c);
+      row(t, cur, mem_p);
+      mem_p = mem_c;
+      cur = nxt;
+    }
   }
 
   if (do_max) {
@@ -231,24 +264,38 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
   return hipGetLastError();
 }
 
-template <int NM, bool SC>
+template <int NM, bool SC, int TH>
 static hipError_t launch_step_t(const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
                                 const int32_t *nbase, int32_t probe, hipStream_t st) {
   unsigned blocks = (unsigned)((tiles + 3) / 4);
-  hipLaunchKernelGGL((k_step<NM, SC>), dim3(blocks), dim3(256), 0, st, d, nmodels, n_off, nbase, probe);
+  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(256), 0, st, d, nmodels, n_off, nbase, probe);
   return hipGetLastError();
 }
 
-hipError_t launch_step(int nmech, bool scalar, const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
-                       const int32_t *nbase, int32_t probe, hipStream_t st) {
+template <int NM, bool SC>
+static hipError_t launch_step_th(int th, const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
+                                 const int32_t *nbase, int32_t probe, hipStream_t st) {
+  switch (th) {
+    case 1: return launch_step_t<NM, SC, 1>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_t<NM, SC, 2>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_t<NM, SC, 4>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 8: return launch_step_t<NM, SC, 8>(d, nmodels, tiles, n_off, nbase, probe, st);
+    default: return launch_step_t<NM, SC, 0>(d, nmodels, tiles, n_off, nbase, probe, st);
+  }
+}
+
+// `th` is the tile height every model of the launch was planned with; 1, 2,
+// 4 and 8 select the fully unrolled kernels, anything else the row march.
+hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, int32_t nmodels, int64_t tiles,
+                       int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
   if (tiles <= 0) return hipSuccess;
-  if (nmech == 0) return launch_step_t<0, true>(d, nmodels, tiles, n_off, nbase, probe, st);
-  if (scalar) return launch_step_t<1, true>(d, nmodels, tiles, n_off, nbase, probe, st);
+  if (nmech == 0) return launch_step_th<0, true>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+  if (scalar) return launch_step_th<1, true>(th, d, nmodels, tiles, n_off, nbase, probe, st);
   switch (nmech) {
-    case 1: return launch_step_t<1, false>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_t<2, false>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 3: return launch_step_t<3, false>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_t<4, false>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 1: return launch_step_th<1, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_th<2, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+    case 3: return launch_step_th<3, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_th<4, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
     default: return hipErrorInvalidValue;
   }
 }

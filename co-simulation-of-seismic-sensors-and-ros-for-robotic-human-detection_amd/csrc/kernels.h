@@ -15,8 +15,8 @@ hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st);
 hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st);
 hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers, float *rowc, float *colc,
                                hipStream_t st);
-hipError_t launch_step(int nmech, bool scalar, const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
-                       const int32_t *nbase, int32_t probe, hipStream_t st);
+hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, int32_t nmodels, int64_t tiles,
+                       int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st);
 hipError_t launch_surface_velocity(const GridDesc *d, int32_t buf, int32_t col, float *out, hipStream_t st);
 hipError_t launch_max_speed(const GridDesc *d, int32_t buf, uint32_t *slot, int64_t cells, hipStream_t st);
 

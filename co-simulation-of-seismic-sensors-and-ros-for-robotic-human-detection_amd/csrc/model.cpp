@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -311,12 +312,21 @@ int profile_first(int n, double h, double offset, double inner, double thick, do
 
 }  // namespace
 
+// Go's %g (strconv.FormatFloat(x, 'g', -1, 64)): the shortest digits that
+// read back exactly, in exponent form only when the decimal exponent is below
+// -4 or at least 6 (C's %g switches at the digit count instead).
 std::string go_g(double x) {
   char buf[64];
-  for (int p = 1; p <= 17; p++) {
-    snprintf(buf, sizeof buf, "%.*g", p, x);
+  if (x == 0) return signbit(x) ? "-0" : "0";
+  if (!isfinite(x)) return isnan(x) ? "NaN" : (x > 0 ? "+Inf" : "-Inf");
+  int p = 1;
+  for (; p < 17; p++) {
+    snprintf(buf, sizeof buf, "%.*e", p - 1, x);
     if (strtod(buf, nullptr) == x) break;
   }
+  snprintf(buf, sizeof buf, "%.*e", p - 1, x);
+  const int e = atoi(strchr(buf, 'e') + 1);
+  if (e >= -4 && e < 6) snprintf(buf, sizeof buf, "%.*f", std::max(p - 1 - e, 0), x);
   return buf;
 }
 

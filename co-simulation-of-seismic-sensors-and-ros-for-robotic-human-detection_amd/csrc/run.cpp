@@ -11,6 +11,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <stdlib.h>
+
 #include <algorithm>
 #include <memory>
 #include <numeric>
@@ -21,6 +23,7 @@
 #include "kernels.h"
 #include "model.h"
 #include "sim.h"
+#include "tune.h"
 
 namespace gfs {
 int api_set_err(int code, const std::string &msg);
@@ -39,6 +42,7 @@ using gfs::Sim;
 struct Group {
   int nmech = 0;
   bool scalar = true;
+  int th = 0;  // tile rows shared by every model of the group
   std::vector<int> order;        // shot indices, longest run first
   std::vector<int64_t> prefix;   // tiles of the first k models
   gfk::GridDesc *d_descs = nullptr;
@@ -66,6 +70,34 @@ int active_models(const Group &G, const std::vector<int32_t> &steps, int64_t n) 
   return k;
 }
 
+// Descriptors of the group's models for tile height th, tile bases in run
+// order, uploaded to the group's device array (allocated on first use).
+int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th) {
+  G.th = th;
+  std::vector<gfk::GridDesc> descs;
+  G.prefix.assign(1, 0);
+  for (int i : G.order) {
+    gfk::GridDesc d = sims[i]->g;
+    d.th = th;
+    d.tiles_z = (d.nz + th - 1) / th;
+    d.tile_base = (int32_t)G.prefix.back();
+    descs.push_back(d);
+    G.prefix.push_back(G.prefix.back() + (int64_t)d.tiles_r * d.tiles_z);
+  }
+  if (G.prefix.back() > (int64_t)INT32_MAX) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: batch has too many tiles");
+  if (!G.d_descs) {
+    HIPCHK(hipMalloc(&G.d_descs, sizeof(gfk::GridDesc) * descs.size()));
+    HIPCHK(hipMalloc(&G.d_nbase, 64));
+  }
+  HIPCHK(hipMemcpy(G.d_descs, descs.data(), sizeof(gfk::GridDesc) * descs.size(), hipMemcpyHostToDevice));
+  if (G.exec) hipGraphExecDestroy(G.exec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  G.graph_models = -1;
+  return GF_OK;
+}
+
 int capture(Group &G, int models, hipStream_t st) {
   if (G.exec) hipGraphExecDestroy(G.exec);
   if (G.graph) hipGraphDestroy(G.graph);
@@ -73,7 +105,7 @@ int capture(Group &G, int models, hipStream_t st) {
   G.graph = nullptr;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < gfs::kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.d_descs, models, G.prefix[models], k, G.d_nbase, 1, st);
+    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, models, G.prefix[models], k, G.d_nbase, 1, st);
     if (e != hipSuccess) {
       hipGraph_t tmp = nullptr;
       hipStreamEndCapture(st, &tmp);
@@ -186,26 +218,36 @@ extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *re
   for (int i = 0; i < n_shots; i++) max_steps = std::max<int64_t>(max_steps, sims[i] ? steps[i] : 0);
   if (!groups.empty()) {
     stream = sims[groups[0].order[0]]->stream;
+    for (auto &sp : sims)
+      if (sp && !rc) {
+        hipError_t e = hipStreamSynchronize(sp->stream);
+        if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
+      }
     for (Group &G : groups) {
       std::stable_sort(G.order.begin(), G.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
-      std::vector<gfk::GridDesc> descs;
-      G.prefix.assign(1, 0);
+      int64_t cols_rows = 0, cell_steps = 0, cells = 0;
       for (int i : G.order) {
-        gfk::GridDesc d = sims[i]->g;
-        d.tile_base = (int32_t)G.prefix.back();
-        descs.push_back(d);
-        G.prefix.push_back(G.prefix.back() + (int64_t)d.tiles_r * d.tiles_z);
+        cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
+        cells += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
+        cell_steps += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
       }
-      if (G.prefix.back() > (int64_t)INT32_MAX) {
-        rc = gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: batch has too many tiles");
-        break;
-      }
-      hipError_t e = hipMalloc(&G.d_descs, sizeof(gfk::GridDesc) * descs.size());
-      if (e == hipSuccess) e = hipMalloc(&G.d_nbase, 64);
-      if (e == hipSuccess) e = hipMemcpy(G.d_descs, descs.data(), sizeof(gfk::GridDesc) * descs.size(), hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        rc = gfs::hip_fail(e, "batch descriptors");
-        break;
+      if ((rc = plan_group(G, sims, gfs::choose_tile_rows_cells(cols_rows)))) break;
+      // Long runs pay for a few trial steps at each tile height.
+      const char *tune_env = getenv("GEOFDTD_TUNE");
+      const bool want_tune = !getenv("GEOFDTD_TILE_ROWS") && (tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL);
+      if (want_tune) {
+        gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(), (int64_t)log2((double)cells)};
+        auto prepare = [&](int th) { return plan_group(G, sims, th) == GF_OK; };
+        auto launch = [&](int th, int n) {
+          return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, (int32_t)G.order.size(), G.prefix.back(), n,
+                                  nullptr, 0, stream);
+        };
+        int best = gft::cached(key);
+        if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
+        if (best && (rc = plan_group(G, sims, best))) break;
+        for (int i : G.order)
+          if ((rc = sims[i]->reset_state())) break;
+        if (rc) break;
       }
     }
     for (auto &sp : sims)
@@ -233,7 +275,7 @@ extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *re
             break;
           }
         } else {
-          hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.d_descs, act, G.prefix[act], (int32_t)n, nullptr, 1, stream);
+          hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, act, G.prefix[act], (int32_t)n, nullptr, 1, stream);
           if (e != hipSuccess) {
             rc = gfs::hip_fail(e, "step launch");
             break;

@@ -20,6 +20,7 @@ constexpr int kGraphSteps = 100;
 int hip_fail(hipError_t e, const char *where);
 int check_device(std::string &why);
 int choose_tile_rows(int nr, int nz);
+int choose_tile_rows_cells(int64_t tiles_r_times_nz);
 
 // Ordered |v| bits back to a speed; NaN bits stay NaN (math.Max semantics).
 inline double bits_to_speed(uint32_t b) {
@@ -79,6 +80,8 @@ struct Sim {
   int launch_one(int64_t n, bool probe);
   int advance(int64_t n, bool use_graph, bool probe);
   int ensure_graph(bool probe);
+  int set_tile_rows(int th);
+  int autotune(int trial_steps, int *chosen);
 };
 
 }  // namespace gfs

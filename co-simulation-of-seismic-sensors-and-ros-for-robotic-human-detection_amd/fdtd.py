@@ -131,6 +131,8 @@ SIGNATURES = {
     "gf_sim_sync_timing": (_I, [C.c_void_p, _PD, _PI64]),
     "gf_sim_read_traces": (_I, [C.c_void_p, _PD, _PI32]),
     "gf_sim_step_bytes": (_I, [C.c_void_p, _PD]),
+    "gf_sim_autotune": (_I, [C.c_void_p, _I32, _PI32]),
+    "gf_sim_set_tile_rows": (_I, [C.c_void_p, _I32]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -362,6 +364,15 @@ class Sim:
         _check(load_library().gf_sim_read_traces(self._h, _dptr(out), C.byref(n)))
         return out
 
+    def Autotune(self, trial_steps: int = 10) -> int:
+        """Pick the tile height by timing; resets to step 0.  Returns it."""
+        th = C.c_int32()
+        _check(load_library().gf_sim_autotune(self._h, trial_steps, C.byref(th)))
+        return th.value
+
+    def SetTileRows(self, rows: int) -> None:
+        _check(load_library().gf_sim_set_tile_rows(self._h, rows))
+
     def StepBytes(self) -> float:
         b = C.c_double()
         _check(load_library().gf_sim_step_bytes(self._h, C.byref(b)))
@@ -511,9 +522,15 @@ def _run_with_progress(sh: Shot) -> Result:
 
 
 def _go_g(x: float) -> str:
-    """Go's %g: shortest round-trip digits."""
-    for p in range(1, 18):
-        t = "%.*g" % (p, x)
-        if float(t) == x:
-            return t
-    return repr(x)
+    """Go's %g: shortest round-trip digits, exponent form only for a decimal
+    exponent below -4 or at least 6."""
+    if x == 0:
+        return "-0" if str(x).startswith("-") else "0"
+    p = 1
+    while p < 17 and float("%.*e" % (p - 1, x)) != x:
+        p += 1
+    t = "%.*e" % (p - 1, x)
+    e = int(t.split("e")[1])
+    if -4 <= e < 6:
+        return "%.*f" % (max(p - 1 - e, 0), x)
+    return t

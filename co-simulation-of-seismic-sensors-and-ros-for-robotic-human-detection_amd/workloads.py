@@ -147,7 +147,7 @@ def c2_stance():
     return s
 
 
-def c2(variant: str = "R", steps: int = 80_000, lead: float = 0.005) -> Workload:
+def c2(variant: str = "R", steps: int = 80_000, lead: float = 0.0) -> Workload:
     """Config 2: layers {1.5 m: Vs 150, 4 m: Vs 250} over a Vs 400 half-space,
     Vp = 2 Vs, rho 1700/1850/2000, Qs 20/40/80; 2048 x 1024 cells at 0.05 m
     with a 32-cell C-PML (MaxRange 100.75 m, Depth 49.55 m); dt = 2.5e-5 s

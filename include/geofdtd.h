@@ -194,6 +194,13 @@ int gf_sim_advance(gf_sim *s, int32_t n, int32_t use_graph);
 int gf_sim_sync_timing(gf_sim *s, double *kernel_ms, int64_t *kernel_launches);
 /* Copy the recorded receiver samples out: out[c * steps_recorded + k]. */
 int gf_sim_read_traces(gf_sim *s, double *out, int32_t *steps_recorded);
+/* Tile shape.  The step kernel's tile height (grid rows per wavefront) is
+ * picked by timing each candidate on this model for trial_steps steps; the
+ * simulation is then reset to step 0 (zero state, cleared receivers).
+ * gf_sim_set_tile_rows fixes it by hand (1, 2, 4, 8 select unrolled kernels,
+ * other values a row-marching kernel).  No reference counterpart. */
+int gf_sim_autotune(gf_sim *s, int32_t trial_steps, int32_t *tile_rows);
+int gf_sim_set_tile_rows(gf_sim *s, int32_t tile_rows);
 /* Algorithmic HBM bytes one step moves (each state array read and written
  * once, plus the C-PML strips): the roofline numerator. */
 int gf_sim_step_bytes(const gf_sim *s, double *bytes);

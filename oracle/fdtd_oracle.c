@@ -72,13 +72,22 @@ static int fail(int code, const char *fmt, ...) {
 
 const char *orc_last_error(void) { return orc_err; }
 
-/* Go's %g: the shortest digit string that round-trips, in exponent form when
- * the decimal exponent is < -4 or >= 6 (strconv.FormatFloat 'g', -1), which
- * is C's %.*g at the smallest precision that reads back exactly. */
+/* Go's %g (strconv.FormatFloat(x, 'g', -1, 64)): the shortest digit string
+ * that reads back exactly, in exponent form only when the decimal exponent is
+ * below -4 or at least 6 -- unlike C's %g, whose switch point is the digit
+ * count. */
 static const char *gofmt_g(double x, char *buf, size_t n) {
-  for (int p = 1; p <= 17; p++) {
-    snprintf(buf, n, "%.*g", p, x);
-    if (strtod(buf, NULL) == x) return buf;
+  int p = 1;
+  for (; p < 17; p++) {
+    snprintf(buf, n, "%.*e", p - 1, x);
+    if (strtod(buf, NULL) == x) break;
+  }
+  snprintf(buf, n, "%.*e", p - 1, x);
+  if (x != 0 && isfinite(x)) {
+    int e = atoi(strchr(buf, 'e') + 1);
+    if (e >= -4 && e < 6) snprintf(buf, n, "%.*f", p - 1 - e > 0 ? p - 1 - e : 0, x);
+  } else if (x == 0) {
+    snprintf(buf, n, "%s", signbit(x) ? "-0" : "0");
   }
   return buf;
 }

@@ -1,10 +1,13 @@
 """GPU parity: the FP32 HIP path against the FP64 oracle on the same inputs.
 
-Tolerance: relative L2 of each receiver trace <= 1e-4 (the target stated for
-FP32 receivers against the FP64 reference scheme).
+Tolerance: relative L2 of each receiver trace <= 1e-4 (FP32 receivers with
+FP32 field state against the FP64 reference scheme).  Traces whose oracle
+peak is below 1e-3 of the largest are left out: they hold only the stencil's
+exponentially small precursor, whose relative error is meaningless.
 """
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -16,35 +19,344 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-4
 
 
-def _report(name, errs):
+def compare(name, got_traces, want_traces, tol=TOL):
+    want = np.asarray(want_traces)
+    got = np.asarray(got_traces)
+    peaks = np.max(np.abs(want), axis=1)
+    errs = [rel_l2(got[k], want[k]) for k in range(len(want)) if peaks[k] > 1e-3 * peaks.max()]
     errs = np.asarray(errs)
-    print(f"{name}: rel-L2 p50 {np.median(errs):.2e} max {errs.max():.2e}", flush=True)
+    print(f"{name}: {len(errs)} traces, rel-L2 p50 {np.median(errs):.2e} p99 {np.quantile(errs, 0.99):.2e} "
+          f"max {errs.max():.2e}", flush=True)
+    assert len(errs) > 0
+    assert errs.max() <= tol, f"{name}: worst rel-L2 {errs.max():.3e}"
+    return errs
+
+
+@pytest.fixture
+def tile_rows(monkeypatch):
+    """Pin the tile height of gf_run for one test."""
+
+    def set_(th):
+        monkeypatch.setenv("GEOFDTD_TILE_ROWS", str(th))
+
+    return set_
+
+
+# ------------------------------------------------------------- config runs
 
 
 def test_c1_reference_physics_full(fdtd, oracle):
-    """Config 1-R, all 10,000 steps, 49 receivers: every trace within 1e-4."""
+    """Config 1-R, all 10,000 steps, 49 receivers."""
     import cosim_amd.workloads as W
 
     w = W.c1("R")
     got = fdtd.Run(w.shot())
     want = oracle.run(w.model, w.force, w.ranges, w.steps)
     assert got.Dt == want.dt and got.Cells == want.cells == (520, 270)
-    errs = [rel_l2(got.Traces[k].Vertical, want.traces[k]) for k in range(len(w.ranges))]
-    _report("c1-R", errs)
-    assert max(errs) <= TOL
     np.testing.assert_array_equal([t.Range for t in got.Traces], want.snapped)
+    compare("c1-R", [t.Vertical for t in got.Traces], want.traces)
 
 
-def test_boussinesq_setup_matches_oracle(fdtd, oracle):
-    """The TestBoussinesqStaticLimit shot (fdtd_test.go:57-102) on both paths."""
+def test_c1_extension_physics_full(fdtd, oracle):
+    """Config 1-X: 3 mechanisms over 5-150 Hz, Qp 60, Qs 30, all 10,000 steps."""
+    import cosim_amd.workloads as W
+
+    w = W.c1("X")
+    got = fdtd.Run(w.shot())
+    want = oracle.run(w.model, w.force, w.ranges, w.steps)
+    assert got.Dt == want.dt
+    compare("c1-X", [t.Vertical for t in got.Traces], want.traces)
+
+
+@pytest.mark.parametrize("variant", ["R", "X"])
+def test_c2_layered_grf_prefix(fdtd, oracle, variant):
+    """Config 2 at full grid (2048 x 1024, three layers, GRF source), the first
+    1,200 of its 80,000 steps (the oracle's share of a test's time)."""
+    import cosim_amd.workloads as W
+
+    w = W.c2(variant, steps=1200)
+    got = fdtd.Run(w.shot())
+    want = oracle.run(w.model, w.force, w.ranges, w.steps)
+    assert got.Cells == (2048, 1024)
+    compare(f"c2-{variant} prefix", [t.Vertical for t in got.Traces], want.traces)
+
+
+# ----------------------------------------------- reference known answers
+
+
+def test_boussinesq_static_limit(fdtd, oracle):
+    """TestBoussinesqStaticLimit (fdtd_test.go:57-102) on the GPU: ratios
+    within 1 % and the traces equal to the oracle's."""
     m = fdtd.Model(Stack=loam(), MaxRange=40, Depth=40, DominantFreq=25)
     m.Spacing = m.SpacingFor(75, 15)
-    dt = fdtd.New(m).Dt()
+    dt = m.Grid()[0]
     steps = int(0.3 / dt)
     sigma = 1 / (2 * math.pi * 25)
     force = [1000 * 0.5 * (1 + math.erf((i * dt - 5 * sigma) / (math.sqrt(2) * sigma))) for i in range(steps)]
     got = fdtd.Run(fdtd.Shot(m, force, [1, 2, 3, 5], steps))
     want = oracle.run(m, force, [1, 2, 3, 5], steps)
-    errs = [rel_l2(got.Traces[k].Vertical, want.traces[k]) for k in range(4)]
-    _report("boussinesq", errs)
-    assert max(errs) <= TOL
+    compare("boussinesq", [t.Vertical for t in got.Traces], want.traces)
+    mu, vp2, vs2 = 1700 * 200.0 ** 2, 500.0 ** 2, 200.0 ** 2
+    nu = (vp2 - 2 * vs2) / (2 * (vp2 - vs2))
+    ratios = []
+    for tr in got.Traces:
+        u = float(np.sum(tr.Vertical) * dt)
+        ratios.append(u / (1000 * (1 - nu) / (2 * math.pi * mu * tr.Range)))
+    print("boussinesq ratios", [f"{r:.4f}" for r in ratios])
+    assert all(abs(r - 1) <= 0.01 for r in ratios)
+    # the reference logged 1.0027, 1.0002, 0.9997, 0.9992
+    np.testing.assert_allclose(ratios, [1.0027, 1.0002, 0.9997, 0.9992], atol=6e-5)
+
+
+def test_stability_limit(fdtd):
+    """TestStabilityLimit (fdtd_test.go:107-135) on the GPU."""
+
+    def run(courant):
+        m = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30, Courant=courant)
+        s = fdtd.New(m)
+        steps = int(0.15 / s.Dt())
+        s.Drive(fdtd.Ricker(1000, 30, s.Dt(), steps))
+        s.StepN(steps)
+        return s.MaxSpeed()
+
+    stable = run(0.5)
+    assert not math.isnan(stable) and stable <= 1
+    bad = run(1.4)
+    assert math.isnan(bad) or bad >= 1e3 * stable
+
+
+def test_divergence_is_reported_like_run(fdtd, oracle):
+    """run.go:118-124: a diverging shot fails with "fdtd: diverged at step N of
+    M", N the first 256-step probe that sees NaN or Inf.  FP32 overflows to Inf
+    long before FP64 does, so the GPU's N is a probe step no later than the
+    oracle's."""
+    import re
+
+    m = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30, Courant=1.4)
+    dt = m.Grid()[0]
+    steps = 6000
+    force = fdtd.Ricker(1000, 30, dt, steps)
+    with pytest.raises(oracle.OracleError) as want:
+        oracle.run(m, force, [1.0], steps)
+    with pytest.raises(fdtd.FdtdError) as got:
+        fdtd.Run(fdtd.Shot(m, force, [1.0], steps))
+    assert got.value.status == fdtd.GF_ERR_DIVERGED
+    pat = r"fdtd: diverged at step (\d+) of 6000"
+    ng = int(re.fullmatch(pat, str(got.value)).group(1))
+    nw = int(re.fullmatch(pat, str(want.value)).group(1))
+    print(f"diverged: GPU at step {ng}, oracle at step {nw}")
+    assert ng % 256 == 0 and 0 < ng <= nw
+
+
+def test_rayleigh_velocity(fdtd):
+    """TestRayleighVelocity (fdtd_test.go:145-177): grid 187.18 m/s logged."""
+    from media import rayleigh_velocity
+    from test_oracle_known_answers import peak_at
+
+    m = fdtd.Model(Stack=loam(), MaxRange=22, Depth=16, DominantFreq=30)
+    m.Spacing = m.SpacingFor(90, 20)
+    dt = m.Grid()[0]
+    steps = int(0.35 / dt)
+    res = fdtd.Run(fdtd.Shot(m, fdtd.Ricker(1000, 30, dt, steps), [8, 10, 12, 14, 16], steps))
+    xs = np.array([t.Range for t in res.Traces])
+    ys = np.array([peak_at(t.Vertical, res.Dt, res.T0)[1] for t in res.Traces])
+    slope = np.polyfit(xs, ys, 1)[0]
+    got = 1 / slope
+    want = rayleigh_velocity(500, 200)
+    print(f"Rayleigh velocity: grid {got:.2f} m/s, secular equation {want:.2f} m/s")
+    assert abs(got / want - 1) <= 0.02
+    assert f"{got:.2f}" == "187.18"
+
+
+def test_pml_reflection(fdtd):
+    """TestPMLReflection (fdtd_test.go:190-220): -71.3 / -70.5 dB logged."""
+
+    def shoot(rmax):
+        m = fdtd.Model(Stack=loam(), MaxRange=rmax, Depth=14, DominantFreq=30, Spacing=0.15)
+        dt = m.Grid()[0]
+        steps = int(0.45 / dt)
+        return fdtd.Run(fdtd.Shot(m, fdtd.Ricker(1000, 30, dt, steps), [4, 8], steps))
+
+    near, far = shoot(12), shoot(36)
+    dbs = []
+    for a, b in zip(near.Traces, far.Traces):
+        peak = np.max(np.abs(b.Vertical))
+        diff = np.max(np.abs(a.Vertical - b.Vertical))
+        dbs.append(20 * math.log10(diff / peak))
+    print("pml reflection dB", [f"{d:.1f}" for d in dbs])
+    assert all(d <= -40 for d in dbs)
+    np.testing.assert_allclose(dbs, [-71.3, -70.5], atol=0.5)
+
+
+def test_attenuation_matches_the_material_model(fdtd, oracle):
+    """TestAttenuationMatchesTheMaterialModel (fdtd_test.go:243-293)."""
+    from media import rayleigh_velocity
+    from test_oracle_known_answers import dft
+
+    sls = fdtd.Fit(25, 30)
+
+    def shoot(relax):
+        m = fdtd.Model(Stack=loam(), Relax=relax, RefFreq=30, MaxRange=26, Depth=18, DominantFreq=30, Spacing=0.12)
+        dt = m.Grid()[0]
+        steps = int(0.45 / dt)
+        return fdtd.Run(fdtd.Shot(m, fdtd.Ricker(1000, 30, dt, steps), [6, 18], steps))
+
+    elastic, lossy = shoot(fdtd.SLS()), shoot(sls)
+    c_r = rayleigh_velocity(500, 200)
+    got_all = []
+    for f in [15, 30, 50]:
+        def ratio(res):
+            return abs(dft(res.Traces[1].Vertical, f, res.Dt, res.T0)) / abs(dft(res.Traces[0].Vertical, f, res.Dt, res.T0))
+
+        got = ratio(lossy) / ratio(elastic)
+        k = 2 * math.pi * f / oracle.sls_velocity(sls, c_r, 30, f)
+        want = math.exp(k.imag * 12)
+        assert abs(got / want - 1) <= 0.05
+        got_all.append(got)
+    np.testing.assert_allclose(got_all, [0.8959, 0.7860, 0.7171], atol=2e-4)
+
+
+def test_zero_relaxation_is_elastic(fdtd, oracle):
+    """TestZeroRelaxationIsElastic (fdtd_test.go:298-316): elastic runs are
+    exactly elastic, and a solid that would amplify is rejected."""
+    m = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    dt = m.Grid()[0]
+    force = fdtd.Ricker(1000, 30, dt, 400)
+    got = fdtd.Run(fdtd.Shot(m, force, [1, 2], 400))
+    want = oracle.run(m, force, [1, 2], 400)
+    compare("elastic", [t.Vertical for t in got.Traces], want.traces)
+    bad = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, Relax=fdtd.SLS(1e-2, 1e-3))
+    with pytest.raises(fdtd.FdtdError, match="fdtd: TauEps must be at least TauSigma for a dissipative solid"):
+        fdtd.New(bad)
+
+
+# ------------------------------------------------------ kernel invariants
+
+
+def _small(fdtd):
+    return dict(Stack=[fdtd.Layer(0, 500, 200, 1700, 25)], MaxRange=9.3, Depth=7.1, Spacing=0.1, DominantFreq=30)
+
+
+@pytest.mark.parametrize("gsls", [0, 3])
+def test_tile_shape_does_not_change_results(fdtd, tile_rows, gsls):
+    """Every tile height gives the same traces: bit-identical within the
+    row-marching kernel (any height), within 1e-6 across kernel variants."""
+    kw = _small(fdtd)
+    if gsls:
+        m = fdtd.Model(**kw, GslsMechanisms=3)
+        m.Stack[0].Qp = 50
+    else:
+        m = fdtd.Model(**kw, Relax=fdtd.Fit(25, 30))
+    dt = m.Grid()[0]
+    steps = 700
+    force = fdtd.Ricker(1000, 30, dt, steps)
+    out = {}
+    for th in [1, 2, 4, 8, 3, 5, 16, 200]:
+        tile_rows(th)
+        r = fdtd.Run(fdtd.Shot(m, force, [0.5, 1.0, 3.0, 6.0], steps))
+        out[th] = np.array([t.Vertical for t in r.Traces])
+    for th in [5, 16, 200]:
+        np.testing.assert_array_equal(out[th], out[3])
+    for th in [1, 2, 4, 8]:
+        assert np.max(np.abs(out[th] - out[3])) <= 1e-6 * np.max(np.abs(out[3]))
+
+
+def test_batch_equals_single_runs(fdtd, tile_rows):
+    """gf_run_batch of uneven models (grid sizes, step counts, physics) gives
+    each model exactly the traces its own gf_run gives."""
+    tile_rows(2)
+    shots = []
+    for i, (rmax, depth, h, relax, gsls, steps) in enumerate([
+        (6.0, 6.0, 0.15, None, 0, 500),
+        (9.3, 5.2, 0.1, (25, 30), 0, 800),
+        (12.0, 8.0, 0.2, (40, 30), 0, 300),
+        (7.5, 7.5, 0.1, None, 3, 650),
+        (5.0, 9.0, 0.12, (30, 30), 0, 800),
+    ]):
+        m = fdtd.Model(Stack=[fdtd.Layer(1.5, 400, 180, 1650, 20), fdtd.Layer(0, 600, 260, 1900, 60)],
+                       MaxRange=rmax, Depth=depth, Spacing=h, DominantFreq=30, GslsMechanisms=gsls)
+        if relax:
+            m.Relax = fdtd.Fit(*relax)
+        dt = m.Grid()[0]
+        shots.append(fdtd.Shot(m, fdtd.Ricker(900, 35, dt, steps), [1.0, 2.5, rmax * 0.6], steps))
+    batch = fdtd.RunBatch(shots)
+    for sh, b in zip(shots, batch):
+        single = fdtd.Run(sh)
+        for tb, ts in zip(b.Traces, single.Traces):
+            np.testing.assert_array_equal(tb.Vertical, ts.Vertical)
+        assert b.Peak == single.Peak
+
+
+def test_batch_reports_each_shots_error(fdtd):
+    good = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    dt = good.Grid()[0]
+    f = fdtd.Ricker(1000, 30, dt, 300)
+    bad_recv = fdtd.Shot(good, f, [7.0], 300)
+    no_steps = fdtd.Shot(good, f, [1.0], 0)
+    res = fdtd.RunBatch([fdtd.Shot(good, f, [1.0], 300), bad_recv, no_steps])
+    assert isinstance(res[0], fdtd.Result)
+    assert str(res[1]) == "fdtd: receiver at 7 m is outside the interior (6 m)"
+    assert str(res[2]) == "fdtd: a shot needs a positive number of steps"
+
+
+# ------------------------------------------------------- stateful Sim API
+
+
+def test_sim_api_matches_oracle(fdtd, oracle):
+    """New / Drive / Step / SurfaceVelocity / RadialVelocity / MaxSpeed / Time
+    / Column, driven step by step as a Go caller would."""
+    m = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    s, o = fdtd.New(m), oracle.OracleSim(m)
+    assert s.Dt() == o.Dt() and s.Cells() == o.Cells()
+    assert s.Column(2.03) == o.Column(2.03) and s.Column(-1) == o.Column(-1) and s.Column(99) == o.Column(99)
+    force = fdtd.Ricker(1000, 30, s.Dt(), 300)
+    s.Drive(force)
+    o.Drive(force)
+    cols = [0, 1, 7, 13]
+    got, want = [], []
+    for n in range(300):
+        s.Step()
+        o.Step()
+        if n % 37 == 0:
+            got.append([s.SurfaceVelocity(c) for c in cols] + [s.RadialVelocity(c) for c in cols] + [s.MaxSpeed()])
+            want.append([o.SurfaceVelocity(c) for c in cols] + [o.RadialVelocity(c) for c in cols] + [o.MaxSpeed()])
+    assert s.Time() == pytest.approx(o.Time(), rel=1e-15)
+    got, want = np.array(got), np.array(want)
+    scale = np.maximum(np.max(np.abs(want), axis=0), 1e-300)
+    assert np.max(np.abs(got - want) / scale) <= 1e-4
+    # StepN records exactly what Step + SurfaceVelocity do
+    rec = s.StepN(50, cols)
+    for k in range(50):
+        o.Step()
+        for j, c in enumerate(cols):
+            assert abs(rec[j, k] - o.SurfaceVelocity(c)) <= 1e-4 * scale[j]
+
+
+def test_reset_replays_the_same_shot(fdtd):
+    m = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    s = fdtd.New(m)
+    s.Drive(fdtd.Ricker(1000, 30, s.Dt(), 400))
+    cols = [3, 10, 20]
+    s.SetReceivers(cols, 400)
+    s.Advance(400, True)
+    a = s.ReadTraces(3)
+    s.Reset()
+    s.Advance(400, False)
+    b = s.ReadTraces(3)
+    np.testing.assert_array_equal(a, b)
+    assert np.max(np.abs(a)) > 0
+
+
+def test_linearity_in_the_force(fdtd):
+    """Doubling the force doubles every trace (a power of two scales every
+    FP32 operation exactly, up to values flushed as denormals); zero force
+    gives an identically zero field."""
+    m = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    dt = m.Grid()[0]
+    f = np.asarray(fdtd.Ricker(1000, 30, dt, 500))
+    a = fdtd.Run(fdtd.Shot(m, f, [1.0, 3.0], 500))
+    b = fdtd.Run(fdtd.Shot(m, 2 * f, [1.0, 3.0], 500))
+    z = fdtd.Run(fdtd.Shot(m, 0 * f, [1.0, 3.0], 500))
+    for ta, tb, tz in zip(a.Traces, b.Traces, z.Traces):
+        assert np.max(np.abs(2 * ta.Vertical - tb.Vertical)) <= 1e-12 * np.max(np.abs(tb.Vertical))
+        assert not np.any(tz.Vertical)
