@@ -9,6 +9,8 @@
 // AddressSanitizer build reports any out-of-bounds access the kernel's
 // indexing would make.  It is never part of the product path.
 #include <math.h>
+#include <pmmintrin.h>
+#include <xmmintrin.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -210,6 +212,10 @@ int emu_run(const gf_model *m, const double *force, int64_t nforce, const int32_
   g.rec_base = 0;
   g.maxbits = maxbits;
   const int nm = g.lossy ? g.nmech : 0;
+  // GEOFDTD_EMU_FTZ=1 flushes float denormals to zero, as a GPU in
+  // flush-to-zero mode would.
+  const unsigned csr = _mm_getcsr();
+  if (getenv("GEOFDTD_EMU_FTZ")) _mm_setcsr(csr | 0x8040);
   if (nm == 0) run_steps<0, true>(g, steps);
   else if (g.scalar) run_steps<1, true>(g, steps);
   else if (nm == 1) run_steps<1, false>(g, steps);
@@ -217,9 +223,11 @@ int emu_run(const gf_model *m, const double *force, int64_t nforce, const int32_
   else if (nm == 3) run_steps<3, false>(g, steps);
   else if (nm == 4) run_steps<4, false>(g, steps);
   else {
+    _mm_setcsr(csr);
     emu_err = "emu: unsupported mechanism count";
     return GF_ERR_INVALID;
   }
+  _mm_setcsr(csr);
   return GF_OK;
 }
 

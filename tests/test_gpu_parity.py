@@ -348,9 +348,12 @@ def test_reset_replays_the_same_shot(fdtd):
 
 
 def test_linearity_in_the_force(fdtd):
-    """Doubling the force doubles every trace (a power of two scales every
-    FP32 operation exactly, up to values flushed as denormals); zero force
-    gives an identically zero field."""
+    """Doubling the force doubles every trace, and zero force gives an
+    identically zero field.  A power of two scales every normal FP32 operation
+    exactly; the stencil's leading edge runs through the denormal range, where
+    it does not, and the GPU traces then differ from exact doubling by a few
+    ULPs of their peak (measured: 2**-32 on a 5.2e-4 peak; the CPU emulation
+    of the kernel shows the same effect once denormals are flushed)."""
     m = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
     dt = m.Grid()[0]
     f = np.asarray(fdtd.Ricker(1000, 30, dt, 500))
@@ -358,5 +361,5 @@ def test_linearity_in_the_force(fdtd):
     b = fdtd.Run(fdtd.Shot(m, 2 * f, [1.0, 3.0], 500))
     z = fdtd.Run(fdtd.Shot(m, 0 * f, [1.0, 3.0], 500))
     for ta, tb, tz in zip(a.Traces, b.Traces, z.Traces):
-        assert np.max(np.abs(2 * ta.Vertical - tb.Vertical)) <= 1e-12 * np.max(np.abs(tb.Vertical))
+        assert np.max(np.abs(2 * ta.Vertical - tb.Vertical)) <= 2e-6 * np.max(np.abs(tb.Vertical))
         assert not np.any(tz.Vertical)
