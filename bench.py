@@ -164,7 +164,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every shot is independent, so "
-              "N GPUs would each run their own shots with no exchange — see DESIGN.md", file=sys.stderr)
+              "N GPUs would each run their own shots with no exchange", file=sys.stderr)
         return 2
 
     pkg = load_package()

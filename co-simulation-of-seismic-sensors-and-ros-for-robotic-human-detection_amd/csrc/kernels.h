@@ -17,6 +17,10 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
                                hipStream_t st);
 hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, int32_t nmodels, int64_t tiles,
                        int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st);
+hipError_t launch_dft_f32(const float *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,
+                          const double *freqs, int32_t nbins, double *out, hipStream_t st);
+hipError_t launch_dft_f64(const double *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,
+                          const double *freqs, int32_t nbins, double *out, hipStream_t st);
 hipError_t launch_surface_velocity(const GridDesc *d, int32_t buf, int32_t col, float *out, hipStream_t st);
 hipError_t launch_max_speed(const GridDesc *d, int32_t buf, uint32_t *slot, int64_t cells, hipStream_t st);
 

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+
+#include <cmath>
 #include <string.h>
 
 #include <stdlib.h>
@@ -22,6 +24,7 @@
 #include "../../include/geofdtd.h"
 #include "kernels.h"
 #include "model.h"
+#include "batch.h"
 #include "sim.h"
 #include "tune.h"
 
@@ -35,9 +38,9 @@ int api_set_err(int code, const std::string &msg);
     if (e_ != hipSuccess) return gfs::hip_fail(e_, #expr); \
   } while (0)
 
-namespace {
-
 using gfs::Sim;
+
+namespace {
 
 struct Group {
   int nmech = 0;
@@ -121,84 +124,81 @@ int capture(Group &G, int models, hipStream_t st) {
 
 }  // namespace
 
-extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
-  if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
-  std::vector<std::unique_ptr<Sim>> sims(n_shots);
-  std::vector<int32_t> st(n_shots, GF_OK), steps(n_shots, 0);
-  std::vector<std::string> msg(n_shots);
-  std::vector<std::vector<int32_t>> cols(n_shots);
-  int first_err = GF_OK;
-  std::string first_msg;
-  auto record = [&](int i, int code, const std::string &m) {
-    st[i] = code;
-    msg[i] = m;
-    if (!first_err) {
-      first_err = code;
-      first_msg = m;
-    }
-  };
+namespace gfb {
 
-  // ---- per-shot set-up, in the reference's order of checks ----
-  for (int i = 0; i < n_shots; i++) {
+int Batch::fail_shot(int i, int code, const std::string &m) {
+  st[i] = code;
+  msg[i] = m;
+  if (!first_err) {
+    first_err = code;
+    first_msg = m;
+  }
+  sims[i].reset();
+  return code;
+}
+
+// Per-shot set-up in the reference's order of checks (run.go:88-108):
+// New's validation, the step count, then each receiver against MaxRange.
+// `ranges[i]` (n_ranges[i] entries) are shot i's receiver ranges.
+void Batch::setup(const gf_shot *shots, int32_t n, const std::vector<std::vector<double>> &ranges) {
+  sims.resize(n);
+  st.assign(n, GF_OK);
+  steps.assign(n, 0);
+  msg.assign(n, std::string());
+  cols.assign(n, {});
+  snapped.assign(n, {});
+  for (int i = 0; i < n; i++) {
     const gf_shot &sh = shots[i];
-    if (!sh.model || (sh.n_ranges > 0 && !sh.ranges) || (sh.n_force > 0 && !sh.force) ||
-        (sh.n_ranges > 0 && !results[i].traces)) {
-      record(i, GF_ERR_ARGUMENT, "geofdtd: null argument in shot");
+    if (!sh.model || (sh.n_force > 0 && !sh.force)) {
+      fail_shot(i, GF_ERR_ARGUMENT, "geofdtd: null argument in shot");
       continue;
     }
     sims[i].reset(new Sim());
     Sim &s = *sims[i];
     int rc = s.init(*sh.model, std::max(sh.steps, 0));
     if (rc) {
-      record(i, rc, gf_last_error());
-      sims[i].reset();
+      fail_shot(i, rc, gf_last_error());
       continue;
     }
     if (sh.steps <= 0) {
-      record(i, GF_ERR_INVALID, "fdtd: a shot needs a positive number of steps");
-      sims[i].reset();
+      fail_shot(i, GF_ERR_INVALID, "fdtd: a shot needs a positive number of steps");
       continue;
     }
     if ((rc = s.set_force(sh.force, sh.n_force))) {
-      record(i, rc, gf_last_error());
-      sims[i].reset();
+      fail_shot(i, rc, gf_last_error());
       continue;
     }
     bool bad = false;
-    for (int r = 0; r < sh.n_ranges && !bad; r++) {
-      int c = (int)round(sh.ranges[r] / s.R.h);
+    for (double r : ranges[i]) {
+      int c = (int)round(r / s.R.h);
       c = std::max(0, std::min(c, s.R.nr - 2));
-      const double snapped = (double)c * s.R.h;
-      if (snapped >= sh.model->max_range_m) {
-        record(i, GF_ERR_INVALID,
-               "fdtd: receiver at " + gfm::go_g(sh.ranges[r]) + " m is outside the interior (" +
-                   gfm::go_g(sh.model->max_range_m) + " m)");
+      const double sn = (double)c * s.R.h;
+      if (sn >= sh.model->max_range_m) {
+        fail_shot(i, GF_ERR_INVALID,
+                  "fdtd: receiver at " + gfm::go_g(r) + " m is outside the interior (" +
+                      gfm::go_g(sh.model->max_range_m) + " m)");
         bad = true;
+        break;
       }
       cols[i].push_back(c);
-      if (results[i].snapped) results[i].snapped[r] = snapped;
+      snapped[i].push_back(sn);
     }
-    if (bad) {
-      sims[i].reset();
-      continue;
-    }
+    if (bad) continue;
     if ((rc = s.set_receivers(cols[i].data(), (int32_t)cols[i].size(), sh.steps, 0)) ||
         (rc = s.reset_max_slots(sh.steps))) {
-      record(i, rc, gf_last_error());
-      sims[i].reset();
+      fail_shot(i, rc, gf_last_error());
       continue;
     }
     steps[i] = sh.steps;
-    results[i].dt = s.R.dt;
-    results[i].t0 = 0.5 * s.R.dt;
-    results[i].nr = s.R.nr;
-    results[i].nz = s.R.nz;
-    results[i].peak = 0;
   }
+}
 
-  // ---- group by kernel variant; one launch per group per step ----
+// All set-up shots, advanced together: one launch per kernel variant per
+// step over the tiles of every model still running, in graph blocks.
+int Batch::run() {
+  const int n = (int)sims.size();
   std::vector<Group> groups;
-  for (int i = 0; i < n_shots; i++) {
+  for (int i = 0; i < n; i++) {
     if (!sims[i]) continue;
     const Sim &s = *sims[i];
     const int nm = s.g.lossy ? s.g.nmech : 0;
@@ -212,126 +212,158 @@ extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *re
     }
     it->order.push_back(i);
   }
-  hipStream_t stream = nullptr;
+  if (groups.empty()) return GF_OK;
   int rc = GF_OK;
   int64_t max_steps = 0;
-  for (int i = 0; i < n_shots; i++) max_steps = std::max<int64_t>(max_steps, sims[i] ? steps[i] : 0);
-  if (!groups.empty()) {
-    stream = sims[groups[0].order[0]]->stream;
-    for (auto &sp : sims)
-      if (sp && !rc) {
-        hipError_t e = hipStreamSynchronize(sp->stream);
-        if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
-      }
-    for (Group &G : groups) {
-      std::stable_sort(G.order.begin(), G.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
-      int64_t cols_rows = 0, cell_steps = 0, cells = 0;
-      for (int i : G.order) {
-        cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
-        cells += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
-        cell_steps += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
-      }
-      if ((rc = plan_group(G, sims, gfs::choose_tile_rows_cells(cols_rows)))) break;
-      // Long runs pay for a few trial steps at each tile height.
-      const char *tune_env = getenv("GEOFDTD_TUNE");
-      const bool want_tune = !getenv("GEOFDTD_TILE_ROWS") && (tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL);
-      if (want_tune) {
-        gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(), (int64_t)log2((double)cells)};
-        auto prepare = [&](int th) { return plan_group(G, sims, th) == GF_OK; };
-        auto launch = [&](int th, int n) {
-          return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, (int32_t)G.order.size(), G.prefix.back(), n,
-                                  nullptr, 0, stream);
-        };
-        int best = gft::cached(key);
-        if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
-        if (best && (rc = plan_group(G, sims, best))) break;
-        for (int i : G.order)
-          if ((rc = sims[i]->reset_state())) break;
-        if (rc) break;
-      }
+  for (int i = 0; i < n; i++) max_steps = std::max<int64_t>(max_steps, sims[i] ? steps[i] : 0);
+  stream = sims[groups[0].order[0]]->stream;
+  for (auto &sp : sims)
+    if (sp && !rc) {
+      hipError_t e = hipStreamSynchronize(sp->stream);
+      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
     }
-    for (auto &sp : sims)
-      if (sp && !rc) {
-        hipError_t e = hipStreamSynchronize(sp->stream);
-        if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
-      }
-
-    // ---- the time loop ----
-    for (int64_t n = 0; n < max_steps && !rc;) {
-      const bool graph_block = n + gfs::kGraphSteps <= max_steps;
-      for (Group &G : groups) {
-        const int act = active_models(G, steps, n);
-        if (act == 0) continue;
-        if (graph_block) {
-          // Re-capture when the running set has shrunk by a tenth of its tiles.
-          if (!G.exec || G.graph_models < act ||
-              (G.prefix[act] < G.prefix[G.graph_models] - G.prefix[G.graph_models] / 10)) {
-            if ((rc = capture(G, act, stream))) break;
-          }
-          hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)n, stream);
-          if (e == hipSuccess) e = hipGraphLaunch(G.exec, stream);
-          if (e != hipSuccess) {
-            rc = gfs::hip_fail(e, "graph launch");
-            break;
-          }
-        } else {
-          hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, act, G.prefix[act], (int32_t)n, nullptr, 1, stream);
-          if (e != hipSuccess) {
-            rc = gfs::hip_fail(e, "step launch");
-            break;
-          }
-        }
-      }
-      n += graph_block ? gfs::kGraphSteps : 1;
+  for (Group &G : groups) {
+    if (rc) break;
+    std::stable_sort(G.order.begin(), G.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    int64_t cols_rows = 0, cell_steps = 0, cells = 0;
+    for (int i : G.order) {
+      cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
+      cells += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
+      cell_steps += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
     }
-    if (!rc) {
-      hipError_t e = hipStreamSynchronize(stream);
-      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
+    if ((rc = plan_group(G, sims, gfs::choose_tile_rows_cells(cols_rows)))) break;
+    // Long runs pay for a few trial steps at each tile height.
+    const char *tune_env = getenv("GEOFDTD_TUNE");
+    const bool want_tune =
+        !getenv("GEOFDTD_TILE_ROWS") && (tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL);
+    if (want_tune) {
+      gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(), (int64_t)log2((double)cells)};
+      auto prepare = [&](int th) { return plan_group(G, sims, th) == GF_OK; };
+      auto launch = [&](int th, int k) {
+        return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, (int32_t)G.order.size(), G.prefix.back(), k,
+                                nullptr, 0, stream);
+      };
+      int best = gft::cached(key);
+      if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
+      if (best && (rc = plan_group(G, sims, best))) break;
+      for (int i : G.order)
+        if ((rc = sims[i]->reset_state())) break;
+      tile_rows = best;
+    } else {
+      tile_rows = G.th;
     }
   }
-  for (Group &G : groups) free_group(G);
-  if (rc) return rc;
-
-  // ---- results: divergence probes, then traces ----
-  for (int i = 0; i < n_shots; i++) {
-    if (!sims[i]) continue;
-    Sim &s = *sims[i];
-    const int64_t slots = (steps[i] - 1) / 256 + 1;
-    std::vector<uint32_t> bits(slots);
-    if (hipMemcpy(bits.data(), s.d_max, sizeof(uint32_t) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
-      record(i, GF_ERR_DEVICE, "geofdtd: reading divergence probes failed");
-      continue;
+  for (auto &sp : sims)
+    if (sp && !rc) {
+      hipError_t e = hipStreamSynchronize(sp->stream);
+      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
     }
-    double peak = 0;
-    bool diverged = false;
-    for (int64_t k = 0; k < slots; k++) {
-      const double p = gfs::bits_to_speed(bits[k]);
-      peak = std::isnan(p) || std::isnan(peak) ? NAN : std::max(peak, p);
-      if (std::isnan(p) || std::isinf(p)) {
-        char buf[128];
-        snprintf(buf, sizeof buf, "fdtd: diverged at step %lld of %d", (long long)(k * 256), steps[i]);
-        record(i, GF_ERR_DIVERGED, buf);
-        diverged = true;
-        break;
+
+  for (int64_t k = 0; k < max_steps && !rc;) {
+    const bool graph_block = k + gfs::kGraphSteps <= max_steps;
+    for (Group &G : groups) {
+      const int act = active_models(G, steps, k);
+      if (act == 0) continue;
+      if (graph_block) {
+        // Re-capture when the running set has shrunk by a tenth of its tiles.
+        if (!G.exec || G.graph_models < act ||
+            (G.prefix[act] < G.prefix[G.graph_models] - G.prefix[G.graph_models] / 10)) {
+          if ((rc = capture(G, act, stream))) break;
+        }
+        hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)k, stream);
+        if (e == hipSuccess) e = hipGraphLaunch(G.exec, stream);
+        if (e != hipSuccess) {
+          rc = gfs::hip_fail(e, "graph launch");
+          break;
+        }
+      } else {
+        hipError_t e =
+            gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, act, G.prefix[act], (int32_t)k, nullptr, 1, stream);
+        if (e != hipSuccess) {
+          rc = gfs::hip_fail(e, "step launch");
+          break;
+        }
       }
     }
-    results[i].peak = peak;
-    if (diverged) continue;
-    const size_t nrec = cols[i].size();
+    k += graph_block ? gfs::kGraphSteps : 1;
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
+  }
+  for (Group &G : groups) free_group(G);
+  return rc;
+}
+
+// The divergence probes of shot i (run.go:118-124): its peak, and whether a
+// probe saw NaN or Inf (then the shot fails as the reference's Run does).
+void Batch::probes(int i, double *peak_out) {
+  Sim &s = *sims[i];
+  const int64_t slots = (steps[i] - 1) / 256 + 1;
+  std::vector<uint32_t> bits(slots);
+  if (hipMemcpy(bits.data(), s.d_max, sizeof(uint32_t) * slots, hipMemcpyDeviceToHost) != hipSuccess) {
+    fail_shot(i, GF_ERR_DEVICE, "geofdtd: reading divergence probes failed");
+    return;
+  }
+  double peak = 0;
+  for (int64_t k = 0; k < slots; k++) {
+    const double p = gfs::bits_to_speed(bits[k]);
+    peak = std::isnan(p) || std::isnan(peak) ? NAN : std::max(peak, p);
+    if (std::isnan(p) || std::isinf(p)) {
+      char buf[128];
+      snprintf(buf, sizeof buf, "fdtd: diverged at step %lld of %d", (long long)(k * 256), steps[i]);
+      if (peak_out) *peak_out = peak;
+      fail_shot(i, GF_ERR_DIVERGED, buf);
+      return;
+    }
+  }
+  if (peak_out) *peak_out = peak;
+}
+
+}  // namespace gfb
+
+extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
+  if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
+  std::vector<std::vector<double>> ranges(n_shots);
+  for (int i = 0; i < n_shots; i++) {
+    if (shots[i].n_ranges > 0 && (!shots[i].ranges || !results[i].traces))
+      return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null ranges or traces in shot");
+    if (shots[i].n_ranges > 0) ranges[i].assign(shots[i].ranges, shots[i].ranges + shots[i].n_ranges);
+  }
+  gfb::Batch B;
+  B.setup(shots, n_shots, ranges);
+  for (int i = 0; i < n_shots; i++) {
+    if (!B.sims[i]) continue;
+    results[i].dt = B.sims[i]->R.dt;
+    results[i].t0 = 0.5 * B.sims[i]->R.dt;
+    results[i].nr = B.sims[i]->R.nr;
+    results[i].nz = B.sims[i]->R.nz;
+    results[i].peak = 0;
+    for (size_t r = 0; r < B.snapped[i].size(); r++)
+      if (results[i].snapped) results[i].snapped[r] = B.snapped[i][r];
+  }
+  int rc = B.run();
+  if (rc) return rc;
+  for (int i = 0; i < n_shots; i++) {
+    if (!B.sims[i]) continue;
+    B.probes(i, &results[i].peak);
+    if (!B.sims[i]) continue;
+    Sim &s = *B.sims[i];
+    const size_t nrec = B.cols[i].size();
     if (nrec > 0) {
-      std::vector<float> tmp((size_t)steps[i] * nrec);
+      std::vector<float> tmp((size_t)B.steps[i] * nrec);
       if (hipMemcpy(tmp.data(), s.d_traces, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost) != hipSuccess) {
-        record(i, GF_ERR_DEVICE, "geofdtd: reading traces failed");
+        B.fail_shot(i, GF_ERR_DEVICE, "geofdtd: reading traces failed");
         continue;
       }
       for (size_t k = 0; k < tmp.size(); k++) results[i].traces[k] = tmp[k];
     }
   }
   for (int i = 0; i < n_shots; i++) {
-    if (statuses) statuses[i] = st[i];
-    results[i].status = st[i];
-    snprintf(results[i].error, sizeof results[i].error, "%s", msg[i].c_str());
+    if (statuses) statuses[i] = B.st[i];
+    results[i].status = B.st[i];
+    snprintf(results[i].error, sizeof results[i].error, "%s", B.msg[i].c_str());
   }
-  if (first_err) return gfs::api_set_err(first_err, first_msg);
+  if (B.first_err) return gfs::api_set_err(B.first_err, B.first_msg);
   return gfs::api_set_err(GF_OK, "");
 }

@@ -101,6 +101,46 @@ class GfResult(C.Structure):
     ]
 
 
+class GfBankHeader(C.Structure):
+    _fields_ = [
+        ("solver", C.c_char_p),
+        ("config_hash", C.c_char_p),
+        ("created", C.c_char_p),
+        ("notes", C.c_char_p),
+        ("medium", C.POINTER(GfLayer)),
+        ("n_medium", C.c_int32),
+        ("sample_rate_hz", C.c_double),
+        ("samples", C.c_int32),
+        ("r_min_m", C.c_double),
+        ("r_max_m", C.c_double),
+        ("count", C.c_int32),
+        ("component", C.c_char_p),
+        ("units", C.c_char_p),
+    ]
+
+
+class GfBankSpec(C.Structure):
+    _fields_ = [
+        ("sample_rate_hz", C.c_double),
+        ("samples", C.c_int32),
+        ("max_freq_hz", C.c_double),
+        ("r_min_m", C.c_double),
+        ("r_max_m", C.c_double),
+        ("count", C.c_int32),
+    ]
+
+
+class GfSoil(C.Structure):
+    _fields_ = [
+        ("layers", GfLayer * 4),
+        ("n_layers", C.c_int32),
+        ("spacing_m", C.c_double),
+        ("max_range_m", C.c_double),
+        ("depth_m", C.c_double),
+        ("pml_cells", C.c_int32),
+    ]
+
+
 # Every symbol include/geofdtd.h declares, with its ctypes signature.
 _D, _I, _I32, _I64 = C.c_double, C.c_int, C.c_int32, C.c_int64
 _PD, _PI32, _PI64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
@@ -133,6 +173,13 @@ SIGNATURES = {
     "gf_sim_step_bytes": (_I, [C.c_void_p, _PD]),
     "gf_sim_autotune": (_I, [C.c_void_p, _I32, _PI32]),
     "gf_sim_set_tile_rows": (_I, [C.c_void_p, _I32]),
+    "gf_bank_write": (_I, [C.c_char_p, C.POINTER(GfBankHeader), C.POINTER(C.c_float)]),
+    "gf_bank_header_json": (_I, [C.POINTER(GfBankHeader), C.c_char_p, _I32]),
+    "gf_bank_generate": (_I, [C.POINTER(GfShot), _I32, C.POINTER(GfBankSpec), C.POINTER(C.c_float), _PD,
+                              C.POINTER(GfResult)]),
+    "gf_soil_sample": (_I, [C.c_uint64, C.c_uint32, C.POINTER(GfSoil)]),
+    "gf_spectrum": (_I, [C.POINTER(C.c_float), _I32, _I32, _D, _D, _PD, _I64, _PD, _I32, _PD]),
+    "gf_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -534,3 +581,133 @@ def _go_g(x: float) -> str:
     if -4 <= e < 6:
         return "%.*f" % (max(p - 1 - e, 0), x)
     return t
+
+
+# ---------------------------------------------------------------- banks
+
+
+@dataclass
+class BankHeader:
+    """bank.Header (internal/bank/format.go:115-131)."""
+
+    Solver: str
+    Medium: List[Layer]
+    SampleRateHz: float
+    Samples: int
+    MinM: float
+    MaxM: float
+    Count: int
+    Component: str = "vertical surface displacement per unit vertical surface point force"
+    Units: str = "m/N"
+    ConfigHash: str = ""
+    Created: str = ""
+    Notes: str = ""
+
+    def to_c(self):
+        layers = (GfLayer * max(1, len(self.Medium)))()
+        for i, l in enumerate(self.Medium):
+            layers[i] = GfLayer(l.Thickness, l.Vp, l.Vs, l.Density, l.Qs, l.Qp)
+        h = GfBankHeader(
+            self.Solver.encode(), self.ConfigHash.encode(), self.Created.encode(), self.Notes.encode(),
+            C.cast(layers, C.POINTER(GfLayer)) if self.Medium else None, len(self.Medium),
+            self.SampleRateHz, self.Samples, self.MinM, self.MaxM, self.Count,
+            self.Component.encode(), self.Units.encode(),
+        )
+        return h, layers
+
+    @property
+    def Bins(self) -> int:
+        return self.Samples // 2 + 1
+
+
+def WriteBank(path: str, h: BankHeader, data: np.ndarray) -> None:
+    """bank.Write: data is complex [Count, Samples/2+1] (stored as float32)."""
+    d = np.ascontiguousarray(np.stack([np.real(data), np.imag(data)], axis=-1), dtype=np.float32)
+    if d.shape != (h.Count, h.Bins, 2):
+        raise FdtdError(GF_ERR_ARGUMENT, f"data is {data.shape}, header implies {(h.Count, h.Bins)}")
+    ch, keep = h.to_c()
+    _check(load_library().gf_bank_write(path.encode(), C.byref(ch), d.ctypes.data_as(C.POINTER(C.c_float))))
+
+
+def BankHeaderJSON(h: BankHeader) -> str:
+    ch, keep = h.to_c()
+    buf = C.create_string_buffer(1 << 16)
+    _check(load_library().gf_bank_header_json(C.byref(ch), buf, len(buf)))
+    return buf.value.decode()
+
+
+@dataclass
+class BankSpec:
+    SampleRateHz: float = 500.0
+    Samples: int = 1024
+    MaxFreqHz: float = 100.0
+    MinM: float = 1.0
+    MaxM: float = 50.0
+    Count: int = 50
+
+
+def GenerateBanks(shots: Sequence[Shot], spec: BankSpec):
+    """FDTD-filled banks: (spectra complex [n, Count, Samples/2+1], snapped
+    ranges [n, Count], per-shot Result-or-FdtdError list)."""
+    lib = load_library()
+    n = len(shots)
+    cshots = (GfShot * n)()
+    keep = []
+    for i, sh in enumerate(shots):
+        c, k = _shot_c(Shot(sh.Model, sh.Force, [], sh.Steps))
+        cshots[i] = c
+        keep.append(k)
+    bins = spec.Samples // 2 + 1
+    out = np.zeros((n, spec.Count, bins, 2), dtype=np.float32)
+    snapped = np.zeros((n, spec.Count), dtype=np.float64)
+    cres = (GfResult * n)()
+    cs = GfBankSpec(spec.SampleRateHz, spec.Samples, spec.MaxFreqHz, spec.MinM, spec.MaxM, spec.Count)
+    rc = lib.gf_bank_generate(cshots, n, C.byref(cs), out.ctypes.data_as(C.POINTER(C.c_float)), _dptr(snapped), cres)
+    status = [cres[i].status for i in range(n)]
+    if rc != GF_OK and not any(status):
+        _check(rc)
+    results: List[object] = []
+    for i in range(n):
+        if status[i] != GF_OK:
+            results.append(FdtdError(status[i], cres[i].error.decode()))
+        else:
+            results.append(Result(cres[i].dt, cres[i].t0, [], (cres[i].nr, cres[i].nz), cres[i].peak))
+    return out[..., 0] + 1j * out[..., 1], snapped, results
+
+
+def Spectrum(traces: np.ndarray, dt: float, t0: float, force: Sequence[float], freqs: Sequence[float]) -> np.ndarray:
+    """G(f) = V(f) / (i w W(f)) of float32 traces [rows, n] on the device."""
+    tr = np.ascontiguousarray(traces, dtype=np.float32)
+    if tr.ndim == 1:
+        tr = tr[None, :]
+    f = np.ascontiguousarray(force, dtype=np.float64)
+    fq = np.ascontiguousarray(freqs, dtype=np.float64)
+    out = np.zeros((tr.shape[0], len(fq), 2), dtype=np.float64)
+    _check(load_library().gf_spectrum(tr.ctypes.data_as(C.POINTER(C.c_float)), tr.shape[0], tr.shape[1], dt, t0,
+                                      _dptr(f), len(f), _dptr(fq), len(fq), _dptr(out)))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+@dataclass
+class Soil:
+    Stack: List[Layer]
+    Spacing: float
+    MaxRange: float
+    Depth: float
+    PMLCells: int
+
+
+def SampleSoil(seed: int, index: int) -> Soil:
+    """Seeded soil model `index` (Philox4x32-10 stream keyed by seed)."""
+    s = GfSoil()
+    _check(load_library().gf_soil_sample(seed, index, C.byref(s)))
+    stack = [Layer(l.thickness_m, l.vp_mps, l.vs_mps, l.density_kgm3, l.qs, l.qp) for l in s.layers[: s.n_layers]]
+    return Soil(stack, s.spacing_m, s.max_range_m, s.depth_m, s.pml_cells)
+
+
+def Philox4x32(counter: Sequence[int], key: Sequence[int]) -> List[int]:
+    c = (C.c_uint32 * 4)(*counter)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load_library().gf_philox4x32(c, k, o)
+    return list(o)

@@ -174,3 +174,41 @@ def c2(variant: str = "R", steps: int = 80_000, lead: float = 0.0) -> Workload:
     force = c2_stance().sample(1.0 / dt, lead, 0.0)
     ranges = [0.5 * k for k in range(1, 101)]
     return Workload(f"c2-{variant}", model, force, ranges, steps)
+
+
+# ----------------------------------------------------- batched bank generation
+
+
+def c3(n_models: int = 256, seed: int = 0, seconds: float = 1.0, variant: str = "X",
+       first: int = 0) -> List[Workload]:
+    """Config 3: `n_models` seeded soils (models first..first+n_models-1 of the
+    stream keyed by `seed`), each with its own cell size, grid extents and
+    time step (Courant 0.5, the reference default), run for `seconds`; a
+    40 Hz Ricker source; receivers on the bank's range grid (1-50 m).  "X":
+    per-layer Q, Qp = 2 Qs, three mechanisms over 5-150 Hz; "R": one scalar
+    SLS at the half-space's Qs (what the reference can express)."""
+    out = []
+    for i in range(first, first + n_models):
+        soil = fd.SampleSoil(seed, i)
+        if variant == "X":
+            model = fd.Model(Stack=soil.Stack, RefFreq=30, GslsMechanisms=3, GslsFmin=5, GslsFmax=150)
+        elif variant == "R":
+            stack = [fd.Layer(l.Thickness, l.Vp, l.Vs, l.Density, l.Qs) for l in soil.Stack]
+            model = fd.Model(Stack=stack, Relax=fd.Fit(soil.Stack[-1].Qs, 30), RefFreq=30)
+        else:
+            raise ValueError(f"unknown variant {variant!r}")
+        model.MaxRange, model.Depth, model.Spacing, model.PMLCells = (
+            soil.MaxRange, soil.Depth, soil.Spacing, soil.PMLCells)
+        model.DominantFreq = 40
+        dt = model.Grid()[0]
+        steps = int(math.ceil(seconds / dt))
+        force = fd.Ricker(1000, 40, dt, steps)
+        out.append(Workload(f"c3-{variant}-{i}", model, force, [], steps))
+    return out
+
+
+def c3_bank_spec(count: int = 50) -> fd.BankSpec:
+    """Bank grid of config 3: 1-50 m, 512 complex bins to 250 Hz (rate 500 Hz,
+    1024 samples -> 513 bins; bin 0 and bins above 100 Hz, the band the grid
+    resolves, left zero)."""
+    return fd.BankSpec(SampleRateHz=500.0, Samples=1024, MaxFreqHz=100.0, MinM=1.0, MaxM=50.0, Count=count)

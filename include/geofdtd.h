@@ -205,6 +205,89 @@ int gf_sim_set_tile_rows(gf_sim *s, int32_t tile_rows);
  * once, plus the C-PML strips): the roofline numerator. */
 int gf_sim_step_bytes(const gf_sim *s, double *bytes);
 
+/* ---- Green's-function banks -------------------------------------------
+ * bank.Header of internal/bank/format.go:115-131 as a flat struct.  Strings
+ * are NUL-terminated; config_hash, created and notes are omitted from the
+ * file when NULL or empty (their omitempty tags), solver never is. */
+typedef struct gf_bank_header {
+  const char *solver;
+  const char *config_hash;
+  const char *created;
+  const char *notes;
+  const gf_layer *medium; /* written as layer.Layer JSON; NULL/0 writes null */
+  int32_t n_medium;
+  double sample_rate_hz; /* bin k is at k * sample_rate_hz / samples */
+  int32_t samples;       /* positive and even; bins = samples / 2 + 1 */
+  double r_min_m;        /* bank.RangeGrid: count ranges, min to max */
+  double r_max_m;
+  int32_t count;
+  const char *component;
+  const char *units;
+} gf_bank_header;
+
+/* bank.Write (internal/bank/format.go:215-240): format v1, the header as Go's
+ * encoding/json writes it, payload data[count][samples/2+1][2] as
+ * little-endian float32 (re, im), range-major.  Validates as
+ * bank.Header.Validate (format.go:142-165), with its messages. */
+int gf_bank_write(const char *path, const gf_bank_header *h, const float *data);
+/* The header JSON gf_bank_write would write (for inspection and tests). */
+int gf_bank_header_json(const gf_bank_header *h, char *out, int32_t cap);
+
+/* The bank grid a generated bank is reduced onto. */
+typedef struct gf_bank_spec {
+  double sample_rate_hz;
+  int32_t samples;
+  double max_freq_hz; /* bins above it, and bin 0, are left zero */
+  double r_min_m;
+  double r_max_m;
+  int32_t count;
+} gf_bank_spec;
+
+/* Fill banks with the FDTD: one shot per soil model, advanced together as in
+ * gf_run_batch, with receivers at the spec's range grid (shot ranges are
+ * ignored).  Each shot's traces stay in HBM and are reduced on the device to
+ * G(f) = V(f) / (i w W(f)) -- vertical surface displacement per unit vertical
+ * point force, m/N, positive down -- at bins 1..max_freq of the spec's
+ * frequency grid.  spectra is [n_shots][count][samples/2+1][2] float32;
+ * snapped [n_shots][count] receives the ranges the receivers sat at (grid
+ * columns); results (may be NULL) receive per-shot dt, peak, status, error.
+ * No reference counterpart: the reference fills banks by wavenumber
+ * integration per bin (cmd/geobank/main.go:128-167); this is the same file
+ * filled by the FDTD. */
+int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *spec, float *spectra,
+                     double *snapped, gf_result *results);
+
+/* The reduction gf_bank_generate applies, on caller traces: for each of
+ * `rows` traces of n samples at t0 + k dt (float32, row-major) and the force
+ * series that drove them (F[k] at k dt), G(f) = V(f) / (i w W(f)) at each of
+ * nbins frequencies, out[row][bin][2] in FP64.  Runs on the device. */
+int gf_spectrum(const float *traces, int32_t rows, int32_t n, double dt, double t0, const double *force,
+                int64_t n_force, const double *freqs, int32_t nbins, double *out);
+
+/* ---- seeded soil models (batched bank generation) ----------------------
+ * Soil model `index` of the stream keyed by `seed`, drawn with Philox4x32-10
+ * (Salmon et al., SC'11): 1-3 layers over a half-space, each with Vs
+ * U[120,400] m/s, Vp/Vs U[1.7,2.5], rho U[1600,2100] kg/m^3, Qs U[10,80],
+ * Qp = 2 Qs, thickness U[0.5,5] m; cell size h = 0.9 min(Vs) / (100 Hz * 10)
+ * (ten cells per shortest Rayleigh wavelength at 100 Hz); a 20-cell C-PML;
+ * an interior of about 218 m by 108 m so that the grid is 600-2048 by
+ * 300-1024 cells.  The same (seed, index) always gives the same model.
+ * No reference counterpart (the reference's sweep axes, internal/sweep,
+ * sample parameters of its analytic model, not grids). */
+typedef struct gf_soil {
+  gf_layer layers[4];
+  int32_t n_layers;
+  double spacing_m;
+  double max_range_m;
+  double depth_m;
+  int32_t pml_cells;
+} gf_soil;
+
+int gf_soil_sample(uint64_t seed, uint32_t index, gf_soil *out);
+
+/* Philox4x32-10 block (counter, key) -> 4 words, for checking the sampler. */
+void gf_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif
