@@ -129,6 +129,80 @@ def cpu_baseline(w, gpu_traces, sample_steps: int, threads: int) -> tuple:
     return base, acc
 
 
+def bench_c3(args, fd) -> int:
+    """Config 3: batched bank generation over seeded soils (one job)."""
+    import numpy as np
+
+    import cosim_amd.workloads as W
+
+    ws = W.c3(n_models=args.c3_models, seconds=args.c3_seconds)
+    spec = W.c3_bank_spec()
+    shots = [w.shot() for w in ws]
+    cells = [w.model.Grid()[1] * w.model.Grid()[2] for w in ws]
+    cell_steps = float(sum(c * w.steps for c, w in zip(cells, ws)))
+    log(f"c3: {len(ws)} models, {sum(cells) / 1e6:.1f} M cells, {cell_steps:.3g} cell-steps")
+    t0 = time.perf_counter()
+    spectra, snapped, res = fd.GenerateBanks(shots, spec)
+    wall = time.perf_counter() - t0
+    st = fd.BatchStats()
+    bad = [r for r in res if isinstance(r, fd.FdtdError)]
+    if bad:
+        raise SystemExit(f"{len(bad)} models failed: {bad[0]}")
+    if not np.all(np.isfinite(spectra)):
+        raise SystemExit("non-finite spectra")
+    traces = len(ws) * spec.Count
+    achieved = st["algorithmic_bytes"] / (st["kernel_ms"] / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": traces / wall,
+        "unit": "traces/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"c3: {len(ws)} seeded soils (Philox seed 0), 600-2048 x 300-1024 cells, "
+                        f"{args.c3_seconds:g} s each, {spec.Count} receivers, banks of {spec.Samples // 2 + 1} bins",
+            "models": len(ws), "cells": sum(cells), "cell_steps": cell_steps,
+            "physics": "GSLS, 3 mechanisms, per-layer Qs, Qp = 2 Qs",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_step over every model still running (one launch per step)",
+            "tile_rows": st["tile_rows"], "algorithmic_bytes": st["algorithmic_bytes"],
+            "step_loop_ms": st["kernel_ms"], "step_launches": st["step_launches"],
+            "avg_launch_us": st["kernel_ms"] * 1e3 / max(st["step_launches"], 1),
+        },
+        "cells_per_s": cell_steps / wall,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = ws[:threads]
+        steps = [args.cpu_sample_steps] * len(sample)
+        t1 = time.perf_counter()
+        O.run_many([w.model for w in sample], [w.force for w in sample], [1.0, 25.0, 50.0], steps, threads)
+        el = time.perf_counter() - t1
+        rate = sum(c * args.cpu_sample_steps for c in cells[:threads]) / el  # cell-steps/s, all threads
+        out["cpu_baseline"] = {
+            "value": traces / (cell_steps / rate), "unit": "traces/s", "cores": threads, "kind": "port",
+            "sample": f"the first {len(sample)} soils, {args.cpu_sample_steps} steps each, one per thread ({el:.1f} s); "
+                      f"scaled to the batch by cell-steps",
+            "cell_steps_per_s": rate,
+        }
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +219,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-steps", type=int, default=0,
                     help="shorten every shot to this many time steps (profiling runs; recorded in config)")
+    ap.add_argument("--c3-models", type=int, default=256, help="config 3: soil models in the batch")
+    ap.add_argument("--c3-seconds", type=float, default=1.0, help="config 3: simulated seconds per model")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the HBM counter "
                          "calibration for rocprofv3 --pmc runs) and exit")
@@ -170,6 +246,9 @@ def main() -> int:
     pkg = load_package()
     fd = pkg.fdtd
     import numpy as np
+
+    if args.workload == "c3":
+        return bench_c3(args, fd)
 
     def measure(name):
         w = workload(name)

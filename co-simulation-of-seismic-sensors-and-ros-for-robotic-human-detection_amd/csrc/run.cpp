@@ -259,8 +259,17 @@ int Batch::run() {
       if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
     }
 
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+              hipEventRecord(e0, stream) != hipSuccess))
+    rc = gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event set-up failed");
+  bytes = 0;
+  for (int i = 0; i < n; i++)
+    if (sims[i]) bytes += gfs::step_bytes(sims[i]->R) * steps[i];
+  launches = 0;
   for (int64_t k = 0; k < max_steps && !rc;) {
     const bool graph_block = k + gfs::kGraphSteps <= max_steps;
+    for (Group &G : groups) launches += active_models(G, steps, k) > 0 ? (graph_block ? gfs::kGraphSteps : 1) : 0;
     for (Group &G : groups) {
       const int act = active_models(G, steps, k);
       if (act == 0) continue;
@@ -288,11 +297,30 @@ int Batch::run() {
     k += graph_block ? gfs::kGraphSteps : 1;
   }
   if (!rc) {
-    hipError_t e = hipStreamSynchronize(stream);
+    hipError_t e = hipEventRecord(e1, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
+    kernel_ms = ms;
   }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   for (Group &G : groups) free_group(G);
+  record_stats(*this);
   return rc;
+}
+
+thread_local gf_batch_stats_t g_stats;
+
+void record_stats(const Batch &b) {
+  g_stats.kernel_ms = b.kernel_ms;
+  g_stats.step_launches = b.launches;
+  g_stats.algorithmic_bytes = b.bytes;
+  g_stats.tile_rows = b.tile_rows;
+  int models = 0;
+  for (auto &s : b.sims) models += s ? 1 : 0;
+  g_stats.models = models;
 }
 
 // The divergence probes of shot i (run.go:118-124): its peak, and whether a
@@ -321,6 +349,12 @@ void Batch::probes(int i, double *peak_out) {
 }
 
 }  // namespace gfb
+
+extern "C" int gf_batch_stats(gf_batch_stats_t *out) {
+  if (!out) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  *out = gfb::g_stats;
+  return gfs::api_set_err(GF_OK, "");
+}
 
 extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
   if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");

@@ -101,6 +101,16 @@ class GfResult(C.Structure):
     ]
 
 
+class GfBatchStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("step_launches", C.c_int64),
+        ("algorithmic_bytes", C.c_double),
+        ("tile_rows", C.c_int32),
+        ("models", C.c_int32),
+    ]
+
+
 class GfBankHeader(C.Structure):
     _fields_ = [
         ("solver", C.c_char_p),
@@ -178,6 +188,7 @@ SIGNATURES = {
     "gf_bank_generate": (_I, [C.POINTER(GfShot), _I32, C.POINTER(GfBankSpec), C.POINTER(C.c_float), _PD,
                               C.POINTER(GfResult)]),
     "gf_soil_sample": (_I, [C.c_uint64, C.c_uint32, C.POINTER(GfSoil)]),
+    "gf_batch_stats": (_I, [C.POINTER(GfBatchStats)]),
     "gf_spectrum": (_I, [C.POINTER(C.c_float), _I32, _I32, _D, _D, _PD, _I64, _PD, _I32, _PD]),
     "gf_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 }
@@ -463,6 +474,14 @@ class Result:
     Traces: List[Trace]
     Cells: tuple
     Peak: float
+
+
+def BatchStats() -> dict:
+    """Timing of this thread's last RunBatch / GenerateBanks."""
+    s = GfBatchStats()
+    _check(load_library().gf_batch_stats(C.byref(s)))
+    return {"kernel_ms": s.kernel_ms, "step_launches": s.step_launches, "algorithmic_bytes": s.algorithmic_bytes,
+            "tile_rows": s.tile_rows, "models": s.models}
 
 
 def _shot_c(sh: Shot):

@@ -179,6 +179,20 @@ int gf_run(const gf_shot *shot, gf_result *res);
 int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results,
                  int32_t *statuses);
 
+/* Timing of the calling thread's last gf_run_batch or gf_bank_generate: the
+ * device time of its step loop (HIP events around it on the solver's
+ * stream), the step kernels launched, the algorithmic HBM bytes of every step
+ * of every model, the tile height and the models run. */
+typedef struct gf_batch_stats_t {
+  double kernel_ms;
+  int64_t step_launches;
+  double algorithmic_bytes;
+  int32_t tile_rows;
+  int32_t models;
+} gf_batch_stats_t;
+
+int gf_batch_stats(gf_batch_stats_t *out);
+
 /* ---- device-resident benchmarking -------------------------------------
  * The timed region of a benchmark must start with inputs already in HBM, so
  * these split gf_run into set-up, advance and read-back.  Receivers record
