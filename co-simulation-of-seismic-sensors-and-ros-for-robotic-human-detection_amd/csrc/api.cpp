@@ -132,7 +132,7 @@ int Sim::autotune(int trial_steps, int *chosen) {
     return err == GF_OK;
   };
   auto launch = [&](int th, int n) {
-    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, th, d_desc, 1, (int64_t)g.tiles_r * g.tiles_z,
+    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, th, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
                             (int32_t)(s0 + n), nullptr, 0, stream);
   };
   int best = gft::cached(key);
@@ -313,7 +313,7 @@ int Sim::reset_max_slots(int64_t steps) {
 }
 
 int Sim::launch_one(int64_t n, bool probe) {
-  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, 1, (int64_t)g.tiles_r * g.tiles_z,
+  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
                           (int32_t)n, nullptr, probe ? 1 : 0, stream));
   return GF_OK;
 }
@@ -361,7 +361,7 @@ int Sim::ensure_graph(bool probe) {
   HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, 1,
+    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, nullptr,
                                     (int64_t)g.tiles_r * g.tiles_z, k, nbase_dev, probe ? 1 : 0, stream);
     if (e != hipSuccess) {
       hipGraph_t tmp;

@@ -76,24 +76,17 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // memory latency per step instead of one per row.  TH == 0: the tile height
 // is g.th at run time and loads run one row ahead of the arithmetic.
 template <int NM, bool SC, int TH>
-__global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, int32_t nmodels, int32_t n_off,
-                                              const int32_t *__restrict__ nbase, int32_t probe) {
+__global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
+                                              int32_t n_off, const int32_t *__restrict__ nbase, int32_t probe) {
   const int lane = threadIdx.x & (kLanes - 1);
   // Wave-uniform by construction; readfirstlane lets the compiler keep the
   // descriptor reads scalar.
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int tile = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
 
-  int m = 0;
-  if (nmodels > 1) {
-    int lo = 0, hi = nmodels - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (gds[mid].tile_base <= tile) lo = mid;
-      else hi = mid - 1;
-    }
-    m = lo;
-  }
+  // A batch's tiles are numbered model by model; the host-built table gives
+  // each tile's model in one load (a single model needs none).
+  const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
   const GridDesc &g = gds[m];
   const int n = n_off + (nbase ? nbase[0] : 0);
   if (n >= g.steps) return;
@@ -265,37 +258,38 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
 }
 
 template <int NM, bool SC, int TH>
-static hipError_t launch_step_t(const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
+static hipError_t launch_step_t(const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
                                 const int32_t *nbase, int32_t probe, hipStream_t st) {
   unsigned blocks = (unsigned)((tiles + 3) / 4);
-  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(256), 0, st, d, nmodels, n_off, nbase, probe);
+  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(256), 0, st, d, tm, n_off, nbase, probe);
   return hipGetLastError();
 }
 
 template <int NM, bool SC>
-static hipError_t launch_step_th(int th, const GridDesc *d, int32_t nmodels, int64_t tiles, int32_t n_off,
+static hipError_t launch_step_th(int th, const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
                                  const int32_t *nbase, int32_t probe, hipStream_t st) {
   switch (th) {
-    case 1: return launch_step_t<NM, SC, 1>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_t<NM, SC, 2>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_t<NM, SC, 4>(d, nmodels, tiles, n_off, nbase, probe, st);
-    case 8: return launch_step_t<NM, SC, 8>(d, nmodels, tiles, n_off, nbase, probe, st);
-    default: return launch_step_t<NM, SC, 0>(d, nmodels, tiles, n_off, nbase, probe, st);
+    case 1: return launch_step_t<NM, SC, 1>(d, tm, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_t<NM, SC, 2>(d, tm, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_t<NM, SC, 4>(d, tm, tiles, n_off, nbase, probe, st);
+    case 8: return launch_step_t<NM, SC, 8>(d, tm, tiles, n_off, nbase, probe, st);
+    default: return launch_step_t<NM, SC, 0>(d, tm, tiles, n_off, nbase, probe, st);
   }
 }
 
 // `th` is the tile height every model of the launch was planned with; 1, 2,
 // 4 and 8 select the fully unrolled kernels, anything else the row march.
-hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, int32_t nmodels, int64_t tiles,
+// `tile_model` maps each tile to its model (null: every tile is model 0).
+hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, const uint16_t *tile_model, int64_t tiles,
                        int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
   if (tiles <= 0) return hipSuccess;
-  if (nmech == 0) return launch_step_th<0, true>(th, d, nmodels, tiles, n_off, nbase, probe, st);
-  if (scalar) return launch_step_th<1, true>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+  if (nmech == 0) return launch_step_th<0, true>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+  if (scalar) return launch_step_th<1, true>(th, d, tile_model, tiles, n_off, nbase, probe, st);
   switch (nmech) {
-    case 1: return launch_step_th<1, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_th<2, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
-    case 3: return launch_step_th<3, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_th<4, false>(th, d, nmodels, tiles, n_off, nbase, probe, st);
+    case 1: return launch_step_th<1, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_th<2, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 3: return launch_step_th<3, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_th<4, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
     default: return hipErrorInvalidValue;
   }
 }

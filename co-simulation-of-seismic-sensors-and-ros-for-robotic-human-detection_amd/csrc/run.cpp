@@ -49,6 +49,8 @@ struct Group {
   std::vector<int> order;        // shot indices, longest run first
   std::vector<int64_t> prefix;   // tiles of the first k models
   gfk::GridDesc *d_descs = nullptr;
+  uint16_t *d_tile_model = nullptr;  // model of each tile (batches of 2+)
+  int64_t tile_model_cap = 0;
   int32_t *d_nbase = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -60,6 +62,8 @@ void free_group(Group &G) {
   if (G.graph) hipGraphDestroy(G.graph);
   if (G.d_descs) hipFree(G.d_descs);
   if (G.d_nbase) hipFree(G.d_nbase);
+  if (G.d_tile_model) hipFree(G.d_tile_model);
+  G.d_tile_model = nullptr;
   G.exec = nullptr;
   G.graph = nullptr;
   G.d_descs = nullptr;
@@ -93,6 +97,23 @@ int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th) {
     HIPCHK(hipMalloc(&G.d_nbase, 64));
   }
   HIPCHK(hipMemcpy(G.d_descs, descs.data(), sizeof(gfk::GridDesc) * descs.size(), hipMemcpyHostToDevice));
+  if (descs.size() > 1) {
+    if (descs.size() > 65535) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: too many models in one batch");
+    // Padded to whole workgroups of 4 tiles: the launch's last workgroup
+    // can hold up to three tiles past the end, which read the last model and
+    // exit on its tile count.
+    const int64_t tiles = (G.prefix.back() + 3) / 4 * 4;
+    std::vector<uint16_t> tm((size_t)tiles, (uint16_t)(descs.size() - 1));
+    for (size_t m = 0; m < descs.size(); m++)
+      std::fill(tm.begin() + G.prefix[m], tm.begin() + G.prefix[m + 1], (uint16_t)m);
+    if (tiles > G.tile_model_cap) {
+      if (G.d_tile_model) (void)hipFree(G.d_tile_model);
+      G.d_tile_model = nullptr;
+      HIPCHK(hipMalloc(&G.d_tile_model, sizeof(uint16_t) * (size_t)tiles));
+      G.tile_model_cap = tiles;
+    }
+    HIPCHK(hipMemcpy(G.d_tile_model, tm.data(), sizeof(uint16_t) * tm.size(), hipMemcpyHostToDevice));
+  }
   if (G.exec) hipGraphExecDestroy(G.exec);
   if (G.graph) hipGraphDestroy(G.graph);
   G.exec = nullptr;
@@ -108,7 +129,7 @@ int capture(Group &G, int models, hipStream_t st) {
   G.graph = nullptr;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < gfs::kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, models, G.prefix[models], k, G.d_nbase, 1, st);
+    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[models], k, G.d_nbase, 1, st);
     if (e != hipSuccess) {
       hipGraph_t tmp = nullptr;
       hipStreamEndCapture(st, &tmp);
@@ -240,8 +261,8 @@ int Batch::run() {
       gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(), (int64_t)log2((double)cells)};
       auto prepare = [&](int th) { return plan_group(G, sims, th) == GF_OK; };
       auto launch = [&](int th, int k) {
-        return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, (int32_t)G.order.size(), G.prefix.back(), k,
-                                nullptr, 0, stream);
+        return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, G.d_tile_model, G.prefix.back(), k, nullptr, 0,
+                                stream);
       };
       int best = gft::cached(key);
       if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
@@ -287,7 +308,8 @@ int Batch::run() {
         }
       } else {
         hipError_t e =
-            gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, act, G.prefix[act], (int32_t)k, nullptr, 1, stream);
+            gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[act], (int32_t)k, nullptr, 1,
+                             stream);
         if (e != hipSuccess) {
           rc = gfs::hip_fail(e, "step launch");
           break;
