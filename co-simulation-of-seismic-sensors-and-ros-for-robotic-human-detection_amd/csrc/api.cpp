@@ -4,7 +4,7 @@
 // C-PML strips, the memory variables, the coefficient tables and the receiver
 // buffers of one model, plus a HIP stream at default priority.  gf_run and
 // gf_run_batch are built on the same pieces.
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>

@@ -20,7 +20,7 @@
 // encoding/json would write it (field order, omitempty, number and string
 // formatting), zero padding to 4096, then little-endian float32 (re, im)
 // pairs, range-major.
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>

@@ -103,8 +103,8 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
   const int rb = n & 1, wb = rb ^ 1;
   const ColConst cc = load_col(g, c);
-  const float *const *A = g.f[rb];
-  float *const *B = g.f[wb];
+  GF_GLOBAL float *const *A = g.f[rb];
+  GF_GLOBAL float *const *B = g.f[wb];
 
   // Carried state, named by row relative to the velocity row t being built.
   float trr_p = 0.f, tqq_p = 0.f;          // old trr, tqq at t-1
@@ -124,8 +124,8 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
              cur.tzz_dn, cur.vr, cur.vz, vrn, vzn);
     if (row_own && own) {
       const size_t k = (size_t)t * g.pitch + c;
-      B[kVr][k] = vrn;
-      B[kVz][k] = vzn;
+      st_stream(&B[kVr][k], vrn);
+      st_stream(&B[kVz][k], vzn);
       if (do_max) mbits = max(mbits, max(abs_bits(vrn), abs_bits(vzn)));
     }
     const float vrn_m = lane_prev(vrn);
@@ -148,10 +148,10 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
         stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
         const size_t k = (size_t)s * g.pitch + c;
-        B[kTrr][k] = nrr;
-        B[kTqq][k] = nqq;
-        B[kTzz][k] = nzz;
-        B[kTrz][k] = nrz;
+        st_stream(&B[kTrr][k], nrr);
+        st_stream(&B[kTqq][k], nqq);
+        st_stream(&B[kTzz][k], nzz);
+        st_stream(&B[kTrz][k], nrz);
         store_mem<NM>(g, s, c, mem_s);
       }
     }
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
 __global__ void k_surface_velocity(const GridDesc *__restrict__ gd, int32_t buf, int32_t col, float *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const GridDesc &g = *gd;
-  const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
+  GF_GLOBAL const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
   const ColConst cc = load_col(g, col);
   const float vrm = col > 0 ? vr[col - 1] : 0.f;
   out[0] = surface_velocity(g, col, cc, vr[col], vrm, vz[col]);
@@ -220,7 +220,7 @@ __global__ void k_surface_velocity(const GridDesc *__restrict__ gd, int32_t buf,
 // Sim.MaxSpeed over both velocity fields of one buffer.
 __global__ void k_max_speed(const GridDesc *__restrict__ gd, int32_t buf, uint32_t *slot) {
   const GridDesc &g = *gd;
-  const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
+  GF_GLOBAL const float *vr = g.f[buf][kVr], *vz = g.f[buf][kVz];
   const int64_t cells = (int64_t)g.nz * g.pitch;
   uint32_t m = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x)

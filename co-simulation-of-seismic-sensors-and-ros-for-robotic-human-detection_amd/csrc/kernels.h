@@ -2,7 +2,7 @@
 // of the parameter blocks the coefficient builder takes.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #include "coef.h"

@@ -6,7 +6,7 @@
 // first, so the models still running are always a prefix of the tile list
 // and the launch shrinks as short runs retire.  Blocks of kGraphSteps steps
 // are replayed from captured HIP graphs.
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdio.h>
 

@@ -1,7 +1,7 @@
 // sim.h — the object behind a gf_sim handle (one model on one GPU).
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>

@@ -32,6 +32,17 @@
 #define GF_HD inline
 #endif
 
+// Device code sees every array the descriptor points to as global memory, so
+// the compiler emits global_load/global_store rather than flat accesses (a
+// flat access also counts against the scalar-load counter, which serialises
+// the kernel's vector loads behind every scalar wait).  Host code sees plain
+// pointers; the layout is the same.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF_GLOBAL __attribute__((address_space(1)))
+#else
+#define GF_GLOBAL
+#endif
+
 namespace gfk {
 
 constexpr int kLanes = 64;   // CDNA wavefront
@@ -83,22 +94,34 @@ struct GridDesc {
   int64_t trace_cap, rec_base;  // traces[r * trace_cap + (n - rec_base)]
   int64_t field_stride;         // floats per padded field allocation
   int64_t mem_stride;           // floats per memory-variable plane (nz*pitch)
-  float *f[2][6];               // [buffer][vr, vz, trr, tqq, tzz, trz] at (0,0)
-  float *vpsi_r[2][2];          // [buffer][psiTrrR (r half), psiTrzR (r int)]
-  float *vpsi_z[2][2];          // [buffer][psiTrzZ (z int), psiTzzZ (z half)]
-  float *spsi_r[2];             // psiVrR (r int), psiVzR (r half); in place
-  float *spsi_z[2];             // psiVzZ (z int), psiVrZ (z half); in place
-  float *mem;                   // [4 components][nmech][nz][pitch]
-  const float *rowc;            // [kRowN + 2 nmech][nz]
-  const float *colc;            // [kColN][pitch]
-  const float *src_tzz;         // -F[n] / A          [nsrc]
-  const float *src_dtdt;        // (F[n+1]-F[n]) / dt  [nsrc]
-  const int32_t *rec_col;       // [nrec]
-  float *traces;                // [nrec][trace_cap]
-  uint32_t *maxbits;            // running-max slots, one per 256 steps
+  GF_GLOBAL float *f[2][6];        // [buffer][vr, vz, trr, tqq, tzz, trz] at (0,0)
+  GF_GLOBAL float *vpsi_r[2][2];   // [buffer][psiTrrR (r half), psiTrzR (r int)]
+  GF_GLOBAL float *vpsi_z[2][2];   // [buffer][psiTrzZ (z int), psiTzzZ (z half)]
+  GF_GLOBAL float *spsi_r[2];      // psiVrR (r int), psiVzR (r half); in place
+  GF_GLOBAL float *spsi_z[2];      // psiVzZ (z int), psiVrZ (z half); in place
+  GF_GLOBAL float *mem;            // [4 components][nmech][nz][pitch]
+  GF_GLOBAL const float *rowc;     // [kRowN + 2 nmech][nz]
+  GF_GLOBAL const float *colc;     // [kColN][pitch]
+  GF_GLOBAL const float *src_tzz;  // -F[n] / A          [nsrc]
+  GF_GLOBAL const float *src_dtdt; // (F[n+1]-F[n]) / dt  [nsrc]
+  GF_GLOBAL const int32_t *rec_col;  // [nrec]
+  GF_GLOBAL float *traces;         // [nrec][trace_cap]
+  GF_GLOBAL uint32_t *maxbits;     // running-max slots, one per 256 steps
 };
 
 enum Field { kVr = 0, kVz, kTrr, kTqq, kTzz, kTrz };
+
+// Stores of state that is not read again within the step.  With
+// GF_NT_STORES the device build marks them non-temporal, so they do not
+// displace the halo rows neighbouring tiles are about to re-read from L2.
+template <typename T>
+GF_HD void st_stream(GF_GLOBAL T *p, T v) {
+#if defined(__HIP_DEVICE_COMPILE__) && defined(GF_NT_STORES)
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
 
 // Column constants a lane keeps in registers for the whole march.
 struct ColConst {
@@ -108,7 +131,7 @@ struct ColConst {
 GF_HD ColConst load_col(const GridDesc &g, int c) {
   ColConst cc = {0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
   if (c >= 0 && c < g.pitch) {
-    const float *t = g.colc;
+    GF_GLOBAL const float *t = g.colc;
     size_t p = (size_t)g.pitch;
     cc.iri = t[kColIri * p + c];
     cc.irh = t[kColIrh * p + c];
@@ -122,13 +145,13 @@ GF_HD ColConst load_col(const GridDesc &g, int c) {
 
 // Field element at (row t, column c); rows -2..nz+1 and columns -1..pitch-1
 // are inside the padded allocation and read as zero outside the grid.
-GF_HD float ld(const float *base, const GridDesc &g, int t, int c) {
+GF_HD float ld(GF_GLOBAL const float *base, const GridDesc &g, int t, int c) {
   return base[(ptrdiff_t)t * g.pitch + c];
 }
 
 // tzz with the prescribed surface traction of step n at the axis cell: the
 // reference writes tzz[0] = -F[n]/A before the velocity update (fdtd.go:152).
-GF_HD float ld_tzz(const float *base, const GridDesc &g, int n, int t, int c) {
+GF_HD float ld_tzz(GF_GLOBAL const float *base, const GridDesc &g, int n, int t, int c) {
   if (t == 0 && c == 0) return n < g.nsrc ? g.src_tzz[n] : 0.f;
   return ld(base, g, t, c);
 }
@@ -139,7 +162,7 @@ struct RowLoad {
   float trr, tqq, trz, tzz_dn, vr, vz;
 };
 
-GF_HD RowLoad load_row(const float *const *A, const GridDesc &g, int n, int t, int c) {
+GF_HD RowLoad load_row(GF_GLOBAL float *const *A, const GridDesc &g, int n, int t, int c) {
   RowLoad L;
   L.trr = ld(A[kTrr], g, t, c);
   L.tqq = ld(A[kTqq], g, t, c);
@@ -236,7 +259,7 @@ GF_HD void store_mem(const GridDesc &g, int s, int c, const MemVars<NM> &m) {
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int q = 0; q < NM; q++) g.mem[(size_t)(k * NM + q) * ms + cell] = m.r[k][q];
+    for (int q = 0; q < NM; q++) st_stream(&g.mem[(size_t)(k * NM + q) * ms + cell], m.r[k][q]);
 }
 
 // New stresses at (s, c): Sim.stresses (fdtd.go:219-307) for one cell.
@@ -251,7 +274,7 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
                   MemVars<NM> &mem) {
   const float invh = g.invh, dt = g.dt;
   const size_t nz = (size_t)g.nz;
-  const float *rc = g.rowc;
+  GF_GLOBAL const float *rc = g.rowc;
   const float lam = rc[kRowLam * nz + s], l2m = rc[kRowL2m * nz + s];
 
   float dvr, vrr;
@@ -263,7 +286,7 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     vrr = 0.5f * (vr + vrm) * cc.iri;
   }
   if (c >= g.first_rint) {
-    float *p = g.spsi_r[0] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
+    GF_GLOBAL float *p = g.spsi_r[0] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
     float psi = cc.bri * *p + cc.ari * dvr;
     *p = psi;
     dvr += psi;
@@ -276,7 +299,7 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
   } else {
     dvz = (vz - vz_up) * invh;
     if (s >= g.first_zint) {
-      float *p = g.spsi_z[0] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
+      GF_GLOBAL float *p = g.spsi_z[0] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
       float psi = rc[kRowBzi * nz + s] * *p + rc[kRowAzi * nz + s] * dvz;
       *p = psi;
       dvz += psi;
@@ -329,14 +352,14 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
   if (s <= g.nz - 2 && c <= g.nr - 2) {
     float dvzdr = (vz1 - vz) * invh;
     if (c >= g.first_rhalf) {
-      float *p = g.spsi_r[1] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
+      GF_GLOBAL float *p = g.spsi_r[1] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
       float psi = cc.brh * *p + cc.arh * dvzdr;
       *p = psi;
       dvzdr += psi;
     }
     float dvrdz = (vr_dn - vr) * invh;
     if (s >= g.first_zhalf) {
-      float *p = g.spsi_z[1] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
+      GF_GLOBAL float *p = g.spsi_z[1] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
       float psi = rc[kRowBzh * nz + s] * *p + rc[kRowAzh * nz + s] * dvrdz;
       *p = psi;
       dvrdz += psi;

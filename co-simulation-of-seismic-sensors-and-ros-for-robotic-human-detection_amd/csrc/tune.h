@@ -8,7 +8,7 @@
 // Results are cached in-process by (variant, models, grid-size bucket).
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #include <functional>

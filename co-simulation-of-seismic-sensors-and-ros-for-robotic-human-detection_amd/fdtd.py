@@ -26,7 +26,8 @@ from typing import Callable, List, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libgeofdtd.so")
+# GEOFDTD_LIBRARY selects another build of the same library (A/B timing).
+LIB_PATH = os.environ.get("GEOFDTD_LIBRARY") or os.path.join(_HERE, "lib", "libgeofdtd.so")
 
 GF_OK, GF_ERR_INVALID, GF_ERR_DIVERGED, GF_ERR_DEVICE, GF_ERR_ARGUMENT = 0, 1, 2, 3, 4
 
