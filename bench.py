@@ -84,6 +84,17 @@ def traffic_from_profiles(workload_name: str):
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
+def traffic_ratio_from_profiles(workload_name: str):
+    """Measured HBM bytes over algorithmic bytes for a workload whose launches
+    differ in size (config 3: the set of running models shrinks), if committed."""
+    path = os.path.join(ROOT, "profiles", f"k_step_{workload_name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d["hbm_bytes_per_launch"] / d["algorithmic_bytes_per_launch"]
+
+
 def cpu_baseline(w, gpu_traces, sample_steps: int, threads: int) -> tuple:
     """Time the FP64 oracle on `threads` concurrent copies of the shot for
     `sample_steps` time steps; scale to whole shots.  Also compares the GPU's
@@ -152,6 +163,8 @@ def bench_c3(args, fd) -> int:
         raise SystemExit("non-finite spectra")
     traces = len(ws) * spec.Count
     achieved = st["algorithmic_bytes"] / (st["kernel_ms"] / 1e3) / 1e9
+    per_launch = st["algorithmic_bytes"] / max(st["step_launches"], 1)
+    ratio = traffic_ratio_from_profiles("c3")
     out = {
         "metric": METRIC,
         "value": traces / wall,
@@ -173,11 +186,12 @@ def bench_c3(args, fd) -> int:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None if ratio is None else ratio * per_launch,
             "kernel": "k_step over every model still running (one launch per step)",
             "tile_rows": st["tile_rows"], "algorithmic_bytes": st["algorithmic_bytes"],
             "step_loop_ms": st["kernel_ms"], "step_launches": st["step_launches"],
             "avg_launch_us": st["kernel_ms"] * 1e3 / max(st["step_launches"], 1),
+            "algorithmic_bytes_per_launch": per_launch,
         },
         "cells_per_s": cell_steps / wall,
     }
@@ -203,6 +217,57 @@ def bench_c3(args, fd) -> int:
     return 0
 
 
+def ceiling(args) -> int:
+    """Bandwidth ceilings: device copies over sizes below and above the 256 MB
+    last-level cache, and the step kernel's rate as its state outgrows it."""
+    import torch
+
+    for mib in (64, 128, 256, 512, 1024, 4096):
+        n = mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record()
+        for _ in range(reps):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        print(json.dumps({"probe": "copy", "mib": mib, "us": us, "gbs": 2 * 4 * n / us / 1e3}), flush=True)
+        del x, y
+        torch.cuda.empty_cache()
+    pkg = load_package()
+    fd = pkg.fdtd
+    import cosim_amd.workloads as W
+
+    steps = 1000
+    for fr, fz in ((0.5, 0.5), (1, 1), (2, 1), (2, 2), (4, 2)):
+        w = W.c2("X", steps)
+        w.model.MaxRange *= fr
+        w.model.Depth *= fz
+        dt, nr, nz = w.model.Grid()
+        sim = fd.New(w.model)
+        sim.Drive(w.force[:steps])
+        sim.SetReceivers([sim.Column(1.0)[0]], steps)
+        rows = {}
+        for th in (1, 2, 4, 8):
+            sim.SetTileRows(th)
+            sim.Reset()
+            sim.Advance(200, True)
+            sim.SyncTiming()
+            sim.Reset()
+            sim.Advance(steps, True)
+            ms, n = sim.SyncTiming()
+            rows[th] = sim.StepBytes() / (ms / 1e3 / n) / 1e9
+        print(json.dumps({"probe": "k_step", "grid": [nr, nz], "state_mb": sim.StepBytes() / 2e6,
+                          "gbs_by_tile_rows": rows}), flush=True)
+        sim.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,7 +289,14 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the HBM counter "
                          "calibration for rocprofv3 --pmc runs) and exit")
+    ap.add_argument("--ceiling", action="store_true",
+                    help="instead of the benchmark, measure what bounds the step kernel: device copies of 64 MiB "
+                         "to 4 GiB, and k_step on the config-2 (GSLS) medium at five grid extents (81 MB to 2.4 GB "
+                         "of state per step), one JSON line each")
     args = ap.parse_args()
+
+    if args.ceiling:
+        return ceiling(args)
 
     if args.calibrate_copy:
         import torch
