@@ -415,6 +415,12 @@ int gf_ricker_spectrum(double peak, double f0, double f, double *re, double *im)
   return ok();
 }
 
+int gf_stack_validate(const gf_layer *stack, int32_t n) {
+  std::string err;
+  int rc = gfm::validate_stack(stack, n, err);
+  return rc ? set_err(rc, err) : ok();
+}
+
 int gf_model_grid(const gf_model *m, double *dt, int32_t *nr, int32_t *nz) {
   if (!m) return set_err(GF_ERR_ARGUMENT, "geofdtd: null model");
   gfm::Resolved R;

@@ -1,6 +1,8 @@
 // model.cpp — see model.h.  Host-only FP64; compiled with -ffp-contract=off.
 #include "model.h"
 
+#include "gofmt.h"
+
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -198,35 +200,6 @@ double layer_qp(const gf_layer &l) { return l.qp > 0 ? l.qp : l.qs; }
 
 double ref_freq(const gf_model &m) { return m.ref_freq_hz > 0 ? m.ref_freq_hz : 30.0; }
 
-int validate_stack(const gf_layer *st, int n, std::string &err) {
-  char buf[512];
-  if (n <= 0 || st == nullptr) {
-    err = "layer: a stack needs at least a half-space";
-    return GF_ERR_INVALID;
-  }
-  for (int i = 0; i < n; i++) {
-    const gf_layer &l = st[i];
-    if (l.vs_mps <= 0) {
-      snprintf(buf, sizeof buf, "layer %d: Vs must be positive, got %s m/s", i, go_g(l.vs_mps).c_str());
-    } else if (l.vp_mps <= 0) {
-      snprintf(buf, sizeof buf, "layer %d: Vp must be positive, got %s m/s", i, go_g(l.vp_mps).c_str());
-    } else if (l.density_kgm3 <= 0) {
-      snprintf(buf, sizeof buf, "layer %d: density must be positive, got %s kg/m^3", i,
-               go_g(l.density_kgm3).c_str());
-    } else if (l.vp_mps <= kSqrt2 * l.vs_mps) {
-      snprintf(buf, sizeof buf, "layer %d: Vp=%s must exceed sqrt(2)*Vs=%s for a non-negative Poisson ratio", i,
-               go_g(l.vp_mps).c_str(), go_g(kSqrt2 * l.vs_mps).c_str());
-    } else if (i < n - 1 && l.thickness_m <= 0) {
-      snprintf(buf, sizeof buf, "layer %d: thickness must be positive, got %s m", i, go_g(l.thickness_m).c_str());
-    } else {
-      continue;
-    }
-    err = buf;
-    return GF_ERR_INVALID;
-  }
-  return GF_OK;
-}
-
 int validate_ext(const gf_model &m, std::string &err) {
   char buf[256];
   if (m.gsls_mechanisms < 0 || m.gsls_mechanisms > 4) {
@@ -312,23 +285,37 @@ int profile_first(int n, double h, double offset, double inner, double thick, do
 
 }  // namespace
 
-// Go's %g (strconv.FormatFloat(x, 'g', -1, 64)): the shortest digits that
-// read back exactly, in exponent form only when the decimal exponent is below
-// -4 or at least 6 (C's %g switches at the digit count instead).
-std::string go_g(double x) {
-  char buf[64];
-  if (x == 0) return signbit(x) ? "-0" : "0";
-  if (!isfinite(x)) return isnan(x) ? "NaN" : (x > 0 ? "+Inf" : "-Inf");
-  int p = 1;
-  for (; p < 17; p++) {
-    snprintf(buf, sizeof buf, "%.*e", p - 1, x);
-    if (strtod(buf, nullptr) == x) break;
+// layer.Stack.Validate (internal/layer/layer.go:53-73), with its messages.
+int validate_stack(const gf_layer *st, int n, std::string &err) {
+  char buf[512];
+  if (n <= 0 || st == nullptr) {
+    err = "layer: a stack needs at least a half-space";
+    return GF_ERR_INVALID;
   }
-  snprintf(buf, sizeof buf, "%.*e", p - 1, x);
-  const int e = atoi(strchr(buf, 'e') + 1);
-  if (e >= -4 && e < 6) snprintf(buf, sizeof buf, "%.*f", std::max(p - 1 - e, 0), x);
-  return buf;
+  for (int i = 0; i < n; i++) {
+    const gf_layer &l = st[i];
+    if (l.vs_mps <= 0) {
+      snprintf(buf, sizeof buf, "layer %d: Vs must be positive, got %s m/s", i, go_g(l.vs_mps).c_str());
+    } else if (l.vp_mps <= 0) {
+      snprintf(buf, sizeof buf, "layer %d: Vp must be positive, got %s m/s", i, go_g(l.vp_mps).c_str());
+    } else if (l.density_kgm3 <= 0) {
+      snprintf(buf, sizeof buf, "layer %d: density must be positive, got %s kg/m^3", i,
+               go_g(l.density_kgm3).c_str());
+    } else if (l.vp_mps <= kSqrt2 * l.vs_mps) {
+      snprintf(buf, sizeof buf, "layer %d: Vp=%s must exceed sqrt(2)*Vs=%s for a non-negative Poisson ratio", i,
+               go_g(l.vp_mps).c_str(), go_g(kSqrt2 * l.vs_mps).c_str());
+    } else if (i < n - 1 && l.thickness_m <= 0) {
+      snprintf(buf, sizeof buf, "layer %d: thickness must be positive, got %s m", i, go_g(l.thickness_m).c_str());
+    } else {
+      continue;
+    }
+    err = buf;
+    return GF_ERR_INVALID;
+  }
+  return GF_OK;
 }
+
+std::string go_g(double x) { return gofmt::g(x); }
 
 int visco_fit(double q, double f0, gf_sls &out, std::string &err) {
   if (q <= 0) {

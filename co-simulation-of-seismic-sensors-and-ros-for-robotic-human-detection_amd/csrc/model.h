@@ -37,6 +37,7 @@ struct Resolved {
 // Status and message in the reference's words; returns GF_OK or an error code.
 int resolve(const gf_model &m, Resolved &out, std::string &err);
 
+int validate_stack(const gf_layer *st, int n, std::string &err);
 int visco_fit(double q, double f0, gf_sls &out, std::string &err);
 int spacing_for(const gf_model &m, double max_freq, double cpw, double &out, std::string &err);
 void ricker(double peak, double f0, double dt, int32_t n, double *out);

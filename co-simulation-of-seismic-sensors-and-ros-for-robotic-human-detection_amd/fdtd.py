@@ -163,6 +163,7 @@ SIGNATURES = {
     "gf_ricker": (_I, [_D, _D, _D, _I32, _PD]),
     "gf_ricker_spectrum": (_I, [_D, _D, _D, _PD, _PD]),
     "gf_model_grid": (_I, [C.POINTER(GfModel), _PD, _PI32, _PI32]),
+    "gf_stack_validate": (_I, [C.POINTER(GfLayer), _I32]),
     "gf_sim_reset": (_I, [C.c_void_p]),
     "gf_sim_create": (_I, [C.POINTER(GfModel), C.POINTER(C.c_void_p)]),
     "gf_sim_destroy": (None, [C.c_void_p]),
@@ -236,6 +237,16 @@ class Layer:
     Density: float = 0.0
     Qs: float = 0.0
     Qp: float = 0.0
+
+
+def ValidateStack(stack: Sequence[Layer]) -> None:
+    """layer.Stack.Validate (internal/layer/layer.go:53-73): raises FdtdError
+    with the reference's message for a physically impossible stack."""
+    lib = load_library()
+    layers = (GfLayer * max(1, len(stack)))()
+    for i, l in enumerate(stack):
+        layers[i] = GfLayer(l.Thickness, l.Vp, l.Vs, l.Density, l.Qs, l.Qp)
+    _check(lib.gf_stack_validate(layers, len(stack)))
 
 
 @dataclass

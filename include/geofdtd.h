@@ -132,6 +132,12 @@ int gf_ricker_spectrum(double peak_n, double f0_hz, double f_hz, double *re,
  * nr, nz, dt) without allocating anything or touching a device. */
 int gf_model_grid(const gf_model *m, double *dt, int32_t *nr, int32_t *nz);
 
+/* layer.Stack.Validate (internal/layer/layer.go:53-73): GF_ERR_INVALID with
+ * the reference's message ("layer 0: Vs must be positive, got -1 m/s", ...)
+ * for a physically impossible stack.  What layer.LoadGolden checks before a
+ * model exists (cmd/geofdtd reads its medium that way). */
+int gf_stack_validate(const gf_layer *stack, int32_t n_layers);
+
 /* ---- *fdtd.Sim ------------------------------------------------------- */
 typedef struct gf_sim gf_sim;
 
