@@ -40,8 +40,9 @@ void philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
 struct Draws {
   uint32_t key[2];
   uint32_t index;
+  uint32_t stream = 0;  // separates the soil families drawn from one seed
   void block(uint32_t b, double u[4]) const {
-    const uint32_t ctr[4] = {index, b, 0, 0};
+    const uint32_t ctr[4] = {index, b, stream, 0};
     uint32_t x[4];
     philox(ctr, key, x);
     for (int k = 0; k < 4; k++) u[k] = ((double)x[k] + 0.5) / 4294967296.0;
@@ -83,5 +84,39 @@ extern "C" int gf_soil_sample(uint64_t seed, uint32_t index, gf_soil *out) {
   out->pml_cells = pml;
   out->max_range_m = (double)(nr - 1 - pml) * h;
   out->depth_m = (double)(nz - 1 - pml) * h;
+  return GF_OK;
+}
+
+extern "C" int gf_soil_sample_layered(uint64_t seed, uint32_t index, gf_soil *out, double params[10]) {
+  if (!out || !params) return GF_ERR_ARGUMENT;
+  memset(out, 0, sizeof *out);
+  Draws d{{(uint32_t)seed, (uint32_t)(seed >> 32)}, index, 1};
+  double u[4], w[4], q[4];
+  d.block(0, u);
+  d.block(1, w);
+  d.block(2, q);
+  const double vs[3] = {120 + 80 * u[0], 200 + 120 * u[1], 320 + 160 * u[2]};
+  const double qs[3] = {10 + 30 * w[0], 20 + 40 * w[1], 40 + 60 * w[2]};
+  const double th[2] = {0.5 + 2.5 * q[0], 2.0 + 4.0 * q[1]};
+  const double ratio = 1.7 + 0.8 * q[2], rho0 = 1600 + 200 * q[3];
+  out->n_layers = 3;
+  for (int l = 0; l < 3; l++) {
+    gf_layer &L = out->layers[l];
+    L.thickness_m = l < 2 ? th[l] : 0.0;
+    L.vs_mps = vs[l];
+    L.vp_mps = ratio * vs[l];
+    L.density_kgm3 = rho0 + 150.0 * l;
+    L.qs = qs[l];
+    L.qp = 2 * qs[l];
+  }
+  const double h = 0.9 * vs[0] / (100.0 * 10.0);
+  const int pml = 20;
+  const int nr = (int)lround(52.0 / h) + 1 + pml, nz = (int)lround(30.0 / h) + 1 + pml;
+  out->spacing_m = h;
+  out->pml_cells = pml;
+  out->max_range_m = (double)(nr - 1 - pml) * h;
+  out->depth_m = (double)(nz - 1 - pml) * h;
+  const double p[10] = {vs[0], vs[1], vs[2], qs[0], qs[1], qs[2], th[0], th[1], ratio, rho0};
+  memcpy(params, p, sizeof p);
   return GF_OK;
 }

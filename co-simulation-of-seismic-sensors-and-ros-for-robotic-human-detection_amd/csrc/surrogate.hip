@@ -1,0 +1,391 @@
+// surrogate.hip — the Green's-function surrogate's kernels for gfx950.
+//
+// The network is a chain of GEMMs (an MLP encoder and a transposed-convolution
+// decoder whose kernel equals its stride, which is a GEMM over (query,
+// position) rows).  Every GEMM, forward and backward, is one kernel:
+//
+//   C[M][N] = sum_k A(m,k) B(k,n),   A, B BF16,  accumulation FP32 on MFMA
+//
+// with A stored either [M][K] or [K][M] and B either [N][K] or [K][N], so the
+// forward product X W^T, the input gradient dZ W and the weight gradient
+// dZ^T X all run on the same tiles.  Tiles are staged in LDS k-contiguous (the
+// MFMA operand layout), so a lane fetches its 8 BF16 of one row with one
+// 16-byte LDS read.  The 32x32x16 BF16 MFMA gives lane l of a 32x32 output
+// tile column l%32 and rows 8(i/4) + 4(l/32) + i%4 for its 16 accumulators.
+//
+// Epilogues are fused: bias + GELU (keeping the FP32 pre-activation for the
+// backward pass), bias only (the output layer), GELU' times the incoming
+// gradient (the backward pass of the layer below), or plain FP32 (weight
+// gradients, one slice per split of K).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "surrogate_kernels.h"
+
+namespace gfsk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBK = 32;        // K per LDS stage (two 32x32x16 MFMA steps)
+constexpr int kLDK = kBK + 8;  // padded LDS row, in BF16 elements (80 bytes)
+
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+  // round to nearest even; NaN stays NaN
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form: 0.5 z (1 + tanh(c (z + a z^3))), c = sqrt(2/pi), a = 0.044715.
+constexpr float kGc = 0.7978845608028654f, kGa = 0.044715f;
+
+__device__ __forceinline__ float tanh_fast(float u) {
+  // 1 - 2 / (1 + e^{2u}): exact limits +-1 when e^{2u} overflows or vanishes
+  return 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+}
+
+__device__ __forceinline__ float gelu(float z) {
+  const float t = tanh_fast(kGc * (z + kGa * z * z * z));
+  return 0.5f * z * (1.0f + t);
+}
+
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float t = tanh_fast(kGc * (z + kGa * z * z * z));
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGc * (1.0f + 3.0f * kGa * z * z);
+}
+
+// One ROWS x kBK tile into LDS as dst[row][k].  TRANS = false: element (r, k)
+// at src[r*ld + k] (k contiguous; 16-byte loads copied as they are).  TRANS =
+// true: element (r, k) at src[k*ld + r] (r contiguous; 16-byte loads along r,
+// scattered into 8 LDS rows).  Rows at or beyond R and k at or beyond kend read
+// as zero; the host guarantees that a 16-byte vector is wholly in or out
+// (K, and R for TRANS, multiples of 8, and 16-byte aligned rows).
+template <int ROWS, bool TRANS>
+__device__ __forceinline__ void load_tile(uint16_t *dst, const uint16_t *__restrict__ src, int64_t ld, int r0,
+                                          int k0, int R, int kend) {
+  constexpr int kVecs = ROWS * kBK / 8;
+  static_assert(kVecs % 256 == 0 || kVecs < 256, "tile size");
+  for (int v = threadIdx.x; v < kVecs; v += 256) {
+    uint4 val = make_uint4(0, 0, 0, 0);
+    if (!TRANS) {
+      const int row = v / (kBK / 8), kv = v % (kBK / 8);
+      const int gr = r0 + row, gk = k0 + kv * 8;
+      if (gr < R && gk < kend) val = *(const uint4 *)(src + (int64_t)gr * ld + gk);
+      *(uint4 *)(dst + row * kLDK + kv * 8) = val;
+    } else {
+      const int k = v / (ROWS / 8), rv = v % (ROWS / 8);
+      const int gk = k0 + k, gr = r0 + rv * 8;
+      if (gk < kend && gr < R) val = *(const uint4 *)(src + (int64_t)gk * ld + gr);
+      const uint16_t *e = (const uint16_t *)&val;
+#pragma unroll
+      for (int j = 0; j < 8; j++) dst[(rv * 8 + j) * kLDK + k] = e[j];
+    }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, bool AT, bool BT, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  constexpr int TM = BM / WM / 32, TN = BN / WN / 32;
+  static_assert(WM * WN == 4 && TM >= 1 && TN >= 1, "4 wavefronts, 32x32 MFMA tiles");
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * kLDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * kLDK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kbeg = blockIdx.z * g.k_split;
+  const int kend = min(g.K, kbeg + g.k_split);
+
+  f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; a++)
+#pragma unroll
+    for (int b = 0; b < TN; b++)
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+
+  for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+    load_tile<BM, AT>(As, g.A, g.lda, m0, k0, g.M, kend);
+    load_tile<BN, !BT>(Bs, g.B, g.ldb, n0, k0, g.N, kend);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kBK / 16; s++) {
+      bf16x8 fa[TM], fb[TN];
+      const int kk = s * 16 + (lane >> 5) * 8;
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+        fa[a] = *(const bf16x8 *)(As + (wm * TM * 32 + a * 32 + (lane & 31)) * kLDK + kk);
+#pragma unroll
+      for (int b = 0; b < TN; b++)
+        fb[b] = *(const bf16x8 *)(Bs + (wn * TN * 32 + b * 32 + (lane & 31)) * kLDK + kk);
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int a = 0; a < TM; a++)
+#pragma unroll
+    for (int b = 0; b < TN; b++) {
+      const int col = n0 + wn * TN * 32 + b * 32 + (lane & 31);
+      if (col >= g.N) continue;
+      const float bias = (EPI == kEpiBiasGelu || EPI == kEpiBias) ? g.bias[col % g.bias_mod] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = m0 + wm * TM * 32 + a * 32 + (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
+        if (row >= g.M) continue;
+        const float v = acc[a][b][i];
+        if (EPI == kEpiF32) {
+          g.cf[(int64_t)blockIdx.z * g.split_stride + (int64_t)row * g.ldc + col] = v;
+        } else if (EPI == kEpiBias) {
+          g.cf[(int64_t)row * g.ldc + col] = v + bias;
+        } else if (EPI == kEpiBiasGelu) {
+          const float z = v + bias;
+          if (g.z) g.z[(int64_t)row * g.ldz + col] = z;
+          g.cb[(int64_t)row * g.ldcb + col] = to_bf16(gelu(z));
+        } else {  // kEpiGeluBwd
+          const float d = v * gelu_grad(g.z[(int64_t)row * g.ldz + col]);
+          g.cf[(int64_t)row * g.ldc + col] = d;
+          g.cb[(int64_t)row * g.ldcb + col] = to_bf16(d);
+        }
+      }
+    }
+}
+
+template <bool AT, bool BT, int EPI>
+hipError_t launch_shape(const GemmArgs &g, int splits, hipStream_t st) {
+  const dim3 block(256);
+  if (g.N > 64) {
+    const dim3 grid((g.N + 127) / 128, (g.M + 127) / 128, splits);
+    hipLaunchKernelGGL((k_gemm<128, 128, 2, 2, AT, BT, EPI>), grid, block, 0, st, g);
+  } else if (g.N > 32) {
+    const dim3 grid((g.N + 63) / 64, (g.M + 127) / 128, splits);
+    hipLaunchKernelGGL((k_gemm<128, 64, 4, 1, AT, BT, EPI>), grid, block, 0, st, g);
+  } else {
+    const dim3 grid((g.N + 31) / 32, (g.M + 127) / 128, splits);
+    hipLaunchKernelGGL((k_gemm<128, 32, 4, 1, AT, BT, EPI>), grid, block, 0, st, g);
+  }
+  return hipGetLastError();
+}
+
+template <int EPI>
+hipError_t launch_layouts(const GemmArgs &g, bool at, bool bt, int splits, hipStream_t st) {
+  if (!at && bt) return launch_shape<false, true, EPI>(g, splits, st);
+  if (!at && !bt) return launch_shape<false, false, EPI>(g, splits, st);
+  if (at && !bt) return launch_shape<true, false, EPI>(g, splits, st);
+  return launch_shape<true, true, EPI>(g, splits, st);
+}
+
+hipError_t launch_gemm(const GemmArgs &g, bool at, bool bt, int epi, int splits, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if ((g.M + 127) / 128 > 65535 || splits < 1 || splits > 65535) return hipErrorInvalidConfiguration;
+  switch (epi) {
+    case kEpiF32: return launch_layouts<kEpiF32>(g, at, bt, splits, st);
+    case kEpiBias: return launch_layouts<kEpiBias>(g, at, bt, splits, st);
+    case kEpiBiasGelu: return launch_layouts<kEpiBiasGelu>(g, at, bt, splits, st);
+    case kEpiGeluBwd: return launch_layouts<kEpiGeluBwd>(g, at, bt, splits, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------ elementwise
+
+__global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = to_bf16(x[i]);
+}
+
+// Rows of `in` ([rows][cols_in] FP32) into the first cols_in columns of
+// `out` ([rows][cols_out] BF16), zero in the rest.
+__global__ void k_pad_rows_bf16(const float *__restrict__ in, int32_t cols_in, uint16_t *__restrict__ out,
+                                int32_t cols_out, int64_t rows) {
+  const int64_t n = rows * cols_out;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols_out;
+    const int c = (int)(i % cols_out);
+    out[i] = c < cols_in ? to_bf16(in[r * cols_in + c]) : (uint16_t)0;
+  }
+}
+
+// Rows `idx[b]` of src ([*][cols] FP32) into dst row b.
+__global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__restrict__ idx, int32_t cols,
+                              float *__restrict__ dst, int32_t rows) {
+  const int64_t n = (int64_t)rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols;
+    const int c = (int)(i % cols);
+    dst[i] = src[(int64_t)idx[r] * cols + c];
+  }
+}
+
+// Relative spectral L2 loss, one workgroup per sample:
+//   loss_b = |o_b - t_b|^2 / |t_b|^2,   d o_b = scale * 2 (o_b - t_b) / |t_b|^2
+// (scale = 1/B gives the gradient of the batch mean).  dF and dB receive the
+// gradient in FP32 and BF16.
+__global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, const float *__restrict__ t, int32_t len,
+                                                float scale, float *__restrict__ loss, float *__restrict__ dF,
+                                                uint16_t *__restrict__ dB) {
+  const int b = blockIdx.x;
+  const float *ob = o + (int64_t)b * len, *tb = t + (int64_t)b * len;
+  float num = 0, den = 0;
+  for (int i = threadIdx.x; i < len; i += 256) {
+    const float d = ob[i] - tb[i];
+    num += d * d;
+    den += tb[i] * tb[i];
+  }
+  __shared__ float sn[256], sd[256];
+  sn[threadIdx.x] = num;
+  sd[threadIdx.x] = den;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sn[threadIdx.x] += sn[threadIdx.x + off];
+      sd[threadIdx.x] += sd[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  const float inv = sd[0] > 0 ? 1.0f / sd[0] : 0.0f;
+  if (threadIdx.x == 0) loss[b] = sn[0] * inv;
+  if (dF) {
+    for (int i = threadIdx.x; i < len; i += 256) {
+      const float g = scale * 2.0f * (ob[i] - tb[i]) * inv;
+      dF[(int64_t)b * len + i] = g;
+      dB[(int64_t)b * len + i] = to_bf16(g);
+    }
+  }
+}
+
+// Column sums of x[rows][cols] over row chunks: part[chunk][col].
+__global__ void k_colsum_part(const float *__restrict__ x, int64_t rows, int32_t cols, int64_t chunk,
+                              float *__restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
+  float s = 0;
+  for (int64_t r = r0; r < r1; r++) s += x[r * cols + c];
+  part[(int64_t)blockIdx.y * cols + c] = s;
+}
+
+// out[j] = sum over chunks and over columns c with c % mod == j of part.
+__global__ void k_colsum_fold(const float *__restrict__ part, int32_t chunks, int32_t cols, int32_t mod,
+                              float *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= mod) return;
+  float s = 0;
+  for (int k = 0; k < chunks; k++)
+    for (int c = j; c < cols; c += mod) s += part[(int64_t)k * cols + c];
+  out[j] = s;
+}
+
+// out[i] = sum_s part[s][i]: the K splits of a weight gradient.
+__global__ void k_sum_splits(const float *__restrict__ part, int32_t splits, int64_t n, float *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0;
+    for (int k = 0; k < splits; k++) s += part[(int64_t)k * n + i];
+    out[i] = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master parameters, with the BF16 copy the
+// GEMMs read refreshed in the same pass.  c1 = 1 - beta1^t, c2 = 1 - beta2^t.
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, uint16_t *__restrict__ pb, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    pb[i] = to_bf16(pi);
+  }
+}
+
+// Normalized network output to the physical spectrum, per query b and bin k:
+//   G = s_b (o_re + i o_im) e^{-i 2 pi f_k w_b},  f_k = k * df
+// (s_b the amplitude scale and w_b the delay the targets were divided by).
+__global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const double *__restrict__ scale,
+                              const double *__restrict__ delay, double df, float *__restrict__ out, int32_t rows) {
+  const int64_t n = (int64_t)rows * bins;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / bins;
+    const int k = (int)(i % bins);
+    const double ph = -2.0 * 3.141592653589793 * (double)k * df * delay[b];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double re = o[2 * i], im = o[2 * i + 1];
+    out[2 * i] = (float)(scale[b] * (re * cs - im * sn));
+    out[2 * i + 1] = (float)(scale[b] * (re * sn + im * cs));
+  }
+}
+
+namespace {
+inline dim3 grid_for(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return dim3((unsigned)(b < 4096 ? b : 4096));
+}
+}  // namespace
+
+hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_to_bf16, grid_for(n), dim3(256), 0, st, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad_rows_bf16(const float *in, int32_t cols_in, uint16_t *out, int32_t cols_out, int64_t rows,
+                                hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pad_rows_bf16, grid_for(rows * cols_out), dim3(256), 0, st, in, cols_in, out, cols_out, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const float *src, const int32_t *idx, int32_t cols, float *dst, int32_t rows,
+                              hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rows, grid_for((int64_t)rows * cols), dim3(256), 0, st, src, idx, cols, dst, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_rel_l2(const float *o, const float *t, int32_t rows, int32_t len, float scale, float *loss,
+                         float *dF, uint16_t *dB, hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rel_l2, dim3(rows), dim3(256), 0, st, o, t, len, scale, loss, dF, dB);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, int32_t chunks,
+                            float *out, hipStream_t st) {
+  const int64_t chunk = (rows + chunks - 1) / chunks;
+  hipLaunchKernelGGL(k_colsum_part, dim3((cols + 255) / 256, chunks), dim3(256), 0, st, x, rows, cols, chunk, part);
+  hipLaunchKernelGGL(k_colsum_fold, dim3((mod + 255) / 256), dim3(256), 0, st, part, chunks, cols, mod, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_splits, grid_for(n), dim3(256), 0, st, part, splits, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t st) {
+  hipLaunchKernelGGL(k_adam, grid_for(n), dim3(256), 0, st, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
+                              float *out, int32_t rows, hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_denormalize, grid_for((int64_t)rows * bins), dim3(256), 0, st, o, bins, scale, delay, df, out,
+                     rows);
+  return hipGetLastError();
+}
+
+}  // namespace gfsk

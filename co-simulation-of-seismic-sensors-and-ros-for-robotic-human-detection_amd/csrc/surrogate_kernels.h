@@ -1,0 +1,56 @@
+// surrogate_kernels.h — launchers of the surrogate's kernels (surrogate.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace gfsk {
+
+enum Epilogue {
+  kEpiF32 = 0,       // cf[split][m][n] = acc                       (weight gradients)
+  kEpiBias = 1,      // cf[m][n] = acc + bias[n % bias_mod]         (output layer)
+  kEpiBiasGelu = 2,  // z[m][n] = acc + bias (if z); cb = bf16(gelu(z))   (hidden layers)
+  kEpiGeluBwd = 3,   // d = acc * gelu'(z[m][n]); cf = d; cb = bf16(d)    (gradient into a hidden layer)
+};
+
+// C[M][N] = sum_k A(m,k) B(k,n) over k in this split's range.  A(m,k) is
+// A[m*lda + k] (at = false) or A[k*lda + m] (at = true); B(k,n) is
+// B[n*ldb + k] (bt = true) or B[k*ldb + n] (bt = false).  BF16 operands as
+// raw uint16 bits.  Requirements (checked by the host code): K a multiple of
+// 8; M a multiple of 8 when at; N a multiple of 8 when !bt; lda, ldb multiples
+// of 8 and both operands 16-byte aligned; k_split a multiple of 32.
+struct GemmArgs {
+  int32_t M = 0, N = 0, K = 0;
+  const uint16_t *A = nullptr;
+  int64_t lda = 0;
+  const uint16_t *B = nullptr;
+  int64_t ldb = 0;
+  float *cf = nullptr;
+  int64_t ldc = 0;
+  uint16_t *cb = nullptr;
+  int64_t ldcb = 0;
+  const float *bias = nullptr;
+  int32_t bias_mod = 1;
+  float *z = nullptr;
+  int64_t ldz = 0;
+  int32_t k_split = 1 << 30;
+  int64_t split_stride = 0;
+};
+
+hipError_t launch_gemm(const GemmArgs &g, bool at, bool bt, int epi, int splits, hipStream_t st);
+hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t st);
+hipError_t launch_pad_rows_bf16(const float *in, int32_t cols_in, uint16_t *out, int32_t cols_out, int64_t rows,
+                                hipStream_t st);
+hipError_t launch_gather_rows(const float *src, const int32_t *idx, int32_t cols, float *dst, int32_t rows,
+                              hipStream_t st);
+hipError_t launch_rel_l2(const float *o, const float *t, int32_t rows, int32_t len, float scale, float *loss,
+                         float *dF, uint16_t *dB, hipStream_t st);
+hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, int32_t chunks,
+                            float *out, hipStream_t st);
+hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float *out, hipStream_t st);
+hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t st);
+hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
+                              float *out, int32_t rows, hipStream_t st);
+
+}  // namespace gfsk

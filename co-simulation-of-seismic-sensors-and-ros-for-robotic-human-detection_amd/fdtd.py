@@ -152,6 +152,46 @@ class GfSoil(C.Structure):
     ]
 
 
+class GfSurConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("seed", C.c_uint64)]
+
+
+class GfSurLayer(C.Structure):
+    _fields_ = [
+        ("k_in", C.c_int32),
+        ("n_out", C.c_int32),
+        ("gelu", C.c_int32),
+        ("rows_per_query", C.c_int32),
+        ("bias_channels", C.c_int32),
+        ("w_offset", C.c_int64),
+        ("b_offset", C.c_int64),
+    ]
+
+
+class GfSurTrainOpts(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_double),
+        ("lr_min", C.c_double),
+        ("beta1", C.c_double),
+        ("beta2", C.c_double),
+        ("eps", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class GfSurStats(C.Structure):
+    _fields_ = [
+        ("forward_ms", C.c_double),
+        ("forward_queries", C.c_int64),
+        ("train_ms", C.c_double),
+        ("train_steps", C.c_int64),
+        ("train_queries", C.c_int64),
+        ("flops_per_query", C.c_double),
+    ]
+
+
 # Every symbol include/geofdtd.h declares, with its ctypes signature.
 _D, _I, _I32, _I64 = C.c_double, C.c_int, C.c_int32, C.c_int64
 _PD, _PI32, _PI64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
@@ -193,6 +233,23 @@ SIGNATURES = {
     "gf_batch_stats": (_I, [C.POINTER(GfBatchStats)]),
     "gf_spectrum": (_I, [C.POINTER(C.c_float), _I32, _I32, _D, _D, _PD, _I64, _PD, _I32, _PD]),
     "gf_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "gf_soil_sample_layered": (_I, [C.c_uint64, C.c_uint32, C.POINTER(GfSoil), _PD]),
+    "gf_sur_create": (_I, [C.POINTER(GfSurConfig), C.POINTER(C.c_void_p)]),
+    "gf_sur_destroy": (None, [C.c_void_p]),
+    "gf_sur_layers": (_I, [C.c_void_p, _PI32]),
+    "gf_sur_layer": (_I, [C.c_void_p, _I32, C.POINTER(GfSurLayer)]),
+    "gf_sur_param_count": (_I, [C.c_void_p, _PI64]),
+    "gf_sur_get_params": (_I, [C.c_void_p, C.POINTER(C.c_float)]),
+    "gf_sur_set_params": (_I, [C.c_void_p, C.POINTER(C.c_float)]),
+    "gf_sur_forward": (_I, [C.c_void_p, C.POINTER(C.c_float), _I32, C.POINTER(C.c_float)]),
+    "gf_sur_predict": (_I, [C.c_void_p, C.POINTER(C.c_float), _PD, _PD, _D, _I32, C.POINTER(C.c_float)]),
+    "gf_sur_loss_grad": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), _I32, _PD,
+                              C.POINTER(C.c_float)]),
+    "gf_sur_train": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), _I64,
+                          C.POINTER(GfSurTrainOpts), C.POINTER(C.c_float)]),
+    "gf_sur_stats": (_I, [C.c_void_p, C.POINTER(GfSurStats)]),
+    "gf_gemm_bf16": (_I, [_I32, _I32, _I32, C.POINTER(C.c_uint16), _I64, _I32, C.POINTER(C.c_uint16), _I64, _I32,
+                          _I32, C.POINTER(C.c_float)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -734,6 +791,17 @@ def SampleSoil(seed: int, index: int) -> Soil:
     _check(load_library().gf_soil_sample(seed, index, C.byref(s)))
     stack = [Layer(l.thickness_m, l.vp_mps, l.vs_mps, l.density_kgm3, l.qs, l.qp) for l in s.layers[: s.n_layers]]
     return Soil(stack, s.spacing_m, s.max_range_m, s.depth_m, s.pml_cells)
+
+
+def SampleLayeredSoil(seed: int, index: int):
+    """Soil model `index` of the layered family the surrogate is trained on,
+    and the 10 numbers that define it {Vs1, Vs2, Vs3, Qs1, Qs2, Qs3, h1, h2,
+    Vp/Vs, rho0}."""
+    s = GfSoil()
+    p = (C.c_double * 10)()
+    _check(load_library().gf_soil_sample_layered(seed, index, C.byref(s), p))
+    stack = [Layer(l.thickness_m, l.vp_mps, l.vs_mps, l.density_kgm3, l.qs, l.qp) for l in s.layers[: s.n_layers]]
+    return Soil(stack, s.spacing_m, s.max_range_m, s.depth_m, s.pml_cells), list(p)
 
 
 def Philox4x32(counter: Sequence[int], key: Sequence[int]) -> List[int]:

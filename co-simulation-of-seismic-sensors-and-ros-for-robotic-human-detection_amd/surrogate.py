@@ -1,0 +1,318 @@
+"""The Green's-function surrogate: a network standing in for the bank lookup.
+
+It maps a layered soil (10 numbers, gf_soil_sample_layered) and a range to
+the bank spectrum G(f) that the FDTD would compute for them (vertical surface
+displacement per unit vertical point force, m/N, 512 bins at 500/1022 Hz
+spacing, zero above 100 Hz).  It is trained on spectra the batched FDTD
+itself generates, runs on the library's BF16 MFMA kernels, and is served
+behind an error gate: a fraction of the soils in each request is re-solved by
+the batched FDTD, and every sampled soil whose relative L2 error exceeds the
+bound is answered by its FDTD solution instead.  No reference counterpart.
+
+Normalization.  The network learns G divided by an analytic prior,
+    T(f) = G(f) * 2 pi mu1 r * exp(+i 2 pi f r / c1),
+mu1 = rho1 Vs1^2 the top layer's shear modulus, c1 = 0.92 Vs1 about its
+Rayleigh speed: the static Boussinesq amplitude ~ 1/(2 pi mu r) and the
+travel-time phase of the surface wave come out, leaving an O(1), slowly
+rotating target.  A per-query scale and a per-bin unit phase factor leave the
+relative L2 unchanged, so the loss is the same in both domains.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import fdtd as fd
+
+FEATURES = 11
+BINS = 512
+SAMPLE_RATE_HZ = 500.0
+SAMPLES = 1022                       # 512 bins, 0 .. 250 Hz
+BIN_HZ = SAMPLE_RATE_HZ / SAMPLES
+MAX_FREQ_HZ = 100.0
+RANGES = (1.0, 50.0, 50)            # bank range grid (min, max, count)
+SECONDS = 1.0
+SOURCE_PEAK_N, SOURCE_F0_HZ = 1000.0, 40.0
+
+# Sampling intervals of gf_soil_sample_layered's 10 numbers.
+PARAM_RANGES = np.array([
+    (120, 200), (200, 320), (320, 480),   # Vs1..3, m/s
+    (10, 40), (20, 60), (40, 100),        # Qs1..3
+    (0.5, 3.0), (2.0, 6.0),               # thicknesses, m
+    (1.7, 2.5),                           # Vp/Vs
+    (1600, 1800),                         # rho0, kg/m^3
+], dtype=np.float64)
+RAYLEIGH_FRACTION = 0.92
+
+
+def bank_spec() -> fd.BankSpec:
+    return fd.BankSpec(SAMPLE_RATE_HZ, SAMPLES, MAX_FREQ_HZ, RANGES[0], RANGES[1], RANGES[2])
+
+
+def features(params: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """[n][11] float32: the 10 soil numbers and log r, each mapped to [-1, 1]."""
+    p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    x = np.empty((p.shape[0], FEATURES))
+    x[:, :10] = 2 * (p - lo) / (hi - lo) - 1
+    x[:, 10] = 2 * np.log(np.asarray(r, dtype=np.float64) / RANGES[0]) / math.log(RANGES[1] / RANGES[0]) - 1
+    return x.astype(np.float32)
+
+
+def prior(params: np.ndarray, r: np.ndarray):
+    """(scale, delay) per query: G = scale * T * exp(-i 2 pi f delay)."""
+    p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    r = np.asarray(r, dtype=np.float64)
+    vs1, rho1 = p[:, 0], p[:, 9]
+    scale = 1.0 / (2 * math.pi * rho1 * vs1 * vs1 * r)
+    delay = r / (RAYLEIGH_FRACTION * vs1)
+    return scale, delay
+
+
+def normalize(G: np.ndarray, scale: np.ndarray, delay: np.ndarray) -> np.ndarray:
+    """Complex spectra [n][512] to the network's target [n][1024] float32."""
+    f = np.arange(BINS) * BIN_HZ
+    T = G / scale[:, None] * np.exp(2j * math.pi * f[None, :] * delay[:, None])
+    out = np.empty((G.shape[0], 2 * BINS), dtype=np.float32)
+    out[:, 0::2] = T.real
+    out[:, 1::2] = T.imag
+    return out
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Per-row |a - b| / |b| over complex (or interleaved real) spectra."""
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    num = np.sqrt(np.sum(np.abs(a - b) ** 2, axis=1))
+    den = np.sqrt(np.sum(np.abs(b) ** 2, axis=1))
+    return num / np.where(den > 0, den, 1.0)
+
+
+# -------------------------------------------------------------- the network
+
+
+class Network:
+    """A gf_sur handle: the surrogate network on the GPU."""
+
+    def __init__(self, max_batch: int = 8192, seed: int = 0):
+        self._lib = fd.load_library()
+        h = C.c_void_p()
+        cfg = fd.GfSurConfig(max_batch, seed)
+        fd._check(self._lib.gf_sur_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.max_batch = max_batch
+
+    def close(self):
+        if self._h:
+            self._lib.gf_sur_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Layers(self) -> List[fd.GfSurLayer]:
+        n = C.c_int32()
+        fd._check(self._lib.gf_sur_layers(self._h, C.byref(n)))
+        out = []
+        for l in range(n.value):
+            y = fd.GfSurLayer()
+            fd._check(self._lib.gf_sur_layer(self._h, l, C.byref(y)))
+            out.append(y)
+        return out
+
+    def ParamCount(self) -> int:
+        n = C.c_int64()
+        fd._check(self._lib.gf_sur_param_count(self._h, C.byref(n)))
+        return n.value
+
+    def Params(self) -> np.ndarray:
+        out = np.zeros(self.ParamCount(), dtype=np.float32)
+        fd._check(self._lib.gf_sur_get_params(self._h, _pf(out)))
+        return out
+
+    def SetParams(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.ParamCount():
+            raise ValueError("parameter count mismatch")
+        fd._check(self._lib.gf_sur_set_params(self._h, _pf(p)))
+
+    def Forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros((len(x), 2 * BINS), dtype=np.float32)
+        fd._check(self._lib.gf_sur_forward(self._h, _pf(x), len(x), _pf(out)))
+        return out
+
+    def Predict(self, x: np.ndarray, scale: np.ndarray, delay: np.ndarray) -> np.ndarray:
+        """Physical spectra [n][512] complex64."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        s = np.ascontiguousarray(scale, dtype=np.float64)
+        d = np.ascontiguousarray(delay, dtype=np.float64)
+        out = np.zeros((len(x), 2 * BINS), dtype=np.float32)
+        fd._check(self._lib.gf_sur_predict(self._h, _pf(x), _pd(s), _pd(d), BIN_HZ, len(x), _pf(out)))
+        return out.view(np.complex64)
+
+    def LossGrad(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.ParamCount(), dtype=np.float32)
+        fd._check(self._lib.gf_sur_loss_grad(self._h, _pf(x), _pf(t), len(x), C.byref(loss), _pf(g)))
+        return loss.value, g
+
+    def Train(self, x: np.ndarray, t: np.ndarray, steps: int, batch: int = 4096, lr: float = 1e-3,
+              lr_min: float = 2e-5, seed: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        opts = fd.GfSurTrainOpts(steps, batch, lr, lr_min, 0.0, 0.0, 0.0, seed)
+        losses = np.zeros(max(steps, 1), dtype=np.float32)
+        fd._check(self._lib.gf_sur_train(self._h, _pf(x), _pf(t), len(x), C.byref(opts), _pf(losses)))
+        return losses[:steps]
+
+    def Stats(self) -> Dict[str, float]:
+        s = fd.GfSurStats()
+        fd._check(self._lib.gf_sur_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in fd.GfSurStats._fields_}
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ------------------------------------------------------------- FDTD data
+
+
+@dataclass
+class SoilSet:
+    """Soils of the layered family and, once solved, their FDTD banks."""
+
+    seed: int
+    indices: np.ndarray                  # soil indices in the family's stream
+    params: np.ndarray                   # [S][10]
+    soils: list
+    spectra: Optional[np.ndarray] = None  # [S][R][512] complex64
+    ranges: Optional[np.ndarray] = None   # [S][R] receiver ranges (grid columns)
+    fdtd_seconds: float = 0.0
+    stats: dict = field(default_factory=dict)
+
+
+def soil_set(seed: int, first: int, count: int) -> SoilSet:
+    idx = np.arange(first, first + count, dtype=np.int64)
+    soils, params = [], []
+    for i in idx:
+        s, p = fd.SampleLayeredSoil(seed, int(i))
+        soils.append(s)
+        params.append(p)
+    return SoilSet(seed, idx, np.asarray(params), soils)
+
+
+def shots_for(soils: Sequence, seconds: float = SECONDS) -> List[fd.Shot]:
+    out = []
+    for s in soils:
+        m = fd.Model(Stack=s.Stack, RefFreq=30, GslsMechanisms=3, GslsFmin=5, GslsFmax=150)
+        m.MaxRange, m.Depth, m.Spacing, m.PMLCells = s.MaxRange, s.Depth, s.Spacing, s.PMLCells
+        m.DominantFreq = SOURCE_F0_HZ
+        dt = m.Grid()[0]
+        steps = int(math.ceil(seconds / dt))
+        out.append(fd.Shot(m, fd.Ricker(SOURCE_PEAK_N, SOURCE_F0_HZ, dt, steps), [], steps))
+    return out
+
+
+def solve(soils: Sequence, seconds: float = SECONDS):
+    """Batched FDTD banks for soils: (spectra [S][R][512] complex64, ranges
+    [S][R], wall seconds, batch statistics)."""
+    t0 = time.perf_counter()
+    spectra, snapped, res = fd.GenerateBanks(shots_for(soils, seconds), bank_spec())
+    wall = time.perf_counter() - t0
+    bad = [r for r in res if isinstance(r, fd.FdtdError)]
+    if bad:
+        raise bad[0]
+    S, R = spectra.shape[0], spectra.shape[1]
+    return spectra.astype(np.complex64), np.asarray(snapped).reshape(S, R), wall, fd.BatchStats()
+
+
+def solve_set(ss: SoilSet, seconds: float = SECONDS) -> SoilSet:
+    ss.spectra, ss.ranges, ss.fdtd_seconds, ss.stats = solve(ss.soils, seconds)
+    return ss
+
+
+def dataset(ss: SoilSet):
+    """Flattened (x [n][11], targets [n][1024], scale, delay, G [n][512]) over
+    every (soil, range) of a solved set."""
+    S, R = ss.ranges.shape
+    p = np.repeat(ss.params, R, axis=0)
+    r = ss.ranges.reshape(-1)
+    x = features(p, r)
+    scale, delay = prior(p, r)
+    G = ss.spectra.reshape(S * R, BINS)
+    return x, normalize(G, scale, delay), scale, delay, G
+
+
+# ------------------------------------------------------------ the service
+
+
+@dataclass
+class GateReport:
+    queries: int
+    soils: int
+    sampled_soils: List[int]
+    failed_soils: List[int]
+    sampled_errors: np.ndarray
+    surrogate_seconds: float
+    gate_seconds: float
+    bound: float
+
+
+def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
+          gate_fraction: float = 0.01, bound: float = 0.02, seed: int = 0,
+          solver: Optional[Callable] = None, predictions: Optional[np.ndarray] = None):
+    """Answer every (soil, range) query of `soils` ([S] soils with their 10
+    numbers `params` [S][10] and receiver ranges [S][R]).
+
+    The network predicts all S*R spectra; then ceil(gate_fraction * S) soils,
+    drawn with `seed`, are re-solved by the batched FDTD (`solver(soils)` ->
+    (spectra [k][R][512], ...), default `solve`), and every sampled soil with a
+    query whose relative L2 against its FDTD solution exceeds `bound` is
+    answered by that solution instead.  `predictions` ([S*R][512]) replaces the
+    network's output, for tests of the gate itself.
+
+    Returns (spectra [S][R][512] complex64, GateReport)."""
+    S, R = ranges.shape
+    p = np.repeat(np.asarray(params), R, axis=0)
+    r = ranges.reshape(-1)
+    t0 = time.perf_counter()
+    if predictions is None:
+        x = features(p, r)
+        scale, delay = prior(p, r)
+        pred = net.Predict(x, scale, delay)
+    else:
+        pred = np.asarray(predictions, dtype=np.complex64)
+    t1 = time.perf_counter()
+    out = pred.reshape(S, R, BINS).copy()
+    k = min(S, int(math.ceil(gate_fraction * S)))
+    rng = np.random.default_rng(seed)
+    sampled = sorted(rng.choice(S, size=k, replace=False).tolist()) if k > 0 else []
+    failed: List[int] = []
+    errs = np.zeros((0,))
+    if sampled:
+        truth = (solver or solve)([soils[i] for i in sampled])[0]
+        e = rel_l2(out[sampled].reshape(k * R, BINS), truth.reshape(k * R, BINS)).reshape(k, R)
+        errs = e.reshape(-1)
+        for j, s in enumerate(sampled):
+            if np.any(e[j] > bound):
+                failed.append(s)
+                out[s] = truth[j]
+    t2 = time.perf_counter()
+    return out, GateReport(S * R, S, sampled, failed, errs, t1 - t0, t2 - t1, bound)

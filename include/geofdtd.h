@@ -305,6 +305,95 @@ typedef struct gf_soil {
 
 int gf_soil_sample(uint64_t seed, uint32_t index, gf_soil *out);
 
+/* Soil model `index` of the layered family the surrogate is trained on,
+ * drawn around config 2 (two layers over a half-space): Vs1 U[120,200],
+ * Vs2 U[200,320], Vs3 U[320,480] m/s; Qs1 U[10,40], Qs2 U[20,60], Qs3
+ * U[40,100], Qp = 2 Qs; thicknesses U[0.5,3] and U[2,6] m; one Vp/Vs U[1.7,
+ * 2.5]; densities rho0, rho0+150, rho0+300 with rho0 U[1600,1800]; h = 0.9
+ * Vs1 / (100 Hz * 10); an interior of 52 m by 30 m and a 20-cell C-PML.
+ * params receives the 10 numbers that define the model, in the order
+ * {Vs1, Vs2, Vs3, Qs1, Qs2, Qs3, h1, h2, Vp/Vs, rho0}.  Philox counter word 2
+ * is 1, so these draws never coincide with gf_soil_sample's.  No reference
+ * counterpart. */
+int gf_soil_sample_layered(uint64_t seed, uint32_t index, gf_soil *out, double params[10]);
+
+/* ---- the Green's-function surrogate -----------------------------------
+ * No reference counterpart: a network standing in for the bank lookup,
+ * trained on spectra the batched FDTD generates, with the FDTD as its
+ * fallback.  Inputs per query: 11 features (the 10 numbers of a layered soil
+ * and the log range, each scaled to about [-1, 1] by the caller).  Output
+ * per query: 512 complex bins, either in the normalized domain the network
+ * is trained in (gf_sur_forward) or as a physical spectrum (gf_sur_predict:
+ * times scale[q] and e^{-i 2 pi f delay[q]}, bin k at k * bin_hz).
+ * Network: an encoder of six 512-wide GELU layers and a decoder of four
+ * transposed convolutions (kernel = stride) from 16 positions x 32 channels
+ * to 512 positions x 2; every product on BF16 MFMA with FP32 accumulation;
+ * FP32 master weights and Adam.  The loss is the relative spectral L2,
+ * mean over queries of |o - t|^2 / |t|^2. */
+typedef struct gf_sur gf_sur;
+
+typedef struct gf_sur_config {
+  int32_t max_batch; /* queries per launch chain; a multiple of 8, at most 16384 */
+  uint64_t seed;     /* weight initialisation */
+} gf_sur_config;
+
+/* One GEMM layer: weights [n_out][k_in] at w_offset and bias_channels
+ * biases at b_offset in the flat FP32 parameter vector; rows_per_query GEMM
+ * rows per query (decoder positions); gelu 0 for the linear output layer. */
+typedef struct gf_sur_layer_t {
+  int32_t k_in, n_out, gelu, rows_per_query, bias_channels;
+  int64_t w_offset, b_offset;
+} gf_sur_layer_t;
+
+typedef struct gf_sur_train_opts {
+  int32_t steps;
+  int32_t batch;        /* a multiple of 8, at most max_batch */
+  double lr, lr_min;    /* cosine decay from lr to lr_min */
+  double beta1, beta2, eps; /* Adam; zero selects 0.9, 0.999, 1e-8 */
+  uint64_t seed;        /* minibatch order */
+} gf_sur_train_opts;
+
+/* Device time of the calling handle's last forward/predict (HIP events
+ * around its kernels, inputs already in HBM) and last training run. */
+typedef struct gf_sur_stats_t {
+  double forward_ms;
+  int64_t forward_queries;
+  double train_ms;
+  int64_t train_steps;
+  int64_t train_queries;
+  double flops_per_query; /* forward GEMM flops, algorithmic */
+} gf_sur_stats_t;
+
+int gf_sur_create(const gf_sur_config *c, gf_sur **out);
+void gf_sur_destroy(gf_sur *h);
+int gf_sur_layers(const gf_sur *h, int32_t *n);
+int gf_sur_layer(const gf_sur *h, int32_t l, gf_sur_layer_t *out);
+int gf_sur_param_count(const gf_sur *h, int64_t *n);
+int gf_sur_get_params(gf_sur *h, float *out);
+/* Replace the parameters (and reset the Adam state). */
+int gf_sur_set_params(gf_sur *h, const float *in);
+/* x [n][11] -> out [n][1024], the normalized output. */
+int gf_sur_forward(gf_sur *h, const float *x, int32_t n, float *out);
+/* x [n][11] -> out [n][512][2], physical spectra. */
+int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double *delay, double bin_hz, int32_t n,
+                   float *out);
+/* Mean loss over n queries (a multiple of 8, at most max_batch) against
+ * targets t [n][1024], and its gradient (flat, like the parameters). */
+int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, int32_t n, double *loss, float *grad);
+/* opts->steps Adam steps on minibatches of the n samples; losses (may be
+ * NULL) receives each step's mean loss. */
+int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_sur_train_opts *opts,
+                 float *losses);
+int gf_sur_stats(const gf_sur *h, gf_sur_stats_t *out);
+
+/* The surrogate's GEMM kernel on its own, for testing: C[M][N] (FP32) =
+ * sum_k A(m,k) B(k,n) with BF16 operands as raw bits; A(m,k) = A[m*lda+k]
+ * (a_trans 0) or A[k*lda+m] (a_trans 1); B(k,n) = B[n*ldb+k] (b_nk 1) or
+ * B[k*ldb+n] (b_nk 0); K split `splits` ways and summed.  K, lda, ldb must be
+ * multiples of 8, and M (a_trans) and N (!b_nk) too. */
+int gf_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda, int32_t a_trans, const uint16_t *B,
+                 int64_t ldb, int32_t b_nk, int32_t splits, float *C);
+
 /* Philox4x32-10 block (counter, key) -> 4 words, for checking the sampler. */
 void gf_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -1,0 +1,193 @@
+"""The surrogate on the GPU: the BF16 MFMA GEMM against FP64, the network's
+forward pass and gradients against the CPU oracle, training, and what the
+trained network gains over solving every query with the FDTD."""
+
+import math
+import time
+
+import numpy as np
+import pytest
+
+import surrogate_oracle as O
+from test_surrogate_oracle import ARCH, layers, random_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import cosim_amd.surrogate as S
+
+    return S
+
+
+def bf16_bits(x):
+    """(raw uint16 bits, the values they hold as float64)."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return r, (r.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K,at,bnk,splits", [
+    (128, 128, 32, 0, 1, 1),
+    (200, 512, 512, 0, 1, 1),     # M not a tile multiple, full encoder width
+    (96, 64, 40, 0, 0, 1),        # K not a multiple of 32, B stored [K][N]
+    (64, 8, 16, 0, 1, 1),         # the output layer's shape
+    (512, 512, 1000, 1, 0, 4),    # a weight gradient: A stored [K][M], split K
+    (32, 16, 4096, 1, 0, 7),
+    (136, 40, 24, 1, 1, 2),
+])
+def test_gemm_matches_fp64_within_fp32_accumulation(fdtd, M, N, K, at, bnk, splits):
+    """Operands are exact BF16 values, so every product is exact in FP32 and
+    the only error is the FP32 accumulation: |C - C64| <= K 2^-24 sum|a b|
+    (a first-order bound, with a factor 2 for the split sums)."""
+    import ctypes as C
+
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    a_bits, a = bf16_bits(rng.normal(size=(K, M) if at else (M, K)))
+    b_bits, b = bf16_bits(rng.normal(size=(N, K) if bnk else (K, N)))
+    A = a.T if at else a          # [M][K]
+    Bm = b.T if bnk else b        # [K][N]
+    want = A @ Bm
+    bound = 2 * K * 2.0 ** -24 * (np.abs(A) @ np.abs(Bm)) + 1e-30
+    out = np.zeros((M, N), dtype=np.float32)
+    lda = M if at else K
+    ldb = K if bnk else N
+    lib = fdtd.load_library()
+    pa = np.ascontiguousarray(a_bits).ctypes.data_as(C.POINTER(C.c_uint16))
+    pb = np.ascontiguousarray(b_bits).ctypes.data_as(C.POINTER(C.c_uint16))
+    print(f"gemm M={M} N={N} K={K} at={at} b_nk={bnk} splits={splits}", flush=True)
+    fdtd._check(lib.gf_gemm_bf16(M, N, K, pa, lda, at, pb, ldb, bnk, splits,
+                                 out.ctypes.data_as(C.POINTER(C.c_float))))
+    err = np.abs(out - want)
+    assert np.all(err <= bound), f"worst excess {np.max(err / bound):.3f} of the bound"
+
+
+def test_gemm_rejects_shapes_it_cannot_vectorise(fdtd):
+    import ctypes as C
+
+    lib = fdtd.load_library()
+    buf = np.zeros(4096, dtype=np.uint16).ctypes.data_as(C.POINTER(C.c_uint16))
+    out = np.zeros(4096, dtype=np.float32).ctypes.data_as(C.POINTER(C.c_float))
+    assert lib.gf_gemm_bf16(16, 16, 12, buf, 16, 0, buf, 16, 1, 1, out) == fdtd.GF_ERR_ARGUMENT
+    assert lib.gf_gemm_bf16(12, 16, 16, buf, 16, 1, buf, 16, 1, 1, out) == fdtd.GF_ERR_ARGUMENT
+
+
+@pytest.fixture(scope="module")
+def net(S):
+    n = S.Network(max_batch=64, seed=3)
+    yield n
+    n.close()
+
+
+def test_layer_table_is_the_documented_architecture(net):
+    L, P = layers(ARCH)
+    got = [(l.k_in, l.n_out, l.gelu, l.rows_per_query, l.bias_channels, l.w_offset, l.b_offset)
+           for l in net.Layers()]
+    assert got == [(l.k, l.n, int(l.gelu), l.rows, l.bias_mod, l.w_off, l.b_off) for l in L]
+    assert net.ParamCount() == P
+
+
+def test_forward_matches_the_oracle(net):
+    L = O.layers_from(None, net)
+    rng = np.random.default_rng(5)
+    p = random_params(L, net.ParamCount(), rng, scale=1.0).astype(np.float32)
+    net.SetParams(p)
+    np.testing.assert_array_equal(net.Params(), p)
+    x = rng.uniform(-1, 1, (40, 11)).astype(np.float32)
+    got = net.Forward(x)
+    want_bf, _ = O.forward(p.astype(np.float64), L, x.astype(np.float64), bf=True)
+    want, _ = O.forward(p.astype(np.float64), L, x.astype(np.float64), bf=False)
+    e_bf = np.linalg.norm(got - want_bf, axis=1) / np.linalg.norm(want_bf, axis=1)
+    e = np.linalg.norm(got - want, axis=1) / np.linalg.norm(want, axis=1)
+    print(f"forward rel-L2 vs BF16-emulating oracle: max {e_bf.max():.2e}; vs FP64: max {e.max():.2e}", flush=True)
+    assert e_bf.max() <= 3e-3
+    assert e.max() <= 3e-2
+
+
+def test_gradients_match_the_oracle(net):
+    """Each layer's weight and bias gradient against the oracle's, as norm-wise
+    relative error: at most 1e-2 against exact FP64 and 5e-3 against the
+    oracle with the GPU's BF16 operand rounding."""
+    L = O.layers_from(None, net)
+    rng = np.random.default_rng(6)
+    p = random_params(L, net.ParamCount(), rng, scale=1.0).astype(np.float32)
+    net.SetParams(p)
+    B = 32
+    x = rng.uniform(-1, 1, (B, 11)).astype(np.float32)
+    t = rng.normal(0, 1, (B, 1024)).astype(np.float32)
+    loss, g = net.LossGrad(x, t)
+    l_bf, g_bf = O.loss_grad(p.astype(np.float64), L, x.astype(np.float64), t.astype(np.float64), bf=True)
+    l64, g64 = O.loss_grad(p.astype(np.float64), L, x.astype(np.float64), t.astype(np.float64), bf=False)
+    assert loss == pytest.approx(l64, rel=2e-2) and loss == pytest.approx(l_bf, rel=2e-3)
+    for i, l in enumerate(L):
+        for name, a, b in (("W", l.w_off, l.w_off + l.n * l.k), ("b", l.b_off, l.b_off + l.bias_mod)):
+            ref, ref_bf, got = g64[a:b], g_bf[a:b], g[a:b].astype(np.float64)
+            r64 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            rbf = np.linalg.norm(got - ref_bf) / np.linalg.norm(ref_bf)
+            print(f"layer {i} d{name}: rel err {r64:.2e} (FP64), {rbf:.2e} (BF16 oracle)", flush=True)
+            assert r64 <= 1e-2 and rbf <= 5e-3, (i, name)
+
+
+def test_predict_applies_the_prior(net, S):
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, (24, 11)).astype(np.float32)
+    scale = rng.uniform(1e-9, 1e-8, 24)
+    delay = rng.uniform(0.0, 0.4, 24)
+    o = net.Forward(x).astype(np.float64).view(np.complex128)
+    f = np.arange(S.BINS) * S.BIN_HZ
+    want = scale[:, None] * o * np.exp(-2j * np.pi * f[None, :] * delay[:, None])
+    got = net.Predict(x, scale, delay)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+
+
+def test_training_reduces_the_loss(S):
+    """Adam on a smooth synthetic map from 11 inputs to spectra: the loss falls
+    by an order of magnitude in a few hundred steps."""
+    n = S.Network(max_batch=256, seed=11)
+    try:
+        rng = np.random.default_rng(8)
+        x = rng.uniform(-1, 1, (2048, 11)).astype(np.float32)
+        k = np.arange(512)
+        amp = np.exp(-k / (80 + 40 * x[:, :1]))
+        ph = 2 * np.pi * k * (0.01 + 0.005 * x[:, 1:2])
+        t = np.empty((2048, 1024), dtype=np.float32)
+        t[:, 0::2] = amp * np.cos(ph)
+        t[:, 1::2] = amp * np.sin(ph)
+        losses = n.Train(x, t, steps=400, batch=256, lr=1e-3, lr_min=1e-4, seed=1)
+        first, last = losses[:10].mean(), losses[-10:].mean()
+        st = n.Stats()
+        print(f"training: loss {first:.3f} -> {last:.4f}; {st['train_ms'] / st['train_steps']:.3f} ms per step",
+              flush=True)
+        assert np.all(np.isfinite(losses))
+        assert last < 0.1 * first
+    finally:
+        n.close()
+
+
+def test_trained_network_gains_over_fdtd_only(S):
+    """On FDTD-generated spectra of the layered family: the trained network's
+    held-out error is a fraction of the untrained one's, and answering a query
+    costs a small fraction of solving it with the batched FDTD."""
+    train = S.solve_set(S.soil_set(0, 0, 48), seconds=1.0)
+    test = S.solve_set(S.soil_set(0, 10_000, 8), seconds=1.0)
+    x, t, _, _, _ = S.dataset(train)
+    xt, tt, sc, dl, Gt = S.dataset(test)
+    n = S.Network(max_batch=2400, seed=5)
+    try:
+        before = S.rel_l2(n.Predict(xt, sc, dl), Gt)
+        losses = n.Train(x, t, steps=600, batch=800, lr=1e-3, lr_min=3e-5, seed=2)
+        after = S.rel_l2(n.Predict(xt, sc, dl), Gt)
+        n.Predict(np.repeat(xt, 6, axis=0), np.repeat(sc, 6), np.repeat(dl, 6))
+        st = n.Stats()
+    finally:
+        n.close()
+    fdtd_per_trace = train.fdtd_seconds / t.shape[0]
+    sur_per_trace = st["forward_ms"] / 1e3 / st["forward_queries"]
+    print(f"held-out rel-L2 median {np.median(before):.3f} -> {np.median(after):.3f} "
+          f"(p90 {np.quantile(after, 0.9):.3f}); train loss {losses[:20].mean():.3f} -> {losses[-20:].mean():.4f}; "
+          f"FDTD {fdtd_per_trace * 1e3:.2f} ms/trace, surrogate {sur_per_trace * 1e6:.2f} us/trace "
+          f"({fdtd_per_trace / sur_per_trace:.0f}x)", flush=True)
+    assert np.median(after) < 0.5 * np.median(before)
+    assert fdtd_per_trace / sur_per_trace >= 100

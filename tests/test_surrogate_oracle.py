@@ -1,0 +1,180 @@
+"""The surrogate on the CPU: its FP64 oracle pinned by finite differences and
+closed forms, the layered soil family, the target normalization, and the
+error gate's fallback with injected predictions.  No GPU."""
+
+import math
+
+import numpy as np
+import pytest
+
+import cosim_amd.fdtd as fd
+import cosim_amd.surrogate as S
+import surrogate_oracle as O
+
+# The network's layer table as the library defines it (checked against a live
+# handle by the GPU tests): (k_in, n_out, gelu, rows per query, bias channels).
+ARCH = [(16, 512, 1, 1, 512)] + [(512, 512, 1, 1, 512)] * 5 + [
+    (32, 64, 1, 16, 32), (32, 64, 1, 32, 32), (32, 32, 1, 64, 16), (16, 8, 0, 128, 2)]
+
+
+def layers(arch=ARCH):
+    out, off = [], 0
+    for k, n, g, rows, bm in arch:
+        w = off
+        off += n * k
+        b = off
+        off += (bm + 7) // 8 * 8
+        out.append(O.Layer(k, n, bool(g), rows, bm, w, b))
+    return out, off
+
+
+def small_arch():
+    """The same structure at small width, so finite differences are cheap."""
+    return [(16, 24, 1, 1, 24), (24, 32, 1, 1, 32), (8, 16, 1, 4, 8), (8, 8, 1, 8, 4), (4, 16, 0, 16, 2)]
+
+
+def random_params(L, P, rng, scale=0.5):
+    p = np.zeros(P)
+    for l in L:
+        fan = 11 if l is L[0] else l.k
+        w = rng.normal(0, scale / math.sqrt(fan), (l.n, l.k))
+        if l is L[0]:
+            w[:, 11:] = 0
+        p[l.w_off: l.w_off + l.n * l.k] = w.ravel()
+        p[l.b_off: l.b_off + l.bias_mod] = rng.normal(0, 0.1, l.bias_mod)
+    return p
+
+
+def test_full_architecture_produces_512_complex_bins():
+    L, P = layers()
+    assert P == 1327320
+    rng = np.random.default_rng(0)
+    p = random_params(L, P, rng)
+    o, _ = O.forward(p, L, rng.uniform(-1, 1, (3, 11)))
+    assert o.shape == (3, 1024)
+    # the decoder's row chaining: each layer's output rows x columns equal
+    # the next layer's input rows x k
+    for a, b in zip(L, L[1:]):
+        assert a.rows * a.n == b.rows * b.k
+
+
+def test_gradient_matches_central_differences():
+    """Exact FP64 backward pass against (f(p+h) - f(p-h)) / 2h, every layer."""
+    arch = small_arch()
+    L, P = layers(arch)
+    # the small decoder ends at 16*16/2 = 128 complex bins
+    rng = np.random.default_rng(1)
+    p = random_params(L, P, rng, scale=1.0)
+    x = rng.uniform(-1, 1, (3, 11))
+    out_len = L[-1].rows * L[-1].n
+    t = rng.normal(0, 1, (3, out_len))
+    f0, g = O.loss_grad(p, L, x, t)
+    h = 1e-6
+    for l in L:
+        idx = list(rng.choice(l.n * l.k, size=6, replace=False) + l.w_off) + [l.b_off, l.b_off + l.bias_mod - 1]
+        for i in idx:
+            if l is L[0] and (i - l.w_off) % l.k >= 11 and i < l.b_off:
+                continue  # padded inputs: zero gradient by construction
+            pp, pm = p.copy(), p.copy()
+            pp[i] += h
+            pm[i] -= h
+            fd_ = (O.loss_grad(pp, L, x, t)[0] - O.loss_grad(pm, L, x, t)[0]) / (2 * h)
+            assert abs(fd_ - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, (i, fd_, g[i])
+
+
+def test_loss_is_the_relative_l2_squared():
+    t = np.array([[3.0, 4.0], [1.0, 0.0]])
+    o = np.array([[3.0, 0.0], [0.0, 0.0]])
+    np.testing.assert_allclose(O.loss(o, t), [16 / 25, 1.0])
+
+
+def test_bf16_rounding_is_nearest_even():
+    one = 1.0
+    ulp = 2.0 ** -7
+    vals = np.array([one, one + ulp / 2, one + 3 * ulp / 2, one + ulp / 4, -2.5, 3.0e38, 1e-39])
+    got = O.bf16(vals)
+    np.testing.assert_array_equal(got[:4], [one, one, one + 2 * ulp, one])
+    assert got[4] == -2.5
+    assert np.isfinite(got[5])
+    assert got[6] != 0 and O.bf16(np.array([2.0 ** -130]))[0] == 2.0 ** -130  # FP32 subnormals keep their bits
+
+
+def test_normalization_keeps_the_relative_l2():
+    """A per-query scale and a per-bin unit phase factor leave |a-b|/|b|
+    unchanged: the loss in the network's domain is the physical one."""
+    rng = np.random.default_rng(2)
+    n = 5
+    params = np.column_stack([rng.uniform(lo, hi, n) for lo, hi in S.PARAM_RANGES])
+    r = rng.uniform(1, 50, n)
+    scale, delay = S.prior(params, r)
+    G = (rng.normal(size=(n, S.BINS)) + 1j * rng.normal(size=(n, S.BINS))) * 1e-9
+    H = G + (rng.normal(size=(n, S.BINS)) + 1j * rng.normal(size=(n, S.BINS))) * 1e-10
+    a, b = S.normalize(H, scale, delay), S.normalize(G, scale, delay)
+    np.testing.assert_allclose(S.rel_l2(a, b), S.rel_l2(H, G), rtol=1e-5)
+    # the prior's amplitude is the static Boussinesq factor 1/(2 pi mu1 r)
+    mu1 = params[:, 9] * params[:, 0] ** 2
+    np.testing.assert_allclose(scale, 1 / (2 * math.pi * mu1 * r))
+    np.testing.assert_allclose(delay, r / (0.92 * params[:, 0]))
+
+
+def test_features_map_the_sampling_box_to_unit_interval():
+    lo, hi = S.PARAM_RANGES[:, 0], S.PARAM_RANGES[:, 1]
+    x = S.features(np.vstack([lo, hi]), np.array([1.0, 50.0]))
+    np.testing.assert_allclose(x, [[-1] * 11, [1] * 11], atol=1e-6)
+
+
+def test_layered_family_is_deterministic_and_inside_its_box():
+    for i in range(40):
+        s1, p1 = fd.SampleLayeredSoil(7, i)
+        s2, p2 = fd.SampleLayeredSoil(7, i)
+        assert p1 == p2 and s1 == s2
+        p = np.asarray(p1)
+        assert np.all(p >= S.PARAM_RANGES[:, 0]) and np.all(p <= S.PARAM_RANGES[:, 1])
+        fd.ValidateStack(s1.Stack)
+        assert [l.Vs for l in s1.Stack] == list(p[:3])
+        assert [l.Qp for l in s1.Stack] == [2 * q for q in p[3:6]]
+        assert s1.Stack[0].Thickness == p[6] and s1.Stack[1].Thickness == p[7]
+        assert [l.Density for l in s1.Stack] == [p[9], p[9] + 150, p[9] + 300]
+        assert s1.Spacing == pytest.approx(0.9 * p[0] / 1000)
+        assert s1.MaxRange > 50.0 and s1.MaxRange < 52.0 + s1.Spacing and s1.PMLCells == 20
+    # a different stream from gf_soil_sample's
+    assert fd.SampleSoil(7, 0).Stack[0].Vs != fd.SampleLayeredSoil(7, 0)[0].Stack[0].Vs
+
+
+def test_gate_falls_back_for_injected_bad_predictions():
+    """The error gate with the network's output replaced: sampled soils whose
+    predictions are off by more than the bound get their FDTD solution, the
+    others keep the prediction.  The solver is a stand-in returning known
+    truth, so this runs without a GPU."""
+    S_, R = 50, 4
+    rng = np.random.default_rng(3)
+    truth = (rng.normal(size=(S_, R, S.BINS)) + 1j * rng.normal(size=(S_, R, S.BINS))).astype(np.complex64)
+    pred = truth.copy()
+    bad = set(range(0, S_, 3))
+    for s in bad:
+        pred[s, 1] *= 1.10   # 10 % off on one range
+    for s in range(1, S_, 3):
+        pred[s] *= 1.005     # 0.5 %: within the bound
+    calls = []
+
+    def solver(soils):
+        calls.append(list(soils))
+        return truth[np.asarray(soils)], None, 0.0, {}
+
+    soils = list(range(S_))
+    params = np.zeros((S_, 10))
+    ranges = np.ones((S_, R))
+    out, rep = S.serve(None, params, ranges, soils, gate_fraction=0.5, bound=0.02, seed=4, solver=solver,
+                       predictions=pred.reshape(S_ * R, S.BINS))
+    assert len(rep.sampled_soils) == 25 and len(calls) == 1 and calls[0] == rep.sampled_soils
+    assert rep.failed_soils == sorted(s for s in rep.sampled_soils if s in bad)
+    for s in range(S_):
+        if s in rep.failed_soils:
+            np.testing.assert_array_equal(out[s], truth[s])
+        else:
+            np.testing.assert_array_equal(out[s], pred[s])
+    assert rep.sampled_errors.shape == (25 * R,)
+    # a gate fraction of 1% samples ceil(0.01 S) soils
+    _, rep1 = S.serve(None, params, ranges, soils, gate_fraction=0.01, solver=solver,
+                      predictions=pred.reshape(S_ * R, S.BINS))
+    assert len(rep1.sampled_soils) == 1
