@@ -311,13 +311,20 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 
 // Normalized network output to the physical spectrum, per query b and bin k:
 //   G = s_b (o_re + i o_im) e^{-i 2 pi f_k w_b},  f_k = k * df
-// (s_b the amplitude scale and w_b the delay the targets were divided by).
+// (s_b the amplitude scale and w_b the delay the targets were divided by);
+// zero outside bins [k_lo, k_hi].
 __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const double *__restrict__ scale,
-                              const double *__restrict__ delay, double df, float *__restrict__ out, int32_t rows) {
+                              const double *__restrict__ delay, double df, int32_t k_lo, int32_t k_hi,
+                              float *__restrict__ out, int32_t rows) {
   const int64_t n = (int64_t)rows * bins;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = i / bins;
     const int k = (int)(i % bins);
+    if (k < k_lo || k > k_hi) {
+      out[2 * i] = 0.0f;
+      out[2 * i + 1] = 0.0f;
+      continue;
+    }
     const double ph = -2.0 * 3.141592653589793 * (double)k * df * delay[b];
     double sn, cs;
     sincos(ph, &sn, &cs);
@@ -381,10 +388,10 @@ hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *p
 }
 
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
-                              float *out, int32_t rows, hipStream_t st) {
+                              int32_t k_lo, int32_t k_hi, float *out, int32_t rows, hipStream_t st) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_denormalize, grid_for((int64_t)rows * bins), dim3(256), 0, st, o, bins, scale, delay, df, out,
-                     rows);
+  hipLaunchKernelGGL(k_denormalize, grid_for((int64_t)rows * bins), dim3(256), 0, st, o, bins, scale, delay, df, k_lo,
+                     k_hi, out, rows);
   return hipGetLastError();
 }
 

@@ -108,6 +108,7 @@ struct gf_sur {
   std::vector<Layer> L;
   int64_t P = 0;
   int max_batch = 0;
+  double band_lo = 0, band_hi = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // parameters and optimizer state
@@ -147,6 +148,8 @@ struct gf_sur {
     L = architecture();
     P = param_count(L);
     max_batch = c.max_batch;
+    band_lo = c.band_lo_hz;
+    band_hi = c.band_hi_hz;
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -416,7 +419,13 @@ int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double 
     HIPCHK(hipMemcpyAsync(h->delay, delay + q0, sizeof(double) * B, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipEventRecord(h->e0, h->st));
     if ((rc = h->stage_features(h->feat, Bp)) || (rc = h->forward(Bp, false))) return rc;
-    HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, h->outp, B, h->st));
+    int k_lo = 0, k_hi = kOut / 2 - 1;
+    if (h->band_hi > 0 && bin_hz > 0) {
+      k_lo = (int)ceil(h->band_lo / bin_hz - 1e-9);
+      k_hi = (int)floor(h->band_hi / bin_hz + 1e-9);
+    }
+    HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->outp, B,
+                                    h->st));
     HIPCHK(hipEventRecord(h->e1, h->st));
     HIPCHK(hipMemcpyAsync(out + (int64_t)q0 * kOut, h->outp, sizeof(float) * B * kOut, hipMemcpyDeviceToHost,
                           h->st));

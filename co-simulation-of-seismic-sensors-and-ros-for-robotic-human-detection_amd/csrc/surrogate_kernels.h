@@ -51,6 +51,6 @@ hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t st);
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
-                              float *out, int32_t rows, hipStream_t st);
+                              int32_t k_lo, int32_t k_hi, float *out, int32_t rows, hipStream_t st);
 
 }  // namespace gfsk

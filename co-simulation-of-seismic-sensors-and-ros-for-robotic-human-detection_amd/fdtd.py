@@ -153,7 +153,8 @@ class GfSoil(C.Structure):
 
 
 class GfSurConfig(C.Structure):
-    _fields_ = [("max_batch", C.c_int32), ("seed", C.c_uint64)]
+    _fields_ = [("max_batch", C.c_int32), ("seed", C.c_uint64), ("band_lo_hz", C.c_double),
+                ("band_hi_hz", C.c_double)]
 
 
 class GfSurLayer(C.Structure):

@@ -310,7 +310,8 @@ int gf_soil_sample(uint64_t seed, uint32_t index, gf_soil *out);
  * Vs2 U[200,320], Vs3 U[320,480] m/s; Qs1 U[10,40], Qs2 U[20,60], Qs3
  * U[40,100], Qp = 2 Qs; thicknesses U[0.5,3] and U[2,6] m; one Vp/Vs U[1.7,
  * 2.5]; densities rho0, rho0+150, rho0+300 with rho0 U[1600,1800]; h = 0.9
- * Vs1 / (100 Hz * 10); an interior of 52 m by 30 m and a 20-cell C-PML.
+ * Vs1 / (100 Hz * 10); an interior of 80 m by 30 m (receivers out to 50 m
+ * stay 30 m from the C-PML) and a 20-cell C-PML.
  * params receives the 10 numbers that define the model, in the order
  * {Vs1, Vs2, Vs3, Qs1, Qs2, Qs3, h1, h2, Vp/Vs, rho0}.  Philox counter word 2
  * is 1, so these draws never coincide with gf_soil_sample's.  No reference
@@ -335,6 +336,8 @@ typedef struct gf_sur gf_sur;
 typedef struct gf_sur_config {
   int32_t max_batch; /* queries per launch chain; a multiple of 8, at most 16384 */
   uint64_t seed;     /* weight initialisation */
+  double band_lo_hz; /* gf_sur_predict leaves bins outside [lo, hi] zero; */
+  double band_hi_hz; /* hi <= 0 keeps every bin */
 } gf_sur_config;
 
 /* One GEMM layer: weights [n_out][k_in] at w_offset and bias_channels

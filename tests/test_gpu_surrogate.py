@@ -130,7 +130,8 @@ def test_gradients_match_the_oracle(net):
             assert r64 <= 1e-2 and rbf <= 5e-3, (i, name)
 
 
-def test_predict_applies_the_prior(net, S):
+def test_predict_applies_the_prior(S):
+    net = S.Network(max_batch=64, seed=3, band=None)
     rng = np.random.default_rng(7)
     x = rng.uniform(-1, 1, (24, 11)).astype(np.float32)
     scale = rng.uniform(1e-9, 1e-8, 24)
@@ -140,6 +141,13 @@ def test_predict_applies_the_prior(net, S):
     want = scale[:, None] * o * np.exp(-2j * np.pi * f[None, :] * delay[:, None])
     got = net.Predict(x, scale, delay)
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    net.close()
+    banded = S.Network(max_batch=64, seed=3)
+    got = banded.Predict(x, scale, delay)
+    m = S.band_mask()
+    assert np.all(got[:, ~m] == 0) and m.sum() == 194
+    np.testing.assert_allclose(got[:, m], want[:, m], rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    banded.close()
 
 
 def test_training_reduces_the_loss(S):
