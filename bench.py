@@ -231,7 +231,8 @@ def bench_surrogate(args, fd) -> int:
     R = S.RANGES[2]
     n_held = -(-args.sur_queries // R)
     log(f"surrogate: FDTD banks for {args.sur_soils} training soils")
-    train = S.solve_set(S.soil_set(0, 0, args.sur_soils))
+    prog = lambda d, n, s: log(f"  FDTD banks {d}/{n} soils, {s:.0f} s")
+    train = S.solve_set(S.soil_set(0, 0, args.sur_soils), progress=prog)
     fdtd_rate = args.sur_soils * R / train.fdtd_seconds
     x, t, _, _, _ = S.dataset(train)
     log(f"  {train.fdtd_seconds:.1f} s ({fdtd_rate:.0f} traces/s); training {args.sur_steps} steps")
@@ -240,7 +241,7 @@ def bench_surrogate(args, fd) -> int:
     tr = net.Stats()
     log(f"  loss {losses[:50].mean():.4f} -> {losses[-50:].mean():.4f} in {tr['train_ms'] / 1e3:.1f} s of device time")
     log(f"held-out: FDTD truth for {n_held} soils (evaluation only)")
-    held = S.solve_set(S.soil_set(0, 1_000_000, n_held))
+    held = S.solve_set(S.soil_set(0, 1_000_000, n_held), progress=prog)
     xh, _, sch, dlh, Gh = S.dataset(held)
     xh, sch, dlh, Gh = xh[:args.sur_queries], sch[:args.sur_queries], dlh[:args.sur_queries], Gh[:args.sur_queries]
     # inference rate: device time of the predict kernels with queries in HBM

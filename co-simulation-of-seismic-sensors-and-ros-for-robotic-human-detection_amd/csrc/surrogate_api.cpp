@@ -4,12 +4,11 @@
 // accumulation, FP32 master weights and Adam state):
 //
 //   encoder  x[11] -> 512 -> 512 -> 512 -> 512 -> 512 -> 512, GELU after each
-//   decoder  the 512 as 16 positions x 32 channels, then transposed
+//   decoder  a dense projection 512 -> 2048, GELU, read as 64 positions x
+//            32 channels (one position per 8 bins), then transposed
 //            convolutions whose kernel equals their stride:
-//              16 -> 32 positions, 32 -> 32 channels, GELU
-//              32 -> 64,           32 -> 32,          GELU
-//              64 -> 128,          32 -> 16,          GELU
-//              128 -> 512,         16 -> 2 (re, im),  linear
+//              64 -> 128 positions, 32 -> 32 channels, GELU
+//              128 -> 512,          32 -> 2 (re, im),  linear
 //
 // A transposed convolution with kernel = stride s maps each input position
 // to s output positions independently, so over rows (query, position) it is
@@ -57,10 +56,9 @@ std::vector<Layer> architecture() {
   std::vector<Layer> L;
   L.push_back({kInPad, 512, 1, 1, 512, 0, 0});
   for (int i = 0; i < 5; i++) L.push_back({512, 512, 1, 1, 512, 0, 0});
-  L.push_back({32, 64, 1, 16, 32, 0, 0});
-  L.push_back({32, 64, 1, 32, 32, 0, 0});
-  L.push_back({32, 32, 1, 64, 16, 0, 0});
-  L.push_back({16, 8, 0, 128, 2, 0, 0});
+  L.push_back({512, 2048, 1, 1, 2048, 0, 0});
+  L.push_back({32, 64, 1, 64, 32, 0, 0});
+  L.push_back({32, 8, 0, 128, 2, 0, 0});
   int64_t off = 0;
   for (auto &l : L) {
     l.w_off = off;
@@ -124,6 +122,7 @@ struct gf_sur {
   float *split = nullptr;
   int64_t split_floats = 0;
   float *colpart = nullptr;
+  int colpart_cols = 0;
   float *feat = nullptr, *tgt = nullptr, *loss = nullptr;
   double *scale = nullptr, *delay = nullptr;
   float *outp = nullptr;
@@ -173,7 +172,8 @@ struct gf_sur {
       HIPCHK(dalloc(&dB[i], B * widest));
     }
     HIPCHK(dalloc(&split, split_floats));
-    HIPCHK(dalloc(&colpart, (int64_t)kColChunks * 512));
+    for (const Layer &y : L) colpart_cols = std::max(colpart_cols, y.n);
+    HIPCHK(dalloc(&colpart, (int64_t)kColChunks * colpart_cols));
     HIPCHK(dalloc(&feat, B * kFeatures));
     HIPCHK(dalloc(&tgt, B * kOut));
     HIPCHK(dalloc(&loss, B));

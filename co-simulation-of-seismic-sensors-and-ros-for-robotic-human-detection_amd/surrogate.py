@@ -248,21 +248,33 @@ def shots_for(soils: Sequence, seconds: float = SECONDS) -> List[fd.Shot]:
     return out
 
 
-def solve(soils: Sequence, seconds: float = SECONDS):
-    """Batched FDTD banks for soils: (spectra [S][R][512] complex64, ranges
-    [S][R], wall seconds, batch statistics)."""
+def solve(soils: Sequence, seconds: float = SECONDS, chunk: int = 256,
+          progress: Optional[Callable[[int, int, float], None]] = None):
+    """Batched FDTD banks for soils, `chunk` soils per batch: (spectra
+    [S][R][512] complex64, ranges [S][R], wall seconds, summed batch
+    statistics).  progress(done, total, seconds) after each batch."""
     t0 = time.perf_counter()
-    spectra, snapped, res = fd.GenerateBanks(shots_for(soils, seconds), bank_spec())
+    parts, snaps = [], []
+    stats = {"kernel_ms": 0.0, "step_launches": 0, "algorithmic_bytes": 0.0}
+    for a in range(0, len(soils), chunk):
+        spectra, snapped, res = fd.GenerateBanks(shots_for(soils[a:a + chunk], seconds), bank_spec())
+        bad = [r for r in res if isinstance(r, fd.FdtdError)]
+        if bad:
+            raise bad[0]
+        parts.append(spectra.astype(np.complex64))
+        snaps.append(np.asarray(snapped))
+        st = fd.BatchStats()
+        for k in stats:
+            stats[k] += st[k]
+        if progress:
+            progress(min(a + chunk, len(soils)), len(soils), time.perf_counter() - t0)
     wall = time.perf_counter() - t0
-    bad = [r for r in res if isinstance(r, fd.FdtdError)]
-    if bad:
-        raise bad[0]
-    S, R = spectra.shape[0], spectra.shape[1]
-    return spectra.astype(np.complex64), np.asarray(snapped).reshape(S, R), wall, fd.BatchStats()
+    spectra = np.concatenate(parts)
+    return spectra, np.concatenate(snaps).reshape(spectra.shape[0], spectra.shape[1]), wall, stats
 
 
-def solve_set(ss: SoilSet, seconds: float = SECONDS) -> SoilSet:
-    ss.spectra, ss.ranges, ss.fdtd_seconds, ss.stats = solve(ss.soils, seconds)
+def solve_set(ss: SoilSet, seconds: float = SECONDS, progress=None) -> SoilSet:
+    ss.spectra, ss.ranges, ss.fdtd_seconds, ss.stats = solve(ss.soils, seconds, progress=progress)
     return ss
 
 

@@ -326,9 +326,10 @@ int gf_soil_sample_layered(uint64_t seed, uint32_t index, gf_soil *out, double p
  * per query: 512 complex bins, either in the normalized domain the network
  * is trained in (gf_sur_forward) or as a physical spectrum (gf_sur_predict:
  * times scale[q] and e^{-i 2 pi f delay[q]}, bin k at k * bin_hz).
- * Network: an encoder of six 512-wide GELU layers and a decoder of four
- * transposed convolutions (kernel = stride) from 16 positions x 32 channels
- * to 512 positions x 2; every product on BF16 MFMA with FP32 accumulation;
+ * Network: an encoder of six 512-wide GELU layers; a decoder of a dense
+ * projection to 64 positions x 32 channels and two transposed convolutions
+ * (kernel = stride) to 512 positions x 2; every product on BF16 MFMA with
+ * FP32 accumulation;
  * FP32 master weights and Adam.  The loss is the relative spectral L2,
  * mean over queries of |o - t|^2 / |t|^2. */
 typedef struct gf_sur gf_sur;

@@ -175,27 +175,33 @@ def test_training_reduces_the_loss(S):
 
 
 def test_trained_network_gains_over_fdtd_only(S):
-    """On FDTD-generated spectra of the layered family: the trained network's
-    held-out error is a fraction of the untrained one's, and answering a query
-    costs a small fraction of solving it with the batched FDTD."""
-    train = S.solve_set(S.soil_set(0, 0, 48), seconds=1.0)
-    test = S.solve_set(S.soil_set(0, 10_000, 8), seconds=1.0)
+    """On FDTD-generated spectra of the layered family, against the method
+    without the network -- the analytic prior alone (the Boussinesq amplitude
+    1/(2 pi mu1 r) with the top layer's Rayleigh delay) -- and against the
+    batched FDTD's cost: the trained network's held-out error is at most half
+    the prior's, and answering a query costs under 1 % of solving it."""
+    train = S.solve_set(S.soil_set(0, 0, 128), seconds=1.0)
+    test = S.solve_set(S.soil_set(0, 10_000, 16), seconds=1.0)
     x, t, _, _, _ = S.dataset(train)
     xt, tt, sc, dl, Gt = S.dataset(test)
-    n = S.Network(max_batch=2400, seed=5)
+    f = np.arange(S.BINS) * S.BIN_HZ
+    prior_only = S.band_limit(sc[:, None] * np.exp(-2j * np.pi * f[None, :] * dl[:, None]))
+    n = S.Network(max_batch=4096, seed=5)
     try:
-        before = S.rel_l2(n.Predict(xt, sc, dl), Gt)
-        losses = n.Train(x, t, steps=600, batch=800, lr=1e-3, lr_min=3e-5, seed=2)
+        untrained = S.rel_l2(n.Predict(xt, sc, dl), Gt)
+        losses = n.Train(x, t, steps=2000, batch=2048, lr=1e-3, lr_min=2e-5, seed=2)
         after = S.rel_l2(n.Predict(xt, sc, dl), Gt)
-        n.Predict(np.repeat(xt, 6, axis=0), np.repeat(sc, 6), np.repeat(dl, 6))
+        n.Predict(np.repeat(xt, 8, axis=0), np.repeat(sc, 8), np.repeat(dl, 8))
         st = n.Stats()
     finally:
         n.close()
+    base = S.rel_l2(prior_only, Gt)
     fdtd_per_trace = train.fdtd_seconds / t.shape[0]
     sur_per_trace = st["forward_ms"] / 1e3 / st["forward_queries"]
-    print(f"held-out rel-L2 median {np.median(before):.3f} -> {np.median(after):.3f} "
-          f"(p90 {np.quantile(after, 0.9):.3f}); train loss {losses[:20].mean():.3f} -> {losses[-20:].mean():.4f}; "
-          f"FDTD {fdtd_per_trace * 1e3:.2f} ms/trace, surrogate {sur_per_trace * 1e6:.2f} us/trace "
-          f"({fdtd_per_trace / sur_per_trace:.0f}x)", flush=True)
-    assert np.median(after) < 0.5 * np.median(before)
+    print(f"held-out rel-L2 median: untrained {np.median(untrained):.3f}, prior only {np.median(base):.3f}, "
+          f"trained {np.median(after):.3f} (p90 {np.quantile(after, 0.9):.3f}); train loss "
+          f"{losses[:20].mean():.3f} -> {losses[-20:].mean():.4f}; FDTD {fdtd_per_trace * 1e3:.2f} ms/trace, "
+          f"surrogate {sur_per_trace * 1e6:.3f} us/trace ({fdtd_per_trace / sur_per_trace:.0f}x)", flush=True)
+    assert np.median(after) <= 0.5 * np.median(base)
+    assert np.median(after) < np.median(untrained)
     assert fdtd_per_trace / sur_per_trace >= 100

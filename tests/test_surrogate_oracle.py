@@ -14,7 +14,7 @@ import surrogate_oracle as O
 # The network's layer table as the library defines it (checked against a live
 # handle by the GPU tests): (k_in, n_out, gelu, rows per query, bias channels).
 ARCH = [(16, 512, 1, 1, 512)] + [(512, 512, 1, 1, 512)] * 5 + [
-    (32, 64, 1, 16, 32), (32, 64, 1, 32, 32), (32, 32, 1, 64, 16), (16, 8, 0, 128, 2)]
+    (512, 2048, 1, 1, 2048), (32, 64, 1, 64, 32), (32, 8, 0, 128, 2)]
 
 
 def layers(arch=ARCH):
@@ -47,7 +47,7 @@ def random_params(L, P, rng, scale=0.5):
 
 def test_full_architecture_produces_512_complex_bins():
     L, P = layers()
-    assert P == 1327320
+    assert P == 2374952
     rng = np.random.default_rng(0)
     p = random_params(L, P, rng)
     o, _ = O.forward(p, L, rng.uniform(-1, 1, (3, 11)))
