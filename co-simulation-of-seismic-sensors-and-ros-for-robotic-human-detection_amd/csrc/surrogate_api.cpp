@@ -3,7 +3,7 @@
 // Network (all GEMMs on the BF16 MFMA kernel of surrogate.hip, FP32
 // accumulation, FP32 master weights and Adam state):
 //
-//   encoder  x[11] -> 512 -> 512 -> 512 -> 512 -> 512 -> 512, GELU after each
+//   encoder  x[15] -> 512 -> 512 -> 512 -> 512 -> 512 -> 512, GELU after each
 //   decoder  a dense projection 512 -> 2048, GELU, read as 64 positions x
 //            32 channels (one position per 8 bins), then transposed
 //            convolutions whose kernel equals their stride:
@@ -17,8 +17,8 @@
 // [rows*s][C_out] input.  No layout pass runs between layers.
 //
 // Weights are stored [N][K] (out x in); a transposed convolution's bias is
-// one per output channel (shared by its s taps).  Inputs: 11 features padded
-// to 16 (the GEMM's K granularity) with zero columns, whose weights stay zero.
+// one per output channel (shared by its s taps).  Inputs: 15 features padded
+// to 16 (the GEMM's K granularity) with a zero column, whose weights stay zero.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <string.h>
@@ -43,7 +43,7 @@ int api_set_err(int code, const std::string &msg);
 
 namespace {
 
-constexpr int kFeatures = 11;
+constexpr int kFeatures = 15;
 constexpr int kInPad = 16;
 constexpr int kOut = 1024;  // 512 bins x (re, im)
 

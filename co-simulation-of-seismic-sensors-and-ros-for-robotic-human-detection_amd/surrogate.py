@@ -30,7 +30,7 @@ import numpy as np
 
 from . import fdtd as fd
 
-FEATURES = 11
+FEATURES = 15
 BINS = 512
 SAMPLE_RATE_HZ = 500.0
 SAMPLES = 1022                       # 512 bins, 0 .. 250 Hz
@@ -60,13 +60,21 @@ def bank_spec() -> fd.BankSpec:
     return fd.BankSpec(SAMPLE_RATE_HZ, SAMPLES, MAX_FREQ_HZ, RANGES[0], RANGES[1], RANGES[2])
 
 
+TRAVEL_MAX_S = RANGES[1] / PARAM_RANGES[0, 0]   # longest r / Vs in the family
+
+
 def features(params: np.ndarray, r: np.ndarray) -> np.ndarray:
-    """[n][11] float32: the 10 soil numbers and log r, each mapped to [-1, 1]."""
+    """[n][15] float32, each mapped to about [-1, 1]: the 10 soil numbers,
+    log r, r, and the shear travel times r / Vs1, r / Vs2, r / Vs3 (the phase
+    of the spectrum is close to linear in these)."""
     p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    r = np.asarray(r, dtype=np.float64)
     lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
     x = np.empty((p.shape[0], FEATURES))
     x[:, :10] = 2 * (p - lo) / (hi - lo) - 1
-    x[:, 10] = 2 * np.log(np.asarray(r, dtype=np.float64) / RANGES[0]) / math.log(RANGES[1] / RANGES[0]) - 1
+    x[:, 10] = 2 * np.log(r / RANGES[0]) / math.log(RANGES[1] / RANGES[0]) - 1
+    x[:, 11] = 2 * r / RANGES[1] - 1
+    x[:, 12:15] = 2 * (r[:, None] / p[:, 0:3]) / TRAVEL_MAX_S - 1
     return x.astype(np.float32)
 
 
@@ -279,7 +287,7 @@ def solve_set(ss: SoilSet, seconds: float = SECONDS, progress=None) -> SoilSet:
 
 
 def dataset(ss: SoilSet):
-    """Flattened (x [n][11], targets [n][1024], scale, delay, G [n][512]) over
+    """Flattened (x [n][15], targets [n][1024], scale, delay, G [n][512]) over
     every (soil, range) of a solved set; G band-limited to BAND_HZ."""
     S, R = ss.ranges.shape
     p = np.repeat(ss.params, R, axis=0)

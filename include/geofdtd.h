@@ -321,8 +321,9 @@ int gf_soil_sample_layered(uint64_t seed, uint32_t index, gf_soil *out, double p
 /* ---- the Green's-function surrogate -----------------------------------
  * No reference counterpart: a network standing in for the bank lookup,
  * trained on spectra the batched FDTD generates, with the FDTD as its
- * fallback.  Inputs per query: 11 features (the 10 numbers of a layered soil
- * and the log range, each scaled to about [-1, 1] by the caller).  Output
+ * fallback.  Inputs per query: 15 features (the 10 numbers of a layered soil,
+ * the log range, the range, and the shear travel times r / Vs of the three
+ * layers, each scaled to about [-1, 1] by the caller).  Output
  * per query: 512 complex bins, either in the normalized domain the network
  * is trained in (gf_sur_forward) or as a physical spectrum (gf_sur_predict:
  * times scale[q] and e^{-i 2 pi f delay[q]}, bin k at k * bin_hz).
@@ -376,9 +377,9 @@ int gf_sur_param_count(const gf_sur *h, int64_t *n);
 int gf_sur_get_params(gf_sur *h, float *out);
 /* Replace the parameters (and reset the Adam state). */
 int gf_sur_set_params(gf_sur *h, const float *in);
-/* x [n][11] -> out [n][1024], the normalized output. */
+/* x [n][15] -> out [n][1024], the normalized output. */
 int gf_sur_forward(gf_sur *h, const float *x, int32_t n, float *out);
-/* x [n][11] -> out [n][512][2], physical spectra. */
+/* x [n][15] -> out [n][512][2], physical spectra. */
 int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double *delay, double bin_hz, int32_t n,
                    float *out);
 /* Mean loss over n queries (a multiple of 8, at most max_batch) against

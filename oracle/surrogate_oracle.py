@@ -27,7 +27,7 @@ import numpy as np
 GELU_C = np.sqrt(2.0 / np.pi)
 GELU_A = 0.044715
 IN_PAD = 16
-FEATURES = 11
+FEATURES = 15
 
 
 @dataclass
@@ -77,7 +77,7 @@ def _b(params, L: Layer):
 
 
 def forward(params: np.ndarray, layers: Sequence[Layer], x: np.ndarray, bf: bool = False):
-    """(output [B][1024], cache) for features x [B][11]."""
+    """(output [B][1024], cache) for features x [B][15]."""
     p = np.asarray(params, dtype=np.float64)
     B = x.shape[0]
     a = np.zeros((B, IN_PAD))
@@ -99,7 +99,7 @@ def loss(o: np.ndarray, t: np.ndarray) -> np.ndarray:
 
 
 def loss_grad(params, layers, x, t, bf: bool = False):
-    """(mean loss, flat gradient) of the batch x [B][11] against t [B][1024]."""
+    """(mean loss, flat gradient) of the batch x [B][15] against t [B][1024]."""
     p = np.asarray(params, dtype=np.float64)
     o, (acts, zs) = forward(p, layers, x, bf)
     B = x.shape[0]

@@ -36,10 +36,10 @@ def small_arch():
 def random_params(L, P, rng, scale=0.5):
     p = np.zeros(P)
     for l in L:
-        fan = 11 if l is L[0] else l.k
+        fan = 15 if l is L[0] else l.k
         w = rng.normal(0, scale / math.sqrt(fan), (l.n, l.k))
         if l is L[0]:
-            w[:, 11:] = 0
+            w[:, 15:] = 0
         p[l.w_off: l.w_off + l.n * l.k] = w.ravel()
         p[l.b_off: l.b_off + l.bias_mod] = rng.normal(0, 0.1, l.bias_mod)
     return p
@@ -50,7 +50,7 @@ def test_full_architecture_produces_512_complex_bins():
     assert P == 2374952
     rng = np.random.default_rng(0)
     p = random_params(L, P, rng)
-    o, _ = O.forward(p, L, rng.uniform(-1, 1, (3, 11)))
+    o, _ = O.forward(p, L, rng.uniform(-1, 1, (3, 15)))
     assert o.shape == (3, 1024)
     # the decoder's row chaining: each layer's output rows x columns equal
     # the next layer's input rows x k
@@ -65,7 +65,7 @@ def test_gradient_matches_central_differences():
     # the small decoder ends at 16*16/2 = 128 complex bins
     rng = np.random.default_rng(1)
     p = random_params(L, P, rng, scale=1.0)
-    x = rng.uniform(-1, 1, (3, 11))
+    x = rng.uniform(-1, 1, (3, 15))
     out_len = L[-1].rows * L[-1].n
     t = rng.normal(0, 1, (3, out_len))
     f0, g = O.loss_grad(p, L, x, t)
@@ -73,7 +73,7 @@ def test_gradient_matches_central_differences():
     for l in L:
         idx = list(rng.choice(l.n * l.k, size=6, replace=False) + l.w_off) + [l.b_off, l.b_off + l.bias_mod - 1]
         for i in idx:
-            if l is L[0] and (i - l.w_off) % l.k >= 11 and i < l.b_off:
+            if l is L[0] and (i - l.w_off) % l.k >= 15 and i < l.b_off:
                 continue  # padded inputs: zero gradient by construction
             pp, pm = p.copy(), p.copy()
             pp[i] += h
@@ -123,7 +123,11 @@ def test_normalization_keeps_the_relative_l2():
 def test_features_map_the_sampling_box_to_unit_interval():
     lo, hi = S.PARAM_RANGES[:, 0], S.PARAM_RANGES[:, 1]
     x = S.features(np.vstack([lo, hi]), np.array([1.0, 50.0]))
-    np.testing.assert_allclose(x, [[-1] * 11, [1] * 11], atol=1e-6)
+    np.testing.assert_allclose(x[:, :11], [[-1] * 11, [1] * 11], atol=1e-6)
+    np.testing.assert_allclose(x[:, 11], [2 / 50 - 1, 1], atol=1e-6)  # r itself
+    # travel times r / Vs: the longest (50 m at the slowest Vs1) maps to 1
+    np.testing.assert_allclose(x[1, 12], 2 * (50 / 200) / (50 / 120) - 1, atol=1e-6)
+    assert np.all(np.abs(x) <= 1 + 1e-6)
 
 
 def test_layered_family_is_deterministic_and_inside_its_box():
