@@ -27,8 +27,8 @@ namespace gfsk {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBK = 32;        // K per LDS stage (two 32x32x16 MFMA steps)
-constexpr int kLDK = kBK + 8;  // padded LDS row, in BF16 elements (80 bytes)
+constexpr int kBK = 64;        // K per LDS stage (four 32x32x16 MFMA steps)
+constexpr int kLDK = kBK + 8;  // padded LDS row, in BF16 elements (144 bytes)
 
 __device__ __forceinline__ uint16_t to_bf16(float x) {
   // round to nearest even; NaN stays NaN
@@ -58,29 +58,58 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGc * (1.0f + 3.0f * kGa * z * z);
 }
 
-// One ROWS x kBK tile into LDS as dst[row][k].  TRANS = false: element (r, k)
-// at src[r*ld + k] (k contiguous; 16-byte loads copied as they are).  TRANS =
-// true: element (r, k) at src[k*ld + r] (r contiguous; 16-byte loads along r,
-// scattered into 8 LDS rows).  Rows at or beyond R and k at or beyond kend read
-// as zero; the host guarantees that a 16-byte vector is wholly in or out
-// (K, and R for TRANS, multiples of 8, and 16-byte aligned rows).
+// One ROWS x kBK tile, moved in two halves so that the global loads of the
+// next stage are in flight while the MFMAs of this one run: fetch_tile reads
+// it into registers (16-byte vectors), store_tile writes them to LDS as
+// dst[row][k].  TRANS = false: element (r, k) at src[r*ld + k] (k
+// contiguous; vectors copied as they are).  TRANS = true: element (r, k) at
+// src[k*ld + r] (r contiguous; vectors along r, scattered into 8 LDS rows).
+// Rows at or beyond R and k at or beyond kend read as zero; the host
+// guarantees that a vector is wholly in or out (K, and R for TRANS, multiples
+// of 8, and 16-byte aligned rows).
+template <int ROWS>
+struct TileRegs {
+  static constexpr int kVecs = ROWS * kBK / 8;
+  static constexpr int kPer = (kVecs + 255) / 256;
+  uint4 v[kPer];
+};
+
 template <int ROWS, bool TRANS>
-__device__ __forceinline__ void load_tile(uint16_t *dst, const uint16_t *__restrict__ src, int64_t ld, int r0,
-                                          int k0, int R, int kend) {
-  constexpr int kVecs = ROWS * kBK / 8;
-  static_assert(kVecs % 256 == 0 || kVecs < 256, "tile size");
-  for (int v = threadIdx.x; v < kVecs; v += 256) {
+__device__ __forceinline__ void fetch_tile(TileRegs<ROWS> &t, const uint16_t *__restrict__ src, int64_t ld, int r0,
+                                           int k0, int R, int kend) {
+  constexpr int kVecs = TileRegs<ROWS>::kVecs;
+#pragma unroll
+  for (int i = 0; i < TileRegs<ROWS>::kPer; i++) {
+    const int v = threadIdx.x + 256 * i;
     uint4 val = make_uint4(0, 0, 0, 0);
+    if (v < kVecs) {
+      if (!TRANS) {
+        const int row = v / (kBK / 8), kv = v % (kBK / 8);
+        const int gr = r0 + row, gk = k0 + kv * 8;
+        if (gr < R && gk < kend) val = *(const uint4 *)(src + (int64_t)gr * ld + gk);
+      } else {
+        const int k = v / (ROWS / 8), rv = v % (ROWS / 8);
+        const int gk = k0 + k, gr = r0 + rv * 8;
+        if (gk < kend && gr < R) val = *(const uint4 *)(src + (int64_t)gk * ld + gr);
+      }
+    }
+    t.v[i] = val;
+  }
+}
+
+template <int ROWS, bool TRANS>
+__device__ __forceinline__ void store_tile(const TileRegs<ROWS> &t, uint16_t *dst) {
+  constexpr int kVecs = TileRegs<ROWS>::kVecs;
+#pragma unroll
+  for (int i = 0; i < TileRegs<ROWS>::kPer; i++) {
+    const int v = threadIdx.x + 256 * i;
+    if (v >= kVecs) break;
     if (!TRANS) {
       const int row = v / (kBK / 8), kv = v % (kBK / 8);
-      const int gr = r0 + row, gk = k0 + kv * 8;
-      if (gr < R && gk < kend) val = *(const uint4 *)(src + (int64_t)gr * ld + gk);
-      *(uint4 *)(dst + row * kLDK + kv * 8) = val;
+      *(uint4 *)(dst + row * kLDK + kv * 8) = t.v[i];
     } else {
       const int k = v / (ROWS / 8), rv = v % (ROWS / 8);
-      const int gk = k0 + k, gr = r0 + rv * 8;
-      if (gk < kend && gr < R) val = *(const uint4 *)(src + (int64_t)gk * ld + gr);
-      const uint16_t *e = (const uint16_t *)&val;
+      const uint16_t *e = (const uint16_t *)&t.v[i];
 #pragma unroll
       for (int j = 0; j < 8; j++) dst[(rv * 8 + j) * kLDK + k] = e[j];
     }
@@ -107,10 +136,22 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
 
+  // software pipeline: the next stage's global loads are issued before this
+  // stage's MFMAs, and land in LDS after them
+  TileRegs<BM> ra;
+  TileRegs<BN> rb;
+  if (kbeg < kend) {
+    fetch_tile<BM, AT>(ra, g.A, g.lda, m0, kbeg, g.M, kend);
+    fetch_tile<BN, !BT>(rb, g.B, g.ldb, n0, kbeg, g.N, kend);
+  }
   for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-    load_tile<BM, AT>(As, g.A, g.lda, m0, k0, g.M, kend);
-    load_tile<BN, !BT>(Bs, g.B, g.ldb, n0, k0, g.N, kend);
+    store_tile<BM, AT>(ra, As);
+    store_tile<BN, !BT>(rb, Bs);
     __syncthreads();
+    if (k0 + kBK < kend) {
+      fetch_tile<BM, AT>(ra, g.A, g.lda, m0, k0 + kBK, g.M, kend);
+      fetch_tile<BN, !BT>(rb, g.B, g.ldb, n0, k0 + kBK, g.N, kend);
+    }
 #pragma unroll
     for (int s = 0; s < kBK / 16; s++) {
       bf16x8 fa[TM], fb[TN];
@@ -161,11 +202,14 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 template <bool AT, bool BT, int EPI>
 hipError_t launch_shape(const GemmArgs &g, int splits, hipStream_t st) {
   const dim3 block(256);
-  if (g.N > 64) {
+  // 128 x 128 tiles when there are enough of them for two workgroups per CU;
+  // otherwise 128 x 64, which doubles the workgroups a CU can interleave
+  const int64_t big_tiles = (int64_t)((g.N + 127) / 128) * ((g.M + 127) / 128) * splits;
+  if (g.N > 64 && big_tiles >= 512) {
     const dim3 grid((g.N + 127) / 128, (g.M + 127) / 128, splits);
     hipLaunchKernelGGL((k_gemm<128, 128, 2, 2, AT, BT, EPI>), grid, block, 0, st, g);
   } else if (g.N > 32) {
-    const dim3 grid((g.N + 63) / 64, (g.M + 127) / 128, splits);
+    const dim3 grid((g.N + 63) / 64, (g.M + 127) / 128, splits);  // also N > 64 with few tiles
     hipLaunchKernelGGL((k_gemm<128, 64, 4, 1, AT, BT, EPI>), grid, block, 0, st, g);
   } else {
     const dim3 grid((g.N + 31) / 32, (g.M + 127) / 128, splits);
@@ -261,15 +305,39 @@ __global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, con
   }
 }
 
-// Column sums of x[rows][cols] over row chunks: part[chunk][col].
-__global__ void k_colsum_part(const float *__restrict__ x, int64_t rows, int32_t cols, int64_t chunk,
-                              float *__restrict__ part) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
-  float s = 0;
-  for (int64_t r = r0; r < r1; r++) s += x[r * cols + c];
-  part[(int64_t)blockIdx.y * cols + c] = s;
+// Column sums of x[rows][cols] over chunks of `chunk` rows, one workgroup
+// per chunk: part[chunk][col].  For cols <= 256 (a divisor of 256) the 256
+// lanes take (column, row phase) pairs so that reads stay coalesced, then
+// fold the phases in LDS; wider rows loop over columns.  The chunk length
+// keeps about 256 chunks in flight, between 16 and 1024 rows each.
+__host__ __device__ inline int64_t colsum_chunk_rows(int64_t rows) {
+  int64_t c = 16;
+  while (c < 1024 && c * 256 < rows) c *= 2;
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ x, int64_t rows, int32_t cols,
+                                                     int64_t chunk, float *__restrict__ part) {
+  const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
+  if (cols <= 256) {
+    const int phases = 256 / cols, c = threadIdx.x % cols, ph = threadIdx.x / cols;
+    float s = 0;
+    if (ph < phases)
+      for (int64_t r = r0 + ph; r < r1; r += phases) s += x[r * cols + c];
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (ph == 0) {
+      for (int p = 1; p < phases; p++) s += red[p * cols + c];
+      part[(int64_t)blockIdx.x * cols + c] = s;
+    }
+  } else {
+    for (int c = threadIdx.x; c < cols; c += 256) {
+      float s = 0;
+      for (int64_t r = r0; r < r1; r++) s += x[r * cols + c];
+      part[(int64_t)blockIdx.x * cols + c] = s;
+    }
+  }
 }
 
 // out[j] = sum over chunks and over columns c with c % mod == j of part.
@@ -334,6 +402,17 @@ __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const d
   }
 }
 
+// out[k][n] = in[n][k] for one layer's BF16 weights (16x16 tiles via LDS).
+__global__ __launch_bounds__(256) void k_transpose_bf16(const uint16_t *__restrict__ in, int32_t n_rows,
+                                                        int32_t k_cols, uint16_t *__restrict__ out) {
+  __shared__ uint16_t t[16][17];
+  const int n0 = blockIdx.y * 16, k0 = blockIdx.x * 16;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  if (n0 + ty < n_rows && k0 + tx < k_cols) t[ty][tx] = in[(int64_t)(n0 + ty) * k_cols + k0 + tx];
+  __syncthreads();
+  if (k0 + ty < k_cols && n0 + tx < n_rows) out[(int64_t)(k0 + ty) * n_rows + n0 + tx] = t[tx][ty];
+}
+
 namespace {
 inline dim3 grid_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
@@ -368,10 +447,16 @@ hipError_t launch_rel_l2(const float *o, const float *t, int32_t rows, int32_t l
   return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, int32_t chunks,
-                            float *out, hipStream_t st) {
-  const int64_t chunk = (rows + chunks - 1) / chunks;
-  hipLaunchKernelGGL(k_colsum_part, dim3((cols + 255) / 256, chunks), dim3(256), 0, st, x, rows, cols, chunk, part);
+int32_t bias_grad_chunks(int64_t rows) {
+  const int64_t c = colsum_chunk_rows(rows);
+  return (int32_t)((rows + c - 1) / c);
+}
+
+hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, float *out,
+                            hipStream_t st) {
+  if (cols <= 256 && 256 % cols != 0) return hipErrorInvalidValue;
+  const int32_t chunks = bias_grad_chunks(rows);
+  hipLaunchKernelGGL(k_colsum_part, dim3(chunks), dim3(256), 0, st, x, rows, cols, colsum_chunk_rows(rows), part);
   hipLaunchKernelGGL(k_colsum_fold, dim3((mod + 255) / 256), dim3(256), 0, st, part, chunks, cols, mod, out);
   return hipGetLastError();
 }
@@ -384,6 +469,13 @@ hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t st) {
   hipLaunchKernelGGL(k_adam, grid_for(n), dim3(256), 0, st, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_cols, uint16_t *out,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_transpose_bf16, dim3((k_cols + 15) / 16, (n_rows + 15) / 16), dim3(256), 0, st, in, n_rows,
+                     k_cols, out);
   return hipGetLastError();
 }
 

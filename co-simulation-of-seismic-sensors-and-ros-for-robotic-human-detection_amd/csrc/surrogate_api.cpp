@@ -112,6 +112,7 @@ struct gf_sur {
   // parameters and optimizer state
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;
+  uint16_t *pbt = nullptr;  // each layer's BF16 weights transposed, [K][N]
   int64_t adam_t = 0;
   // activations: X[l] is the BF16 input of layer l (X[L] the last GELU output
   // is not needed); Z[l] the FP32 pre-activation (the output for the last)
@@ -122,7 +123,6 @@ struct gf_sur {
   float *split = nullptr;
   int64_t split_floats = 0;
   float *colpart = nullptr;
-  int colpart_cols = 0;
   float *feat = nullptr, *tgt = nullptr, *loss = nullptr;
   double *scale = nullptr, *delay = nullptr;
   float *outp = nullptr;
@@ -130,7 +130,7 @@ struct gf_sur {
 
   ~gf_sur() {
     hipStreamSynchronize(st);
-    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)dF[0], (void *)dF[1],
+    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)dF[0], (void *)dF[1],
                     (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)feat, (void *)tgt,
                     (void *)loss, (void *)scale, (void *)delay, (void *)outp})
       if (q) hipFree(q);
@@ -157,6 +157,7 @@ struct gf_sur {
     HIPCHK(dalloc(&m, P));
     HIPCHK(dalloc(&v, P));
     HIPCHK(dalloc(&pb, P));
+    HIPCHK(dalloc(&pbt, P));
     const int64_t B = max_batch;
     X.assign(L.size(), nullptr);
     Z.assign(L.size(), nullptr);
@@ -172,8 +173,13 @@ struct gf_sur {
       HIPCHK(dalloc(&dB[i], B * widest));
     }
     HIPCHK(dalloc(&split, split_floats));
-    for (const Layer &y : L) colpart_cols = std::max(colpart_cols, y.n);
-    HIPCHK(dalloc(&colpart, (int64_t)kColChunks * colpart_cols));
+    // chunk counts are at most max(256, rows / 1024) for any row count up to
+    // the largest batch (the chunk length grows with the rows)
+    int64_t colpart_floats = 1;
+    for (const Layer &y : L)
+      colpart_floats = std::max<int64_t>(colpart_floats,
+                                         std::max<int64_t>(256, (B * y.rows + 1023) / 1024) * (int64_t)y.n);
+    HIPCHK(dalloc(&colpart, colpart_floats));
     HIPCHK(dalloc(&feat, B * kFeatures));
     HIPCHK(dalloc(&tgt, B * kOut));
     HIPCHK(dalloc(&loss, B));
@@ -182,8 +188,6 @@ struct gf_sur {
     HIPCHK(dalloc(&outp, B * kOut));
     return initialise(c.seed);
   }
-
-  static constexpr int kColChunks = 64;
 
   // K splits of a weight-gradient GEMM over `rows` = B * L[l].rows: enough
   // workgroups to cover the CUs, at least 256 rows per split.
@@ -215,8 +219,16 @@ struct gf_sur {
     HIPCHK(hipMemcpyAsync(pb, b.data(), sizeof(uint16_t) * P, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(m, 0, sizeof(float) * P, st));
     HIPCHK(hipMemsetAsync(v, 0, sizeof(float) * P, st));
+    int rc = transpose_weights();
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(st));
     adam_t = 0;
+    return GF_OK;
+  }
+
+  // pbt[layer] = pb[layer]^T, the B operand of the input-gradient GEMMs
+  int transpose_weights() {
+    for (const Layer &y : L) HIPCHK(gfsk::launch_transpose_bf16(pb + y.w_off, y.n, y.k, pbt + y.w_off, st));
     return GF_OK;
   }
 
@@ -276,7 +288,7 @@ struct gf_sur {
       w.split_stride = (int64_t)y.n * y.k;
       HIPCHK(gfsk::launch_gemm(w, true, false, gfsk::kEpiF32, splits, st));
       HIPCHK(gfsk::launch_sum_splits(split, splits, (int64_t)y.n * y.k, g + y.w_off, st));
-      HIPCHK(gfsk::launch_bias_grad(dF[cur], rows, y.n, y.bias_mod, colpart, kColChunks, g + y.b_off, st));
+      HIPCHK(gfsk::launch_bias_grad(dF[cur], rows, y.n, y.bias_mod, colpart, g + y.b_off, st));
       if (l == 0) break;
       // dX = dZ W, times GELU'(Z[l-1]) of the layer below
       gfsk::GemmArgs x;
@@ -285,15 +297,15 @@ struct gf_sur {
       x.K = y.n;
       x.A = dB[cur];
       x.lda = y.n;
-      x.B = pb + y.w_off;
-      x.ldb = y.k;
+      x.B = pbt + y.w_off;  // B(k=n_out, n=k_in) = W[n_out][k_in] = W^T[k_in][n_out]: k contiguous
+      x.ldb = y.n;
       x.z = Z[l - 1];
       x.ldz = y.k;
       x.cf = dF[1 - cur];
       x.ldc = y.k;
       x.cb = dB[1 - cur];
       x.ldcb = y.k;
-      HIPCHK(gfsk::launch_gemm(x, false, false, gfsk::kEpiGeluBwd, 1, st));
+      HIPCHK(gfsk::launch_gemm(x, false, true, gfsk::kEpiGeluBwd, 1, st));
       cur = 1 - cur;
     }
     return GF_OK;
@@ -531,6 +543,10 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_
     if ((e = gfsk::launch_adam(h->p, h->g, h->m, h->v, h->pb, h->P, (float)lr, (float)b1, (float)b2, (float)eps,
                                (float)c1, (float)c2, h->st)))
       return fail(e, "adam");
+    if ((rc = h->transpose_weights())) {
+      cleanup();
+      return rc;
+    }
     if ((e = hipEventRecord(h->e1, h->st))) return fail(e, "event");
     if ((e = hipEventSynchronize(h->e1))) return fail(e, "train step");
     float ms = 0;
