@@ -340,15 +340,23 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ x
   }
 }
 
-// out[j] = sum over chunks and over columns c with c % mod == j of part.
-__global__ void k_colsum_fold(const float *__restrict__ part, int32_t chunks, int32_t cols, int32_t mod,
-                              float *__restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= mod) return;
+// out[j] = sum over chunks and over columns c with c % mod == j of part: one
+// workgroup per channel j, its lanes striding over the (chunk, tap) entries,
+// then an LDS tree.
+__global__ __launch_bounds__(256) void k_colsum_fold(const float *__restrict__ part, int32_t chunks, int32_t cols,
+                                                     int32_t mod, float *__restrict__ out) {
+  const int j = blockIdx.x;
+  const int taps = cols / mod;
   float s = 0;
-  for (int k = 0; k < chunks; k++)
-    for (int c = j; c < cols; c += mod) s += part[(int64_t)k * cols + c];
-  out[j] = s;
+  for (int e = threadIdx.x; e < chunks * taps; e += 256) s += part[(int64_t)(e / taps) * cols + j + (e % taps) * mod];
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
 }
 
 // out[i] = sum_s part[s][i]: the K splits of a weight gradient.
@@ -457,7 +465,7 @@ hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t 
   if (cols <= 256 && 256 % cols != 0) return hipErrorInvalidValue;
   const int32_t chunks = bias_grad_chunks(rows);
   hipLaunchKernelGGL(k_colsum_part, dim3(chunks), dim3(256), 0, st, x, rows, cols, colsum_chunk_rows(rows), part);
-  hipLaunchKernelGGL(k_colsum_fold, dim3((mod + 255) / 256), dim3(256), 0, st, part, chunks, cols, mod, out);
+  hipLaunchKernelGGL(k_colsum_fold, dim3(mod), dim3(256), 0, st, part, chunks, cols, mod, out);
   return hipGetLastError();
 }
 
