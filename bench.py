@@ -228,6 +228,8 @@ def bench_surrogate(args, fd) -> int:
 
     import cosim_amd.surrogate as S
 
+    if args.sur_speed_only:
+        return surrogate_speed(args, S)
     R = S.RANGES[2]
     n_held = -(-args.sur_queries // R)
     log(f"surrogate: FDTD banks for {args.sur_soils} training soils")
@@ -282,7 +284,7 @@ def bench_surrogate(args, fd) -> int:
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-            "kernel": "surrogate forward (10 BF16 MFMA GEMMs + output de-normalization)",
+            "kernel": "surrogate forward (9 BF16 MFMA GEMMs + output de-normalization)",
             "flops_per_query": tr["flops_per_query"],
         },
         "surrogate": {
@@ -303,6 +305,36 @@ def bench_surrogate(args, fd) -> int:
     }
     net.close()
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def surrogate_speed(args, S) -> int:
+    """The surrogate's kernels alone, on synthetic inputs and targets (their
+    speed does not depend on the weights): inference over --sur-queries at
+    batch 8192, then 50 training steps at batch 4096."""
+    import numpy as np
+
+    rng = np.random.default_rng(0)
+    net = S.Network(max_batch=8192, seed=1)
+    Q = args.sur_queries
+    x = rng.uniform(-1, 1, (Q, S.FEATURES)).astype(np.float32)
+    sc, dl = rng.uniform(1e-9, 1e-8, Q), rng.uniform(0, 0.4, Q)
+    for _ in range(3):
+        net.Predict(x, sc, dl)
+    inf = net.Stats()
+    rate = inf["forward_queries"] / (inf["forward_ms"] / 1e3)
+    t = rng.normal(0, 1, (16384, 2 * S.BINS)).astype(np.float32)
+    net.Train(x[:16384], t, steps=50, batch=4096, lr=1e-4, lr_min=1e-5, seed=1)
+    tr = net.Stats()
+    net.close()
+    achieved = inf["flops_per_query"] * rate / 1e12
+    print(json.dumps({
+        "probe": "surrogate kernels", "queries": Q, "inference_traces_per_s": rate,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "flops_per_query": inf["flops_per_query"]},
+        "train_ms_per_step": tr["train_ms"] / tr["train_steps"],
+        "train_tflops": 3 * inf["flops_per_query"] * tr["train_queries"] / (tr["train_ms"] / 1e3) / 1e12,
+    }), flush=True)
     return 0
 
 
@@ -379,6 +411,8 @@ def main() -> int:
     ap.add_argument("--sur-queries", type=int, default=65536, help="surrogate: held-out queries served")
     ap.add_argument("--sur-steps", type=int, default=10000, help="surrogate: Adam steps (batch 4096)")
     ap.add_argument("--sur-lr", type=float, default=1e-3, help="surrogate: peak learning rate")
+    ap.add_argument("--sur-speed-only", action="store_true",
+                    help="surrogate: time the kernels on synthetic inputs only (no FDTD data, no accuracy)")
     ap.add_argument("--c3-seconds", type=float, default=1.0, help="config 3: simulated seconds per model")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB (the HBM counter "
