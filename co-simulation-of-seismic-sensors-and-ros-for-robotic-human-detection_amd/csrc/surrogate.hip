@@ -268,20 +268,26 @@ __global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__re
   }
 }
 
-// Relative spectral L2 loss, one workgroup per sample:
-//   loss_b = |o_b - t_b|^2 / |t_b|^2,   d o_b = scale * 2 (o_b - t_b) / |t_b|^2
-// (scale = 1/B gives the gradient of the batch mean).  dF and dB receive the
-// gradient in FP32 and BF16.
-__global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, const float *__restrict__ t, int32_t len,
-                                                float scale, float *__restrict__ loss, float *__restrict__ dF,
+// The loss, one workgroup per sample (scale = 1/B gives the gradient of the
+// batch mean; dF and dB receive the gradient in FP32 and BF16):
+//   w == null: relative spectral L2,  loss_b = |o_b - t_b|^2 / |t_b|^2,
+//              d o_b = scale * 2 (o_b - t_b) / |t_b|^2
+//   w != null: weighted squares,      loss_b = sum_i w_i (o_i - t_i)^2,
+//              d o_i = scale * 2 w_i (o_i - t_i)
+// (with outputs (log-amplitude, phase) and w = |T|^2 / sum |T|^2 per bin, the
+// weighted form is the relative L2 to first order in the error).
+__global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, const float *__restrict__ t,
+                                                const float *__restrict__ w, int32_t len, float scale,
+                                                float *__restrict__ loss, float *__restrict__ dF,
                                                 uint16_t *__restrict__ dB) {
   const int b = blockIdx.x;
   const float *ob = o + (int64_t)b * len, *tb = t + (int64_t)b * len;
+  const float *wb = w ? w + (int64_t)b * len : nullptr;
   float num = 0, den = 0;
   for (int i = threadIdx.x; i < len; i += 256) {
     const float d = ob[i] - tb[i];
-    num += d * d;
-    den += tb[i] * tb[i];
+    num += wb ? wb[i] * d * d : d * d;
+    den += wb ? 0.0f : tb[i] * tb[i];
   }
   __shared__ float sn[256], sd[256];
   sn[threadIdx.x] = num;
@@ -294,11 +300,11 @@ __global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, con
     }
     __syncthreads();
   }
-  const float inv = sd[0] > 0 ? 1.0f / sd[0] : 0.0f;
+  const float inv = wb ? 1.0f : (sd[0] > 0 ? 1.0f / sd[0] : 0.0f);
   if (threadIdx.x == 0) loss[b] = sn[0] * inv;
   if (dF) {
     for (int i = threadIdx.x; i < len; i += 256) {
-      const float g = scale * 2.0f * (ob[i] - tb[i]) * inv;
+      const float g = scale * 2.0f * (ob[i] - tb[i]) * (wb ? wb[i] : inv);
       dF[(int64_t)b * len + i] = g;
       dB[(int64_t)b * len + i] = to_bf16(g);
     }
@@ -386,12 +392,13 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 }
 
 // Normalized network output to the physical spectrum, per query b and bin k:
-//   G = s_b (o_re + i o_im) e^{-i 2 pi f_k w_b},  f_k = k * df
-// (s_b the amplitude scale and w_b the delay the targets were divided by);
-// zero outside bins [k_lo, k_hi].
+//   G = s_b T e^{-i 2 pi f_k w_b},  f_k = k * df,
+// T = o_0 + i o_1 (cartesian) or e^{o_0 + i o_1} (polar: log-amplitude and
+// phase), s_b the amplitude scale and w_b the delay the targets were divided
+// by; zero outside bins [k_lo, k_hi].
 __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const double *__restrict__ scale,
                               const double *__restrict__ delay, double df, int32_t k_lo, int32_t k_hi,
-                              float *__restrict__ out, int32_t rows) {
+                              int32_t polar, float *__restrict__ out, int32_t rows) {
   const int64_t n = (int64_t)rows * bins;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = i / bins;
@@ -401,10 +408,15 @@ __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const d
       out[2 * i + 1] = 0.0f;
       continue;
     }
-    const double ph = -2.0 * 3.141592653589793 * (double)k * df * delay[b];
+    double ph = -2.0 * 3.141592653589793 * (double)k * df * delay[b];
+    double re = o[2 * i], im = o[2 * i + 1];
+    if (polar) {
+      ph += im;
+      re = exp(re);
+      im = 0.0;
+    }
     double sn, cs;
     sincos(ph, &sn, &cs);
-    const double re = o[2 * i], im = o[2 * i + 1];
     out[2 * i] = (float)(scale[b] * (re * cs - im * sn));
     out[2 * i + 1] = (float)(scale[b] * (re * sn + im * cs));
   }
@@ -448,10 +460,10 @@ hipError_t launch_gather_rows(const float *src, const int32_t *idx, int32_t cols
   return hipGetLastError();
 }
 
-hipError_t launch_rel_l2(const float *o, const float *t, int32_t rows, int32_t len, float scale, float *loss,
-                         float *dF, uint16_t *dB, hipStream_t st) {
+hipError_t launch_rel_l2(const float *o, const float *t, const float *w, int32_t rows, int32_t len, float scale,
+                         float *loss, float *dF, uint16_t *dB, hipStream_t st) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rel_l2, dim3(rows), dim3(256), 0, st, o, t, len, scale, loss, dF, dB);
+  hipLaunchKernelGGL(k_rel_l2, dim3(rows), dim3(256), 0, st, o, t, w, len, scale, loss, dF, dB);
   return hipGetLastError();
 }
 
@@ -488,10 +500,10 @@ hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_c
 }
 
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
-                              int32_t k_lo, int32_t k_hi, float *out, int32_t rows, hipStream_t st) {
+                              int32_t k_lo, int32_t k_hi, int32_t polar, float *out, int32_t rows, hipStream_t st) {
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_denormalize, grid_for((int64_t)rows * bins), dim3(256), 0, st, o, bins, scale, delay, df, k_lo,
-                     k_hi, out, rows);
+                     k_hi, polar, out, rows);
   return hipGetLastError();
 }
 

@@ -107,6 +107,7 @@ struct gf_sur {
   int64_t P = 0;
   int max_batch = 0;
   double band_lo = 0, band_hi = 0;
+  int polar = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // parameters and optimizer state
@@ -123,7 +124,7 @@ struct gf_sur {
   float *split = nullptr;
   int64_t split_floats = 0;
   float *colpart = nullptr;
-  float *feat = nullptr, *tgt = nullptr, *loss = nullptr;
+  float *feat = nullptr, *tgt = nullptr, *wgt = nullptr, *loss = nullptr;
   double *scale = nullptr, *delay = nullptr;
   float *outp = nullptr;
   gf_sur_stats_t stats{};
@@ -131,7 +132,7 @@ struct gf_sur {
   ~gf_sur() {
     hipStreamSynchronize(st);
     for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)dF[0], (void *)dF[1],
-                    (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)feat, (void *)tgt,
+                    (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)feat, (void *)tgt, (void *)wgt,
                     (void *)loss, (void *)scale, (void *)delay, (void *)outp})
       if (q) hipFree(q);
     for (auto q : X)
@@ -149,6 +150,7 @@ struct gf_sur {
     max_batch = c.max_batch;
     band_lo = c.band_lo_hz;
     band_hi = c.band_hi_hz;
+    polar = c.polar;
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -182,6 +184,7 @@ struct gf_sur {
     HIPCHK(dalloc(&colpart, colpart_floats));
     HIPCHK(dalloc(&feat, B * kFeatures));
     HIPCHK(dalloc(&tgt, B * kOut));
+    HIPCHK(dalloc(&wgt, B * kOut));
     HIPCHK(dalloc(&loss, B));
     HIPCHK(dalloc(&scale, B));
     HIPCHK(dalloc(&delay, B));
@@ -436,8 +439,8 @@ int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double 
       k_lo = (int)ceil(h->band_lo / bin_hz - 1e-9);
       k_hi = (int)floor(h->band_hi / bin_hz + 1e-9);
     }
-    HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->outp, B,
-                                    h->st));
+    HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->polar,
+                                    h->outp, B, h->st));
     HIPCHK(hipEventRecord(h->e1, h->st));
     HIPCHK(hipMemcpyAsync(out + (int64_t)q0 * kOut, h->outp, sizeof(float) * B * kOut, hipMemcpyDeviceToHost,
                           h->st));
@@ -451,7 +454,7 @@ int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double 
   return gfs::api_set_err(GF_OK, "");
 }
 
-int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, int32_t n, double *loss, float *grad) {
+int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, const float *w, int32_t n, double *loss, float *grad) {
   int rc = check_batch(h, n);
   if (rc) return rc;
   if (n <= 0 || n > h->max_batch || n % 8 != 0)
@@ -459,8 +462,10 @@ int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, int32_t n, doubl
   if (!x || !t || !loss) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
   HIPCHK(hipMemcpyAsync(h->feat, x, sizeof(float) * n * kFeatures, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemcpyAsync(h->tgt, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st));
+  if (w) HIPCHK(hipMemcpyAsync(h->wgt, w, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st));
   if ((rc = h->stage_features(h->feat, n)) || (rc = h->forward(n, true))) return rc;
-  HIPCHK(gfsk::launch_rel_l2(h->Z.back(), h->tgt, n, kOut, 1.0f / n, h->loss, h->dF[0], h->dB[0], h->st));
+  HIPCHK(gfsk::launch_rel_l2(h->Z.back(), h->tgt, w ? h->wgt : nullptr, n, kOut, 1.0f / n, h->loss, h->dF[0],
+                             h->dB[0], h->st));
   if ((rc = h->backward(n))) return rc;
   std::vector<float> lb(n);
   HIPCHK(hipMemcpyAsync(lb.data(), h->loss, sizeof(float) * n, hipMemcpyDeviceToHost, h->st));
@@ -472,7 +477,8 @@ int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, int32_t n, doubl
   return gfs::api_set_err(GF_OK, "");
 }
 
-int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_sur_train_opts *o, float *losses) {
+int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int64_t n, const gf_sur_train_opts *o,
+                 float *losses) {
   int rc = check_batch(h, n);
   if (rc) return rc;
   if (!o || !x || !t) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
@@ -480,12 +486,13 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_
   if (B < 8 || B % 8 != 0 || B > h->max_batch || n < B || o->steps < 0)
     return gfs::api_set_err(GF_ERR_ARGUMENT,
                             "geofdtd: train batch must be a multiple of 8, at most max_batch and the sample count");
-  float *dx = nullptr, *dt = nullptr, *dls = nullptr;
+  float *dx = nullptr, *dt = nullptr, *dw = nullptr, *dls = nullptr;
   int32_t *didx = nullptr, *hidx = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(h->st);
     if (dx) hipFree(dx);
     if (dt) hipFree(dt);
+    if (dw) hipFree(dw);
     if (dls) hipFree(dls);
     if (didx) hipFree(didx);
     if (hidx) hipHostFree(hidx);
@@ -496,11 +503,13 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_
   };
   hipError_t e;
   if ((e = dalloc(&dx, (size_t)n * kFeatures)) || (e = dalloc(&dt, (size_t)n * kOut)) ||
+      (w && (e = dalloc(&dw, (size_t)n * kOut))) ||
       (e = dalloc(&dls, (size_t)std::max(o->steps, 1) * B)) || (e = dalloc(&didx, (size_t)B)) ||
       (e = hipHostMalloc((void **)&hidx, sizeof(int32_t) * B, hipHostMallocDefault)))
     return fail(e, "train allocation");
   if ((e = hipMemcpyAsync(dx, x, sizeof(float) * n * kFeatures, hipMemcpyHostToDevice, h->st)) ||
-      (e = hipMemcpyAsync(dt, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st)))
+      (e = hipMemcpyAsync(dt, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st)) ||
+      (w && (e = hipMemcpyAsync(dw, w, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st))))
     return fail(e, "train upload");
   // epochs of a seeded permutation; a batch never straddles two epochs
   std::vector<int32_t> perm(n);
@@ -522,14 +531,15 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_
       return fail(e, "train indices");
     if ((e = hipEventRecord(h->e0, h->st))) return fail(e, "event");
     if ((e = gfsk::launch_gather_rows(dx, didx, kFeatures, h->feat, B, h->st)) ||
-        (e = gfsk::launch_gather_rows(dt, didx, kOut, h->tgt, B, h->st)))
+        (e = gfsk::launch_gather_rows(dt, didx, kOut, h->tgt, B, h->st)) ||
+        (w && (e = gfsk::launch_gather_rows(dw, didx, kOut, h->wgt, B, h->st))))
       return fail(e, "gather");
     if ((rc = h->stage_features(h->feat, B)) || (rc = h->forward(B, true))) {
       cleanup();
       return rc;
     }
-    if ((e = gfsk::launch_rel_l2(h->Z.back(), h->tgt, B, kOut, 1.0f / B, dls + (int64_t)s * B, h->dF[0], h->dB[0],
-                                 h->st)))
+    if ((e = gfsk::launch_rel_l2(h->Z.back(), h->tgt, w ? h->wgt : nullptr, B, kOut, 1.0f / B, dls + (int64_t)s * B,
+                                 h->dF[0], h->dB[0], h->st)))
       return fail(e, "loss");
     if ((rc = h->backward(B))) {
       cleanup();

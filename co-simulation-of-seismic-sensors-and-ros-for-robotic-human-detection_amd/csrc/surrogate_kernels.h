@@ -43,8 +43,8 @@ hipError_t launch_pad_rows_bf16(const float *in, int32_t cols_in, uint16_t *out,
                                 hipStream_t st);
 hipError_t launch_gather_rows(const float *src, const int32_t *idx, int32_t cols, float *dst, int32_t rows,
                               hipStream_t st);
-hipError_t launch_rel_l2(const float *o, const float *t, int32_t rows, int32_t len, float scale, float *loss,
-                         float *dF, uint16_t *dB, hipStream_t st);
+hipError_t launch_rel_l2(const float *o, const float *t, const float *w, int32_t rows, int32_t len, float scale,
+                         float *loss, float *dF, uint16_t *dB, hipStream_t st);
 // Bias gradient: column sums of x[rows][cols] folded onto mod channels.
 // part needs bias_grad_chunks(rows) * cols floats; cols <= 256 must divide 256.
 int32_t bias_grad_chunks(int64_t rows);
@@ -56,6 +56,6 @@ hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *p
 // out[k][n] = in[n][k] (BF16).
 hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_cols, uint16_t *out, hipStream_t st);
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
-                              int32_t k_lo, int32_t k_hi, float *out, int32_t rows, hipStream_t st);
+                              int32_t k_lo, int32_t k_hi, int32_t polar, float *out, int32_t rows, hipStream_t st);
 
 }  // namespace gfsk

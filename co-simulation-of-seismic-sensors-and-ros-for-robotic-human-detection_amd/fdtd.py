@@ -154,7 +154,7 @@ class GfSoil(C.Structure):
 
 class GfSurConfig(C.Structure):
     _fields_ = [("max_batch", C.c_int32), ("seed", C.c_uint64), ("band_lo_hz", C.c_double),
-                ("band_hi_hz", C.c_double)]
+                ("band_hi_hz", C.c_double), ("polar", C.c_int32)]
 
 
 class GfSurLayer(C.Structure):
@@ -244,9 +244,9 @@ SIGNATURES = {
     "gf_sur_set_params": (_I, [C.c_void_p, C.POINTER(C.c_float)]),
     "gf_sur_forward": (_I, [C.c_void_p, C.POINTER(C.c_float), _I32, C.POINTER(C.c_float)]),
     "gf_sur_predict": (_I, [C.c_void_p, C.POINTER(C.c_float), _PD, _PD, _D, _I32, C.POINTER(C.c_float)]),
-    "gf_sur_loss_grad": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), _I32, _PD,
-                              C.POINTER(C.c_float)]),
-    "gf_sur_train": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), _I64,
+    "gf_sur_loss_grad": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), _I32,
+                              _PD, C.POINTER(C.c_float)]),
+    "gf_sur_train": (_I, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), _I64,
                           C.POINTER(GfSurTrainOpts), C.POINTER(C.c_float)]),
     "gf_sur_stats": (_I, [C.c_void_p, C.POINTER(GfSurStats)]),
     "gf_gemm_bf16": (_I, [_I32, _I32, _I32, C.POINTER(C.c_uint16), _I64, _I32, C.POINTER(C.c_uint16), _I64, _I32,

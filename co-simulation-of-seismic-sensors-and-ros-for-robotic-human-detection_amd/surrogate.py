@@ -111,6 +111,32 @@ def normalize(G: np.ndarray, scale: np.ndarray, delay: np.ndarray) -> np.ndarray
     return out
 
 
+def polar_targets(t: np.ndarray):
+    """Cartesian normalized targets [n][1024] (from `normalize`) to the polar
+    form the network can also learn: per bin (log|T|, unwrapped phase), and
+    weights w = |T|^2 / sum |T|^2 on both entries of in-band bins (zero out
+    of band), so that sum w ((a - log|T|)^2 + (phi - arg T)^2) is the relative
+    L2 of e^{a + i phi} against T to first order.  The phase is unwrapped
+    across the band from its first bin; log|T| is floored at 1e-3 of the
+    query's largest |T| (where the weight is negligible anyway)."""
+    T = t[:, 0::2].astype(np.float64) + 1j * t[:, 1::2]
+    m = band_mask()
+    mag = np.abs(T)
+    floor = 1e-3 * np.maximum(mag.max(axis=1, keepdims=True), 1e-30)
+    a = np.log(np.maximum(mag, floor))
+    ph = np.zeros_like(a)
+    ph[:, m] = np.unwrap(np.angle(T[:, m]), axis=1)
+    w = np.where(m[None, :], mag ** 2, 0.0)
+    w /= np.maximum(w.sum(axis=1, keepdims=True), 1e-300)
+    out = np.empty_like(t, dtype=np.float32)
+    out[:, 0::2] = np.where(m[None, :], a, 0.0)
+    out[:, 1::2] = ph
+    wt = np.empty_like(out)
+    wt[:, 0::2] = w
+    wt[:, 1::2] = w
+    return out, wt
+
+
 def rel_l2(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """Per-row |a - b| / |b| over complex (or interleaved real) spectra."""
     a = np.asarray(a).reshape(len(a), -1)
@@ -126,10 +152,11 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 class Network:
     """A gf_sur handle: the surrogate network on the GPU."""
 
-    def __init__(self, max_batch: int = 8192, seed: int = 0, band=BAND_HZ):
+    def __init__(self, max_batch: int = 8192, seed: int = 0, band=BAND_HZ, polar: bool = False):
         self._lib = fd.load_library()
         h = C.c_void_p()
-        cfg = fd.GfSurConfig(max_batch, seed, band[0] if band else 0.0, band[1] if band else 0.0)
+        cfg = fd.GfSurConfig(max_batch, seed, band[0] if band else 0.0, band[1] if band else 0.0, int(polar))
+        self.polar = polar
         fd._check(self._lib.gf_sur_create(C.byref(cfg), C.byref(h)))
         self._h = h
         self.max_batch = max_batch
@@ -186,21 +213,25 @@ class Network:
         fd._check(self._lib.gf_sur_predict(self._h, _pf(x), _pd(s), _pd(d), BIN_HZ, len(x), _pf(out)))
         return out.view(np.complex64)
 
-    def LossGrad(self, x: np.ndarray, t: np.ndarray):
+    def LossGrad(self, x: np.ndarray, t: np.ndarray, w: Optional[np.ndarray] = None):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        keep = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+        wp = None if keep is None else _pf(keep)
         loss = C.c_double()
         g = np.zeros(self.ParamCount(), dtype=np.float32)
-        fd._check(self._lib.gf_sur_loss_grad(self._h, _pf(x), _pf(t), len(x), C.byref(loss), _pf(g)))
+        fd._check(self._lib.gf_sur_loss_grad(self._h, _pf(x), _pf(t), wp, len(x), C.byref(loss), _pf(g)))
         return loss.value, g
 
     def Train(self, x: np.ndarray, t: np.ndarray, steps: int, batch: int = 4096, lr: float = 1e-3,
-              lr_min: float = 2e-5, seed: int = 0) -> np.ndarray:
+              lr_min: float = 2e-5, seed: int = 0, w: Optional[np.ndarray] = None) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
+        keep = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
         opts = fd.GfSurTrainOpts(steps, batch, lr, lr_min, 0.0, 0.0, 0.0, seed)
         losses = np.zeros(max(steps, 1), dtype=np.float32)
-        fd._check(self._lib.gf_sur_train(self._h, _pf(x), _pf(t), len(x), C.byref(opts), _pf(losses)))
+        fd._check(self._lib.gf_sur_train(self._h, _pf(x), _pf(t), None if keep is None else _pf(keep), len(x),
+                                         C.byref(opts), _pf(losses)))
         return losses[:steps]
 
     def Stats(self) -> Dict[str, float]:

@@ -340,6 +340,7 @@ typedef struct gf_sur_config {
   uint64_t seed;     /* weight initialisation */
   double band_lo_hz; /* gf_sur_predict leaves bins outside [lo, hi] zero; */
   double band_hi_hz; /* hi <= 0 keeps every bin */
+  int32_t polar;     /* outputs per bin: 0 (re, im); 1 (log-amplitude, phase) */
 } gf_sur_config;
 
 /* One GEMM layer: weights [n_out][k_in] at w_offset and bias_channels
@@ -383,11 +384,15 @@ int gf_sur_forward(gf_sur *h, const float *x, int32_t n, float *out);
 int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double *delay, double bin_hz, int32_t n,
                    float *out);
 /* Mean loss over n queries (a multiple of 8, at most max_batch) against
- * targets t [n][1024], and its gradient (flat, like the parameters). */
-int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, int32_t n, double *loss, float *grad);
-/* opts->steps Adam steps on minibatches of the n samples; losses (may be
- * NULL) receives each step's mean loss. */
-int gf_sur_train(gf_sur *h, const float *x, const float *t, int64_t n, const gf_sur_train_opts *opts,
+ * targets t [n][1024], and its gradient (flat, like the parameters).  w NULL:
+ * relative L2 |o - t|^2 / |t|^2 per query; otherwise the weighted squares
+ * sum_i w_i (o_i - t_i)^2 with weights w [n][1024]. */
+int gf_sur_loss_grad(gf_sur *h, const float *x, const float *t, const float *w, int32_t n, double *loss,
+                     float *grad);
+/* opts->steps Adam steps on minibatches of the n samples, with the loss of
+ * gf_sur_loss_grad (w may be NULL); losses (may be NULL) receives each step's
+ * mean loss. */
+int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int64_t n, const gf_sur_train_opts *opts,
                  float *losses);
 int gf_sur_stats(const gf_sur *h, gf_sur_stats_t *out);
 

@@ -98,14 +98,20 @@ def loss(o: np.ndarray, t: np.ndarray) -> np.ndarray:
     return np.sum((o - t) ** 2, axis=1) / np.sum(t ** 2, axis=1)
 
 
-def loss_grad(params, layers, x, t, bf: bool = False):
-    """(mean loss, flat gradient) of the batch x [B][15] against t [B][1024]."""
+def loss_grad(params, layers, x, t, bf: bool = False, w=None):
+    """(mean loss, flat gradient) of the batch x [B][15] against t [B][1024]:
+    the relative L2 |o - t|^2 / |t|^2 per query, or with weights w the
+    weighted squares sum_i w_i (o_i - t_i)^2."""
     p = np.asarray(params, dtype=np.float64)
     o, (acts, zs) = forward(p, layers, x, bf)
     B = x.shape[0]
-    den = np.sum(t ** 2, axis=1)
-    L_b = np.sum((o - t) ** 2, axis=1) / den
-    d = 2.0 * (o - t) / den[:, None] / B
+    if w is None:
+        den = np.sum(t ** 2, axis=1)
+        L_b = np.sum((o - t) ** 2, axis=1) / den
+        d = 2.0 * (o - t) / den[:, None] / B
+    else:
+        L_b = np.sum(w * (o - t) ** 2, axis=1)
+        d = 2.0 * w * (o - t) / B
     q = bf16 if bf else (lambda v: v)
     g = np.zeros_like(p)
     for i in range(len(layers) - 1, -1, -1):

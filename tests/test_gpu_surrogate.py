@@ -130,6 +130,41 @@ def test_gradients_match_the_oracle(net):
             assert r64 <= 1e-2 and rbf <= 5e-3, (i, name)
 
 
+def test_weighted_gradients_match_the_oracle(net):
+    L = O.layers_from(None, net)
+    rng = np.random.default_rng(12)
+    p = random_params(L, net.ParamCount(), rng, scale=1.0).astype(np.float32)
+    net.SetParams(p)
+    B = 16
+    x = rng.uniform(-1, 1, (B, 15)).astype(np.float32)
+    t = rng.normal(0, 1, (B, 1024)).astype(np.float32)
+    w = (rng.uniform(0, 1, (B, 1024)) / 512).astype(np.float32)
+    loss, g = net.LossGrad(x, t, w)
+    l_bf, g_bf = O.loss_grad(p.astype(np.float64), L, x.astype(np.float64), t.astype(np.float64), bf=True,
+                             w=w.astype(np.float64))
+    assert loss == pytest.approx(l_bf, rel=2e-3)
+    for i, l in enumerate(L):
+        a, b = l.w_off, l.w_off + l.n * l.k
+        r = np.linalg.norm(g[a:b] - g_bf[a:b]) / np.linalg.norm(g_bf[a:b])
+        assert r <= 5e-3, (i, r)
+
+
+def test_polar_predict_exponentiates(S):
+    net = S.Network(max_batch=64, seed=3, polar=True)
+    rng = np.random.default_rng(13)
+    x = rng.uniform(-1, 1, (16, 15)).astype(np.float32)
+    scale = rng.uniform(1e-9, 1e-8, 16)
+    delay = rng.uniform(0.0, 0.4, 16)
+    o = net.Forward(x).astype(np.float64)
+    f = np.arange(S.BINS) * S.BIN_HZ
+    want = scale[:, None] * np.exp(o[:, 0::2] + 1j * (o[:, 1::2] - 2 * np.pi * f[None, :] * delay[:, None]))
+    got = net.Predict(x, scale, delay)
+    m = S.band_mask()
+    assert np.all(got[:, ~m] == 0)
+    np.testing.assert_allclose(got[:, m], want[:, m], rtol=2e-5, atol=1e-6 * np.abs(want).max())
+    net.close()
+
+
 def test_predict_applies_the_prior(S):
     net = S.Network(max_batch=64, seed=3, band=None)
     rng = np.random.default_rng(7)

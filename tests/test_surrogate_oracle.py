@@ -185,3 +185,51 @@ def test_gate_falls_back_for_injected_bad_predictions():
     _, rep1 = S.serve(None, params, ranges, soils, gate_fraction=0.01, solver=solver,
                       predictions=pred.reshape(S_ * R, S.BINS))
     assert len(rep1.sampled_soils) == 1
+
+
+def test_polar_targets_reconstruct_the_spectrum_and_the_loss():
+    """exp(a + i phi) gives back the normalized target in band, and the
+    weighted squares of a small perturbation of (a, phi) equal the relative L2
+    squared of the complex spectrum to first order."""
+    rng = np.random.default_rng(9)
+    n = 4
+    params = np.column_stack([rng.uniform(lo, hi, n) for lo, hi in S.PARAM_RANGES])
+    r = rng.uniform(1, 50, n)
+    scale, delay = S.prior(params, r)
+    f = np.arange(S.BINS) * S.BIN_HZ
+    G = scale[:, None] * (1 + 0.3 * np.cos(f / 7.0))[None, :] * np.exp(-2j * np.pi * f[None, :] * (delay[:, None] + 0.01))
+    t = S.normalize(G, scale, delay)
+    pt, w = S.polar_targets(t)
+    m = S.band_mask()
+    T = t[:, 0::2] + 1j * t[:, 1::2]
+    rec = np.exp(pt[:, 0::2] + 1j * pt[:, 1::2])
+    np.testing.assert_allclose(rec[:, m], T[:, m], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(w.sum(axis=1), 2.0, rtol=1e-6)   # both entries of each bin
+    assert np.all(w[:, 0::2][:, ~m] == 0)
+    # first-order equivalence of the weighted squares and the relative L2
+    eps = 1e-3 * rng.normal(size=pt.shape)
+    P = np.exp((pt + eps)[:, 0::2] + 1j * (pt + eps)[:, 1::2])
+    rel2 = np.sum(np.abs(P[:, m] - T[:, m]) ** 2, axis=1) / np.sum(np.abs(T[:, m]) ** 2, axis=1)
+    wsq = np.sum(w * eps ** 2, axis=1)
+    np.testing.assert_allclose(wsq, rel2, rtol=1e-2)
+
+
+def test_weighted_gradient_matches_central_differences():
+    L, P = layers(small_arch())
+    rng = np.random.default_rng(10)
+    p = random_params(L, P, rng, scale=1.0)
+    x = rng.uniform(-1, 1, (3, 15))
+    out_len = L[-1].rows * L[-1].n
+    t = rng.normal(0, 1, (3, out_len))
+    w = rng.uniform(0, 1, (3, out_len)) / out_len
+    _, g = O.loss_grad(p, L, x, t, w=w)
+    h = 1e-6
+    for l in L:
+        for i in list(rng.choice(l.n * l.k, size=4, replace=False) + l.w_off) + [l.b_off]:
+            if l is L[0] and (i - l.w_off) % l.k >= 15 and i < l.b_off:
+                continue
+            pp, pm = p.copy(), p.copy()
+            pp[i] += h
+            pm[i] -= h
+            fd_ = (O.loss_grad(pp, L, x, t, w=w)[0] - O.loss_grad(pm, L, x, t, w=w)[0]) / (2 * h)
+            assert abs(fd_ - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd_, g[i])
