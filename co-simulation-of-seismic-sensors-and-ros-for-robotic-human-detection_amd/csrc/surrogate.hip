@@ -19,6 +19,7 @@
 // gradients, one slice per split of K).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "surrogate_kernels.h"
 
@@ -27,8 +28,8 @@ namespace gfsk {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBK = 64;        // K per LDS stage (four 32x32x16 MFMA steps)
-constexpr int kLDK = kBK + 8;  // padded LDS row, in BF16 elements (144 bytes)
+// K per LDS stage is a template parameter BK (a multiple of 16: BK/16 MFMA
+// steps per stage); LDS rows are padded to BK + 8 elements.
 
 __device__ __forceinline__ uint16_t to_bf16(float x) {
   // round to nearest even; NaN stays NaN
@@ -67,24 +68,24 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // Rows at or beyond R and k at or beyond kend read as zero; the host
 // guarantees that a vector is wholly in or out (K, and R for TRANS, multiples
 // of 8, and 16-byte aligned rows).
-template <int ROWS>
+template <int ROWS, int BK>
 struct TileRegs {
-  static constexpr int kVecs = ROWS * kBK / 8;
+  static constexpr int kVecs = ROWS * BK / 8;
   static constexpr int kPer = (kVecs + 255) / 256;
   uint4 v[kPer];
 };
 
-template <int ROWS, bool TRANS>
-__device__ __forceinline__ void fetch_tile(TileRegs<ROWS> &t, const uint16_t *__restrict__ src, int64_t ld, int r0,
-                                           int k0, int R, int kend) {
-  constexpr int kVecs = TileRegs<ROWS>::kVecs;
+template <int ROWS, bool TRANS, int BK>
+__device__ __forceinline__ void fetch_tile(TileRegs<ROWS, BK> &t, const uint16_t *__restrict__ src, int64_t ld,
+                                           int r0, int k0, int R, int kend) {
+  constexpr int kVecs = TileRegs<ROWS, BK>::kVecs;
 #pragma unroll
-  for (int i = 0; i < TileRegs<ROWS>::kPer; i++) {
+  for (int i = 0; i < TileRegs<ROWS, BK>::kPer; i++) {
     const int v = threadIdx.x + 256 * i;
     uint4 val = make_uint4(0, 0, 0, 0);
     if (v < kVecs) {
       if (!TRANS) {
-        const int row = v / (kBK / 8), kv = v % (kBK / 8);
+        const int row = v / (BK / 8), kv = v % (BK / 8);
         const int gr = r0 + row, gk = k0 + kv * 8;
         if (gr < R && gk < kend) val = *(const uint4 *)(src + (int64_t)gr * ld + gk);
       } else {
@@ -97,29 +98,30 @@ __device__ __forceinline__ void fetch_tile(TileRegs<ROWS> &t, const uint16_t *__
   }
 }
 
-template <int ROWS, bool TRANS>
-__device__ __forceinline__ void store_tile(const TileRegs<ROWS> &t, uint16_t *dst) {
-  constexpr int kVecs = TileRegs<ROWS>::kVecs;
+template <int ROWS, bool TRANS, int BK>
+__device__ __forceinline__ void store_tile(const TileRegs<ROWS, BK> &t, uint16_t *dst) {
+  constexpr int kVecs = TileRegs<ROWS, BK>::kVecs, LDK = BK + 8;
 #pragma unroll
-  for (int i = 0; i < TileRegs<ROWS>::kPer; i++) {
+  for (int i = 0; i < TileRegs<ROWS, BK>::kPer; i++) {
     const int v = threadIdx.x + 256 * i;
     if (v >= kVecs) break;
     if (!TRANS) {
-      const int row = v / (kBK / 8), kv = v % (kBK / 8);
-      *(uint4 *)(dst + row * kLDK + kv * 8) = t.v[i];
+      const int row = v / (BK / 8), kv = v % (BK / 8);
+      *(uint4 *)(dst + row * LDK + kv * 8) = t.v[i];
     } else {
       const int k = v / (ROWS / 8), rv = v % (ROWS / 8);
       const uint16_t *e = (const uint16_t *)&t.v[i];
 #pragma unroll
-      for (int j = 0; j < 8; j++) dst[(rv * 8 + j) * kLDK + k] = e[j];
+      for (int j = 0; j < 8; j++) dst[(rv * 8 + j) * LDK + k] = e[j];
     }
   }
 }
 
-template <int BM, int BN, int WM, int WN, bool AT, bool BT, int EPI>
+template <int BM, int BN, int BK, int WM, int WN, bool AT, bool BT, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  constexpr int TM = BM / WM / 32, TN = BN / WN / 32;
+  constexpr int TM = BM / WM / 32, TN = BN / WN / 32, kLDK = BK + 8;
   static_assert(WM * WN == 4 && TM >= 1 && TN >= 1, "4 wavefronts, 32x32 MFMA tiles");
+  static_assert(BK % 16 == 0, "whole MFMA steps per stage");
   __shared__ __attribute__((aligned(16))) uint16_t As[BM * kLDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * kLDK];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -138,22 +140,22 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 
   // software pipeline: the next stage's global loads are issued before this
   // stage's MFMAs, and land in LDS after them
-  TileRegs<BM> ra;
-  TileRegs<BN> rb;
+  TileRegs<BM, BK> ra;
+  TileRegs<BN, BK> rb;
   if (kbeg < kend) {
-    fetch_tile<BM, AT>(ra, g.A, g.lda, m0, kbeg, g.M, kend);
-    fetch_tile<BN, !BT>(rb, g.B, g.ldb, n0, kbeg, g.N, kend);
+    fetch_tile<BM, AT, BK>(ra, g.A, g.lda, m0, kbeg, g.M, kend);
+    fetch_tile<BN, !BT, BK>(rb, g.B, g.ldb, n0, kbeg, g.N, kend);
   }
-  for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-    store_tile<BM, AT>(ra, As);
-    store_tile<BN, !BT>(rb, Bs);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    store_tile<BM, AT, BK>(ra, As);
+    store_tile<BN, !BT, BK>(rb, Bs);
     __syncthreads();
-    if (k0 + kBK < kend) {
-      fetch_tile<BM, AT>(ra, g.A, g.lda, m0, k0 + kBK, g.M, kend);
-      fetch_tile<BN, !BT>(rb, g.B, g.ldb, n0, k0 + kBK, g.N, kend);
+    if (k0 + BK < kend) {
+      fetch_tile<BM, AT, BK>(ra, g.A, g.lda, m0, k0 + BK, g.M, kend);
+      fetch_tile<BN, !BT, BK>(rb, g.B, g.ldb, n0, k0 + BK, g.N, kend);
     }
 #pragma unroll
-    for (int s = 0; s < kBK / 16; s++) {
+    for (int s = 0; s < BK / 16; s++) {
       bf16x8 fa[TM], fb[TN];
       const int kk = s * 16 + (lane >> 5) * 8;
 #pragma unroll
@@ -199,23 +201,41 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
 }
 
+// Tile shapes (M x N x K-stage, wavefronts along M x N).  The default picks by
+// N and by how many workgroups a shape gives; GEOFDTD_GEMM_SHAPE forces one of
+// the shapes for N > 32 (a tuning aid: every shape is correct for every size).
+template <int BM, int BN, int BK, int WM, int WN, bool AT, bool BT, int EPI>
+hipError_t launch_tiles(const GemmArgs &g, int splits, hipStream_t st) {
+  const dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, splits);
+  hipLaunchKernelGGL((k_gemm<BM, BN, BK, WM, WN, AT, BT, EPI>), grid, dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+inline int forced_shape() {
+  static const int v = [] {
+    const char *e = getenv("GEOFDTD_GEMM_SHAPE");
+    return e ? atoi(e) : -1;
+  }();
+  return v;
+}
+
 template <bool AT, bool BT, int EPI>
 hipError_t launch_shape(const GemmArgs &g, int splits, hipStream_t st) {
-  const dim3 block(256);
-  // 128 x 128 tiles when there are enough of them for two workgroups per CU;
-  // otherwise 128 x 64, which doubles the workgroups a CU can interleave
-  const int64_t big_tiles = (int64_t)((g.N + 127) / 128) * ((g.M + 127) / 128) * splits;
-  if (g.N > 64 && big_tiles >= 512) {
-    const dim3 grid((g.N + 127) / 128, (g.M + 127) / 128, splits);
-    hipLaunchKernelGGL((k_gemm<128, 128, 2, 2, AT, BT, EPI>), grid, block, 0, st, g);
-  } else if (g.N > 32) {
-    const dim3 grid((g.N + 63) / 64, (g.M + 127) / 128, splits);  // also N > 64 with few tiles
-    hipLaunchKernelGGL((k_gemm<128, 64, 4, 1, AT, BT, EPI>), grid, block, 0, st, g);
-  } else {
-    const dim3 grid((g.N + 31) / 32, (g.M + 127) / 128, splits);
-    hipLaunchKernelGGL((k_gemm<128, 32, 4, 1, AT, BT, EPI>), grid, block, 0, st, g);
+  if (g.N <= 32) return launch_tiles<128, 32, 64, 4, 1, AT, BT, EPI>(g, splits, st);
+  switch (forced_shape()) {
+    case 0: return launch_tiles<128, 128, 64, 2, 2, AT, BT, EPI>(g, splits, st);
+    case 1: return launch_tiles<128, 64, 64, 4, 1, AT, BT, EPI>(g, splits, st);
+    case 2: return launch_tiles<128, 128, 128, 2, 2, AT, BT, EPI>(g, splits, st);
+    case 3: return launch_tiles<128, 64, 128, 4, 1, AT, BT, EPI>(g, splits, st);
+    case 4: return launch_tiles<64, 64, 64, 2, 2, AT, BT, EPI>(g, splits, st);
+    case 5: return launch_tiles<64, 64, 128, 2, 2, AT, BT, EPI>(g, splits, st);
+    default: break;
   }
-  return hipGetLastError();
+  // 64 x 64 tiles, four wavefronts of 32 x 32: the network's GEMMs are short
+  // in K (16-2048) and latency-bound, and small tiles let a CU interleave up
+  // to eight workgroups (measured best of the shapes above for inference at
+  // batch 8192 and for training at 4096)
+  return launch_tiles<64, 64, 64, 2, 2, AT, BT, EPI>(g, splits, st);
 }
 
 template <int EPI>
