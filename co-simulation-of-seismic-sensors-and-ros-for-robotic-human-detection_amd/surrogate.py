@@ -63,11 +63,39 @@ def bank_spec() -> fd.BankSpec:
 TRAVEL_MAX_S = RANGES[1] / PARAM_RANGES[0, 0]   # longest r / Vs in the family
 
 
-def features(params: np.ndarray, r: np.ndarray) -> np.ndarray:
-    """[n][15] float32, each mapped to about [-1, 1]: the 10 soil numbers,
-    log r, r, and the shear travel times r / Vs1, r / Vs2, r / Vs3 (the phase
-    of the spectrum is close to linear in these)."""
+def grid_spacing(params: np.ndarray) -> np.ndarray:
+    """The family's cell size, h = 0.9 Vs1 / (100 Hz * 10), as the sampler
+    computes it."""
+    return 0.9 * np.atleast_2d(np.asarray(params, dtype=np.float64))[:, 0] / (100.0 * 10.0)
+
+
+def grid_thicknesses(params: np.ndarray) -> np.ndarray:
+    """[n][2]: the layer thicknesses the grid realizes.  A grid node at depth
+    j h belongs to the layer whose top-to-bottom span holds it, an interface
+    node going to the layer below (internal/fdtd/model.go:200-209), so layer 1
+    holds the nodes with j h < h1 and layers 1-2 those with j h < h1 + h2;
+    each realized thickness is its node count times h."""
     p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    h = grid_spacing(p)
+
+    def nodes_above(z):
+        n = np.ceil(z / h)
+        n = np.where((n - 1) * h >= z, n - 1, n)  # the count of j >= 0 with j h < z, exactly
+        return np.where(n * h < z, n + 1, n)
+
+    n1 = nodes_above(p[:, 6])
+    n12 = nodes_above(p[:, 6] + p[:, 7])
+    return np.column_stack([n1 * h, (n12 - n1) * h])
+
+
+def features(params: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """[n][15] float32, each mapped to about [-1, 1]: the 10 soil numbers
+    (with the two thicknesses as the grid realizes them, which is what the
+    FDTD spectra depend on: the drawn values move a layer boundary by up to
+    half a cell), log r, r, and the shear travel times r / Vs1, r / Vs2,
+    r / Vs3 (the phase of the spectrum is close to linear in these)."""
+    p = np.array(np.atleast_2d(np.asarray(params, dtype=np.float64)), copy=True)
+    p[:, 6:8] = grid_thicknesses(p)
     r = np.asarray(r, dtype=np.float64)
     lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
     x = np.empty((p.shape[0], FEATURES))

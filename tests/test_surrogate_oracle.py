@@ -123,11 +123,35 @@ def test_normalization_keeps_the_relative_l2():
 def test_features_map_the_sampling_box_to_unit_interval():
     lo, hi = S.PARAM_RANGES[:, 0], S.PARAM_RANGES[:, 1]
     x = S.features(np.vstack([lo, hi]), np.array([1.0, 50.0]))
-    np.testing.assert_allclose(x[:, :11], [[-1] * 11, [1] * 11], atol=1e-6)
+    keep = [0, 1, 2, 3, 4, 5, 8, 9, 10]
+    np.testing.assert_allclose(x[:, keep], [[-1] * 9, [1] * 9], atol=1e-6)
+    # thicknesses as the grid realizes them: within a cell of the drawn value
+    h = S.grid_spacing(np.vstack([lo, hi]))
+    for row, p in enumerate([lo, hi]):
+        for k in (6, 7):
+            span = hi[k] - lo[k]
+            got = lo[k] + (x[row, k] + 1) / 2 * span
+            assert abs(got - p[k]) <= h[row] + 1e-9
     np.testing.assert_allclose(x[:, 11], [2 / 50 - 1, 1], atol=1e-6)  # r itself
     # travel times r / Vs: the longest (50 m at the slowest Vs1) maps to 1
     np.testing.assert_allclose(x[1, 12], 2 * (50 / 200) / (50 / 120) - 1, atol=1e-6)
-    assert np.all(np.abs(x) <= 1 + 1e-6)
+    assert np.all(np.abs(x[:, [0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 13, 14]]) <= 1 + 1e-6)
+
+
+def test_grid_thicknesses_follow_the_node_rule():
+    """Node j (depth j h) is in layer 1 iff j h < h1 (an interface node goes
+    to the layer below); checked by counting nodes directly, including
+    thicknesses that are exact multiples of h."""
+    rng = np.random.default_rng(11)
+    P = np.column_stack([rng.uniform(lo, hi, 200) for lo, hi in S.PARAM_RANGES])
+    h = S.grid_spacing(P)
+    P[:5, 6] = h[:5] * np.array([4, 7, 10, 13, 16])   # exact multiples
+    t = S.grid_thicknesses(P)
+    for i in range(len(P)):
+        j = np.arange(0, 200)
+        n1 = int(np.sum(j * h[i] < P[i, 6]))
+        n12 = int(np.sum(j * h[i] < P[i, 6] + P[i, 7]))
+        assert t[i, 0] == n1 * h[i] and t[i, 1] == (n12 - n1) * h[i]
 
 
 def test_layered_family_is_deterministic_and_inside_its_box():
