@@ -409,8 +409,8 @@ def main() -> int:
     ap.add_argument("--c3-models", type=int, default=256, help="config 3: soil models in the batch")
     ap.add_argument("--sur-soils", type=int, default=4096, help="surrogate: training soils (FDTD-solved)")
     ap.add_argument("--sur-queries", type=int, default=65536, help="surrogate: held-out queries served")
-    ap.add_argument("--sur-steps", type=int, default=10000, help="surrogate: Adam steps (batch 4096)")
-    ap.add_argument("--sur-lr", type=float, default=1e-3, help="surrogate: peak learning rate")
+    ap.add_argument("--sur-steps", type=int, default=60000, help="surrogate: Adam steps (batch 4096)")
+    ap.add_argument("--sur-lr", type=float, default=2e-3, help="surrogate: peak learning rate")
     ap.add_argument("--sur-speed-only", action="store_true",
                     help="surrogate: time the kernels on synthetic inputs only (no FDTD data, no accuracy)")
     ap.add_argument("--c3-seconds", type=float, default=1.0, help="config 3: simulated seconds per model")
