@@ -303,6 +303,24 @@ def bench_surrogate(args, fd) -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        # the FP64 CPU oracle's forward pass on the trained weights, on a
+        # sample of the same queries (numpy matmuls on the host's cores)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import surrogate_oracle as SO
+
+        layers = SO.layers_from(None, net)
+        p = net.Params().astype(np.float64)
+        sample = xh[:2048].astype(np.float64)
+        t1 = time.perf_counter()
+        SO.forward(p, layers, sample)
+        el = time.perf_counter() - t1
+        out["cpu_baseline"] = {
+            "value": len(sample) / el, "unit": "traces/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "sample": f"FP64 forward of the trained network on {len(sample)} of the queries ({el:.2f} s, numpy BLAS "
+                      f"on every host core)",
+        }
+        out["speedup_vs_cpu"] = inf_rate / out["cpu_baseline"]["value"]
     net.close()
     print(json.dumps(out), flush=True)
     return 0
