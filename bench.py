@@ -236,8 +236,11 @@ def bench_surrogate(args, fd) -> int:
     prog = lambda d, n, s: log(f"  FDTD banks {d}/{n} soils, {s:.0f} s")
     train = S.solve_set(S.soil_set(0, 0, args.sur_soils), progress=prog)
     fdtd_rate = args.sur_soils * R / train.fdtd_seconds
-    x, t, _, _, _ = S.dataset(train)
-    log(f"  {train.fdtd_seconds:.1f} s ({fdtd_rate:.0f} traces/s); training {args.sur_steps} steps")
+    qc = S.soil_qc(train)
+    x, t, _, _, _ = S.dataset(S.subset(train, qc))
+    limit = S.norm_bound(t)
+    log(f"  {train.fdtd_seconds:.1f} s ({fdtd_rate:.0f} traces/s); {qc.sum()} of {len(qc)} soils pass the bank QC; "
+        f"training {args.sur_steps} steps")
     net = S.Network(max_batch=8192, seed=1)
     losses = net.Train(x, t, steps=args.sur_steps, batch=4096, lr=args.sur_lr, lr_min=args.sur_lr / 50, seed=2)
     tr = net.Stats()
@@ -254,10 +257,14 @@ def bench_surrogate(args, fd) -> int:
     err = S.rel_l2(pred, Gh)
     # the served job: every query answered, 1 % of soils re-solved by the FDTD
     t0 = time.perf_counter()
-    served, rep = S.serve(net, held.params, held.ranges, held.soils, gate_fraction=0.01, bound=0.02, seed=3)
+    served, rep = S.serve(net, held.params, held.ranges, held.soils, gate_fraction=0.01, bound=0.02, seed=3,
+                          norm_limit=limit)
     wall = time.perf_counter() - t0
     served = served.reshape(-1, S.BINS)[:args.sur_queries]
     err_served = S.rel_l2(served, Gh)
+    held_qc = S.soil_qc(held)
+    qmask = np.repeat(held_qc, R)[:args.sur_queries]
+    err_qc = err[qmask]
     fb = set(rep.failed_soils)
     achieved = tr["flops_per_query"] * inf_rate / 1e12
     out = {
@@ -296,6 +303,9 @@ def bench_surrogate(args, fd) -> int:
             "fraction_within_2pct": float(np.mean(err <= 0.02)),
             "served_rel_l2_p50": float(np.median(err_served)), "served_rel_l2_p99": float(np.quantile(err_served, 0.99)),
             "gate_sampled_soils": len(rep.sampled_soils), "gate_fallback_soils": len(fb),
+            "magnitude_guard_soils": len(rep.guarded_soils), "train_soils_passing_qc": int(qc.sum()),
+            "held_out_soils_passing_qc": int(held_qc.sum()),
+            "rel_l2_p50_qc_soils": float(np.median(err_qc)), "rel_l2_p99_qc_soils": float(np.quantile(err_qc, 0.99)),
             "gate_seconds": rep.gate_seconds, "surrogate_seconds": rep.surrogate_seconds,
             "train_loss_first": float(losses[:50].mean()), "train_loss_last": float(losses[-50:].mean()),
             "train_ms_per_step": tr["train_ms"] / max(tr["train_steps"], 1),

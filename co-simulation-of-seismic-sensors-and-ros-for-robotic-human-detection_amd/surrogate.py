@@ -345,6 +345,38 @@ def solve_set(ss: SoilSet, seconds: float = SECONDS, progress=None) -> SoilSet:
     return ss
 
 
+QC_PEAK_OVER_MEDIAN = 30.0
+
+
+def soil_qc(ss: SoilSet) -> np.ndarray:
+    """[S] bool: soils whose every bank spectrum looks physical.  The late-time
+    growth a C-PML can develop in a soil with a thick soft top layer reaches
+    every receiver of that soil and shows as an in-band |T| (the normalized
+    target) peaking far above its median; such a bank is not a fit target."""
+    S_, R = ss.ranges.shape
+    p = np.repeat(ss.params, R, axis=0)
+    r = ss.ranges.reshape(-1)
+    scale, delay = prior(p, r)
+    t = normalize(ss.spectra.reshape(S_ * R, BINS), scale, delay)
+    mag = np.abs(t[:, 0::2] + 1j * t[:, 1::2])[:, band_mask()]
+    ok = mag.max(axis=1) <= QC_PEAK_OVER_MEDIAN * np.median(mag, axis=1)
+    return ok.reshape(S_, R).all(axis=1)
+
+
+def subset(ss: SoilSet, keep: np.ndarray) -> SoilSet:
+    k = np.flatnonzero(keep)
+    return SoilSet(ss.seed, ss.indices[k], ss.params[k], [ss.soils[i] for i in k],
+                   None if ss.spectra is None else ss.spectra[k], None if ss.ranges is None else ss.ranges[k],
+                   ss.fdtd_seconds, ss.stats)
+
+
+def norm_bound(t: np.ndarray, factor: float = 4.0) -> float:
+    """The service's magnitude guard for a network trained on targets t: a
+    prediction whose normalized norm exceeds `factor` times the largest
+    training target norm is outside what the network has learned."""
+    return float(factor * np.max(np.linalg.norm(t, axis=1)))
+
+
 def dataset(ss: SoilSet):
     """Flattened (x [n][15], targets [n][1024], scale, delay, G [n][512]) over
     every (soil, range) of a solved set; G band-limited to BAND_HZ."""
@@ -370,11 +402,13 @@ class GateReport:
     surrogate_seconds: float
     gate_seconds: float
     bound: float
+    guarded_soils: List[int] = field(default_factory=list)
 
 
 def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
           gate_fraction: float = 0.01, bound: float = 0.02, seed: int = 0,
-          solver: Optional[Callable] = None, predictions: Optional[np.ndarray] = None):
+          solver: Optional[Callable] = None, predictions: Optional[np.ndarray] = None,
+          norm_limit: Optional[float] = None):
     """Answer every (soil, range) query of `soils` ([S] soils with their 10
     numbers `params` [S][10] and receiver ranges [S][R]).
 
@@ -382,7 +416,9 @@ def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
     drawn with `seed`, are re-solved by the batched FDTD (`solver(soils)` ->
     (spectra [k][R][512], ...), default `solve`), and every sampled soil with a
     query whose relative L2 against its FDTD solution exceeds `bound` is
-    answered by that solution instead.  `predictions` ([S*R][512]) replaces the
+    answered by that solution instead.  With `norm_limit` (see norm_bound),
+    every soil with a prediction whose normalized norm exceeds it is re-solved
+    and answered by the FDTD as well.  `predictions` ([S*R][512]) replaces the
     network's output, for tests of the gate itself.
 
     Returns (spectra [S][R][512] complex64, GateReport)."""
@@ -401,17 +437,28 @@ def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
     k = min(S, int(math.ceil(gate_fraction * S)))
     rng = np.random.default_rng(seed)
     sampled = sorted(rng.choice(S, size=k, replace=False).tolist()) if k > 0 else []
+    guarded: List[int] = []
+    if norm_limit is not None:
+        p = np.repeat(np.asarray(params), R, axis=0)
+        scale, _ = prior(p, ranges.reshape(-1))
+        nrm = (np.linalg.norm(pred.reshape(S * R, BINS), axis=1) / scale).reshape(S, R)
+        guarded = sorted(set(np.flatnonzero(np.any(nrm > norm_limit, axis=1)).tolist()) - set(sampled))
     failed: List[int] = []
     errs = np.zeros((0,))
-    if sampled:
-        truth = (solver or solve)([soils[i] for i in sampled])[0]
+    resolve = sampled + guarded
+    if resolve:
+        truth = (solver or solve)([soils[i] for i in resolve])[0]
         if predictions is None:
             truth = band_limit(truth)
-        e = rel_l2(out[sampled].reshape(k * R, BINS), truth.reshape(k * R, BINS)).reshape(k, R)
-        errs = e.reshape(-1)
-        for j, s in enumerate(sampled):
-            if np.any(e[j] > bound):
-                failed.append(s)
-                out[s] = truth[j]
+        k = len(sampled)
+        if k:
+            e = rel_l2(out[sampled].reshape(k * R, BINS), truth[:k].reshape(k * R, BINS)).reshape(k, R)
+            errs = e.reshape(-1)
+            for j, s in enumerate(sampled):
+                if np.any(e[j] > bound):
+                    failed.append(s)
+                    out[s] = truth[j]
+        for j, s in enumerate(guarded):
+            out[s] = truth[k + j]
     t2 = time.perf_counter()
-    return out, GateReport(S * R, S, sampled, failed, errs, t1 - t0, t2 - t1, bound)
+    return out, GateReport(S * R, S, sampled, failed, errs, t1 - t0, t2 - t1, bound, guarded)

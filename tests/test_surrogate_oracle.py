@@ -257,3 +257,42 @@ def test_weighted_gradient_matches_central_differences():
             pm[i] -= h
             fd_ = (O.loss_grad(pp, L, x, t, w=w)[0] - O.loss_grad(pm, L, x, t, w=w)[0]) / (2 * h)
             assert abs(fd_ - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd_, g[i])
+
+
+def test_magnitude_guard_sends_blown_up_soils_to_the_fdtd():
+    """A soil with a prediction whose normalized norm exceeds the limit is
+    re-solved and answered by the FDTD even when the 1 % sample misses it."""
+    S_, R = 40, 3
+    rng = np.random.default_rng(14)
+    params = np.column_stack([rng.uniform(lo, hi, S_) for lo, hi in S.PARAM_RANGES])
+    ranges = np.tile([2.0, 10.0, 40.0], (S_, 1))
+    scale, _ = S.prior(np.repeat(params, R, axis=0), ranges.reshape(-1))
+    truth = (rng.normal(size=(S_, R, S.BINS)) + 1j * rng.normal(size=(S_, R, S.BINS))).astype(np.complex64)
+    truth *= scale.reshape(S_, R, 1).astype(np.float32)        # normalized norm ~ sqrt(2 * 512)
+    pred = truth.copy()
+    blown = [5, 17, 33]
+    for s in blown:
+        pred[s, 2] *= 100.0
+    calls = []
+
+    def solver(soils):
+        calls.append(list(soils))
+        return truth[np.asarray(soils)], None, 0.0, {}
+
+    limit = 4.0 * np.sqrt(2 * S.BINS) * 1.2
+    out, rep = S.serve(None, params, ranges, list(range(S_)), gate_fraction=0.01, seed=1, solver=solver,
+                       predictions=pred.reshape(S_ * R, S.BINS), norm_limit=limit)
+    assert sorted(set(rep.guarded_soils) | set(rep.sampled_soils)) >= sorted(blown)
+    assert set(blown) <= set(rep.guarded_soils) | set(rep.sampled_soils)
+    assert len(calls) == 1 and sorted(calls[0]) == sorted(rep.sampled_soils + rep.guarded_soils)
+    for s in blown:
+        np.testing.assert_array_equal(out[s], truth[s])
+    others = [s for s in range(S_) if s not in rep.guarded_soils and s not in rep.failed_soils]
+    for s in others:
+        np.testing.assert_array_equal(out[s], pred[s])
+    # without the limit, unsampled blown-up soils keep their predictions
+    out2, rep2 = S.serve(None, params, ranges, list(range(S_)), gate_fraction=0.01, seed=1, solver=solver,
+                         predictions=pred.reshape(S_ * R, S.BINS))
+    assert rep2.guarded_soils == []
+    missed = [s for s in blown if s not in rep2.sampled_soils]
+    assert missed and all(np.array_equal(out2[s], pred[s]) for s in missed)
