@@ -282,7 +282,6 @@ def test_magnitude_guard_sends_blown_up_soils_to_the_fdtd():
     limit = 4.0 * np.sqrt(2 * S.BINS) * 1.2
     out, rep = S.serve(None, params, ranges, list(range(S_)), gate_fraction=0.01, seed=1, solver=solver,
                        predictions=pred.reshape(S_ * R, S.BINS), norm_limit=limit)
-    assert sorted(set(rep.guarded_soils) | set(rep.sampled_soils)) >= sorted(blown)
     assert set(blown) <= set(rep.guarded_soils) | set(rep.sampled_soils)
     assert len(calls) == 1 and sorted(calls[0]) == sorted(rep.sampled_soils + rep.guarded_soils)
     for s in blown:
