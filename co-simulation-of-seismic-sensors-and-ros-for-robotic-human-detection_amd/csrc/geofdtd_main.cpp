@@ -682,7 +682,7 @@ int run(int argc, char **argv) {
     fail("-compare needs attenuation: with no loss the Rayleigh pole sits on the wavenumber integration path "
          "and the reference does not exist");
   }
-  if (*compare) fail("-compare needs the frequency-wavenumber solver, which this build does not include");
+  if (*compare) fail("-compare needs the frequency-wavenumber solver, which libgeofdtd does not include");
 
   gf_model m{};
   m.stack = g.layers.data();

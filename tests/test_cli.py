@@ -72,7 +72,7 @@ def test_compare_is_refused_after_the_reference_checks():
     rc, err = geofdtd("-model", HOMOGENEOUS, "-compare", "-q", "0")
     assert rc == 1 and err.startswith("geofdtd: -compare needs attenuation: with no loss the Rayleigh pole")
     rc, err = geofdtd("-model", HOMOGENEOUS, "-compare")
-    assert (rc, err) == (1, "geofdtd: -compare needs the frequency-wavenumber solver, which this build does "
+    assert (rc, err) == (1, "geofdtd: -compare needs the frequency-wavenumber solver, which libgeofdtd does "
                             "not include\n")
 
 
