@@ -214,86 +214,138 @@ void Batch::setup(const gf_shot *shots, int32_t n, const std::vector<std::vector
   }
 }
 
-// All set-up shots, advanced together: one launch per kernel variant per
-// step over the tiles of every model still running, in graph blocks.
+// State bytes of one model: every array a step reads (the other half of the
+// double buffers is written): the figure that decides what stays in cache.
+double state_bytes(const Sim &s) { return gfs::step_bytes(s.R) / 2; }
+
+// Models of one kernel variant are advanced in chunks whose combined state fits
+// the GPU's last-level cache (256 MB Infinity Cache): a chunk runs all of its
+// steps before the next one starts, so its arrays are served from cache step
+// after step instead of streaming every model's state from HBM on every step.
+// Soils are independent, so the order changes nothing in any result.
+// GEOFDTD_CACHE_MB sets the budget; 0 advances every model together.
+double cache_budget_bytes() {
+  const char *e = getenv("GEOFDTD_CACHE_MB");
+  const double mb = e ? atof(e) : 160.0;
+  return mb * 1e6;
+}
+
+// All set-up shots: one launch per step over the tiles of every running
+// model of a chunk, in graph blocks, chunk after chunk.
 int Batch::run() {
   const int n = (int)sims.size();
-  std::vector<Group> groups;
+  std::vector<Group> variants;
   for (int i = 0; i < n; i++) {
     if (!sims[i]) continue;
     const Sim &s = *sims[i];
     const int nm = s.g.lossy ? s.g.nmech : 0;
     const bool sc = s.g.scalar != 0;
-    auto it = std::find_if(groups.begin(), groups.end(), [&](const Group &G) { return G.nmech == nm && G.scalar == sc; });
-    if (it == groups.end()) {
-      groups.emplace_back();
-      groups.back().nmech = nm;
-      groups.back().scalar = sc;
-      it = groups.end() - 1;
+    auto it = std::find_if(variants.begin(), variants.end(),
+                           [&](const Group &G) { return G.nmech == nm && G.scalar == sc; });
+    if (it == variants.end()) {
+      variants.emplace_back();
+      variants.back().nmech = nm;
+      variants.back().scalar = sc;
+      it = variants.end() - 1;
     }
     it->order.push_back(i);
   }
-  if (groups.empty()) return GF_OK;
+  if (variants.empty()) return GF_OK;
   int rc = GF_OK;
-  int64_t max_steps = 0;
-  for (int i = 0; i < n; i++) max_steps = std::max<int64_t>(max_steps, sims[i] ? steps[i] : 0);
-  stream = sims[groups[0].order[0]]->stream;
-  for (auto &sp : sims)
-    if (sp && !rc) {
-      hipError_t e = hipStreamSynchronize(sp->stream);
-      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
-    }
-  for (Group &G : groups) {
-    if (rc) break;
-    std::stable_sort(G.order.begin(), G.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
-    int64_t cols_rows = 0, cell_steps = 0, cells = 0;
-    for (int i : G.order) {
-      cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
-      cells += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
-      cell_steps += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
-    }
-    if ((rc = plan_group(G, sims, gfs::choose_tile_rows_cells(cols_rows)))) break;
-    // Long runs pay for a few trial steps at each tile height.
-    const char *tune_env = getenv("GEOFDTD_TUNE");
-    const bool want_tune =
-        !getenv("GEOFDTD_TILE_ROWS") && (tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL);
-    if (want_tune) {
-      gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(), (int64_t)log2((double)cells)};
-      auto prepare = [&](int th) { return plan_group(G, sims, th) == GF_OK; };
-      auto launch = [&](int th, int k) {
-        return gfk::launch_step(G.nmech, G.scalar, th, G.d_descs, G.d_tile_model, G.prefix.back(), k, nullptr, 0,
-                                stream);
-      };
-      int best = gft::cached(key);
-      if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
-      if (best && (rc = plan_group(G, sims, best))) break;
-      for (int i : G.order)
-        if ((rc = sims[i]->reset_state())) break;
-      tile_rows = best;
-    } else {
-      tile_rows = G.th;
-    }
-  }
+  stream = sims[variants[0].order[0]]->stream;
   for (auto &sp : sims)
     if (sp && !rc) {
       hipError_t e = hipStreamSynchronize(sp->stream);
       if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
     }
 
+  // chunks: models sorted by step count (the running set is a prefix), cut
+  // where the cache budget would be exceeded
+  const double budget = cache_budget_bytes();
+  struct Chunk {
+    Group G;
+    int variant;
+  };
+  std::vector<Chunk> chunks;
+  std::vector<int64_t> variant_cell_steps(variants.size(), 0), variant_cells(variants.size(), 0);
+  for (size_t v = 0; v < variants.size(); v++) {
+    Group &V = variants[v];
+    std::stable_sort(V.order.begin(), V.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    double acc = 0;
+    for (int i : V.order) {
+      const double b = state_bytes(*sims[i]);
+      if (chunks.empty() || chunks.back().variant != (int)v || (budget > 0 && acc + b > budget)) {
+        chunks.push_back(Chunk{Group(), (int)v});
+        chunks.back().G.nmech = V.nmech;
+        chunks.back().G.scalar = V.scalar;
+        acc = 0;
+      }
+      chunks.back().G.order.push_back(i);
+      acc += b;
+      variant_cells[v] += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
+      variant_cell_steps[v] += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
+    }
+  }
+
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
-              hipEventRecord(e0, stream) != hipSuccess))
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event set-up failed");
   bytes = 0;
   for (int i = 0; i < n; i++)
     if (sims[i]) bytes += gfs::step_bytes(sims[i]->R) * steps[i];
   launches = 0;
-  for (int64_t k = 0; k < max_steps && !rc;) {
-    const bool graph_block = k + gfs::kGraphSteps <= max_steps;
-    for (Group &G : groups) launches += active_models(G, steps, k) > 0 ? (graph_block ? gfs::kGraphSteps : 1) : 0;
-    for (Group &G : groups) {
+  double total_ms = 0;
+  std::vector<int> variant_th(variants.size(), 0);  // tile height chosen for each variant
+
+  for (Chunk &C : chunks) {
+    if (rc) break;
+    Group &G = C.G;
+    int64_t cols_rows = 0;
+    for (int i : G.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
+    int th = variant_th[C.variant];
+    if ((rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows)))) break;
+    if (!th) {
+      // Long runs pay for a few trial steps at each tile height, once per
+      // variant, on its first chunk.
+      const char *tune_env = getenv("GEOFDTD_TUNE");
+      const bool want_tune = !getenv("GEOFDTD_TILE_ROWS") &&
+                             (tune_env ? atoi(tune_env) != 0 : variant_cell_steps[C.variant] >= 5000000000LL);
+      if (want_tune) {
+        gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(),
+                     (int64_t)log2((double)std::max<int64_t>(1, variant_cells[C.variant]))};
+        auto prepare = [&](int t) { return plan_group(G, sims, t) == GF_OK; };
+        auto launch = [&](int t, int k) {
+          return gfk::launch_step(G.nmech, G.scalar, t, G.d_descs, G.d_tile_model, G.prefix.back(), k, nullptr, 0,
+                                  stream);
+        };
+        int best = gft::cached(key);
+        if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
+        if (best && (rc = plan_group(G, sims, best))) break;
+        for (int i : G.order)
+          if ((rc = sims[i]->reset_state())) break;
+        if (rc) break;
+      }
+      variant_th[C.variant] = G.th;
+    }
+    tile_rows = G.th;
+    for (int i : G.order) {
+      hipError_t e = hipStreamSynchronize(sims[i]->stream);
+      if (e != hipSuccess) {
+        rc = gfs::hip_fail(e, "batch set-up");
+        break;
+      }
+    }
+    if (rc) break;
+    const int64_t max_steps = steps[G.order.front()];
+    if (hipEventRecord(e0, stream) != hipSuccess) {
+      rc = gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event record failed");
+      break;
+    }
+    for (int64_t k = 0; k < max_steps && !rc;) {
+      const bool graph_block = k + gfs::kGraphSteps <= max_steps;
       const int act = active_models(G, steps, k);
-      if (act == 0) continue;
+      if (act == 0) break;
+      launches += graph_block ? gfs::kGraphSteps : 1;
       if (graph_block) {
         // Re-capture when the running set has shrunk by a tenth of its tiles.
         if (!G.exec || G.graph_models < act ||
@@ -315,20 +367,22 @@ int Batch::run() {
           break;
         }
       }
+      k += graph_block ? gfs::kGraphSteps : 1;
     }
-    k += graph_block ? gfs::kGraphSteps : 1;
+    if (!rc) {
+      hipError_t e = hipEventRecord(e1, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
+      total_ms += ms;
+    }
+    free_group(G);
   }
-  if (!rc) {
-    hipError_t e = hipEventRecord(e1, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
-    kernel_ms = ms;
-  }
+  kernel_ms = total_ms;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  for (Group &G : groups) free_group(G);
+  for (Chunk &C : chunks) free_group(C.G);
   record_stats(*this);
   return rc;
 }
