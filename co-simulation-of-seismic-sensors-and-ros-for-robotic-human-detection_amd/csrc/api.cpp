@@ -95,6 +95,7 @@ int choose_tile_rows(int nr, int nz) {
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 Sim::~Sim() {
+  if (stream) (void)hipStreamSynchronize(stream);  // nothing of ours may still run when it is freed
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   if (arena) hipFree(arena);
@@ -112,6 +113,8 @@ Sim::~Sim() {
 
 int Sim::set_tile_rows(int th) {
   if (th < 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: tile rows must be positive");
+  // launches in flight read the graph and the descriptor about to change
+  HIPCHK(hipStreamSynchronize(stream));
   g.th = th;
   g.tiles_z = (g.nz + th - 1) / th;
   if (exec) hipGraphExecDestroy(exec);
@@ -354,11 +357,12 @@ int Sim::advance(int64_t n, bool use_graph, bool probe) {
 // of a captured node are fixed, the step number is not).
 int Sim::ensure_graph(bool probe) {
   if (exec && graph_probe == probe) return GF_OK;
+  // the old graph may still be running: wait before destroying it
+  HIPCHK(hipStreamSynchronize(stream));
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   exec = nullptr;
   graph = nullptr;
-  HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; k++) {
     hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, nullptr,

@@ -79,7 +79,19 @@ int active_models(const Group &G, const std::vector<int32_t> &steps, int64_t n) 
 
 // Descriptors of the group's models for tile height th, tile bases in run
 // order, uploaded to the group's device array (allocated on first use).
-int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th) {
+// Destroys a group's graph only once the stream it was launched on has
+// finished with it: an executable graph must not be destroyed while a launch
+// of it may still be running.
+int drop_graph(Group &G, hipStream_t st) {
+  if (G.exec || G.graph) HIPCHK(hipStreamSynchronize(st));
+  if (G.exec) hipGraphExecDestroy(G.exec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  return GF_OK;
+}
+
+int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th, hipStream_t st) {
   G.th = th;
   std::vector<gfk::GridDesc> descs;
   G.prefix.assign(1, 0);
@@ -114,19 +126,15 @@ int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th) {
     }
     HIPCHK(hipMemcpy(G.d_tile_model, tm.data(), sizeof(uint16_t) * tm.size(), hipMemcpyHostToDevice));
   }
-  if (G.exec) hipGraphExecDestroy(G.exec);
-  if (G.graph) hipGraphDestroy(G.graph);
-  G.exec = nullptr;
-  G.graph = nullptr;
+  int rc = drop_graph(G, st);
+  if (rc) return rc;
   G.graph_models = -1;
   return GF_OK;
 }
 
 int capture(Group &G, int models, hipStream_t st) {
-  if (G.exec) hipGraphExecDestroy(G.exec);
-  if (G.graph) hipGraphDestroy(G.graph);
-  G.exec = nullptr;
-  G.graph = nullptr;
+  int rc = drop_graph(G, st);
+  if (rc) return rc;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < gfs::kGraphSteps; k++) {
     hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[models], k, G.d_nbase, 1, st);
@@ -303,7 +311,7 @@ int Batch::run() {
     int64_t cols_rows = 0;
     for (int i : G.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
     int th = variant_th[C.variant];
-    if ((rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows)))) break;
+    if ((rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows), stream))) break;
     if (!th) {
       // Long runs pay for a few trial steps at each tile height, once per
       // variant, on its first chunk.
@@ -313,14 +321,14 @@ int Batch::run() {
       if (want_tune) {
         gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(),
                      (int64_t)log2((double)std::max<int64_t>(1, variant_cells[C.variant]))};
-        auto prepare = [&](int t) { return plan_group(G, sims, t) == GF_OK; };
+        auto prepare = [&](int t) { return plan_group(G, sims, t, stream) == GF_OK; };
         auto launch = [&](int t, int k) {
           return gfk::launch_step(G.nmech, G.scalar, t, G.d_descs, G.d_tile_model, G.prefix.back(), k, nullptr, 0,
                                   stream);
         };
         int best = gft::cached(key);
         if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
-        if (best && (rc = plan_group(G, sims, best))) break;
+        if (best && (rc = plan_group(G, sims, best, stream))) break;
         for (int i : G.order)
           if ((rc = sims[i]->reset_state())) break;
         if (rc) break;
@@ -380,6 +388,9 @@ int Batch::run() {
     free_group(G);
   }
   kernel_ms = total_ms;
+  // an error path can leave launches in flight: let them finish before any
+  // graph or table of theirs is freed
+  (void)hipStreamSynchronize(stream);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   for (Chunk &C : chunks) free_group(C.G);
