@@ -193,6 +193,38 @@ class GfSurStats(C.Structure):
     ]
 
 
+class GfMedium(C.Structure):
+    _fields_ = [
+        ("stack", C.POINTER(GfLayer)),
+        ("n_layers", C.c_int32),
+        ("ref_freq_hz", C.c_double),
+        ("default_q", C.c_double),
+        ("has_relax", C.c_int32),
+        ("relax", GfSls),
+    ]
+
+
+class GfIntegration(C.Structure):
+    _fields_ = [("samples", C.c_int32), ("kmax_factor", C.c_double)]
+
+
+class GfFkGrid(C.Structure):
+    _fields_ = [("kmax", C.c_double), ("samples", C.c_int32)]
+
+
+class GfFkStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("prologue_ms", C.c_double),
+        ("quadrature_ms", C.c_double),
+        ("fold_ms", C.c_double),
+        ("samples", C.c_int64),
+        ("weights", C.c_int64),
+        ("frequencies", C.c_int32),
+        ("launches", C.c_int32),
+    ]
+
+
 # Every symbol include/geofdtd.h declares, with its ctypes signature.
 _D, _I, _I32, _I64 = C.c_double, C.c_int, C.c_int32, C.c_int64
 _PD, _PI32, _PI64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
@@ -251,6 +283,14 @@ SIGNATURES = {
     "gf_sur_stats": (_I, [C.c_void_p, C.POINTER(GfSurStats)]),
     "gf_gemm_bf16": (_I, [_I32, _I32, _I32, C.POINTER(C.c_uint16), _I64, _I32, C.POINTER(C.c_uint16), _I64, _I32,
                           _I32, C.POINTER(C.c_float)]),
+    "gf_fk_surface_response": (_I, [C.POINTER(GfMedium), _PD, _PD, _I64, _PD]),
+    "gf_fk_static_coefficient": (_I, [C.POINTER(GfMedium), _D, _D, _PD]),
+    "gf_fk_grid_for": (_I, [C.POINTER(GfMedium), _PD, _I32, _D, GfIntegration, C.POINTER(GfFkGrid)]),
+    "gf_fk_check_layer_phase": (_I, [C.POINTER(GfMedium), _D]),
+    "gf_fk_vertical_displacement": (_I, [C.POINTER(GfMedium), _D, _D, GfIntegration, _PD]),
+    "gf_fk_vertical_displacement_multi": (_I, [C.POINTER(GfMedium), _PD, _I32, _PD, _I32, GfIntegration, _PD]),
+    "gf_fk_synthesise": (_I, [C.POINTER(GfMedium), _PD, _I32, _D, _D, _D, _D, _D, _I32, _I32, _PD]),
+    "gf_fk_stats": (_I, [C.POINTER(GfFkStats)]),
 }
 
 _lib: Optional[C.CDLL] = None

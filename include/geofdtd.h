@@ -407,6 +407,89 @@ int gf_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda
 /* Philox4x32-10 block (counter, key) -> 4 words, for checking the sampler. */
 void gf_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
 
+/* ---- frequency-wavenumber solver (the Go package internal/fk) ----------
+ * The layered boundary-value solve and the Hankel quadrature run on the GPU
+ * in float64, one wavenumber sample per thread and many frequencies per
+ * launch.  The host checks arguments and chooses grids exactly as the Go
+ * code does, so errors carry the Go text.  A stack holds at most
+ * GF_FK_MAX_LAYERS entries. */
+#define GF_FK_MAX_LAYERS 16
+
+/* fk.Medium (internal/fk/fk.go:35-53).  ref_freq_hz 0 -> 30 Hz; default_q 0
+ * -> 30, used by layers whose qs is 0.  has_relax != 0 is Relax != nil: one
+ * SLS applied to every layer in place of Kjartansson's constant-Q law.
+ * gf_layer.qp is not read: the solver takes Qp = 9/4 Qs (fk.go:119-120). */
+typedef struct gf_medium {
+  const gf_layer *stack;
+  int32_t n_layers;
+  double ref_freq_hz;
+  double default_q;
+  int32_t has_relax;
+  gf_sls relax;
+} gf_medium;
+
+/* fk.Integration (fk.go:309-331): samples 0 -> 20000 for one range, or the
+ * grid's own choice for many; kmax_factor 0 -> 30. */
+typedef struct gf_integration {
+  int32_t samples;
+  double kmax_factor;
+} gf_integration;
+
+/* fk.Grid (fk.go:439-443). */
+typedef struct gf_fk_grid {
+  double kmax;
+  int32_t samples;
+} gf_fk_grid;
+
+/* Medium.SurfaceResponse (fk.go:276-307) at n points (k[i], f_hz[i]):
+ * out [n][2] (re, im), m/N per unit Hankel measure.  The first point in order
+ * that the Go call would reject sets the error. */
+int gf_fk_surface_response(const gf_medium *m, const double *k, const double *f_hz, int64_t n, double *out);
+
+/* Medium.StaticCoefficient (fk.go:347-353): k_far * SurfaceResponse(k_far, f). */
+int gf_fk_static_coefficient(const gf_medium *m, double f_hz, double k_far, double out[2]);
+
+/* Medium.GridFor (fk.go:456-489). */
+int gf_fk_grid_for(const gf_medium *m, const double *ranges, int32_t n_ranges, double f_hz,
+                   gf_integration opt, gf_fk_grid *out);
+
+/* Medium.checkLayerPhase (fk.go:253-268), MaxLayerPhase 16. */
+int gf_fk_check_layer_phase(const gf_medium *m, double f_hz);
+
+/* Medium.VerticalDisplacement (fk.go:378-431): out (re, im), m/N. */
+int gf_fk_vertical_displacement(const gf_medium *m, double r_m, double f_hz, gf_integration opt, double out[2]);
+
+/* Medium.VerticalDisplacementMulti (fk.go:500-550) for n_freqs frequencies at
+ * once: out [n_freqs][n_ranges][2].  Each frequency gets the grid and error
+ * checks one Go call at that frequency would; the first frequency in order
+ * whose Go call would fail sets the error, and nothing is written then. */
+int gf_fk_vertical_displacement_multi(const gf_medium *m, const double *ranges, int32_t n_ranges,
+                                      const double *f_hz, int32_t n_freqs, gf_integration opt, double *out);
+
+/* synthesise in cmd/geofdtd (main.go:168-200): the f-k response to a Ricker
+ * force (peak_n, f0_hz) at each range, as vertical velocity on the instants
+ * t0 + i*dt, i < n, band-limited to band_hz, with `samples` wavenumber
+ * samples (0: the grid's choice).  The medium is stack + relax with
+ * has_relax set, as geofdtd -compare builds it.  out [n_ranges][n]. */
+int gf_fk_synthesise(const gf_medium *m, const double *ranges, int32_t n_ranges, double peak_n, double f0_hz,
+                     double band_hz, double dt, double t0, int32_t n, int32_t samples, double *out);
+
+/* Counters of the last f-k integration on the calling thread (HIP events
+ * on its stream): the per-frequency prologue (layer constants and the static
+ * coefficient), the fused solve + Hankel-sum kernel, and the fold of the
+ * partial sums; and the work done. */
+typedef struct gf_fk_stats_t {
+  double kernel_ms;     /* prologue + quadrature + fold                      */
+  double prologue_ms;
+  double quadrature_ms;
+  double fold_ms;
+  int64_t samples;      /* wavenumber samples solved, over all frequencies  */
+  int64_t weights;      /* samples x ranges Hankel terms J0(k r) summed      */
+  int32_t frequencies;
+  int32_t launches;
+} gf_fk_stats_t;
+int gf_fk_stats(gf_fk_stats_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,7 @@ def lib():
     fd = _fdtd_module()
     L = C.CDLL(LIB_PATH)
     M = C.POINTER(fd.GfModel)
+    MED = C.POINTER(fd.GfMedium)
     sls = fd.GfSls
     sig = {
         "orc_last_error": (C.c_char_p, []),
@@ -80,6 +81,15 @@ def lib():
             C.c_int,
             [M, _I32, C.POINTER(_PD), C.POINTER(_I64), _PD, _I32, _PI32, C.POINTER(_PD), _I32, _PI32],
         ),
+        "orc_fk_surface_response": (C.c_int, [MED, _D, _D, _PD, _PD]),
+        "orc_fk_static_coefficient": (C.c_int, [MED, _D, _D, _PD, _PD]),
+        "orc_fk_check_layer_phase": (C.c_int, [MED, _D]),
+        "orc_fk_vertical_displacement": (C.c_int, [MED, _D, _D, fd.GfIntegration, _PD, _PD]),
+        "orc_fk_grid_for": (C.c_int, [MED, _PD, C.c_int, _D, fd.GfIntegration, _PD, C.POINTER(C.c_longlong)]),
+        "orc_fk_vertical_displacement_multi": (C.c_int, [MED, _PD, C.c_int, _D, fd.GfIntegration, _PD]),
+        "orc_fk_synthesise_coeff": (C.c_int, [MED, _PD, C.c_int, _D, _D, _D, _D, _D, C.c_int, C.c_int, _PD,
+                                              C.POINTER(C.c_int)]),
+        "orc_fk_complex_velocity": (None, [MED, _D, _D, _D, _PD, _PD]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -322,3 +332,83 @@ def spacing_for(model, fmax, cpw):
     out = C.c_double()
     _check(lib().orc_spacing_for(C.byref(cm), fmax, cpw, C.byref(out)))
     return out.value
+
+
+# ------------------------------------------------------------ the f-k solver
+# Each function takes the product's fk.Medium (for its struct layout only)
+# and an fk.Integration-like object with Samples / KMaxFactor.
+
+
+def _integ(opt):
+    fd = _fdtd_module()
+    if opt is None:
+        return fd.GfIntegration(0, 0.0)
+    return fd.GfIntegration(int(opt.Samples), float(opt.KMaxFactor))
+
+
+def fk_surface_response(medium, k, f):
+    m, keep = medium.to_c()
+    re, im = C.c_double(), C.c_double()
+    _check(lib().orc_fk_surface_response(C.byref(m), k, f, C.byref(re), C.byref(im)))
+    return complex(re.value, im.value)
+
+
+def fk_static_coefficient(medium, f, k_far):
+    m, keep = medium.to_c()
+    re, im = C.c_double(), C.c_double()
+    _check(lib().orc_fk_static_coefficient(C.byref(m), f, k_far, C.byref(re), C.byref(im)))
+    return complex(re.value, im.value)
+
+
+def fk_check_layer_phase(medium, f):
+    m, keep = medium.to_c()
+    _check(lib().orc_fk_check_layer_phase(C.byref(m), f))
+
+
+def fk_vertical_displacement(medium, r, f, opt=None):
+    m, keep = medium.to_c()
+    re, im = C.c_double(), C.c_double()
+    _check(lib().orc_fk_vertical_displacement(C.byref(m), r, f, _integ(opt), C.byref(re), C.byref(im)))
+    return complex(re.value, im.value)
+
+
+def fk_grid_for(medium, ranges, f, opt=None):
+    m, keep = medium.to_c()
+    r = np.ascontiguousarray(ranges, dtype=np.float64)
+    kmax, n = C.c_double(), C.c_longlong()
+    _check(lib().orc_fk_grid_for(C.byref(m), _p(r) if len(r) else None, len(r), f, _integ(opt), C.byref(kmax),
+                                 C.byref(n)))
+    return kmax.value, n.value
+
+
+def fk_vertical_displacement_multi(medium, ranges, f, opt=None):
+    m, keep = medium.to_c()
+    r = np.ascontiguousarray(ranges, dtype=np.float64)
+    out = np.zeros((max(len(r), 1), 2))
+    _check(lib().orc_fk_vertical_displacement_multi(C.byref(m), _p(r) if len(r) else None, len(r), f, _integ(opt),
+                                                    _p(out)))
+    return out[: len(r), 0] + 1j * out[: len(r), 1]
+
+
+def fk_complex_velocity(medium, v, q, f):
+    m, keep = medium.to_c()
+    re, im = C.c_double(), C.c_double()
+    lib().orc_fk_complex_velocity(C.byref(m), v, q, f, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def fk_synthesise(medium, ranges, peak, f0, band, dt, t0, n, samples=0):
+    """synthesise (cmd/geofdtd/main.go:168-200): the oracle's spectra, then
+    numpy's inverse real FFT in place of dsp.IRFFT (the same transform:
+    unnormalised inverse times 1/N); [ranges][n]."""
+    m, keep = medium.to_c()
+    r = np.ascontiguousarray(ranges, dtype=np.float64)
+    nfft = C.c_int()
+    _check(lib().orc_fk_synthesise_coeff(C.byref(m), _p(r), len(r), peak, f0, band, dt, t0, n, samples, None,
+                                         C.byref(nfft)))
+    N = nfft.value
+    coeff = np.zeros((len(r), N // 2 + 1, 2))
+    _check(lib().orc_fk_synthesise_coeff(C.byref(m), _p(r), len(r), peak, f0, band, dt, t0, n, samples, _p(coeff),
+                                         C.byref(nfft)))
+    X = coeff[..., 0] + 1j * coeff[..., 1]
+    return np.fft.irfft(X, N, axis=1)[:, :n]

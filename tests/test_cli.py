@@ -68,12 +68,10 @@ def test_ranges_are_parsed_as_the_reference_does():
     assert (rc, err) == (1, 'geofdtd: parsing range "": strconv.ParseFloat: parsing "": invalid syntax\n')
 
 
-def test_compare_is_refused_after_the_reference_checks():
+def test_compare_needs_attenuation():
     rc, err = geofdtd("-model", HOMOGENEOUS, "-compare", "-q", "0")
-    assert rc == 1 and err.startswith("geofdtd: -compare needs attenuation: with no loss the Rayleigh pole")
-    rc, err = geofdtd("-model", HOMOGENEOUS, "-compare")
-    assert (rc, err) == (1, "geofdtd: -compare needs the frequency-wavenumber solver, which libgeofdtd does "
-                            "not include\n")
+    assert (rc, err) == (1, "geofdtd: -compare needs attenuation: with no loss the Rayleigh pole sits on the "
+                            "wavenumber integration path and the reference does not exist\n")
 
 
 def test_medium_file_errors(tmp_path):

@@ -70,3 +70,40 @@ def test_csv_matches_the_oracle(fdtd, oracle, tmp_path, medium, args, ranges, se
         for s in r:
             mant = s.lstrip("-").split("e")[0].replace(".", "").lstrip("0")
             assert len(mant) <= 8, s
+
+
+def test_compare_adds_the_fk_traces_of_the_oracle(fdtd, oracle, tmp_path):
+    """-compare (main.go:123-135, 168-200, 240-259): fk_r<range> columns equal
+    the oracle's synthesis at the snapped ranges to the 8 written digits, and
+    the summary table's rms residual is what the two columns give."""
+    path = os.path.join(MEDIA, "homogeneous.json")
+    out = tmp_path / "cmp.csv"
+    print("geofdtd -compare homogeneous", flush=True)
+    p = subprocess.run([BIN, "-model", path, "-o", str(out), "-compare"], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    ranges = [4.0, 8.0, 16.0]
+    m, dt, steps, force = reference_setup(fdtd, path, ranges)
+    want_fdtd = oracle.run(m, force, ranges, steps)
+    fk = __import__("conftest").load_package().fk
+    med = fk.Medium(Stack=m.Stack, Relax=m.Relax, RefFreq=30.0)
+    want = oracle.fk_synthesise(med, want_fdtd.snapped, 1000.0, 25.0, 3.5 * 25.0, dt, 0.5 * dt, steps)
+    rows = list(csv.reader(open(out, newline="")))
+    names = [f"{r:.2f}" for r in want_fdtd.snapped]
+    assert rows[0] == ["time_s"] + [f"fdtd_r{n}" for n in names] + [f"fk_r{n}" for n in names]
+    body = rows[1:]
+    assert len(body) == steps
+    grid = np.array([[float(r[1 + k]) for r in body] for k in range(3)])
+    ref = np.array([[float(r[4 + k]) for r in body] for k in range(3)])
+    for k in range(3):
+        assert np.max(np.abs(ref[k] - want[k])) <= 1e-7 * np.max(np.abs(want[k])), k
+    lines = p.stderr.splitlines()
+    i = lines.index("     range     grid peak      f-k peak     ratio       rms")
+    for k in range(3):
+        cols = lines[i + 1 + k].split()
+        rms = 100 * math.sqrt(np.sum((grid[k] - ref[k]) ** 2) / np.sum(ref[k] ** 2))
+        print(lines[i + 1 + k], flush=True)
+        assert cols[0] == f"{want_fdtd.snapped[k]:.2f}m"
+        assert abs(float(cols[4].rstrip("%")) - rms) < 0.011
+        # V5: the grid and the wavenumber integral describe the same field.
+        assert rms < 15.0
+    assert any(l.startswith("  reference ") and l.endswith(" s") for l in lines)

@@ -1,0 +1,177 @@
+"""The GPU f-k solver against the FP64 oracle, and the properties the
+reference's f-k tests assert, run on the GPU path itself.
+
+Tolerances: the GPU computes in float64 with fused multiply-adds and the
+device's libm, the oracle in float64 without fusion; the two differ by
+rounding, amplified where the layered solve is ill-conditioned (large k*h,
+stiff media at large k).  Point responses agree to 1e-9 relative, integrated
+displacements to 1e-9 relative to the largest value over the ranges of a
+frequency, synthesised traces to 1e-9 of their peak.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+pkg = load_package()
+fd, fk = pkg.fdtd, pkg.fk
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+
+LOAM = [fd.Layer(0, 500, 200, 1700, 25)]
+LAYERED = [fd.Layer(1.5, 300, 150, 1700, 20), fd.Layer(4, 500, 250, 1850, 40), fd.Layer(0, 800, 400, 2000, 80)]
+SLS = fd.Fit(25, 30)
+
+MEDIA = {
+    "loam": fk.Medium(Stack=LOAM),
+    "loam-lossless": fk.Medium(Stack=[fd.Layer(0, 500, 200, 1700, 1e9)], DefaultQ=1e9),
+    "layered": fk.Medium(Stack=LAYERED),
+    "layered-sls": fk.Medium(Stack=LAYERED, Relax=SLS, RefFreq=30),
+    "rock-over-soil": fk.Medium(Stack=[fd.Layer(3, 400, 160, 1600, 0), fd.Layer(0, 2200, 1100, 2400, 0)],
+                                DefaultQ=12),
+}
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b)))
+
+
+@pytest.mark.parametrize("name", list(MEDIA))
+def test_surface_response_equals_the_oracle(oracle, name):
+    m = MEDIA[name]
+    print(f"[fk] surface response {name}", flush=True)
+    ks = np.array([1e-4, 0.003, 0.05, 0.3, 1.0, 2.5, 7.0, 20.0, 60.0])
+    fs = np.array([0.5, 5.0, 20.0, 47.3, 90.0])
+    K, F = np.meshgrid(ks, fs)
+    got = m.SurfaceResponses(K, F)
+    for i in range(K.shape[0]):
+        for j in range(K.shape[1]):
+            want = oracle.fk_surface_response(m, K[i, j], F[i, j])
+            assert abs(got[i, j] - want) <= 1e-9 * abs(want), (K[i, j], F[i, j], got[i, j], want)
+
+
+@pytest.mark.parametrize("name", ["loam", "layered", "layered-sls", "rock-over-soil"])
+def test_multi_equals_the_oracle(oracle, name):
+    m = MEDIA[name]
+    ranges = [0.5, 1.0, 2.25, 7.0, 16.0, 33.3, 50.0]
+    freqs = [3.0, 12.5, 30.0, 61.0, 80.0]
+    print(f"[fk] multi {name}", flush=True)
+    got = m.VerticalDisplacementMulti(ranges, freqs)
+    assert got.shape == (len(freqs), len(ranges))
+    for i, f in enumerate(freqs):
+        want = oracle.fk_vertical_displacement_multi(m, ranges, f)
+        assert rel(got[i], want) < 1e-9, (f, rel(got[i], want))
+    st = fk.Stats()
+    assert st["frequencies"] == len(freqs)
+    assert st["samples"] == sum(m.GridFor(ranges, f).Samples for f in freqs)
+    assert st["weights"] == st["samples"] * len(ranges)
+    assert st["kernel_ms"] > 0
+
+
+def test_single_range_and_static_coefficient_equal_the_oracle(oracle):
+    m = MEDIA["layered"]
+    for r, f, opt in [(3.0, 25.0, fk.Integration()), (12.0, 70.0, fk.Integration(Samples=3000, KMaxFactor=20))]:
+        got = m.VerticalDisplacement(r, f, opt)
+        want = oracle.fk_vertical_displacement(m, r, f, opt)
+        assert abs(got - want) <= 1e-9 * abs(want)
+    got = m.StaticCoefficient(40.0, 300.0)
+    want = oracle.fk_static_coefficient(m, 40.0, 300.0)
+    assert abs(got - want) <= 1e-9 * abs(want)
+
+
+def test_a_batch_equals_its_frequencies_one_at_a_time():
+    m = MEDIA["layered-sls"]
+    ranges = [1.0, 4.0, 9.5]
+    freqs = [7.0, 19.0, 44.0, 71.0]
+    batch = m.VerticalDisplacementMulti(ranges, freqs)
+    for i, f in enumerate(freqs):
+        one = m.VerticalDisplacementMulti(ranges, f)
+        assert np.array_equal(batch[i], one)
+
+
+@pytest.mark.parametrize("name,vp,vs,rho", [("loam", 500, 200, 1700), ("dry sand", 350, 150, 1600),
+                                            ("weathered rock", 1800, 800, 2300)])
+def test_boussinesq_limits_on_the_gpu(name, vp, vs, rho):
+    m = fk.Medium(Stack=[fd.Layer(0, vp, vs, rho, 1e9)], DefaultQ=1e9)
+    nu = (vp * vp - 2 * vs * vs) / (2 * (vp * vp - vs * vs))
+    coef = (1 - nu) / (TWO_PI * rho * vs * vs)
+    u = m.SurfaceResponses([1.0, 10.0, 200.0], 0.05)
+    assert np.all(np.abs(np.array([1.0, 10.0, 200.0]) * np.abs(u) - coef) / coef < 1e-3)
+    if name == "loam":
+        for r in (2, 5, 10, 20):
+            got = abs(m.VerticalDisplacement(r, 0.02, fk.Integration(Samples=8000)))
+            assert abs(got - coef / r) / (coef / r) < 0.02
+
+
+def test_quadrature_is_second_order_on_the_gpu():
+    m = MEDIA["loam"]
+    r, f = 16.0, 60.0
+    n0 = m.GridFor([r], f).Samples
+    vd = lambda n: m.VerticalDisplacement(r, f, fk.Integration(Samples=n))
+    ref = (4 * vd(32 * n0) - vd(16 * n0)) / 3
+    errs = [abs(vd(n) - ref) / abs(ref) for n in (n0, 2 * n0, 4 * n0)]
+    assert errs[0] < 1e-3
+    assert all(math.log2(a / b) > 1.5 for a, b in zip(errs, errs[1:]))
+
+
+def test_identical_layers_equal_their_half_space_on_the_gpu():
+    half = fk.Medium(Stack=LOAM, DefaultQ=30)
+    lay = fd.Layer(2, 500, 200, 1700, 25)
+    split = fk.Medium(Stack=[lay, lay, fd.Layer(0, 500, 200, 1700, 25)], DefaultQ=30)
+    a = half.VerticalDisplacementMulti([10.0], [10.0, 30.0, 100.0, 200.0])
+    b = split.VerticalDisplacementMulti([10.0], [10.0, 30.0, 100.0, 200.0])
+    assert np.all(np.abs(a - b) / np.abs(a) < 1e-5)
+
+
+ERRORS = [
+    ("multi", dict(ranges=[3.0, 0.0], freqs=[20.0]), "fk: range must be positive, got 0 m"),
+    ("multi", dict(ranges=[], freqs=[20.0]), "fk: no ranges given"),
+    ("multi", dict(ranges=[3.0], freqs=[20.0, -5.0]), None),
+    ("multi", dict(ranges=[3.0], freqs=[20.0, 600.0]), None),
+    ("vd", dict(r=0.0, f=10.0), "fk: range must be positive, got 0 m"),
+    ("vd", dict(r=5.0, f=0.0), "fk: frequency must be positive, got 0 Hz"),
+    ("surface", dict(k=0.0, f=10.0), "fk: wavenumber and frequency must be positive, got 0 and 10"),
+]
+
+
+@pytest.mark.parametrize("kind,args,message", ERRORS)
+def test_errors_carry_the_reference_text(oracle, kind, args, message):
+    m = MEDIA["layered"]
+    with pytest.raises(fd.FdtdError) as got:
+        if kind == "multi":
+            m.VerticalDisplacementMulti(args["ranges"], args["freqs"])
+        elif kind == "vd":
+            m.VerticalDisplacement(args["r"], args["f"])
+        else:
+            m.SurfaceResponse(args["k"], args["f"])
+    if message is None:
+        # The first frequency the Go calls would fail on, as the oracle says.
+        with pytest.raises(oracle.OracleError) as want:
+            for f in args["freqs"]:
+                oracle.fk_vertical_displacement_multi(m, args["ranges"], f)
+        message = str(want.value)
+    assert str(got.value) == message
+    bad = fk.Medium(Stack=[fd.Layer(0, 100, 200, 1700)])
+    with pytest.raises(fd.FdtdError, match=r"layer 0: Vp=100 must exceed sqrt\(2\)\*Vs=282.842712474619"):
+        bad.VerticalDisplacementMulti([1.0], 10.0)
+
+
+def test_synthesise_equals_the_oracle(oracle):
+    # geofdtd -compare's reference on a small case: loam with a fitted SLS,
+    # a 40 Hz Ricker, dt 2e-4, 1500 instants (nfft 2048), band 140 Hz.
+    dt, n, f0, band, peak = 2e-4, 1500, 40.0, 140.0, 800.0
+    ranges = [1.0, 2.5, 6.0]
+    sls = fd.Fit(30, 30)
+    t0 = dt / 2
+    print("[fk] synthesise", flush=True)
+    got = fk.Synthesise(LOAM, sls, 30, ranges, peak, f0, band, dt, t0, n, 0)
+    med = fk.Medium(Stack=LOAM, Relax=sls, RefFreq=30)
+    want = oracle.fk_synthesise(med, ranges, peak, f0, band, dt, t0, n)
+    assert got.shape == want.shape == (3, n)
+    for i in range(3):
+        assert np.max(np.abs(got[i] - want[i])) <= 1e-9 * np.max(np.abs(want[i])), i
