@@ -513,3 +513,67 @@ int orc_fk_synthesise_coeff(const gf_medium *m, const double *ranges, int n_rang
   free(u);
   return GF_OK;
 }
+
+/* The quadrature of VerticalDisplacementMulti on a given grid (k_max, n) and
+ * static-coefficient wavenumber k_far, summed in the order the GPU kernels
+ * sum (libgeofdtd's fk.hip): samples in workgroups of 256; within one, for
+ * each group of G = min(256, ranges left) ranges, S = 256 / G interleaved
+ * slices (sample ii in slice ii mod S) summed in order, then the slices in
+ * order; the workgroup partials are added in order to C/r - (dk/2) C.  Every
+ * term is the Go code's w * J0(k r) * dk * (k u - C); only the order of the
+ * additions differs from the reference's sequential sum, so that comparing
+ * the GPU with this isolates per-term rounding.  out [n_ranges][2]. */
+int orc_fk_quadrature_workgroup_order(const gf_medium *m, const double *ranges, int n_ranges, double f, double k_max,
+                                      long long n, double k_far, double *out) {
+  enum { B = 256 };
+  cx c;
+  int rc = orc_fk_static_coefficient(m, f, k_far, &c.re, &c.im);
+  if (rc) return rc;
+  double dk = k_max / (double)n;
+  cx *acc = (cx *)malloc(sizeof(cx) * (size_t)n_ranges);
+  cx *rem = (cx *)malloc(sizeof(cx) * B);
+  double *kk = (double *)malloc(sizeof(double) * B), *ww = (double *)malloc(sizeof(double) * B);
+  cx *slice = (cx *)malloc(sizeof(cx) * B);
+  if (!acc || !rem || !kk || !ww || !slice) {
+    free(acc), free(rem), free(kk), free(ww), free(slice);
+    return fail(GF_ERR_ARGUMENT, "oracle: out of memory");
+  }
+  for (int j = 0; j < n_ranges; j++) acc[j] = csub(go_cdiv(c, C(ranges[j], 0)), cmul(C(0.5 * dk, 0), c));
+  for (long long first = 1; first <= n; first += B) {
+    int cnt = (int)((n - first + 1) < B ? (n - first + 1) : B);
+    for (int t = 0; t < cnt; t++) {
+      long long i = first + t;
+      double k = (double)i * dk;
+      cx u;
+      rc = surface(m, k, f, &u);
+      if (rc) {
+        free(acc), free(rem), free(kk), free(ww), free(slice);
+        return rc;
+      }
+      kk[t] = k;
+      ww[t] = i == n ? 0.5 : 1.0;
+      rem[t] = csub(cmul(C(k, 0), u), c);
+    }
+    for (int g0 = 0; g0 < n_ranges; g0 += B) {
+      int G = (n_ranges - g0) < B ? (n_ranges - g0) : B;
+      int S = B / G;
+      for (int jr = 0; jr < G; jr++) {
+        double r = ranges[g0 + jr];
+        for (int s = 0; s < S; s++) {
+          cx a = C(0, 0);
+          for (int ii = s; ii < cnt; ii += S) a = cadd(a, cmul(C(ww[ii] * j0(kk[ii] * r) * dk, 0), rem[ii]));
+          slice[s] = a;
+        }
+        cx tot = slice[0];
+        for (int s = 1; s < S; s++) tot = cadd(tot, slice[s]);
+        acc[g0 + jr] = cadd(acc[g0 + jr], tot);
+      }
+    }
+  }
+  for (int j = 0; j < n_ranges; j++) {
+    out[2 * j] = acc[j].re;
+    out[2 * j + 1] = acc[j].im;
+  }
+  free(acc), free(rem), free(kk), free(ww), free(slice);
+  return GF_OK;
+}

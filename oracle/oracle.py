@@ -33,16 +33,31 @@ def build() -> str:
     return LIB_PATH
 
 
-def lib():
-    global _lib
+_lib_fma = None
+LIB_FMA_PATH = os.path.join(HERE, "lib", "liboracle_fdtd_fma.so")
+
+
+def lib(fma: bool = False):
+    """The oracle library; fma=True: the same code built with fused
+    multiply-adds (for measuring sensitivity to per-operation rounding)."""
+    global _lib, _lib_fma
+    if fma:
+        if _lib_fma is None:
+            if not os.path.exists(LIB_FMA_PATH):
+                build()
+            _lib_fma = _load(LIB_FMA_PATH)
+        return _lib_fma
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    from importlib import import_module
+    _lib = _load(LIB_PATH)
+    return _lib
 
+
+def _load(path):
     fd = _fdtd_module()
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(path)
     M = C.POINTER(fd.GfModel)
     MED = C.POINTER(fd.GfMedium)
     sls = fd.GfSls
@@ -90,12 +105,12 @@ def lib():
         "orc_fk_synthesise_coeff": (C.c_int, [MED, _PD, C.c_int, _D, _D, _D, _D, _D, C.c_int, C.c_int, _PD,
                                               C.POINTER(C.c_int)]),
         "orc_fk_complex_velocity": (None, [MED, _D, _D, _D, _PD, _PD]),
+        "orc_fk_quadrature_workgroup_order": (C.c_int, [MED, _PD, C.c_int, _D, _D, C.c_longlong, _D, _PD]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
     return L
 
 
@@ -346,17 +361,17 @@ def _integ(opt):
     return fd.GfIntegration(int(opt.Samples), float(opt.KMaxFactor))
 
 
-def fk_surface_response(medium, k, f):
+def fk_surface_response(medium, k, f, fma=False):
     m, keep = medium.to_c()
     re, im = C.c_double(), C.c_double()
-    _check(lib().orc_fk_surface_response(C.byref(m), k, f, C.byref(re), C.byref(im)))
+    _check(lib(fma).orc_fk_surface_response(C.byref(m), k, f, C.byref(re), C.byref(im)))
     return complex(re.value, im.value)
 
 
-def fk_static_coefficient(medium, f, k_far):
+def fk_static_coefficient(medium, f, k_far, fma=False):
     m, keep = medium.to_c()
     re, im = C.c_double(), C.c_double()
-    _check(lib().orc_fk_static_coefficient(C.byref(m), f, k_far, C.byref(re), C.byref(im)))
+    _check(lib(fma).orc_fk_static_coefficient(C.byref(m), f, k_far, C.byref(re), C.byref(im)))
     return complex(re.value, im.value)
 
 
@@ -365,10 +380,10 @@ def fk_check_layer_phase(medium, f):
     _check(lib().orc_fk_check_layer_phase(C.byref(m), f))
 
 
-def fk_vertical_displacement(medium, r, f, opt=None):
+def fk_vertical_displacement(medium, r, f, opt=None, fma=False):
     m, keep = medium.to_c()
     re, im = C.c_double(), C.c_double()
-    _check(lib().orc_fk_vertical_displacement(C.byref(m), r, f, _integ(opt), C.byref(re), C.byref(im)))
+    _check(lib(fma).orc_fk_vertical_displacement(C.byref(m), r, f, _integ(opt), C.byref(re), C.byref(im)))
     return complex(re.value, im.value)
 
 
@@ -412,3 +427,31 @@ def fk_synthesise(medium, ranges, peak, f0, band, dt, t0, n, samples=0):
                                          C.byref(nfft)))
     X = coeff[..., 0] + 1j * coeff[..., 1]
     return np.fft.irfft(X, N, axis=1)[:, :n]
+
+
+def _kbody(medium, f):
+    return 2 * 3.141592653589793 * f / min(l.Vs for l in medium.Stack)
+
+
+def fk_multi_workgroup_order(medium, ranges, f, opt=None, fma=False):
+    """VerticalDisplacementMulti's grid and terms, summed in the GPU's order."""
+    kmax, n = fk_grid_for(medium, ranges, f, opt)
+    k_far = max(200 * _kbody(medium, f), 400 / min(ranges))
+    return _quad_wg(medium, ranges, f, kmax, n, k_far, fma)
+
+
+def fk_vd_workgroup_order(medium, r, f, opt=None, fma=False):
+    """VerticalDisplacement's grid and terms, summed in the GPU's order."""
+    kb = _kbody(medium, f)
+    factor = opt.KMaxFactor if opt is not None and opt.KMaxFactor > 0 else 30.0
+    n = opt.Samples if opt is not None and opt.Samples > 0 else 20000
+    kmax = max(factor * kb, 40 / r)
+    return complex(_quad_wg(medium, [r], f, kmax, n, max(200 * kb, 400 / r), fma)[0])
+
+
+def _quad_wg(medium, ranges, f, kmax, n, k_far, fma=False):
+    m, keep = medium.to_c()
+    r = np.ascontiguousarray(ranges, dtype=np.float64)
+    out = np.zeros((len(r), 2))
+    _check(lib(fma).orc_fk_quadrature_workgroup_order(C.byref(m), _p(r), len(r), f, kmax, n, k_far, _p(out)))
+    return out[:, 0] + 1j * out[:, 1]

@@ -275,3 +275,16 @@ def test_host_guard_and_errors_equal_the_oracle(fdtd, oracle):
     with pytest.raises(fdtd.FdtdError) as e:
         fk.Medium(Stack=[]).GridFor([1], 10)
     assert str(e.value) == "layer: a stack needs at least a half-space"
+
+
+def test_workgroup_order_sum_equals_the_go_order(oracle):
+    """The oracle's GPU-order quadrature (the comparison basis of the GPU
+    tests) differs from the reference's sequential sum only in rounding."""
+    m = fk.Medium(Stack=[fd.Layer(1.5, 300, 150, 1700, 20), fd.Layer(0, 800, 400, 2000, 80)])
+    for ranges, f in (([0.5, 3.0, 20.0], 4.0), ([float(r) for r in range(1, 301)], 35.0)):
+        a = oracle.fk_multi_workgroup_order(m, ranges, f)
+        b = oracle.fk_vertical_displacement_multi(m, ranges, f)
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < 1e-12
+    a = oracle.fk_vd_workgroup_order(m, 7.0, 22.0, fk.Integration(Samples=777))
+    b = oracle.fk_vertical_displacement(m, 7.0, 22.0, fk.Integration(Samples=777))
+    assert abs(a - b) / abs(b) < 1e-12

@@ -1,12 +1,21 @@
 """The GPU f-k solver against the FP64 oracle, and the properties the
 reference's f-k tests assert, run on the GPU path itself.
 
-Tolerances: the GPU computes in float64 with fused multiply-adds and the
-device's libm, the oracle in float64 without fusion; the two differ by
-rounding, amplified where the layered solve is ill-conditioned (large k*h,
-stiff media at large k).  Point responses agree to 1e-9 relative, integrated
-displacements to 1e-9 relative to the largest value over the ranges of a
-frequency, synthesised traces to 1e-9 of their peak.
+Tolerances: the GPU computes in float64 with fused multiply-adds, the
+device's libm and the quadrature summed per workgroup; the oracle in float64
+without fusion, glibc's libm and a sequential sum.  The layered solve
+amplifies rounding where it is ill-conditioned: past k*h = 16
+(fk.go:225-247), where P and SV eigenvectors nearly coincide (k far above
+the body-wave wavenumbers, fk_test.go:44-54), and where the integral cancels
+(low frequencies).  Each comparison is therefore calibrated by the oracle's
+own sensitivity to rounding: the oracle built with fused multiply-adds
+(oracle/Makefile) and the oracle on the medium moved by about one ulp give
+two measures of it, and the GPU must agree with the oracle within 10 times
+the larger, or 1e-11 relative, whichever is larger.  Measured, the GPU's
+difference is within a factor 2 of the fused build's everywhere.  Where the
+solve is well conditioned (point responses with k*h <= 16) the bound is a
+flat 1e-9.  Integrated results are compared with the oracle summing in the
+GPU's order, and that order is checked against the Go order separately.
 """
 
 import math
@@ -41,6 +50,22 @@ def rel(a, b):
     return np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b)))
 
 
+def nudged(m, e):
+    """The medium with every velocity and density scaled by (1 + e)."""
+    st = [fd.Layer(l.Thickness, l.Vp * (1 + e), l.Vs * (1 + e), l.Density * (1 + e), l.Qs, l.Qp) for l in m.Stack]
+    return fk.Medium(Stack=st, RefFreq=m.RefFreq, DefaultQ=m.DefaultQ, Relax=m.Relax)
+
+
+NUDGES = (2.3e-16, -3.4e-16)
+
+
+def bound(call, m, want):
+    """10x the oracle's own rounding sensitivity, at least 1e-11.  `call`
+    takes (medium, fma) and evaluates the oracle."""
+    spread = max([rel(call(m, True), want)] + [rel(call(nudged(m, e), False), want) for e in NUDGES])
+    return max(10 * spread, 1e-11)
+
+
 @pytest.mark.parametrize("name", list(MEDIA))
 def test_surface_response_equals_the_oracle(oracle, name):
     m = MEDIA[name]
@@ -49,10 +74,21 @@ def test_surface_response_equals_the_oracle(oracle, name):
     fs = np.array([0.5, 5.0, 20.0, 47.3, 90.0])
     K, F = np.meshgrid(ks, fs)
     got = m.SurfaceResponses(K, F)
+    # Past k*h = 16 (the reference's MaxLayerPhase, fk.go:248) the direct
+    # propagator amplifies rounding (fk.go:225-247), so there the two FP64
+    # evaluations agree less closely.
+    hmax = max([l.Thickness for l in m.Stack[:-1]], default=0.0)
+    worst = {True: 0.0, False: 0.0}
     for i in range(K.shape[0]):
         for j in range(K.shape[1]):
-            want = oracle.fk_surface_response(m, K[i, j], F[i, j])
-            assert abs(got[i, j] - want) <= 1e-9 * abs(want), (K[i, j], F[i, j], got[i, j], want)
+            k, f = K[i, j], F[i, j]
+            want = oracle.fk_surface_response(m, k, f)
+            err = abs(got[i, j] - want) / abs(want)
+            well = k * hmax <= 16
+            worst[well] = max(worst[well], err)
+            tol = 1e-9 if well else bound(lambda mm, fma: oracle.fk_surface_response(mm, k, f, fma), m, want)
+            assert err <= tol, (k, f, got[i, j], want, tol)
+    print(f"[fk] {name}: worst rel diff {worst[True]:.1e} (k*h <= 16), {worst[False]:.1e} beyond", flush=True)
 
 
 @pytest.mark.parametrize("name", ["loam", "layered", "layered-sls", "rock-over-soil"])
@@ -64,8 +100,13 @@ def test_multi_equals_the_oracle(oracle, name):
     got = m.VerticalDisplacementMulti(ranges, freqs)
     assert got.shape == (len(freqs), len(ranges))
     for i, f in enumerate(freqs):
-        want = oracle.fk_vertical_displacement_multi(m, ranges, f)
-        assert rel(got[i], want) < 1e-9, (f, rel(got[i], want))
+        call = lambda mm, fma: oracle.fk_multi_workgroup_order(mm, ranges, f, None, fma)
+        want = call(m, False)
+        tol = bound(call, m, want)
+        print(f"[fk] {name} {f} Hz: rel diff {rel(got[i], want):.1e}, bound {tol:.1e}", flush=True)
+        assert rel(got[i], want) <= tol, (f, rel(got[i], want), tol)
+        # The GPU's summation order against the reference's sequential one.
+        assert rel(want, oracle.fk_vertical_displacement_multi(m, ranges, f)) < 1e-12
     st = fk.Stats()
     assert st["frequencies"] == len(freqs)
     assert st["samples"] == sum(m.GridFor(ranges, f).Samples for f in freqs)
@@ -77,11 +118,14 @@ def test_single_range_and_static_coefficient_equal_the_oracle(oracle):
     m = MEDIA["layered"]
     for r, f, opt in [(3.0, 25.0, fk.Integration()), (12.0, 70.0, fk.Integration(Samples=3000, KMaxFactor=20))]:
         got = m.VerticalDisplacement(r, f, opt)
-        want = oracle.fk_vertical_displacement(m, r, f, opt)
-        assert abs(got - want) <= 1e-9 * abs(want)
+        call = lambda mm, fma: oracle.fk_vd_workgroup_order(mm, r, f, opt, fma)
+        want = call(m, False)
+        assert abs(got - want) / abs(want) <= bound(call, m, want)
+        assert abs(want - oracle.fk_vertical_displacement(m, r, f, opt)) / abs(want) < 1e-12
     got = m.StaticCoefficient(40.0, 300.0)
-    want = oracle.fk_static_coefficient(m, 40.0, 300.0)
-    assert abs(got - want) <= 1e-9 * abs(want)
+    call = lambda mm, fma: oracle.fk_static_coefficient(mm, 40.0, 300.0, fma)
+    want = call(m, False)
+    assert abs(got - want) / abs(want) <= bound(call, m, want)
 
 
 def test_a_batch_equals_its_frequencies_one_at_a_time():
