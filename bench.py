@@ -222,6 +222,91 @@ def bench_c3(args, fd) -> int:
     return 0
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fk_workload(fd):
+    """The frequency-wavenumber bank of the config-2 layered medium: geobank's
+    frequency grid (2000 Hz, 4096 samples: 0.488 Hz bins, cmd/geobank/main.go)
+    up to 80 Hz (the layer-phase guard allows 83 Hz for the 4 m layer), 50
+    ranges 1-50 m, Kjartansson constant Q from each layer's Qs."""
+    fd_ = load_package().fdtd
+    vs, rho, qs, th = [150.0, 250.0, 400.0], [1700.0, 1850.0, 2000.0], [20.0, 40.0, 80.0], [1.5, 4.0, 0.0]
+    stack = [fd_.Layer(th[i], 2 * vs[i], vs[i], rho[i], qs[i]) for i in range(3)]
+    medium = load_package().fk.Medium(Stack=stack)
+    bin_hz = 2000.0 / 4096
+    freqs = [b * bin_hz for b in range(1, int(80.0 / bin_hz) + 1)]
+    ranges = [float(r) for r in range(1, 51)]
+    return medium, freqs, ranges
+
+
+def bench_fk(args, fd, standalone=True):
+    """f-k bank generation (the reference's bank filler, cmd/geobank over
+    fk.VerticalDisplacementMulti): every frequency of one bank in one set of
+    launches; a step is one bank."""
+    import numpy as np
+
+    fk = load_package().fk
+    medium, freqs, ranges = fk_workload(fd)
+    for i in range(args.warmup):
+        medium.VerticalDisplacementMulti(ranges, freqs)
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for i in range(args.steps):
+        g = medium.VerticalDisplacementMulti(ranges, freqs)
+        kernel_ms += fk.Stats()["kernel_ms"]
+    wall = time.perf_counter() - t0
+    st = fk.Stats()
+    if not np.all(np.isfinite(g)):
+        raise SystemExit("non-finite f-k bank")
+    log(f"fk: {len(freqs)} bins x {len(ranges)} ranges, {st['samples']} samples per bank, "
+        f"{args.steps} banks in {wall:.3f} s ({kernel_ms:.1f} ms of kernels)")
+    res = {
+        "value": len(ranges) * args.steps / wall,
+        "unit": "traces/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": {
+            "workload": "fk: one Green's-function bank of the config-2 layered medium by wavenumber integration; "
+                        f"{len(freqs)} bins of 0.488 Hz to 80 Hz, {len(ranges)} ranges 1-50 m",
+            "bins": len(freqs), "ranges": len(ranges), "samples_per_bank": st["samples"],
+            "hankel_terms_per_bank": st["weights"],
+        },
+        "kernel_ms_per_bank": kernel_ms / args.steps,
+        "quadrature_ms_per_bank": st["quadrature_ms"],
+        "samples_per_s": st["samples"] * args.steps / wall,
+        "dtype": "f64",
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from concurrent.futures import ThreadPoolExecutor
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = freqs[::10]
+        n_all = sum(medium.GridFor(ranges, f).Samples for f in freqs)
+        n_sample = sum(medium.GridFor(ranges, f).Samples for f in sample)
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda f: O.fk_vertical_displacement_multi(medium, ranges, f), sample))
+        el = time.perf_counter() - t1
+        bank_s = el * n_all / n_sample
+        res["cpu_baseline"] = {
+            "value": len(ranges) / bank_s, "unit": "traces/s", "cores": threads, "kind": "port",
+            "sample": f"every 10th bin ({len(sample)} bins, {n_sample} samples) over {threads} threads in "
+                      f"{el:.1f} s, scaled to the bank by wavenumber samples",
+        }
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    if standalone:
+        out = {"metric": METRIC, "value": res["value"], "unit": "traces/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": res["config"], "roofline": None, **{k: v for k, v in res.items() if k not in (
+                   "value", "unit", "ms_per_step", "config", "dtype")}}
+        print(json.dumps(out), flush=True)
+        return 0
+    return res
+
+
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -430,10 +515,12 @@ def main() -> int:
     ap.add_argument("--workload", default="c2-X",
                     help="c2-X (default: config 2 as stated, per-layer Q), c2-R (the reference's one-SLS physics), "
                          "c1-R, c1-X, c3 (batched bank generation), surrogate (train on FDTD banks, serve held-out "
-                         "queries behind the FDTD error gate)")
+                         "queries behind the FDTD error gate), fk (one bank by wavenumber integration)")
     ap.add_argument("--also", default="c2-R", help="second workload measured the same way and reported under "
                     "'also' (empty to skip)")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
+    ap.add_argument("--no-fk", dest="fk", action="store_false",
+                    help="skip the f-k bank line reported under 'fk' (the wavenumber-integration bank filler)")
     ap.add_argument("--cpu-sample-steps", type=int, default=400)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu_count)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -484,6 +571,8 @@ def main() -> int:
         return bench_c3(args, fd)
     if args.workload == "surrogate":
         return bench_surrogate(args, fd)
+    if args.workload == "fk":
+        return bench_fk(args, fd)
 
     def measure(name):
         w = workload(name)
@@ -573,6 +662,8 @@ def main() -> int:
     if args.also:
         _, _, also = measure(args.also)
         out["also"] = {"workload": args.also, **also}
+    if args.fk:
+        out["fk"] = bench_fk(args, fd, standalone=False)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -19,8 +19,8 @@
 //                   LDS; the workgroup then forms the Hankel sums for every
 //                   range from LDS (threads split as ranges x sample slices)
 //                   and writes one partial sum per range.
-//   k_fk_fold       one thread per (frequency, range): C/r - dk/2 C plus the
-//                   workgroup partials in sample order.
+//   k_fk_fold       one wavefront per (frequency, range): C/r - dk/2 C plus
+//                   the workgroup partials, summed in a fixed order.
 // The solve is FP64-VALU work with register-resident 4x4 matrices; HBM
 // traffic is a few bytes per sample (the partials), so the bound is the FP64
 // vector rate.  Row exchanges in the eliminations are selects over
@@ -437,19 +437,33 @@ __global__ __launch_bounds__(kBlock) void k_fk_quadrature(Medium M, const Job *_
   }
 }
 
-// out[j][r] = C/r - (dk/2) C + the partials of job j's workgroups in order.
-__global__ void k_fk_fold(const Job *__restrict__ jobs, int32_t njobs, const dcx *__restrict__ cstat,
-                          const double *__restrict__ ranges, int32_t nranges, const dcx *__restrict__ partial,
-                          dcx *__restrict__ out) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)njobs * nranges) return;
-  int j = (int)(idx / nranges), r = (int)(idx % nranges);
+// out[j][r] = C/r - (dk/2) C + the sum of job j's workgroup partials, one
+// wavefront per (job, range): lane l adds partials l, l+64, l+128, ... in
+// order, then the 64 lane sums are added pairwise at distances 32, 16, ...,
+// 1 (lane l takes lane l+d), a fixed order, so results do not depend on
+// timing.  Low frequencies have thousands of workgroups; one thread per sum
+// left those chains serial.
+__global__ __launch_bounds__(256) void k_fk_fold(const Job *__restrict__ jobs, int32_t njobs,
+                                                 const dcx *__restrict__ cstat, const double *__restrict__ ranges,
+                                                 int32_t nranges, const dcx *__restrict__ partial,
+                                                 dcx *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (idx >= (int64_t)njobs * nranges) return;  // whole wavefronts only
+  const int j = (int)(idx / nranges), r = (int)(idx % nranges);
   const Job J = jobs[j];
-  const dcx C = cstat[j];
-  dcx acc = csub(cdiv(C, mk(ranges[r], 0)), rmul(0.5 * J.dk, C));
-  int nb = (J.n + kBlock - 1) / kBlock;
-  for (int b = 0; b < nb; b++) acc = cadd(acc, partial[(int64_t)(J.first_block + b) * nranges + r]);
-  out[idx] = acc;
+  const int nb = (J.n + kBlock - 1) / kBlock;
+  dcx s = mk(0, 0);
+  for (int b = lane; b < nb; b += 64) s = cadd(s, partial[(int64_t)(J.first_block + b) * nranges + r]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    double re = __shfl_down(s.re, d, 64), im = __shfl_down(s.im, d, 64);
+    if (lane < d) s = cadd(s, mk(re, im));
+  }
+  if (lane == 0) {
+    const dcx C = cstat[j];
+    out[idx] = cadd(csub(cdiv(C, mk(ranges[r], 0)), rmul(0.5 * J.dk, C)), s);
+  }
 }
 
 // synthesise's spectra (cmd/geofdtd/main.go:186-193):
@@ -613,9 +627,8 @@ int launch_multi(const Medium &M, const Job *jobs, int32_t njobs, const double *
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(g_ws.ev[2], st));
-  int64_t nout = (int64_t)njobs * nranges;
-  k_fk_fold<<<(unsigned)((nout + 255) / 256), 256, 0, st>>>(B.jobs, njobs, B.cstat, B.ranges, nranges, B.partial,
-                                                            B.out);
+  int64_t nout = (int64_t)njobs * nranges;  // one wavefront each, four per workgroup
+  k_fk_fold<<<(unsigned)((nout + 3) / 4), 256, 0, st>>>(B.jobs, njobs, B.cstat, B.ranges, nranges, B.partial, B.out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(g_ws.ev[3], st));
   std::vector<int32_t> far(njobs);

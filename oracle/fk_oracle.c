@@ -519,7 +519,9 @@ int orc_fk_synthesise_coeff(const gf_medium *m, const double *ranges, int n_rang
  * sum (libgeofdtd's fk.hip): samples in workgroups of 256; within one, for
  * each group of G = min(256, ranges left) ranges, S = 256 / G interleaved
  * slices (sample ii in slice ii mod S) summed in order, then the slices in
- * order; the workgroup partials are added in order to C/r - (dk/2) C.  Every
+ * order; the workgroup partials are folded by 64 lanes (lane l takes every
+ * 64th partial from l, then lane sums pairwise at distances 32 .. 1) and
+ * added to C/r - (dk/2) C.  Every
  * term is the Go code's w * J0(k r) * dk * (k u - C); only the order of the
  * additions differs from the reference's sequential sum, so that comparing
  * the GPU with this isolates per-term rounding.  out [n_ranges][2]. */
@@ -538,7 +540,12 @@ int orc_fk_quadrature_workgroup_order(const gf_medium *m, const double *ranges, 
     free(acc), free(rem), free(kk), free(ww), free(slice);
     return fail(GF_ERR_ARGUMENT, "oracle: out of memory");
   }
-  for (int j = 0; j < n_ranges; j++) acc[j] = csub(go_cdiv(c, C(ranges[j], 0)), cmul(C(0.5 * dk, 0), c));
+  long long nb = (n + B - 1) / B;
+  cx *part = (cx *)malloc(sizeof(cx) * (size_t)nb * (size_t)n_ranges);
+  if (!part) {
+    free(acc), free(rem), free(kk), free(ww), free(slice);
+    return fail(GF_ERR_ARGUMENT, "oracle: out of memory");
+  }
   for (long long first = 1; first <= n; first += B) {
     int cnt = (int)((n - first + 1) < B ? (n - first + 1) : B);
     for (int t = 0; t < cnt; t++) {
@@ -566,14 +573,24 @@ int orc_fk_quadrature_workgroup_order(const gf_medium *m, const double *ranges, 
         }
         cx tot = slice[0];
         for (int s = 1; s < S; s++) tot = cadd(tot, slice[s]);
-        acc[g0 + jr] = cadd(acc[g0 + jr], tot);
+        part[(size_t)((first - 1) / B) * n_ranges + g0 + jr] = tot;
       }
     }
   }
+  /* The fold: 64 lanes each add every 64th partial in order, then the lane
+   * sums are added pairwise at distances 32, 16, ..., 1. */
   for (int j = 0; j < n_ranges; j++) {
+    cx lane[64];
+    for (int l = 0; l < 64; l++) {
+      lane[l] = C(0, 0);
+      for (long long b = l; b < nb; b += 64) lane[l] = cadd(lane[l], part[(size_t)b * n_ranges + j]);
+    }
+    for (int d = 32; d >= 1; d >>= 1)
+      for (int l = 0; l < d; l++) lane[l] = cadd(lane[l], lane[l + d]);
+    acc[j] = cadd(csub(go_cdiv(c, C(ranges[j], 0)), cmul(C(0.5 * dk, 0), c)), lane[0]);
     out[2 * j] = acc[j].re;
     out[2 * j + 1] = acc[j].im;
   }
-  free(acc), free(rem), free(kk), free(ww), free(slice);
+  free(acc), free(rem), free(kk), free(ww), free(slice), free(part);
   return GF_OK;
 }

@@ -288,3 +288,18 @@ def test_workgroup_order_sum_equals_the_go_order(oracle):
     a = oracle.fk_vd_workgroup_order(m, 7.0, 22.0, fk.Integration(Samples=777))
     b = oracle.fk_vertical_displacement(m, 7.0, 22.0, fk.Integration(Samples=777))
     assert abs(a - b) / abs(b) < 1e-12
+
+
+def test_v5_comparison_measures_a_known_lag_and_scale():
+    """The V5 comparison helper (v5_test.go:95-145) on a trace that is the
+    reference delayed by 7.3 samples and scaled by 1.1."""
+    from test_gpu_v5 import compare
+
+    dt = 1e-4
+    t = np.arange(4000) * dt
+    want = np.exp(-((t - 0.1) / 0.01) ** 2) * np.sin(2 * np.pi * 40 * t)
+    got = 1.1 * np.exp(-((t - 0.1 - 7.3 * dt) / 0.01) ** 2) * np.sin(2 * np.pi * 40 * (t - 7.3 * dt))
+    peak, lag, residual = compare(got, want, dt)
+    assert abs(peak - 1.1) < 1e-3
+    assert abs(lag - 7.3 * dt) < 0.05 * dt
+    assert residual < 0.15

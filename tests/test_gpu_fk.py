@@ -11,7 +11,8 @@ the body-wave wavenumbers, fk_test.go:44-54), and where the integral cancels
 own sensitivity to rounding: the oracle built with fused multiply-adds
 (oracle/Makefile) and the oracle on the medium moved by about one ulp give
 two measures of it, and the GPU must agree with the oracle within 10 times
-the larger, or 1e-11 relative, whichever is larger.  Measured, the GPU's
+the larger, or 1e-10 relative (what the two libms alone leave in
+well-conditioned cases), whichever is larger.  Measured, the GPU's
 difference is within a factor 2 of the fused build's everywhere.  Where the
 solve is well conditioned (point responses with k*h <= 16) the bound is a
 flat 1e-9.  Integrated results are compared with the oracle summing in the
@@ -60,10 +61,10 @@ NUDGES = (2.3e-16, -3.4e-16)
 
 
 def bound(call, m, want):
-    """10x the oracle's own rounding sensitivity, at least 1e-11.  `call`
+    """10x the oracle's own rounding sensitivity, at least 1e-10.  `call`
     takes (medium, fma) and evaluates the oracle."""
     spread = max([rel(call(m, True), want)] + [rel(call(nudged(m, e), False), want) for e in NUDGES])
-    return max(10 * spread, 1e-11)
+    return max(10 * spread, 1e-10)
 
 
 @pytest.mark.parametrize("name", list(MEDIA))
