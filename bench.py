@@ -259,6 +259,13 @@ def bench_fk(args, fd, standalone=True):
     st = fk.Stats()
     if not np.all(np.isfinite(g)):
         raise SystemExit("non-finite f-k bank")
+    # FP64 VALU work per bank, counted by rocprofv3 (executed, not algorithmic:
+    # the f-k solve has no closed-form operation count to quote).
+    prof = os.path.join(ROOT, "profiles", "fk_bank_c2_fp64.json")
+    flops = None
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            flops = json.load(fh)
     log(f"fk: {len(freqs)} bins x {len(ranges)} ranges, {st['samples']} samples per bank, "
         f"{args.steps} banks in {wall:.3f} s ({kernel_ms:.1f} ms of kernels)")
     res = {
@@ -272,10 +279,21 @@ def bench_fk(args, fd, standalone=True):
             "hankel_terms_per_bank": st["weights"],
         },
         "kernel_ms_per_bank": kernel_ms / args.steps,
-        "quadrature_ms_per_bank": st["quadrature_ms"],
+        "solve_ms_per_bank": st["solve_ms"],
+        "hankel_ms_per_bank": st["hankel_ms"],
         "samples_per_s": st["samples"] * args.steps / wall,
         "dtype": "f64",
+        "roofline": None,
     }
+    if flops:
+        t = (st["solve_ms"] + st["hankel_ms"]) / 1e3
+        achieved = (flops["solve_fp64_flop_per_bank"] + flops["hankel_fp64_flop_per_bank"]) / t / 1e12
+        res["roofline"] = {
+            "bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "k_fk_solve + k_fk_hankel (last bank's HIP-event times)",
+            "flop_count": "executed FP64 VALU FLOP from rocprofv3 counters (profiles/fk_bank_c2_fp64.json)",
+        }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -300,7 +318,7 @@ def bench_fk(args, fd, standalone=True):
         out = {"metric": METRIC, "value": res["value"], "unit": "traces/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-               "config": res["config"], "roofline": None, **{k: v for k, v in res.items() if k not in (
+               "config": res["config"], **{k: v for k, v in res.items() if k not in (
                    "value", "unit", "ms_per_step", "config", "dtype")}}
         print(json.dumps(out), flush=True)
         return 0

@@ -13,12 +13,14 @@
 //   k_fk_prologue   one thread per frequency: each layer's complex constants
 //                   (propsOf, fk.go:114-125) and the static coefficient
 //                   C = kfar * u(kfar) (fk.go:347-353).
-//   k_fk_quadrature one workgroup per 256 consecutive samples of one
-//                   frequency (all frequencies of a call in one launch): each
-//                   thread solves its sample and leaves (k, w, k u - C) in
-//                   LDS; the workgroup then forms the Hankel sums for every
-//                   range from LDS (threads split as ranges x sample slices)
-//                   and writes one partial sum per range.
+//   k_fk_solve      one thread per wavenumber sample, in workgroups of 256
+//                   consecutive samples of one frequency (all frequencies of
+//                   a call in one launch): the layered solve, k u - C stored.
+//   k_fk_hankel     the same workgroups: the 256 stored values into LDS, then
+//                   the Hankel sums for every range (threads split as ranges
+//                   x sample slices), one partial sum per range.  Apart from
+//                   the solve so that the J0 evaluations run at full
+//                   occupancy while the solve uses every register.
 //   k_fk_fold       one wavefront per (frequency, range): C/r - dk/2 C plus
 //                   the workgroup partials, summed in a fixed order.
 // The solve is FP64-VALU work with register-resident 4x4 matrices; HBM
@@ -384,13 +386,47 @@ __global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fk_quadrature(Medium M, const Job *__restrict__ jobs,
-                                                          const int32_t *__restrict__ blk_job,
-                                                          const Props *__restrict__ props,
-                                                          const dcx *__restrict__ cstat,
-                                                          const double *__restrict__ ranges, int32_t nranges,
-                                                          dcx *__restrict__ partial,
-                                                          unsigned long long *__restrict__ err) {
+// Each thread solves one wavenumber sample and stores k u - C; samples past
+// a job's n store 0.  rem is laid out like the workgroups: 256 per workgroup.
+// Workgroups of k_fk_solve resident per CU, which sets its register budget:
+// 1 keeps every matrix in registers (one wavefront per SIMD); 2 halves the
+// budget and spills about 90 registers to scratch, and is faster (measured
+// on the config-2 bank: 2.06 ms against 2.65 ms), because two wavefronts hide
+// the latency of the FP64 divide and square-root sequences.
+#ifndef GF_FK_SOLVE_GROUPS
+#define GF_FK_SOLVE_GROUPS 2
+#endif
+
+__global__ __launch_bounds__(kBlock, GF_FK_SOLVE_GROUPS) void k_fk_solve(Medium M, const Job *__restrict__ jobs,
+                                                     const int32_t *__restrict__ blk_job,
+                                                     const Props *__restrict__ props, const dcx *__restrict__ cstat,
+                                                     dcx *__restrict__ rem_out, unsigned long long *__restrict__ err) {
+  const int t = threadIdx.x;
+  const int j = blk_job[blockIdx.x];
+  const Job J = jobs[j];
+  const int i = (blockIdx.x - J.first_block) * kBlock + 1 + t;
+  dcx rem = mk(0, 0);
+  if (i <= J.n) {
+    const double k = (double)i * J.dk;
+    dcx u;
+    int st = surface_response(props + (int64_t)j * M.nl, M.nl, k, 2 * kPi * J.f, M.mu_ref, u);
+    if (st) atomicMin(&err[j], (unsigned long long)i * 2 + (unsigned long long)(st - 1));
+    rem = csub(rmul(k, u), cstat[j]);
+  }
+  rem_out[(int64_t)blockIdx.x * kBlock + t] = rem;
+}
+
+// The Hankel sums of one workgroup's 256 samples for every range, from LDS:
+// threads split as (range within a group of G = min(256, ranges left),
+// slice s < S = 256 / G); slice s takes samples s, s+S, ... in order, then
+// the slices are added in order.  One partial per (workgroup, range).  Kept
+// apart from the solve so that it runs at full occupancy: the solve needs
+// every register a lane can have, the J0 evaluations need latency hiding.
+__global__ __launch_bounds__(kBlock) void k_fk_hankel(const Job *__restrict__ jobs,
+                                                      const int32_t *__restrict__ blk_job,
+                                                      const dcx *__restrict__ rem_in,
+                                                      const double *__restrict__ ranges, int32_t nranges,
+                                                      dcx *__restrict__ partial) {
   __shared__ dcx s_rem[kBlock];
   __shared__ double s_k[kBlock], s_w[kBlock];
   __shared__ dcx s_part[kBlock];
@@ -399,19 +435,9 @@ __global__ __launch_bounds__(kBlock) void k_fk_quadrature(Medium M, const Job *_
   const Job J = jobs[j];
   const int first = (blockIdx.x - J.first_block) * kBlock + 1;
   const int i = first + t;
-  const dcx C = cstat[j];
-  double k = (double)i * J.dk, w = 0;
-  dcx rem = mk(0, 0);
-  if (i <= J.n) {
-    dcx u;
-    int st = surface_response(props + (int64_t)j * M.nl, M.nl, k, 2 * kPi * J.f, M.mu_ref, u);
-    if (st) atomicMin(&err[j], (unsigned long long)i * 2 + (unsigned long long)(st - 1));
-    w = i == J.n ? 0.5 : 1.0;
-    rem = csub(rmul(k, u), C);
-  }
-  s_rem[t] = rem;
-  s_k[t] = k;
-  s_w[t] = w;
+  s_rem[t] = rem_in[(int64_t)blockIdx.x * kBlock + t];
+  s_k[t] = (double)i * J.dk;
+  s_w[t] = i > J.n ? 0.0 : i == J.n ? 0.5 : 1.0;
   __syncthreads();
   const int cnt = min(kBlock, J.n - first + 1);
   for (int g0 = 0; g0 < nranges; g0 += kBlock) {
@@ -518,7 +544,7 @@ __global__ void k_fk_irdft(const dcx *__restrict__ coeff, int32_t nbins, int32_t
 
 struct Workspace {
   hipStream_t st = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   void *buf = nullptr;
   size_t cap = 0;
 };
@@ -562,7 +588,7 @@ struct MultiBufs {
   Job *jobs;
   int32_t *blk_job;
   Props *props;
-  dcx *ufar, *cstat, *partial, *out;
+  dcx *ufar, *cstat, *partial, *out, *rem;
   int32_t *far;
   unsigned long long *err;
   double *ranges;
@@ -594,6 +620,7 @@ int launch_multi(const Medium &M, const Job *jobs, int32_t njobs, const double *
   size_t o_err = cv.take(sizeof(unsigned long long) * njobs);
   size_t o_r = cv.take(sizeof(double) * nranges);
   size_t o_part = cv.take(sizeof(dcx) * (size_t)std::max<int64_t>(blocks, 1) * nranges);
+  size_t o_rem = cv.take(sizeof(dcx) * (size_t)std::max<int64_t>(blocks, 1) * kBlock);
   size_t o_out = cv.take(sizeof(dcx) * (size_t)njobs * nranges);
   size_t o_coeff = cv.take(sizeof(dcx) * (size_t)std::max<int64_t>(extra_coeff, 1));
   size_t o_trace = cv.take(sizeof(double) * (size_t)std::max<int64_t>(extra_trace, 1));
@@ -608,6 +635,7 @@ int launch_multi(const Medium &M, const Job *jobs, int32_t njobs, const double *
   B.err = (unsigned long long *)(base + o_err);
   B.ranges = (double *)(base + o_r);
   B.partial = (dcx *)(base + o_part);
+  B.rem = (dcx *)(base + o_rem);
   B.out = (dcx *)(base + o_out);
   B.coeff = (dcx *)(base + o_coeff);
   B.trace = (double *)(base + o_trace);
@@ -622,15 +650,19 @@ int launch_multi(const Medium &M, const Job *jobs, int32_t njobs, const double *
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(g_ws.ev[1], st));
   if (blocks > 0) {
-    k_fk_quadrature<<<(unsigned)blocks, kBlock, 0, st>>>(M, B.jobs, B.blk_job, B.props, B.cstat, B.ranges, nranges,
-                                                         B.partial, B.err);
+    k_fk_solve<<<(unsigned)blocks, kBlock, 0, st>>>(M, B.jobs, B.blk_job, B.props, B.cstat, B.rem, B.err);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(g_ws.ev[2], st));
+  if (blocks > 0) {
+    k_fk_hankel<<<(unsigned)blocks, kBlock, 0, st>>>(B.jobs, B.blk_job, B.rem, B.ranges, nranges, B.partial);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(g_ws.ev[3], st));
   int64_t nout = (int64_t)njobs * nranges;  // one wavefront each, four per workgroup
   k_fk_fold<<<(unsigned)((nout + 3) / 4), 256, 0, st>>>(B.jobs, njobs, B.cstat, B.ranges, nranges, B.partial, B.out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(g_ws.ev[3], st));
+  HIPCHK(hipEventRecord(g_ws.ev[4], st));
   std::vector<int32_t> far(njobs);
   std::vector<unsigned long long> err(njobs);
   HIPCHK(hipMemcpyAsync(far.data(), B.far, sizeof(int32_t) * njobs, hipMemcpyDeviceToHost, st));
@@ -641,14 +673,16 @@ int launch_multi(const Medium &M, const Job *jobs, int32_t njobs, const double *
     status[j].sample = err[j];
   }
   if (tm) {
-    float a = 0, b = 0, c = 0;
+    float a = 0, b = 0, c = 0, d = 0;
     HIPCHK(hipEventElapsedTime(&a, g_ws.ev[0], g_ws.ev[1]));
     HIPCHK(hipEventElapsedTime(&b, g_ws.ev[1], g_ws.ev[2]));
     HIPCHK(hipEventElapsedTime(&c, g_ws.ev[2], g_ws.ev[3]));
+    HIPCHK(hipEventElapsedTime(&d, g_ws.ev[3], g_ws.ev[4]));
     tm->prologue_ms = a;
-    tm->quadrature_ms = b;
-    tm->fold_ms = c;
-    tm->launches = 3 + (blocks > 0);
+    tm->solve_ms = b;
+    tm->hankel_ms = c;
+    tm->fold_ms = d;
+    tm->launches = 3 + 2 * (blocks > 0);
   }
   return 0;
 }

@@ -266,9 +266,10 @@ void record_stats(const std::vector<gfw::Job> &jobs, int32_t nranges, const gfw:
   gf_fk_stats_t s;
   memset(&s, 0, sizeof s);
   s.prologue_ms = t.prologue_ms;
-  s.quadrature_ms = t.quadrature_ms;
+  s.solve_ms = t.solve_ms;
+  s.hankel_ms = t.hankel_ms;
   s.fold_ms = t.fold_ms;
-  s.kernel_ms = t.prologue_ms + t.quadrature_ms + t.fold_ms;
+  s.kernel_ms = t.prologue_ms + t.solve_ms + t.hankel_ms + t.fold_ms;
   for (const auto &j : jobs) s.samples += j.n;
   s.weights = s.samples * nranges;
   s.frequencies = (int32_t)jobs.size();

@@ -39,7 +39,7 @@ struct JobStatus {
 };
 
 struct Timing {
-  double prologue_ms, quadrature_ms, fold_ms;
+  double prologue_ms, solve_ms, hankel_ms, fold_ms;
   int32_t launches;
 };
 
@@ -47,8 +47,7 @@ struct Timing {
 int surface_points(const Medium &M, const double *k, const double *f, int64_t n, double *out, int32_t *status);
 
 // VerticalDisplacementMulti for every job: out [njobs][nranges][2] on the
-// host (may be null when only device_out is wanted).  When keep_device is
-// set the result stays on the device for synthesise().
+// host (may be null when only the statuses are wanted).
 int multi(const Medium &M, const Job *jobs, int32_t njobs, const double *ranges, int32_t nranges, double *out,
           JobStatus *status, Timing *t);
 

@@ -216,7 +216,8 @@ class GfFkStats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double),
         ("prologue_ms", C.c_double),
-        ("quadrature_ms", C.c_double),
+        ("solve_ms", C.c_double),
+        ("hankel_ms", C.c_double),
         ("fold_ms", C.c_double),
         ("samples", C.c_int64),
         ("weights", C.c_int64),

@@ -476,12 +476,13 @@ int gf_fk_synthesise(const gf_medium *m, const double *ranges, int32_t n_ranges,
 
 /* Counters of the last f-k integration on the calling thread (HIP events
  * on its stream): the per-frequency prologue (layer constants and the static
- * coefficient), the fused solve + Hankel-sum kernel, and the fold of the
- * partial sums; and the work done. */
+ * coefficient), the per-sample layered solve, the Hankel sums, and the fold
+ * of the partial sums; and the work done. */
 typedef struct gf_fk_stats_t {
-  double kernel_ms;     /* prologue + quadrature + fold                      */
+  double kernel_ms;     /* prologue + solve + hankel + fold                  */
   double prologue_ms;
-  double quadrature_ms;
+  double solve_ms;
+  double hankel_ms;
   double fold_ms;
   int64_t samples;      /* wavenumber samples solved, over all frequencies  */
   int64_t weights;      /* samples x ranges Hankel terms J0(k r) summed      */
