@@ -325,6 +325,68 @@ def bench_fk(args, fd, standalone=True):
     return res
 
 
+def bench_bank_lookup(args, fd):
+    """Batched bank lookup (bank.Bank.Response at every bin for many ranges):
+    the reference's benchmark bank (bank_test.go:358-382: 270 ranges 1-40 m,
+    4096 samples, so 2049 bins), 8192 query ranges per call; a step is one
+    call.  `value` is responses per second of kernel time with the result in
+    HBM; the PCIe-inclusive rate (FP64 results copied to the host) beside it."""
+    import numpy as np
+
+    bkm = load_package().bank
+    count, samples = 270, 4096
+    bins = samples // 2 + 1
+    i = np.arange(count)[:, None]
+    k = np.arange(bins)[None, :]
+    z = (1.0 / (i + 1)) + 1j * (k * 1e-4)
+    h = fd.BankHeader(Solver="bench", Medium=[fd.Layer(0, 500, 200, 1700, 25)], SampleRateHz=2000, Samples=samples,
+                      MinM=1, MaxM=40, Count=count, Component="x", Units="m/N")
+    dev = bkm.Bank(h, z)
+    nq = 8192
+    r = np.random.default_rng(0).uniform(1, 40, nq)
+    for _ in range(args.warmup):
+        dev.Spectra(r)
+    kernel_ms, t0 = 0.0, time.perf_counter()
+    for _ in range(args.steps):
+        dev.Spectra(r)
+        kernel_ms += dev.Stats()["kernel_ms"]
+    wall = time.perf_counter() - t0
+    st = dev.Stats()
+    dev.close()
+    per_call = nq * bins
+    achieved = st["bytes"] / (st["kernel_ms"] / 1e3) / 1e9
+    res = {
+        "value": per_call * args.steps / (kernel_ms / 1e3), "unit": "responses/s",
+        "pcie_inclusive_responses_per_s": per_call * args.steps / wall,
+        "config": {"workload": f"bank lookup: {count} ranges x {bins} bins bank, {nq} query ranges x every bin "
+                               "per call", "responses_per_call": per_call},
+        "kernel_ms_per_call": kernel_ms / args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_bank_spectra",
+                     "algorithmic_bytes_per_launch": st["bytes"],
+                     "bytes_rule": "two range rows of log|G| and phase (FP64) read and one FP64 complex written "
+                                   "per value: 48 B"},
+        "dtype": "f64",
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from concurrent.futures import ThreadPoolExecutor
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        data = np.stack([z.real, z.imag], axis=-1).astype(np.float32)
+        ob = O.OracleBank(1, 40, data)
+        sample = np.array_split(r[:1024], threads)
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(ob.Spectra, sample))
+        el = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": 1024 * bins / el, "unit": "responses/s", "cores": threads, "kind": "port",
+                               "sample": f"1024 of the query ranges x every bin over {threads} threads ({el:.2f} s)"}
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -533,7 +595,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c2-X",
                     help="c2-X (default: config 2 as stated, per-layer Q), c2-R (the reference's one-SLS physics), "
                          "c1-R, c1-X, c3 (batched bank generation), surrogate (train on FDTD banks, serve held-out "
-                         "queries behind the FDTD error gate), fk (one bank by wavenumber integration)")
+                         "queries behind the FDTD error gate), fk (one bank by wavenumber integration), bank-lookup "
+                         "(batched interpolation in a resident bank)")
     ap.add_argument("--also", default="c2-R", help="second workload measured the same way and reported under "
                     "'also' (empty to skip)")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
@@ -591,6 +654,12 @@ def main() -> int:
         return bench_surrogate(args, fd)
     if args.workload == "fk":
         return bench_fk(args, fd)
+    if args.workload == "bank-lookup":
+        res = bench_bank_lookup(args, fd)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+                          **res}), flush=True)
+        return 0
 
     def measure(name):
         w = workload(name)
@@ -682,6 +751,7 @@ def main() -> int:
         out["also"] = {"workload": args.also, **also}
     if args.fk:
         out["fk"] = bench_fk(args, fd, standalone=False)
+        out["bank_lookup"] = bench_bank_lookup(args, fd)
     print(json.dumps(out), flush=True)
     return 0
 

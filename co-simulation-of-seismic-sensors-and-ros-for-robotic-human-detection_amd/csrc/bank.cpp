@@ -177,6 +177,11 @@ double range_at(const gf_bank_spec &s, int i) {
 
 }  // namespace
 
+namespace gfs {
+// bank.Header.Validate, for the device bank lookup.
+int bank_validate_header(const gf_bank_header &h, std::string &err) { return validate_header(h, err); }
+}  // namespace gfs
+
 #define HIPCHK(expr)                                       \
   do {                                                     \
     hipError_t e_ = (expr);                                \

@@ -1,0 +1,332 @@
+// bank_lookup.hip — bank.Bank's interpolation (internal/bank/interp.go of the
+// reference) on gfx950: a bank resident in HBM and batched lookups.
+//
+// Layout: the payload [count][bins] float2 (range-major, as the file stores
+// it) and two FP64 tables of the same shape, log|G| and the phase unwrapped
+// along range (Bank.prepare, interp.go:64-101).  Zero values carry log|G| =
+// -1e308 and stay zero on lookup.
+//
+//   k_bank_prepare  one thread per bin walks the range axis (the unwrapping is
+//                   a running sum along range); consecutive threads read
+//                   consecutive bins of one range row, so every step is one
+//                   coalesced row read.
+//   k_bank_points   one thread per (range, bin) query (Bank.Response).
+//   k_bank_spectra  one thread per (query range, bin): the bracket of the
+//                   query range, then two rows of each table read at
+//                   consecutive bins and one complex value written.
+// A lookup reads 32 B of tables and writes 16 B per value: HBM-bound.
+//
+// Arithmetic is Go's: cmplx.Abs as math.Hypot, math.Round half away from
+// zero, cmplx.Rect as r cos + i r sin; log, atan2, exp and sincos are the
+// device library's.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/geofdtd.h"
+#include "model.h"
+
+namespace gfs {
+int hip_fail(hipError_t e, const char *where);
+int api_set_err(int code, const std::string &msg);
+int check_device(std::string &why);
+int bank_validate_header(const gf_bank_header &h, std::string &err);
+}  // namespace gfs
+
+#define HIPCHK(expr)                                       \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return gfs::hip_fail(e_, #expr); \
+  } while (0)
+
+struct gf_bank_dev {
+  double r_min = 0, r_max = 0;
+  int32_t count = 0, bins = 0;
+  float2 *data = nullptr;
+  double *logmag = nullptr, *phase = nullptr;
+  double *q_r = nullptr;
+  int32_t *q_bin = nullptr;
+  double2 *out = nullptr;
+  size_t q_cap = 0, out_cap = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double last_ms = 0, last_bytes = 0;
+};
+
+namespace {
+
+constexpr double kNegInf = -1e308;
+constexpr double kPi = 3.141592653589793;
+
+__device__ __forceinline__ double hypot_go(double p, double q) {
+  p = fabs(p);
+  q = fabs(q);
+  if (isinf(p) || isinf(q)) return INFINITY;
+  if (p < q) {
+    double t = p;
+    p = q;
+    q = t;
+  }
+  if (p == 0) return 0;
+  q = q / p;
+  return p * sqrt(1 + q * q);
+}
+
+__global__ void k_bank_prepare(int32_t count, int32_t bins, const float2 *__restrict__ data,
+                               double *__restrict__ logmag, double *__restrict__ phase) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= bins) return;
+  double prev = 0, unwrapped = 0;
+  for (int i = 0; i < count; i++) {
+    const int64_t idx = (int64_t)i * bins + k;
+    const float2 v = data[idx];
+    const double re = v.x, im = v.y;
+    const double mag = hypot_go(re, im);
+    if (mag == 0) {
+      logmag[idx] = kNegInf;
+      phase[idx] = unwrapped;
+      continue;
+    }
+    logmag[idx] = log(mag);
+    const double w = atan2(im, re);
+    if (i == 0) {
+      unwrapped = w;
+    } else {
+      double d = w - prev;
+      d -= 2 * kPi * round(d / (2 * kPi));
+      unwrapped += d;
+    }
+    prev = w;
+    phase[idx] = unwrapped;
+  }
+}
+
+// Bank.bracket (format.go:295-309) and the interpolation of Bank.Response
+// (interp.go:38-53) for a range already checked to lie in the bank.
+__device__ __forceinline__ double2 lookup(double r, int bin, double r_min, double r_max, int32_t count, int32_t bins,
+                                          const double *__restrict__ logmag, const double *__restrict__ phase) {
+  int lo = 0;
+  double frac = 0;
+  if (count > 1) {
+    const double spacing = (r_max - r_min) / (double)(count - 1);
+    const double x = (r - r_min) / spacing;
+    lo = (int)floor(x);
+    if (lo >= count - 1) {
+      lo = count - 2;
+      frac = 1;
+    } else {
+      frac = x - (double)lo;
+    }
+  }
+  const int64_t i0 = (int64_t)lo * bins + bin;
+  const int64_t i1 = count == 1 ? i0 : (int64_t)(lo + 1) * bins + bin;
+  const double l0 = logmag[i0], l1 = logmag[i1];
+  if (l0 == kNegInf || l1 == kNegInf) return make_double2(0, 0);
+  const double p0 = phase[i0], p1 = phase[i1];
+  const double lm = l0 + frac * (l1 - l0);
+  const double ph = p0 + frac * (p1 - p0);
+  const double m = exp(lm);
+  double s, c;
+  sincos(ph, &s, &c);
+  return make_double2(m * c, m * s);
+}
+
+__global__ void k_bank_points(const double *__restrict__ r, const int32_t *__restrict__ bin, int64_t n,
+                              double r_min, double r_max, int32_t count, int32_t bins,
+                              const double *__restrict__ logmag, const double *__restrict__ phase,
+                              double2 *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = lookup(r[i], bin[i], r_min, r_max, count, bins, logmag, phase);
+}
+
+__global__ void k_bank_spectra(const double *__restrict__ r, int32_t nq, double r_min, double r_max, int32_t count,
+                               int32_t bins, const double *__restrict__ logmag, const double *__restrict__ phase,
+                               double2 *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (k >= bins || q >= nq) return;
+  out[(int64_t)q * bins + k] = lookup(r[q], k, r_min, r_max, count, bins, logmag, phase);
+}
+
+int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
+int ok() { return gfs::api_set_err(GF_OK, ""); }
+
+// The Go error of Response(r, bin), or GF_OK (interp.go:30-36, format.go:297-299).
+int check_query(const gf_bank_dev *b, double r, int32_t bin, std::string &err) {
+  if (bin < 0 || bin >= b->bins) {
+    err = "bank: bin " + std::to_string(bin) + " out of range [0, " + std::to_string(b->bins) + ")";
+    return GF_ERR_INVALID;
+  }
+  if (r < b->r_min || r > b->r_max) {
+    err = "bank: range " + gfm::go_g(r) + " m is outside the bank's " + gfm::go_g(b->r_min) + " to " +
+          gfm::go_g(b->r_max) + " m";
+    return GF_ERR_INVALID;
+  }
+  return GF_OK;
+}
+
+int reserve(gf_bank_dev *b, size_t nq, size_t nout) {
+  if (b->q_cap < nq) {
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (b->q_r) HIPCHK(hipFree(b->q_r));
+    if (b->q_bin) HIPCHK(hipFree(b->q_bin));
+    b->q_r = nullptr;
+    b->q_bin = nullptr;
+    HIPCHK(hipMalloc(&b->q_r, sizeof(double) * nq));
+    HIPCHK(hipMalloc(&b->q_bin, sizeof(int32_t) * nq));
+    b->q_cap = nq;
+  }
+  if (b->out_cap < nout) {
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (b->out) HIPCHK(hipFree(b->out));
+    b->out = nullptr;
+    HIPCHK(hipMalloc(&b->out, sizeof(double2) * nout));
+    b->out_cap = nout;
+  }
+  return GF_OK;
+}
+
+void release(gf_bank_dev *b) {
+  if (b->st) hipStreamSynchronize(b->st);
+  hipFree(b->data);
+  hipFree(b->logmag);
+  hipFree(b->phase);
+  hipFree(b->q_r);
+  hipFree(b->q_bin);
+  hipFree(b->out);
+  if (b->e0) hipEventDestroy(b->e0);
+  if (b->e1) hipEventDestroy(b->e1);
+  if (b->st) hipStreamDestroy(b->st);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gf_bank_dev_open(const gf_bank_header *h, const float *payload, gf_bank_dev **out) {
+  if (!h || !payload || !out) return fail(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  *out = nullptr;
+  std::string err;
+  int rc = gfs::bank_validate_header(*h, err);
+  if (rc) return fail(rc, err);
+  if ((rc = gfs::check_device(err))) return fail(rc, err);
+  gf_bank_dev *b = new gf_bank_dev;
+  b->r_min = h->r_min_m;
+  b->r_max = h->r_max_m;
+  b->count = h->count;
+  b->bins = h->samples / 2 + 1;
+  const size_t n = (size_t)b->count * b->bins;
+  auto bail = [&](hipError_t e, const char *where) {
+    release(b);
+    return gfs::hip_fail(e, where);
+  };
+  hipError_t e;
+  if ((e = hipStreamCreate(&b->st)) != hipSuccess) return bail(e, "hipStreamCreate");
+  if ((e = hipEventCreate(&b->e0)) != hipSuccess || (e = hipEventCreate(&b->e1)) != hipSuccess)
+    return bail(e, "hipEventCreate");
+  if ((e = hipMalloc(&b->data, sizeof(float2) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->logmag, sizeof(double) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->phase, sizeof(double) * n)) != hipSuccess)
+    return bail(e, "hipMalloc");
+  if ((e = hipMemcpyAsync(b->data, payload, sizeof(float2) * n, hipMemcpyHostToDevice, b->st)) != hipSuccess)
+    return bail(e, "hipMemcpyAsync");
+  k_bank_prepare<<<(b->bins + 255) / 256, 256, 0, b->st>>>(b->count, b->bins, b->data, b->logmag, b->phase);
+  if ((e = hipGetLastError()) != hipSuccess) return bail(e, "k_bank_prepare");
+  if ((e = hipStreamSynchronize(b->st)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+  *out = b;
+  return ok();
+}
+
+void gf_bank_dev_close(gf_bank_dev *b) {
+  if (b) release(b);
+}
+
+int gf_bank_dev_response(gf_bank_dev *b, const double *r_m, const int32_t *bin, int64_t n, double *out) {
+  if (!b) return fail(GF_ERR_ARGUMENT, "geofdtd: null bank handle");
+  if (n < 0 || (n > 0 && (!r_m || !bin || !out))) return fail(GF_ERR_ARGUMENT, "geofdtd: bad query arrays");
+  std::string err;
+  for (int64_t i = 0; i < n; i++)
+    if (int rc = check_query(b, r_m[i], bin[i], err)) return fail(rc, err);
+  if (n == 0) return ok();
+  if (int rc = reserve(b, (size_t)n, (size_t)n)) return rc;
+  HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->q_bin, bin, sizeof(int32_t) * n, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipEventRecord(b->e0, b->st));
+  k_bank_points<<<(unsigned)((n + 255) / 256), 256, 0, b->st>>>(b->q_r, b->q_bin, n, b->r_min, b->r_max, b->count,
+                                                                 b->bins, b->logmag, b->phase, b->out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->e1, b->st));
+  HIPCHK(hipMemcpyAsync(out, b->out, sizeof(double2) * n, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->last_ms = ms;
+  b->last_bytes = (double)n * (32.0 + 16.0 + 12.0);
+  return ok();
+}
+
+int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *out) {
+  if (!b) return fail(GF_ERR_ARGUMENT, "geofdtd: null bank handle");
+  if (n < 0 || (n > 0 && (!r_m || !out))) return fail(GF_ERR_ARGUMENT, "geofdtd: bad query arrays");
+  std::string err;
+  for (int32_t i = 0; i < n; i++)
+    if (int rc = check_query(b, r_m[i], 0, err)) return fail(rc, err);
+  if (n == 0) return ok();
+  if (n > 65535) return fail(GF_ERR_ARGUMENT, "geofdtd: at most 65535 ranges per spectra call");
+  const size_t nout = (size_t)n * b->bins;
+  if (int rc = reserve(b, (size_t)n, nout)) return rc;
+  HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipEventRecord(b->e0, b->st));
+  dim3 grid((b->bins + 255) / 256, n);
+  k_bank_spectra<<<grid, 256, 0, b->st>>>(b->q_r, n, b->r_min, b->r_max, b->count, b->bins, b->logmag, b->phase,
+                                          b->out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->e1, b->st));
+  HIPCHK(hipMemcpyAsync(out, b->out, sizeof(double2) * nout, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->last_ms = ms;
+  b->last_bytes = (double)nout * (32.0 + 16.0);
+  return ok();
+}
+
+int gf_bank_dev_stats(const gf_bank_dev *b, double *kernel_ms, double *bytes) {
+  if (!b || !kernel_ms || !bytes) return fail(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  *kernel_ms = b->last_ms;
+  *bytes = b->last_bytes;
+  return ok();
+}
+
+double gf_range_nyquist(double slowest, double max_freq) {
+  if (max_freq <= 0) return INFINITY;
+  return slowest / (2 * max_freq);
+}
+
+int gf_bank_check_range_sampling(const gf_bank_header *h, double max_freq) {
+  if (!h) return fail(GF_ERR_ARGUMENT, "geofdtd: null header");
+  if (h->count <= 1 || h->n_medium <= 0 || !h->medium) return ok();
+  double slowest = h->medium[0].vs_mps;
+  for (int i = 0; i < h->n_medium; i++) slowest = fmin(slowest, h->medium[i].vs_mps);
+  const double limit = gf_range_nyquist(0.87 * slowest, max_freq) / 2;
+  const double got = (h->r_max_m - h->r_min_m) / (double)(h->count - 1);
+  if (got > limit) {
+    char b1[64], b2[64];
+    snprintf(b1, sizeof b1, "%.4f", got);
+    if (std::isinf(limit))
+      snprintf(b2, sizeof b2, "+Inf");
+    else
+      snprintf(b2, sizeof b2, "%.4f", limit);
+    return fail(GF_ERR_INVALID, std::string("bank: range spacing ") + b1 + " m is too coarse to unwrap phase up to " +
+                                    gfm::go_g(max_freq) + " Hz; need under " + b2 + " m");
+  }
+  return ok();
+}
+
+}  // extern "C"

@@ -284,6 +284,42 @@ int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *
 int gf_spectrum(const float *traces, int32_t rows, int32_t n, double dt, double t0, const double *force,
                 int64_t n_force, const double *freqs, int32_t nbins, double *out);
 
+/* ---- bank lookup: bank.Bank's interpolation on the device --------------
+ * (internal/bank/interp.go).  The payload is copied to HBM once and the
+ * log-magnitude and range-unwrapped phase tables of Bank.prepare are built
+ * there (one thread per bin walking the range axis); lookups run one thread
+ * per output value.  A handle is single-threaded, like *bank.Bank's lazily
+ * built tables. */
+typedef struct gf_bank_dev gf_bank_dev;
+
+/* bank.New and Set of a whole payload [count][samples/2+1][2] float32,
+ * range-major (format.go:167-194), then Bank.prepare (interp.go:64-101).
+ * The header is validated as bank.Header.Validate (format.go:142-165). */
+int gf_bank_dev_open(const gf_bank_header *h, const float *payload, gf_bank_dev **out);
+void gf_bank_dev_close(gf_bank_dev *b);
+
+/* Bank.Response (interp.go:29-54) at n points (r_m[i], bin[i]): out [n][2]
+ * (re, im), linear in log-magnitude and unwrapped phase between the two
+ * bracketing grid ranges.  The first point whose Go call would fail sets the
+ * error ("bank: bin %d out of range [0, %d)", "bank: range %g m is outside
+ * the bank's %g to %g m") and nothing is written. */
+int gf_bank_dev_response(gf_bank_dev *b, const double *r_m, const int32_t *bin, int64_t n, double *out);
+
+/* Bank.Response at every bin for each of n ranges: out [n][samples/2+1][2],
+ * the spectra a synthesis at those ranges reads. */
+int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *out);
+
+/* Kernel time (HIP events) and algorithmic HBM bytes of the last lookup on
+ * this handle: table reads (two range rows of log-magnitude and phase per
+ * value, FP64) plus the FP64 complex values written. */
+int gf_bank_dev_stats(const gf_bank_dev *b, double *kernel_ms, double *bytes);
+
+/* bank.RangeNyquist and Header.CheckRangeSampling (interp.go:120-141):
+ * whether the range grid is fine enough to unwrap phase up to max_freq_hz
+ * for the header's medium. */
+double gf_range_nyquist(double slowest_velocity_mps, double max_freq_hz);
+int gf_bank_check_range_sampling(const gf_bank_header *h, double max_freq_hz);
+
 /* ---- seeded soil models (batched bank generation) ----------------------
  * Soil model `index` of the stream keyed by `seed`, drawn with Philox4x32-10
  * (Salmon et al., SC'11): 1-3 layers over a half-space, each with Vs

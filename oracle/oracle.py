@@ -105,6 +105,9 @@ def _load(path):
         "orc_fk_synthesise_coeff": (C.c_int, [MED, _PD, C.c_int, _D, _D, _D, _D, _D, C.c_int, C.c_int, _PD,
                                               C.POINTER(C.c_int)]),
         "orc_fk_complex_velocity": (None, [MED, _D, _D, _D, _PD, _PD]),
+        "orc_bank_prepare": (None, [C.c_int, C.c_int, C.POINTER(C.c_float), _PD, _PD]),
+        "orc_bank_response": (C.c_int, [_D, _D, C.c_int, C.c_int, _PD, _PD, _D, C.c_int, _PD, _PD]),
+        "orc_bank_spectra": (C.c_int, [_D, _D, C.c_int, C.c_int, _PD, _PD, _PD, C.c_int, _PD]),
         "orc_fk_quadrature_workgroup_order": (C.c_int, [MED, _PD, C.c_int, _D, _D, C.c_longlong, _D, _PD]),
     }
     for name, (res, args) in sig.items():
@@ -455,3 +458,42 @@ def _quad_wg(medium, ranges, f, kmax, n, k_far, fma=False):
     out = np.zeros((len(r), 2))
     _check(lib(fma).orc_fk_quadrature_workgroup_order(C.byref(m), _p(r), len(r), f, kmax, n, k_far, _p(out)))
     return out[:, 0] + 1j * out[:, 1]
+
+
+# ------------------------------------------------------- bank interpolation
+
+
+class OracleBank:
+    """bank.Bank's interpolation (internal/bank/interp.go) over a payload
+    [count][bins][2] float32 on the range grid r_min..r_max."""
+
+    def __init__(self, r_min, r_max, payload):
+        self.data = np.ascontiguousarray(payload, dtype=np.float32)
+        self.count, self.bins = self.data.shape[0], self.data.shape[1]
+        self.r_min, self.r_max = float(r_min), float(r_max)
+        self.logmag = np.zeros((self.count, self.bins))
+        self.phase = np.zeros((self.count, self.bins))
+        lib().orc_bank_prepare(self.count, self.bins, self.data.ctypes.data_as(C.POINTER(C.c_float)),
+                               _p(self.logmag), _p(self.phase))
+
+    def range_at(self, i):
+        """RangeGrid.At (format.go:52-57)."""
+        if self.count <= 1:
+            return self.r_min
+        return self.r_min + (self.r_max - self.r_min) * i / (self.count - 1)
+
+    def At(self, i, k):
+        return complex(float(self.data[i, k, 0]), float(self.data[i, k, 1]))
+
+    def Spectra(self, ranges):
+        r = np.ascontiguousarray(ranges, dtype=np.float64)
+        out = np.zeros((len(r), self.bins, 2))
+        _check(lib().orc_bank_spectra(self.r_min, self.r_max, self.count, self.bins, _p(self.logmag),
+                                      _p(self.phase), _p(r), len(r), _p(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def Response(self, r, k):
+        re, im = C.c_double(), C.c_double()
+        _check(lib().orc_bank_response(self.r_min, self.r_max, self.count, self.bins, _p(self.logmag),
+                                       _p(self.phase), r, int(k), C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
