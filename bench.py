@@ -390,6 +390,67 @@ def bench_bank_lookup(args, fd):
     return res
 
 
+def bench_stream(args, fd):
+    """Streaming convolution (stream.Convolver) for a 64-receiver array, each
+    with its own 1 s response at 2 kHz (2000 taps): the latency of one
+    20-sample chunk (10 ms of signal, the reference's default) for every
+    receiver, and the offline rate of ProcessAll over 60 s of input."""
+    import numpy as np
+
+    stm = load_package().stream
+    R, L, chunk, fs = 64, 2000, 20, 2000.0
+    i = np.arange(L)
+    h = np.stack([np.exp(-i / (200 + 10 * r)) * np.sin(i * (0.05 + 0.003 * r) + r) for r in range(R)])
+    x = np.random.default_rng(1).standard_normal((R, int(60 * fs)))
+    c = stm.NewConvolver(h, chunk)
+    for k in range(20):
+        c.Process(x[:, k * chunk:(k + 1) * chunk])
+    n_chunks = 500
+    t0 = time.perf_counter()
+    kern = 0.0
+    for k in range(n_chunks):
+        c.Process(x[:, k * chunk:(k + 1) * chunk])
+        kern += c.KernelMs()
+    wall = time.perf_counter() - t0
+    c.Reset()
+    t1 = time.perf_counter()
+    c.ProcessAll(x)
+    wall_all = time.perf_counter() - t1
+    k_all = c.KernelMs()
+    c.close()
+    macs = float(R) * x.shape[1] * L
+    achieved = 2 * macs / (k_all / 1e3) / 1e12
+    res = {
+        "value": R * x.shape[1] / (k_all / 1e3), "unit": "output samples/s (ProcessAll, kernel time)",
+        "chunk_latency_us": wall / n_chunks * 1e6, "chunk_kernel_us": kern / n_chunks * 1e3,
+        "real_time_factor": (chunk / fs) / (wall / n_chunks),
+        "process_all_wall_s": wall_all,
+        "config": {"workload": f"stream: {R} receivers, {L}-tap responses, chunks of {chunk} samples at 2 kHz; "
+                               "ProcessAll over 60 s"},
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_fir (ProcessAll)",
+                     "flop_rule": "2 x taps per output sample"},
+        "dtype": "f64",
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import stream_oracle as SO
+        from concurrent.futures import ThreadPoolExecutor
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n = 4000
+        t2 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda r: SO.Convolver(h[r], chunk).ProcessAll(x[r, :n]), range(threads)))
+        el = time.perf_counter() - t2
+        res["cpu_baseline"] = {"value": threads * n / el, "unit": "output samples/s", "cores": threads,
+                               "kind": "port", "sample": f"{threads} receivers x {n} samples in {chunk}-sample "
+                                                         f"chunks, one receiver per thread ({el:.2f} s; numpy "
+                                                         "FFTs, Python per chunk)"}
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    return res
+
+
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -599,7 +660,7 @@ def main() -> int:
                     help="c2-X (default: config 2 as stated, per-layer Q), c2-R (the reference's one-SLS physics), "
                          "c1-R, c1-X, c3 (batched bank generation), surrogate (train on FDTD banks, serve held-out "
                          "queries behind the FDTD error gate), fk (one bank by wavenumber integration), bank-lookup "
-                         "(batched interpolation in a resident bank)")
+                         "(batched interpolation in a resident bank), stream (chunked convolution)")
     ap.add_argument("--also", default="c2-R", help="second workload measured the same way and reported under "
                     "'also' (empty to skip)")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
@@ -657,6 +718,11 @@ def main() -> int:
         return bench_surrogate(args, fd)
     if args.workload == "fk":
         return bench_fk(args, fd)
+    if args.workload == "stream":
+        res = bench_stream(args, fd)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "data": "synthetic", **res}), flush=True)
+        return 0
     if args.workload == "bank-lookup":
         res = bench_bank_lookup(args, fd)
         print(json.dumps({"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -755,6 +821,7 @@ def main() -> int:
     if args.fk:
         out["fk"] = bench_fk(args, fd, standalone=False)
         out["bank_lookup"] = bench_bank_lookup(args, fd)
+        out["stream"] = bench_stream(args, fd)
     print(json.dumps(out), flush=True)
     return 0
 

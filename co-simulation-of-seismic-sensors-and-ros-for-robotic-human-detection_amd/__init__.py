@@ -4,6 +4,7 @@ internal/fdtd behind the C ABI in include/geofdtd.h).
     fdtd      the reference's fdtd package API over libgeofdtd.so (ctypes)
     fk        the reference's fk package (frequency-wavenumber solver) API
     bank      bank.Bank's interpolation on the device (batched lookups)
+    stream    stream.Convolver for many receivers (chunked convolution)
     csrc/     HIP kernels for gfx950 and the C ABI
 
 The directory name is not a Python identifier; load it with
@@ -13,5 +14,6 @@ importlib (see tests/conftest.py) under any alias.
 from . import fdtd  # noqa: F401
 from . import bank  # noqa: F401
 from . import fk  # noqa: F401
+from . import stream  # noqa: F401
 
-__all__ = ["fdtd", "fk", "bank"]
+__all__ = ["fdtd", "fk", "bank", "stream"]

@@ -298,6 +298,13 @@ SIGNATURES = {
     "gf_bank_dev_spectra": (_I, [C.c_void_p, _PD, _I32, _PD]),
     "gf_bank_dev_stats": (_I, [C.c_void_p, _PD, _PD]),
     "gf_range_nyquist": (_D, [_D, _D]),
+    "gf_convolver_create": (_I, [_PD, _I32, _I32, _I32, C.POINTER(C.c_void_p)]),
+    "gf_convolver_destroy": (None, [C.c_void_p]),
+    "gf_convolver_info": (_I, [C.c_void_p, _PI32, _PI32, _PI32]),
+    "gf_convolver_process": (_I, [C.c_void_p, _PD, _I32, _PD, _I32]),
+    "gf_convolver_reset": (_I, [C.c_void_p]),
+    "gf_convolver_process_all": (_I, [C.c_void_p, _PD, _I64, _PD]),
+    "gf_convolver_stats": (_I, [C.c_void_p, _PD]),
     "gf_bank_check_range_sampling": (_I, [C.POINTER(GfBankHeader), _D]),
 }
 

@@ -320,6 +320,42 @@ int gf_bank_dev_stats(const gf_bank_dev *b, double *kernel_ms, double *bytes);
 double gf_range_nyquist(double slowest_velocity_mps, double max_freq_hz);
 int gf_bank_check_range_sampling(const gf_bank_header *h, double max_freq_hz);
 
+/* ---- streaming convolution (the Go package internal/stream) ------------
+ * stream.Convolver for many receivers at once: each receiver convolves its
+ * own impulse response with its input, delivered a chunk at a time, and the
+ * output equals the whole-record convolution truncated to the input length.
+ * On the device the state is a ring of each receiver's past input in HBM and
+ * each output sample is the direct sum over the response (LDS-tiled); the
+ * reference's partitioned overlap-add is the same linear map, so results
+ * agree to rounding (the reference's own bound: 1e-9 of the peak). */
+typedef struct gf_convolver gf_convolver;
+
+/* stream.NewConvolver (stream.go:72-107): impulse [n_recv][len] (one
+ * response per receiver, all the same length), chunk samples per Process. */
+int gf_convolver_create(const double *impulse, int32_t n_recv, int32_t len, int32_t chunk, gf_convolver **out);
+void gf_convolver_destroy(gf_convolver *c);
+
+/* ChunkSize and Partitions (stream.go:109-112), and the receiver count. */
+int gf_convolver_info(const gf_convolver *c, int32_t *chunk, int32_t *partitions, int32_t *n_recv);
+
+/* Convolver.Process (stream.go:120-162) for every receiver: in and out are
+ * [n_recv][len_in] and [n_recv][len_out]; both lengths must equal the chunk
+ * size ("stream: chunk length mismatch: in %d, out %d, want %d"); in and out
+ * may alias. */
+int gf_convolver_process(gf_convolver *c, const double *in, int32_t len_in, double *out, int32_t len_out);
+
+/* Convolver.Reset (stream.go:164-171). */
+int gf_convolver_reset(gf_convolver *c);
+
+/* Convolver.ProcessAll (stream.go:175-186) for every receiver: in and out
+ * [n_recv][n]; the input is padded to whole chunks, processed from the
+ * current state in one pass on the device, and the state left as after those
+ * chunks. */
+int gf_convolver_process_all(gf_convolver *c, const double *in, int64_t n, double *out);
+
+/* Kernel time (HIP events) of the last Process or ProcessAll. */
+int gf_convolver_stats(const gf_convolver *c, double *kernel_ms);
+
 /* ---- seeded soil models (batched bank generation) ----------------------
  * Soil model `index` of the stream keyed by `seed`, drawn with Philox4x32-10
  * (Salmon et al., SC'11): 1-3 layers over a half-space, each with Vs
