@@ -17,10 +17,10 @@
 //
 //   k_ring_write  one thread per (receiver, sample): the newest min(n, M)
 //                 samples into the ring (older ones would be overwritten).
-//   k_fir<RING>   one workgroup per (receiver, 256 outputs): taps in tiles of
-//                 256 staged in LDS with the 511 input samples they touch, each
-//                 thread accumulating its output over the tile's taps in
-//                 order.  RING reads the ring (Process); otherwise a linear
+//   k_fir<RING>   one workgroup per (receiver, 1024 outputs): taps in tiles of
+//                 256 staged in LDS with the 1279 input samples they touch,
+//                 each thread accumulating four consecutive outputs over the
+//                 tile's taps in order from a sliding register window.  RING reads the ring (Process); otherwise a linear
 //                 buffer of the last len - 1 samples followed by the new input
 //                 (ProcessAll over a whole record in one launch).
 // Work: len multiply-adds per output sample, FP64; for long records the
@@ -82,23 +82,32 @@ __global__ void k_ring_write(double *__restrict__ ring, int64_t M, int64_t s0, c
 // out[r][t] = sum_{j < len} h[r][j] x_r(s0 + t - j), t < n.  RING: x_r(s) =
 // src[r][s mod M]; else x_r(s) = src[r][s - s0 + len - 1] (a linear buffer
 // that starts len - 1 samples before s0), with row stride ld.
+//
+// A workgroup makes kOut = 4 x 256 consecutive outputs; thread u owns outputs
+// 4u..4u+3.  Per tile of kT taps the inputs they meet (kOut + kT - 1) are
+// staged in LDS; the thread keeps a sliding window of four inputs in
+// registers, so each tap costs one broadcast tap read, one input read and
+// four multiply-adds.  Each output still sums its taps in order.
+constexpr int kR = 4;
+constexpr int kOut = kR * kT;
+
 template <bool RING>
 __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_t len, const double *__restrict__ src,
                                             int64_t ld, int64_t M, int64_t s0, int64_t n,
                                             double *__restrict__ out, int64_t ldo) {
   __shared__ double sh[kT];
-  __shared__ double sx[2 * kT - 1];
+  __shared__ double sx[kOut + kT - 1];
   const int r = blockIdx.y;
-  const int t = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * kT;
+  const int u = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kOut;
   const double *hr = h + (int64_t)r * len;
   const double *xr = src + (int64_t)r * ld;
-  double acc = 0;
+  double acc[kR] = {0, 0, 0, 0};
   for (int j0 = 0; j0 < len; j0 += kT) {
     // Taps j0..j0+kT-1 and the inputs they meet: s = s0 + t0 - j0 - (kT-1) + m.
     const int jt = min(kT, len - j0);
-    sh[t] = t < jt ? hr[j0 + t] : 0.0;
-    for (int m = t; m < 2 * kT - 1; m += kT) {
+    sh[u] = u < jt ? hr[j0 + u] : 0.0;
+    for (int m = u; m < kOut + kT - 1; m += kT) {
       const int64_t s = s0 + t0 - j0 - (kT - 1) + m;
       double v;
       if (RING) {
@@ -110,11 +119,28 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
       sx[m] = v;
     }
     __syncthreads();
-    // x(s0 + t0 + t - j) for j = j0 + q sits at sx[t - q + kT - 1].
-    for (int q = 0; q < jt; q++) acc = fma(sh[q], sx[t - q + kT - 1], acc);
+    // x(s0 + t0 + t - j) for j = j0 + q sits at sx[t - q + kT - 1]; for the
+    // thread's outputs t = kR u + i that is w[i] = sx[kR u + i - q + kT - 1].
+    const int b = kR * u + kT - 1;
+    double w[kR];
+#pragma unroll
+    for (int i = 0; i < kR; i++) w[i] = sx[b + i];
+    for (int q = 0; q < jt; q++) {
+      const double hq = sh[q];
+#pragma unroll
+      for (int i = 0; i < kR; i++) acc[i] = fma(hq, w[i], acc[i]);
+      // Slide the window one tap: w[i] <- sx[b + i - (q + 1)].
+#pragma unroll
+      for (int i = kR - 1; i > 0; i--) w[i] = w[i - 1];
+      w[0] = q + 1 < jt ? sx[b - q - 1] : 0.0;
+    }
     __syncthreads();
   }
-  if (t0 + t < n) out[(int64_t)r * ldo + t0 + t] = acc;
+#pragma unroll
+  for (int i = 0; i < kR; i++) {
+    const int64_t t = t0 + kR * u + i;
+    if (t < n) out[(int64_t)r * ldo + t] = acc[i];
+  }
 }
 
 int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
@@ -226,7 +252,7 @@ int gf_convolver_process(gf_convolver *c, const double *in, int32_t len_in, doub
   dim3 gw((unsigned)((C + 255) / 256), c->nrecv);
   k_ring_write<<<gw, 256, 0, c->st>>>(c->x, c->ring, c->pos, c->in, C, C);
   HIPCHK(hipGetLastError());
-  dim3 gf((unsigned)((C + kT - 1) / kT), c->nrecv);
+  dim3 gf((unsigned)((C + kOut - 1) / kOut), c->nrecv);
   k_fir<true><<<gf, kT, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->ring, c->pos, C, c->out, C);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->e1, c->st));
@@ -272,7 +298,7 @@ int gf_convolver_process_all(gf_convolver *c, const double *in, int64_t n, doubl
   for (int r = 0; r < c->nrecv; r++) memcpy(&host[(size_t)r * ld + (L - 1)], in + (size_t)r * n, sizeof(double) * n);
   HIPCHK(hipMemcpyAsync(c->in, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipEventRecord(c->e0, c->st));
-  dim3 gf((unsigned)((npad + kT - 1) / kT), c->nrecv);
+  dim3 gf((unsigned)((npad + kOut - 1) / kOut), c->nrecv);
   k_fir<false><<<gf, kT, 0, c->st>>>(c->h, (int32_t)L, c->in, ld, c->ring, c->pos, npad, c->out, npad);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->e1, c->st));
