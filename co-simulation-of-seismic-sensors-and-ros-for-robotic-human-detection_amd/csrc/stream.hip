@@ -17,6 +17,9 @@
 //
 //   k_ring_write  one thread per (receiver, sample): the newest min(n, M)
 //                 samples into the ring (older ones would be overwritten).
+//   k_fir_chunk   a streaming chunk of at most 64 samples: one workgroup per
+//                 receiver, its lanes split over outputs and tap slices, so
+//                 the serial chain is a fraction of the response.
 //   k_fir<RING>   one workgroup per (receiver, 1024 outputs): taps in tiles of
 //                 256 staged in LDS with the 1279 input samples they touch,
 //                 each thread accumulating four consecutive outputs over the
@@ -143,6 +146,32 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
   }
 }
 
+// The latency form for one streaming chunk (n <= 64 outputs): a workgroup
+// per receiver, lanes split as np outputs (n rounded up to a power of two)
+// x S = 256 / np tap slices.  Slice q sums taps q, q + S, q + 2S, ... in order
+// straight from the ring (it sits in L2); the S slice sums are then added in
+// order.  The serial chain per lane is len / S taps instead of len.
+__global__ __launch_bounds__(256) void k_fir_chunk(const double *__restrict__ h, int32_t len,
+                                                   const double *__restrict__ ring, int64_t M, int64_t s0, int32_t n,
+                                                   int32_t np, double *__restrict__ out) {
+  __shared__ double part[256];
+  const int r = blockIdx.x;
+  const int t = threadIdx.x % np, q = threadIdx.x / np;
+  const int S = 256 / np;
+  const double *hr = h + (int64_t)r * len;
+  const double *xr = ring + (int64_t)r * M;
+  double acc = 0;
+  if (t < n)
+    for (int j = q; j < len; j += S) acc = fma(hr[j], xr[(s0 + t - j) & (M - 1)], acc);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  if (q == 0 && t < n) {
+    double tot = part[t];
+    for (int k = 1; k < S; k++) tot += part[k * np + t];
+    out[(int64_t)r * n + t] = tot;
+  }
+}
+
 int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
 int ok() { return gfs::api_set_err(GF_OK, ""); }
 
@@ -252,8 +281,14 @@ int gf_convolver_process(gf_convolver *c, const double *in, int32_t len_in, doub
   dim3 gw((unsigned)((C + 255) / 256), c->nrecv);
   k_ring_write<<<gw, 256, 0, c->st>>>(c->x, c->ring, c->pos, c->in, C, C);
   HIPCHK(hipGetLastError());
-  dim3 gf((unsigned)((C + kOut - 1) / kOut), c->nrecv);
-  k_fir<true><<<gf, kT, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->ring, c->pos, C, c->out, C);
+  if (C <= 64) {
+    int np = 1;
+    while (np < C) np <<= 1;
+    k_fir_chunk<<<c->nrecv, 256, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->pos, (int32_t)C, np, c->out);
+  } else {
+    dim3 gf((unsigned)((C + kOut - 1) / kOut), c->nrecv);
+    k_fir<true><<<gf, kT, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->ring, c->pos, C, c->out, C);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->e1, c->st));
   HIPCHK(hipMemcpyAsync(out, c->out, sizeof(double) * c->nrecv * C, hipMemcpyDeviceToHost, c->st));

@@ -77,3 +77,20 @@ def test_chunk_length_is_checked():
         c.Process(np.zeros(3))
     assert str(e.value) == "stream: chunk length mismatch: in 3, out 4, want 4"
     c.close()
+
+
+@pytest.mark.parametrize("chunk", [1, 64, 65, 300])
+def test_process_chunk_by_chunk_at_both_kernel_forms(chunk):
+    """Chunks of at most 64 samples take the latency kernel, longer ones the
+    tiled kernel over the ring; both must follow the oracle chunk by chunk."""
+    impulse = decaying(900, 150, 0.13)
+    x = np.random.default_rng(chunk).standard_normal(6 * chunk + 5)
+    c = st.NewConvolver(np.stack([impulse, -0.5 * impulse]), chunk)
+    o = S.Convolver(impulse, chunk)
+    for k in range(6):
+        seg = x[k * chunk:(k + 1) * chunk]
+        got = c.Process(np.stack([seg, seg]))
+        want = o.Process(seg)
+        assert close(got[0], want), (chunk, k)
+        assert close(got[1], -0.5 * want), (chunk, k)
+    c.close()
