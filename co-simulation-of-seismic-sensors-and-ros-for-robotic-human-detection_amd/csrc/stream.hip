@@ -57,6 +57,7 @@ struct gf_convolver {
   double *x = nullptr;     // [nrecv][ring]
   double *in = nullptr;    // staging [nrecv][cap_in]
   double *out = nullptr;   // [nrecv][cap_out]
+  double *pin_in = nullptr, *pin_out = nullptr;  // pinned host chunks [nrecv][chunk]
   size_t cap_in = 0, cap_out = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -151,18 +152,26 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
 // x S = 256 / np tap slices.  Slice q sums taps q, q + S, q + 2S, ... in order
 // straight from the ring (it sits in L2); the S slice sums are then added in
 // order.  The serial chain per lane is len / S taps instead of len.
-__global__ __launch_bounds__(256) void k_fir_chunk(const double *__restrict__ h, int32_t len,
-                                                   const double *__restrict__ ring, int64_t M, int64_t s0, int32_t n,
-                                                   int32_t np, double *__restrict__ out) {
+//
+// The workgroup first writes its receiver's new chunk into the ring (one
+// launch per chunk, not two); __syncthreads makes those writes visible to
+// the workgroup's own reads.
+__global__ __launch_bounds__(256) void k_fir_chunk(const double *__restrict__ h, int32_t len, double *ring, int64_t M,
+                                                   int64_t s0, const double *__restrict__ in, int32_t n, int32_t np,
+                                                   double *__restrict__ out) {
   __shared__ double part[256];
   const int r = blockIdx.x;
   const int t = threadIdx.x % np, q = threadIdx.x / np;
   const int S = 256 / np;
   const double *hr = h + (int64_t)r * len;
-  const double *xr = ring + (int64_t)r * M;
+  double *xr = ring + (int64_t)r * M;
+  if (threadIdx.x < n) xr[(s0 + threadIdx.x) & (M - 1)] = in[(int64_t)r * n + threadIdx.x];
+  __syncthreads();
   double acc = 0;
-  if (t < n)
+  if (t < n) {
+#pragma unroll 8
     for (int j = q; j < len; j += S) acc = fma(hr[j], xr[(s0 + t - j) & (M - 1)], acc);
+  }
   part[threadIdx.x] = acc;
   __syncthreads();
   if (q == 0 && t < n) {
@@ -199,6 +208,8 @@ void release(gf_convolver *c) {
   hipFree(c->x);
   hipFree(c->in);
   hipFree(c->out);
+  if (c->pin_in) hipHostFree(c->pin_in);
+  if (c->pin_out) hipHostFree(c->pin_out);
   if (c->e0) hipEventDestroy(c->e0);
   if (c->e1) hipEventDestroy(c->e1);
   if (c->st) hipStreamDestroy(c->st);
@@ -276,23 +287,32 @@ int gf_convolver_process(gf_convolver *c, const double *in, int32_t len_in, doub
   if (!in || !out) return fail(GF_ERR_ARGUMENT, "geofdtd: null chunk");
   const int64_t C = c->chunk;
   if (int rc = grow(c, (size_t)c->nrecv * C, (size_t)c->nrecv * C)) return rc;
-  HIPCHK(hipMemcpyAsync(c->in, in, sizeof(double) * c->nrecv * C, hipMemcpyHostToDevice, c->st));
+  // Chunks go through pinned host buffers: a pageable copy costs a staging
+  // round trip that would dominate a 20-sample chunk's latency.
+  const size_t bytes = sizeof(double) * c->nrecv * C;
+  if (!c->pin_in) {
+    HIPCHK(hipHostMalloc((void **)&c->pin_in, bytes, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->pin_out, bytes, hipHostMallocDefault));
+  }
+  memcpy(c->pin_in, in, bytes);
+  HIPCHK(hipMemcpyAsync(c->in, c->pin_in, bytes, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipEventRecord(c->e0, c->st));
-  dim3 gw((unsigned)((C + 255) / 256), c->nrecv);
-  k_ring_write<<<gw, 256, 0, c->st>>>(c->x, c->ring, c->pos, c->in, C, C);
-  HIPCHK(hipGetLastError());
   if (C <= 64) {
     int np = 1;
     while (np < C) np <<= 1;
-    k_fir_chunk<<<c->nrecv, 256, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->pos, (int32_t)C, np, c->out);
+    k_fir_chunk<<<c->nrecv, 256, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->pos, c->in, (int32_t)C, np, c->out);
   } else {
+    dim3 gw((unsigned)((C + 255) / 256), c->nrecv);
+    k_ring_write<<<gw, 256, 0, c->st>>>(c->x, c->ring, c->pos, c->in, C, C);
+    HIPCHK(hipGetLastError());
     dim3 gf((unsigned)((C + kOut - 1) / kOut), c->nrecv);
     k_fir<true><<<gf, kT, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->ring, c->pos, C, c->out, C);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->e1, c->st));
-  HIPCHK(hipMemcpyAsync(out, c->out, sizeof(double) * c->nrecv * C, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->pin_out, c->out, bytes, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  memcpy(out, c->pin_out, bytes);
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->e0, c->e1));
   c->last_ms = ms;
