@@ -13,7 +13,9 @@
 //   k_bank_points   one thread per (range, bin) query (Bank.Response).
 //   k_bank_spectra  one thread per (query range, bin): the bracket of the
 //                   query range, then two rows of each table read at
-//                   consecutive bins and one complex value written.
+//                   consecutive bins and one complex value written.  Queries
+//                   run in range order, so table rows are fetched once per
+//                   L2 rather than once per query.
 // A lookup reads 32 B of tables and writes 16 B per value: HBM-bound.
 //
 // Arithmetic is Go's: cmplx.Abs as math.Hypot, math.Round half away from
@@ -24,6 +26,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -144,13 +147,17 @@ __global__ void k_bank_points(const double *__restrict__ r, const int32_t *__res
   out[i] = lookup(r[i], bin[i], r_min, r_max, count, bins, logmag, phase);
 }
 
-__global__ void k_bank_spectra(const double *__restrict__ r, int32_t nq, double r_min, double r_max, int32_t count,
-                               int32_t bins, const double *__restrict__ logmag, const double *__restrict__ phase,
+// Queries arrive sorted by range (slot q holds caller query dest[q]), so
+// neighbouring workgroups read the same two table rows and each XCD's L2
+// fetches a row pair once for all the queries that bracket into it.
+__global__ void k_bank_spectra(const double *__restrict__ r, const int32_t *__restrict__ dest, int32_t nq,
+                               double r_min, double r_max, int32_t count, int32_t bins,
+                               const double *__restrict__ logmag, const double *__restrict__ phase,
                                double2 *__restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (k >= bins || q >= nq) return;
-  out[(int64_t)q * bins + k] = lookup(r[q], k, r_min, r_max, count, bins, logmag, phase);
+  out[(int64_t)dest[q] * bins + k] = lookup(r[q], k, r_min, r_max, count, bins, logmag, phase);
 }
 
 int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
@@ -281,11 +288,18 @@ int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *ou
   if (n > 65535) return fail(GF_ERR_ARGUMENT, "geofdtd: at most 65535 ranges per spectra call");
   const size_t nout = (size_t)n * b->bins;
   if (int rc = reserve(b, (size_t)n, nout)) return rc;
-  HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
+  // Sorted by range, stable, with each query's output row.
+  std::vector<int32_t> order((size_t)n);
+  for (int32_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) { return r_m[a] < r_m[c]; });
+  std::vector<double> sorted((size_t)n);
+  for (int32_t i = 0; i < n; i++) sorted[i] = r_m[order[i]];
+  HIPCHK(hipMemcpyAsync(b->q_r, sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->q_bin, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipEventRecord(b->e0, b->st));
   dim3 grid((b->bins + 255) / 256, n);
-  k_bank_spectra<<<grid, 256, 0, b->st>>>(b->q_r, n, b->r_min, b->r_max, b->count, b->bins, b->logmag, b->phase,
-                                          b->out);
+  k_bank_spectra<<<grid, 256, 0, b->st>>>(b->q_r, b->q_bin, n, b->r_min, b->r_max, b->count, b->bins, b->logmag,
+                                          b->phase, b->out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->e1, b->st));
   HIPCHK(hipMemcpyAsync(out, b->out, sizeof(double2) * nout, hipMemcpyDeviceToHost, b->st));
