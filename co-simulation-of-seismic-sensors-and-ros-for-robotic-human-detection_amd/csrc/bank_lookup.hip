@@ -16,7 +16,8 @@
 //                   consecutive bins and one complex value written.  Queries
 //                   run in range order, so table rows are fetched once per
 //                   L2 rather than once per query.
-// A lookup reads 32 B of tables and writes 16 B per value: HBM-bound.
+// A value reads 32 B of tables (mostly from L2 once queries run in range
+// order) and writes 16 B; the FP64 exp and sincos per value bound it.
 //
 // Arithmetic is Go's: cmplx.Abs as math.Hypot, math.Round half away from
 // zero, cmplx.Rect as r cos + i r sin; log, atan2, exp and sincos are the
