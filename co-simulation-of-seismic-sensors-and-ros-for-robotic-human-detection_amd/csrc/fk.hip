@@ -422,7 +422,14 @@ __global__ __launch_bounds__(kBlock, GF_FK_SOLVE_GROUPS) void k_fk_solve(Medium 
 // the slices are added in order.  One partial per (workgroup, range).  Kept
 // apart from the solve so that it runs at full occupancy: the solve needs
 // every register a lane can have, the J0 evaluations need latency hiding.
-__global__ __launch_bounds__(kBlock) void k_fk_hankel(const Job *__restrict__ jobs,
+// Workgroups of k_fk_hankel resident per CU; 1 leaves the register count to
+// the compiler (130, three wavefronts per SIMD).  4 (128 registers) measured
+// no faster.
+#ifndef GF_FK_HANKEL_GROUPS
+#define GF_FK_HANKEL_GROUPS 1
+#endif
+
+__global__ __launch_bounds__(kBlock, GF_FK_HANKEL_GROUPS) void k_fk_hankel(const Job *__restrict__ jobs,
                                                       const int32_t *__restrict__ blk_job,
                                                       const dcx *__restrict__ rem_in,
                                                       const double *__restrict__ ranges, int32_t nranges,
