@@ -6,6 +6,8 @@ interpolation (internal/bank/interp.go of the reference) over the C ABI.
     Bank.Response(r, bin)     interp.go:29-54
     Bank.Responses(r, bin)    the same at many points, one launch
     Bank.Spectra(ranges)      every bin at each range, one launch
+    Bank.ImpulseResponses     sensing.FromBank (propagation.go:79-112) through
+                              dsp.CausalImpulseResponseAt (dsp.go:232-260)
     RangeNyquist, CheckRangeSampling      interp.go:120-141
 
 Errors the Go code returns raise fdtd.FdtdError with the same text.  Every
@@ -75,6 +77,21 @@ class Bank:
         out = np.zeros((max(rr.size, 1), self.Bins, 2))
         fdtd._check(fdtd.load_library().gf_bank_dev_spectra(self._h, fdtd._dptr(rr), rr.size, fdtd._dptr(out)))
         return out[: rr.size, :, 0] + 1j * out[: rr.size, :, 1]
+
+    def ImpulseResponses(self, ranges: Sequence[float], length: int, fs: float, sensor=None) -> np.ndarray:
+        """sensing.FromBank's vertical velocity response through
+        dsp.CausalImpulseResponseAt at the bank's transform size, for each
+        range: [len(ranges)][length].  sensor: complex per bin, or None."""
+        rr = np.ascontiguousarray(ranges, dtype=np.float64)
+        sen = None
+        if sensor is not None:
+            z = np.asarray(sensor, dtype=complex)
+            sen = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1))
+        out = np.zeros((max(rr.size, 1), max(int(length), 1)))
+        fdtd._check(fdtd.load_library().gf_bank_dev_impulse(
+            self._h, fdtd._dptr(rr), rr.size, int(length), fs, fdtd._dptr(sen) if sen is not None else None,
+            fdtd._dptr(out)))
+        return out[: rr.size, : int(length)]
 
     def Stats(self) -> dict:
         ms, b = C.c_double(), C.c_double()

@@ -48,7 +48,7 @@ int bank_validate_header(const gf_bank_header &h, std::string &err);
   } while (0)
 
 struct gf_bank_dev {
-  double r_min = 0, r_max = 0;
+  double r_min = 0, r_max = 0, rate = 0;
   int32_t count = 0, bins = 0;
   float2 *data = nullptr;
   double *logmag = nullptr, *phase = nullptr;
@@ -161,6 +161,64 @@ __global__ void k_bank_spectra(const double *__restrict__ r, const int32_t *__re
   out[(int64_t)dest[q] * bins + k] = lookup(r[q], k, r_min, r_max, count, bins, logmag, phase);
 }
 
+// The spectra CausalImpulseResponseAt inverts (dsp.go:245-258) for the
+// banked vertical velocity response (propagation.go:89-107): bin k at
+// f = k fs / N is (i 2 pi f) G(r, k), times the sensor, tapered by a raised
+// cosine above 0.8 of Nyquist; bin 0 is zero (f <= 0 answers 0).
+__global__ void k_bank_impulse_coeff(const double *__restrict__ r, int32_t nq, double r_min, double r_max,
+                                     int32_t count, int32_t bins, const double *__restrict__ logmag,
+                                     const double *__restrict__ phase, const double2 *__restrict__ sensor, double fs,
+                                     int32_t nfft, double bank_df, double2 *__restrict__ coeff) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (k >= bins || q >= nq) return;
+  const double df = fs / (double)nfft;
+  const double f = (double)k * df;
+  // The bank bin this frequency lands on (checked on the host); past the
+  // bank's last bin the response is 0 (propagation.go:96-98).
+  const double bin = round(f / bank_df);
+  double2 h = make_double2(0, 0);
+  if (f > 0 && bin >= 0 && bin < bins) {
+    const double2 u = lookup(r[q], (int)bin, r_min, r_max, count, bins, logmag, phase);
+    const double w = 2 * kPi * f;
+    h = make_double2(0.0 * u.x - w * u.y, 0.0 * u.y + w * u.x);  // complex(0, w) * u
+  }
+  if (sensor) {
+    const double2 g = sensor[k];
+    h = make_double2(h.x * g.x - h.y * g.y, h.x * g.y + h.y * g.x);
+  }
+  const double nyq = fs / 2, from = 0.8 * nyq;
+  if (f > from) {
+    const double t = 0.5 * (1 + cos(kPi * (f - from) / (nyq - from)));
+    h = make_double2(h.x * t - h.y * 0.0, h.x * 0.0 + h.y * t);  // h * complex(t, 0)
+  }
+  coeff[(int64_t)q * bins + k] = h;
+}
+
+// dsp.IRFFT (dsp.go:42-50) of an nfft-point spectrum, first `length`
+// samples: x_t = (1/N)(Re X_0 + Re X_{N/2} (-1)^t + 2 sum_k Re(X_k e^{2 pi i k t / N})),
+// the phase k t mod N reduced exactly in integers.
+__global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, int32_t nq, int32_t nfft,
+                              int32_t length, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (t >= length || q >= nq) return;
+  const double2 *X = coeff + (int64_t)q * bins;
+  double s = X[0].x;
+  for (int k = 1; k < bins; k++) {
+    const double2 x = X[k];
+    if (2 * k == nfft) {
+      s += (t & 1) ? -x.x : x.x;
+      continue;
+    }
+    const int64_t ph = ((int64_t)k * t) % nfft;
+    double sn, cs;
+    sincospi(2.0 * (double)ph / (double)nfft, &sn, &cs);
+    s += 2 * (x.x * cs - x.y * sn);
+  }
+  out[(int64_t)q * length + t] = s * (1 / (double)nfft);
+}
+
 int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
 int ok() { return gfs::api_set_err(GF_OK, ""); }
 
@@ -227,6 +285,7 @@ int gf_bank_dev_open(const gf_bank_header *h, const float *payload, gf_bank_dev 
   gf_bank_dev *b = new gf_bank_dev;
   b->r_min = h->r_min_m;
   b->r_max = h->r_max_m;
+  b->rate = h->sample_rate_hz;
   b->count = h->count;
   b->bins = h->samples / 2 + 1;
   const size_t n = (size_t)b->count * b->bins;
@@ -309,6 +368,66 @@ int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *ou
   HIPCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
   b->last_ms = ms;
   b->last_bytes = (double)nout * (32.0 + 16.0);
+  return ok();
+}
+
+int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t length, double fs_hz,
+                        const double *sensor, double *out) {
+  if (!b) return fail(GF_ERR_ARGUMENT, "geofdtd: null bank handle");
+  if (n < 0 || (n > 0 && (!r_m || !out))) return fail(GF_ERR_ARGUMENT, "geofdtd: bad query arrays");
+  // CausalImpulseResponseAt's checks (dsp.go:233-244), transform size = the bank's samples.
+  const int32_t nfft = 2 * (b->bins - 1);
+  if (length <= 0)
+    return fail(GF_ERR_INVALID, "dsp: impulse response length must be positive, got " + std::to_string(length));
+  if (fs_hz <= 0) return fail(GF_ERR_INVALID, "dsp: sample rate must be positive, got " + gfm::go_g(fs_hz));
+  if (nfft < length)
+    return fail(GF_ERR_INVALID, "dsp: transform size " + std::to_string(nfft) + " is shorter than the " +
+                                    std::to_string(length) + " samples to keep");
+  if (n > 65535) return fail(GF_ERR_ARGUMENT, "geofdtd: at most 65535 ranges per impulse call");
+  // For each range in order, the Go loop asks for bins 0..N/2: the grid check
+  // of propagation.go:94-102 at every f > 0, and Bank.Response's range check.
+  const double bank_df = b->rate / (double)nfft, df = fs_hz / (double)nfft;
+  std::string err;
+  for (int32_t q = 0; q < n; q++) {
+    for (int32_t k = 1; k < b->bins; k++) {
+      const double f = (double)k * df;
+      const double x = f / bank_df;
+      const double bin = round(x);
+      if (bin < 0 || bin >= b->bins) continue;  // answered 0
+      if (fabs(x - bin) > 1e-6)
+        return fail(GF_ERR_INVALID, "sensing: " + gfm::go_g(f) + " Hz does not land on the bank's frequency grid "
+                                    "(spacing " + gfm::go_g(bank_df) + " Hz)");
+      if (int rc = check_query(b, r_m[q], (int32_t)bin, err)) return fail(rc, err);
+      break;  // the range check is the same for every bin
+    }
+  }
+  if (n == 0) return ok();
+  const size_t ncoef = (size_t)n * b->bins;
+  if (int rc = reserve(b, (size_t)n, ncoef + ((size_t)n * length + 1) / 2 + 1)) return rc;
+  double2 *coeff = b->out;
+  double *trace = (double *)(b->out + ncoef);
+  double2 *d_sensor = nullptr;
+  if (sensor) {
+    HIPCHK(hipMalloc(&d_sensor, sizeof(double2) * b->bins));
+    HIPCHK(hipMemcpyAsync(d_sensor, sensor, sizeof(double2) * b->bins, hipMemcpyHostToDevice, b->st));
+  }
+  HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipEventRecord(b->e0, b->st));
+  dim3 g1((b->bins + 255) / 256, n);
+  k_bank_impulse_coeff<<<g1, 256, 0, b->st>>>(b->q_r, n, b->r_min, b->r_max, b->count, b->bins, b->logmag, b->phase,
+                                              d_sensor, fs_hz, nfft, bank_df, coeff);
+  HIPCHK(hipGetLastError());
+  dim3 g2((length + 255) / 256, n);
+  k_irdft_first<<<g2, 256, 0, b->st>>>(coeff, b->bins, n, nfft, length, trace);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->e1, b->st));
+  HIPCHK(hipMemcpyAsync(out, trace, sizeof(double) * (size_t)n * length, hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  if (d_sensor) HIPCHK(hipFree(d_sensor));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->last_ms = ms;
+  b->last_bytes = (double)ncoef * 48.0 + (double)n * length * 8.0;
   return ok();
 }
 

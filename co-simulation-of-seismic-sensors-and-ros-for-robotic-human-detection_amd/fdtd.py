@@ -297,6 +297,7 @@ SIGNATURES = {
     "gf_bank_dev_response": (_I, [C.c_void_p, _PD, _PI32, _I64, _PD]),
     "gf_bank_dev_spectra": (_I, [C.c_void_p, _PD, _I32, _PD]),
     "gf_bank_dev_stats": (_I, [C.c_void_p, _PD, _PD]),
+    "gf_bank_dev_impulse": (_I, [C.c_void_p, _PD, _I32, _I32, _D, _PD, _PD]),
     "gf_range_nyquist": (_D, [_D, _D]),
     "gf_convolver_create": (_I, [_PD, _I32, _I32, _I32, C.POINTER(C.c_void_p)]),
     "gf_convolver_destroy": (None, [C.c_void_p]),

@@ -309,6 +309,18 @@ int gf_bank_dev_response(gf_bank_dev *b, const double *r_m, const int32_t *bin, 
  * the spectra a synthesis at those ranges reads. */
 int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *out);
 
+/* The impulse responses runtime synthesis convolves with, from the bank:
+ * sensing.FromBank's VerticalVelocityResponse (propagation.go:89-107: i w
+ * times Bank.Response on the bank's own frequency grid) through
+ * dsp.CausalImpulseResponseAt with the bank's transform size
+ * (dsp.go:232-260: taper over the top fifth of the band, inverse real
+ * transform, first `length` samples).  sensor (may be NULL: 1) multiplies
+ * each bin, [samples/2+1][2], as the engine multiplies by the geophone's
+ * response.  fs_hz must put every bin on the bank's grid.  out [n][length];
+ * the first range whose Go call fails sets the error. */
+int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t length, double fs_hz,
+                        const double *sensor, double *out);
+
 /* Kernel time (HIP events) and algorithmic HBM bytes of the last lookup on
  * this handle: table reads (two range rows of log-magnitude and phase per
  * value, FP64) plus the FP64 complex values written. */

@@ -497,3 +497,38 @@ class OracleBank:
         _check(lib().orc_bank_response(self.r_min, self.r_max, self.count, self.bins, _p(self.logmag),
                                        _p(self.phase), r, int(k), C.byref(re), C.byref(im)))
         return complex(re.value, im.value)
+
+
+def bank_impulse(ob, rate, ranges, length, fs, sensor=None):
+    """sensing.FromBank's VerticalVelocityResponse (propagation.go:89-107)
+    through dsp.CausalImpulseResponseAt at the bank's transform size
+    (dsp.go:232-260), on the oracle bank; numpy's irfft in place of dsp.IRFFT.
+    [len(ranges)][length]."""
+    import math
+
+    n = 2 * (ob.bins - 1)
+    bank_df = rate / n
+    df, nyq = fs / n, fs / 2
+    frm = 0.8 * nyq
+    out = np.zeros((len(ranges), length))
+    for q, r in enumerate(ranges):
+        coeff = np.zeros(n // 2 + 1, dtype=complex)
+        for k in range(n // 2 + 1):
+            f = k * df
+            h = 0j
+            if f > 0:
+                x = f / bank_df
+                b = math.floor(x + 0.5)
+                if 0 <= b < ob.bins:
+                    if abs(x - b) > 1e-6:
+                        raise OracleError(1, f"sensing: off grid at {f} Hz")
+                    u = ob.Response(r, b)
+                    w = 2 * math.pi * f
+                    h = complex(0.0 * u.real - w * u.imag, 0.0 * u.imag + w * u.real)
+            if sensor is not None:
+                h = h * complex(sensor[k])
+            if f > frm:
+                h = h * (0.5 * (1 + math.cos(math.pi * (f - frm) / (nyq - frm))))
+            coeff[k] = h
+        out[q] = np.fft.irfft(coeff, n)[:length]
+    return out

@@ -193,3 +193,24 @@ def test_host_range_sampling_rule():
     bk.CheckRangeSampling(h, 20)
     assert abs(bk.RangeNyquist(180, 300) - 0.3) < 1e-12
     assert math.isinf(bk.RangeNyquist(180, 0))
+
+
+def test_bank_impulse_of_a_unit_velocity_bank_is_the_taper(oracle):
+    """A bank holding 1/(i 2 pi f) at every bin has a vertical velocity
+    response of 1, so its causal impulse response is the inverse transform of
+    dsp.CausalImpulseResponseAt's taper alone (dsp.go:245-258) -- a closed form
+    that pins the i w factor, the taper and the scaling of the composition."""
+    rate, samples = 2000.0, 512
+    bins = samples // 2 + 1
+    f = np.arange(bins) * rate / samples
+    z = np.zeros((3, bins), dtype=complex)
+    z[:, 1:] = 1 / (1j * 2 * np.pi * f[1:])
+    data = np.stack([z.real, z.imag], axis=-1).astype(np.float32)
+    ob = oracle.OracleBank(1, 3, data)
+    got = oracle.bank_impulse(ob, rate, [1.0, 2.5], 300, rate)
+    nyq, frm = rate / 2, 0.8 * rate / 2
+    taper = np.where(f > frm, 0.5 * (1 + np.cos(np.pi * (f - frm) / (nyq - frm))), 1.0)
+    taper[0] = 0.0
+    want = np.fft.irfft(taper, samples)[:300]
+    for q in range(2):
+        assert np.max(np.abs(got[q] - want)) <= 1e-6 * np.max(np.abs(want))

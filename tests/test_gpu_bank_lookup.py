@@ -132,3 +132,45 @@ def test_a_large_bank_equals_the_oracle(oracle):
     print(f"[bank] 4096 ranges x {bins} bins: {st['kernel_ms']:.3f} ms, {st['bytes'] / st['kernel_ms'] / 1e6:.0f} GB/s",
           flush=True)
     dev.close()
+
+
+def test_impulse_responses_equal_the_oracle(oracle):
+    """FromBank + CausalImpulseResponseAt on the device against the oracle's
+    composition: the f-k bank of smallBank, a geophone-like sensor, ranges
+    between grid points; then the chain into the streaming convolver."""
+    data = small_bank(2, 12, 101, 100)
+    dev = device_bank(data, 2, 12)
+    ob = oracle.OracleBank(2, 12, data)
+    f = np.arange(BINS) * RATE / SAMPLES
+    sensor = 28.8 * (1j * f / 4.5) ** 2 / (1 + 1j * 0.7 * (f / 4.5) - (f / 4.5) ** 2)  # 4.5 Hz, damping 0.35
+    ranges = [2.0, 3.37, 7.0, 11.9]
+    for sen in (None, sensor):
+        got = dev.ImpulseResponses(ranges, 400, RATE, sen)
+        want = oracle.bank_impulse(ob, RATE, ranges, 400, RATE, sen)
+        for q in range(len(ranges)):
+            assert np.max(np.abs(got[q] - want[q])) <= 1e-12 * np.max(np.abs(want[q])), (q, sen is None)
+    # A bank grid twice as fine as the transform's (fs = 2 x rate): every
+    # frequency still lands on a bin, the upper half answers 0.
+    got = dev.ImpulseResponses([5.0], 256, 2 * RATE)
+    want = oracle.bank_impulse(ob, RATE, [5.0], 256, 2 * RATE)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    with pytest.raises(fd.FdtdError) as e:
+        dev.ImpulseResponses([5.0], 256, 1.5 * RATE)
+    assert str(e.value) == "sensing: 5.859375 Hz does not land on the bank's frequency grid (spacing 3.90625 Hz)"
+    with pytest.raises(fd.FdtdError) as e:
+        dev.ImpulseResponses([5.0, 13.0], 256, RATE)
+    assert str(e.value) == "bank: range 13 m is outside the bank's 2 to 12 m"
+    with pytest.raises(fd.FdtdError) as e:
+        dev.ImpulseResponses([5.0], 600, RATE)
+    assert str(e.value) == "dsp: transform size 512 is shorter than the 600 samples to keep"
+    # The chain: bank -> impulse responses -> convolvers, all on the device.
+    st = load_package().stream
+    irs = dev.ImpulseResponses(ranges, 400, RATE, sensor)
+    conv = st.NewConvolver(irs, 20)
+    x = np.random.default_rng(2).standard_normal((len(ranges), 1000))
+    y = np.concatenate([conv.Process(x[:, k * 20:(k + 1) * 20]) for k in range(50)], axis=1)
+    for q in range(len(ranges)):
+        want_y = np.convolve(x[q], irs[q])[:1000]
+        assert np.max(np.abs(y[q] - want_y)) <= 1e-12 * np.max(np.abs(want_y))
+    conv.close()
+    dev.close()
