@@ -25,6 +25,7 @@
 
 namespace gfs {
 int api_set_err(int code, const std::string &msg);
+int bank_validate_header(const gf_bank_header &h, std::string &err);
 }
 
 namespace {
@@ -470,6 +471,50 @@ int gf_fk_synthesise(const gf_medium *m, const double *ranges, int32_t n_ranges,
     return rc;
   record_stats(jobs, n_ranges, t);
   if ((rc = first_error(jobs, st, GF_OK, ""))) return rc;
+  return ok();
+}
+
+int gf_fk_bank_fill(const gf_medium *m, const gf_bank_header *h, double max_freq_hz, float *payload) {
+  std::string err;
+  if (!h || !payload) return fail(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  int rc = check_medium(m, err);
+  if (rc) return fail(rc, err);
+  if ((rc = gfs::bank_validate_header(*h, err))) return fail(rc, err);
+  if ((rc = gf_bank_check_range_sampling(h, max_freq_hz))) return rc;
+  const int32_t count = h->count, bins = h->samples / 2 + 1;
+  std::vector<double> ranges((size_t)count);
+  for (int32_t i = 0; i < count; i++)  // RangeGrid.At (format.go:52-57)
+    ranges[i] = count <= 1 ? h->r_min_m : h->r_min_m + (h->r_max_m - h->r_min_m) * (double)i / (double)(count - 1);
+  // topBin (main.go:115-119) and FrequencyAt (format.go:137-139).
+  int32_t top = (int32_t)(max_freq_hz / (h->sample_rate_hz / (double)h->samples));
+  if (top >= bins) top = bins - 1;
+  memset(payload, 0, sizeof(float) * 2 * (size_t)count * bins);
+  if (top < 1) return ok();
+  std::vector<double> freqs((size_t)top);
+  for (int32_t k = 1; k <= top; k++) freqs[k - 1] = (double)k * h->sample_rate_hz / (double)h->samples;
+  std::vector<double> vals((size_t)top * count * 2);
+  gf_integration opt{0, 0};
+  rc = gf_fk_vertical_displacement_multi(m, ranges.data(), count, freqs.data(), top, opt, vals.data());
+  if (rc) {
+    // Which bin: the first whose own call fails, as the error names it.
+    for (int32_t k = 1; k <= top; k++) {
+      std::vector<double> one((size_t)count * 2);
+      if (gf_fk_vertical_displacement_multi(m, ranges.data(), count, &freqs[k - 1], 1, opt, one.data()) != GF_OK) {
+        const std::string inner = gf_last_error();
+        char head[96];
+        snprintf(head, sizeof head, "bin %d (%.3f Hz): ", k, freqs[k - 1]);
+        return fail(GF_ERR_INVALID, head + inner);
+      }
+    }
+    return rc;
+  }
+  for (int32_t k = 1; k <= top; k++)
+    for (int32_t i = 0; i < count; i++) {
+      const double *v = &vals[((size_t)(k - 1) * count + i) * 2];
+      float *o = payload + 2 * ((size_t)i * bins + k);
+      o[0] = (float)v[0];  // bank.Set: float32(real(v)), float32(imag(v))
+      o[1] = (float)v[1];
+    }
   return ok();
 }
 

@@ -292,6 +292,7 @@ SIGNATURES = {
     "gf_fk_vertical_displacement_multi": (_I, [C.POINTER(GfMedium), _PD, _I32, _PD, _I32, GfIntegration, _PD]),
     "gf_fk_synthesise": (_I, [C.POINTER(GfMedium), _PD, _I32, _D, _D, _D, _D, _D, _I32, _I32, _PD]),
     "gf_fk_stats": (_I, [C.POINTER(GfFkStats)]),
+    "gf_fk_bank_fill": (_I, [C.POINTER(GfMedium), C.POINTER(GfBankHeader), _D, C.POINTER(C.c_float)]),
     "gf_bank_dev_open": (_I, [C.POINTER(GfBankHeader), C.POINTER(C.c_float), C.POINTER(C.c_void_p)]),
     "gf_bank_dev_close": (None, [C.c_void_p]),
     "gf_bank_dev_response": (_I, [C.c_void_p, _PD, _PI32, _I64, _PD]),

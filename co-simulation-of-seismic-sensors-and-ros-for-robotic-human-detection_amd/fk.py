@@ -14,6 +14,7 @@ The names and argument meanings follow internal/fk of the reference:
     Medium.checkLayerPhase    fk.go:253-268
     Integration, Grid, MaxLayerPhase
     Synthesise                synthesise in cmd/geofdtd/main.go:168-200
+    FillBank                  cmd/geobank's build loop (main.go:100-175)
 
 Every solve runs on the GPU in float64; errors the Go code returns raise
 fdtd.FdtdError with the same text.  There is no CPU fallback.
@@ -149,6 +150,18 @@ def Synthesise(stack: Sequence[Layer], sls: SLS, refFreq: float, ranges: Sequenc
     fdtd._check(lib.gf_fk_synthesise(C.byref(m), fdtd._dptr(r) if r.size else None, r.size, peak, f0, band, dt, t0,
                                      n, samples, fdtd._dptr(out)))
     return out[: r.size, :n]
+
+
+def FillBank(medium: Medium, h: "fdtd.BankHeader", max_freq: float) -> np.ndarray:
+    """cmd/geobank's build (main.go:100-175): the bank payload for header h by
+    wavenumber integration, bins 1..max_freq, as complex [Count][Bins]
+    (float32 values, as bank.Set stores them)."""
+    lib = fdtd.load_library()
+    m, keep = medium.to_c()
+    ch, keep_h = h.to_c()
+    out = np.zeros((h.Count, h.Bins, 2), dtype=np.float32)
+    fdtd._check(lib.gf_fk_bank_fill(C.byref(m), C.byref(ch), max_freq, out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out[..., 0].astype(np.float64) + 1j * out[..., 1].astype(np.float64)
 
 
 def Stats() -> dict:

@@ -558,6 +558,17 @@ int gf_fk_vertical_displacement_multi(const gf_medium *m, const double *ranges, 
 int gf_fk_synthesise(const gf_medium *m, const double *ranges, int32_t n_ranges, double peak_n, double f0_hz,
                      double band_hz, double dt, double t0, int32_t n, int32_t samples, double *out);
 
+/* The wavenumber-integration fill of a bank, as cmd/geobank's build does it
+ * (main.go:100-175): h validated as bank.Header.Validate, its range grid
+ * checked with CheckRangeSampling at max_freq_hz, then bins 1..top (top =
+ * int(max_freq / bin spacing), at most the last bin) from
+ * VerticalDisplacementMulti at every grid range with the default
+ * integration, all bins in one call; bin 0 and the bins above top stay zero.
+ * payload [count][samples/2+1][2] float32, as bank.Set stores it.  A failing
+ * bin reports "bin %d (%.3f Hz): " and the solver's error, for the lowest
+ * failing bin (geobank's worker pool reports whichever finishes first). */
+int gf_fk_bank_fill(const gf_medium *m, const gf_bank_header *h, double max_freq_hz, float *payload);
+
 /* Counters of the last f-k integration on the calling thread (HIP events
  * on its stream): the per-frequency prologue (layer constants and the static
  * coefficient), the per-sample layered solve, the Hankel sums, and the fold

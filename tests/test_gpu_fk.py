@@ -220,3 +220,36 @@ def test_synthesise_equals_the_oracle(oracle):
     assert got.shape == want.shape == (3, n)
     for i in range(3):
         assert np.max(np.abs(got[i] - want[i])) <= 1e-9 * np.max(np.abs(want[i])), i
+
+
+def test_fill_bank_equals_the_oracle_bank():
+    """cmd/geobank's build loop on the device (gf_fk_bank_fill) against the
+    oracle filling the same grid as the reference's smallBank does, to float32
+    storage; bin 0 and the bins above the band stay zero; the range-sampling
+    check refuses a grid too coarse for the band."""
+    from test_bank_lookup_oracle import small_bank
+
+    layers = [fd.Layer(0, 500, 200, 1700, 25)]
+    h = fd.BankHeader(Solver="geosim fk (layered wavenumber integration)", Medium=layers, SampleRateHz=2000,
+                      Samples=512, MinM=2, MaxM=12, Count=101,
+                      Component="vertical surface displacement per unit vertical surface point force")
+    print("[fk] fill bank 101 ranges x 25 bins", flush=True)
+    got = fk.FillBank(fk.Medium(Stack=layers, DefaultQ=30), h, 100)
+    d = small_bank(2, 12, 101, 100)
+    want = d[..., 0].astype(np.float64) + 1j * d[..., 1].astype(np.float64)
+    assert got.shape == want.shape == (101, 257)
+    assert np.all(got[:, 0] == 0) and np.all(got[:, 26:] == 0)
+    for k in range(1, 26):
+        assert np.max(np.abs(got[:, k] - want[:, k])) <= 1e-6 * np.max(np.abs(want[:, k])), k
+    coarse = fd.BankHeader(Solver="t", Medium=layers, SampleRateHz=2000, Samples=512, MinM=1, MaxM=40, Count=40,
+                           Component="x")
+    with pytest.raises(fd.FdtdError) as e:
+        fk.FillBank(fk.Medium(Stack=layers), coarse, 300)
+    assert str(e.value) == ("bank: range spacing 1.0000 m is too coarse to unwrap phase up to 300 Hz; "
+                            "need under 0.1450 m")
+    layered = [fd.Layer(2, 500, 200, 1700, 25), fd.Layer(0, 900, 400, 2000, 60)]
+    hl = fd.BankHeader(Solver="t", Medium=layered, SampleRateHz=2000, Samples=512, MinM=2, MaxM=4, Count=41,
+                       Component="x")
+    with pytest.raises(fd.FdtdError) as e:
+        fk.FillBank(fk.Medium(Stack=layered), hl, 400)
+    assert str(e.value).startswith("bin 57 (222.656 Hz): fk: layer 0 is 2 m thick, which is k*h = 16.1 at 222.65625 Hz")
