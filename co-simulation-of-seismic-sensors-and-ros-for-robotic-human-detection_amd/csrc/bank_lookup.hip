@@ -273,6 +273,61 @@ void release(gf_bank_dev *b) {
 
 }  // namespace
 
+namespace gfbank {
+
+int32_t bins(const gf_bank_dev *b) { return b->bins; }
+double rate(const gf_bank_dev *b) { return b->rate; }
+
+// CausalImpulseResponseAt's checks (dsp.go:233-244) with the bank's
+// transform size, then, for each range in order, what the Go loop over bins
+// 0..N/2 would reject first: the grid check of propagation.go:94-102 at the
+// first f > 0 that lands inside the bank, and Bank.Response's range check
+// there (the same for every bin).
+int impulse_check(const gf_bank_dev *b, const double *r_m, int32_t n, int32_t length, double fs_hz) {
+  const int32_t nfft = 2 * (b->bins - 1);
+  if (length <= 0)
+    return fail(GF_ERR_INVALID, "dsp: impulse response length must be positive, got " + std::to_string(length));
+  if (fs_hz <= 0) return fail(GF_ERR_INVALID, "dsp: sample rate must be positive, got " + gfm::go_g(fs_hz));
+  if (nfft < length)
+    return fail(GF_ERR_INVALID, "dsp: transform size " + std::to_string(nfft) + " is shorter than the " +
+                                    std::to_string(length) + " samples to keep");
+  const double bank_df = b->rate / (double)nfft, df = fs_hz / (double)nfft;
+  std::string err;
+  for (int32_t q = 0; q < n; q++) {
+    for (int32_t k = 1; k < b->bins; k++) {
+      const double f = (double)k * df;
+      const double x = f / bank_df;
+      const double bin = round(x);
+      if (bin < 0 || bin >= b->bins) continue;  // answered 0
+      if (fabs(x - bin) > 1e-6)
+        return fail(GF_ERR_INVALID, "sensing: " + gfm::go_g(f) + " Hz does not land on the bank's frequency grid "
+                                    "(spacing " + gfm::go_g(bank_df) + " Hz)");
+      if (int rc = check_query(b, r_m[q], (int32_t)bin, err)) return fail(rc, err);
+      break;
+    }
+  }
+  return GF_OK;
+}
+
+// The two kernels of an impulse-response build for n ranges already in HBM
+// (d_r), on stream st: spectra into d_coeff [n][bins], the first `length`
+// samples of their inverse transforms into d_out [n][length].
+hipError_t impulse_launch(const gf_bank_dev *b, const double *d_r, int32_t n, int32_t length, double fs_hz,
+                          const double2 *d_sensor, double2 *d_coeff, double *d_out, hipStream_t st) {
+  const int32_t nfft = 2 * (b->bins - 1);
+  const double bank_df = b->rate / (double)nfft;
+  dim3 g1((b->bins + 255) / 256, n);
+  k_bank_impulse_coeff<<<g1, 256, 0, st>>>(d_r, n, b->r_min, b->r_max, b->count, b->bins, b->logmag, b->phase,
+                                           d_sensor, fs_hz, nfft, bank_df, d_coeff);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  dim3 g2((length + 255) / 256, n);
+  k_irdft_first<<<g2, 256, 0, st>>>(d_coeff, b->bins, n, nfft, length, d_out);
+  return hipGetLastError();
+}
+
+}  // namespace gfbank
+
 extern "C" {
 
 int gf_bank_dev_open(const gf_bank_header *h, const float *payload, gf_bank_dev **out) {
@@ -375,32 +430,8 @@ int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t le
                         const double *sensor, double *out) {
   if (!b) return fail(GF_ERR_ARGUMENT, "geofdtd: null bank handle");
   if (n < 0 || (n > 0 && (!r_m || !out))) return fail(GF_ERR_ARGUMENT, "geofdtd: bad query arrays");
-  // CausalImpulseResponseAt's checks (dsp.go:233-244), transform size = the bank's samples.
-  const int32_t nfft = 2 * (b->bins - 1);
-  if (length <= 0)
-    return fail(GF_ERR_INVALID, "dsp: impulse response length must be positive, got " + std::to_string(length));
-  if (fs_hz <= 0) return fail(GF_ERR_INVALID, "dsp: sample rate must be positive, got " + gfm::go_g(fs_hz));
-  if (nfft < length)
-    return fail(GF_ERR_INVALID, "dsp: transform size " + std::to_string(nfft) + " is shorter than the " +
-                                    std::to_string(length) + " samples to keep");
   if (n > 65535) return fail(GF_ERR_ARGUMENT, "geofdtd: at most 65535 ranges per impulse call");
-  // For each range in order, the Go loop asks for bins 0..N/2: the grid check
-  // of propagation.go:94-102 at every f > 0, and Bank.Response's range check.
-  const double bank_df = b->rate / (double)nfft, df = fs_hz / (double)nfft;
-  std::string err;
-  for (int32_t q = 0; q < n; q++) {
-    for (int32_t k = 1; k < b->bins; k++) {
-      const double f = (double)k * df;
-      const double x = f / bank_df;
-      const double bin = round(x);
-      if (bin < 0 || bin >= b->bins) continue;  // answered 0
-      if (fabs(x - bin) > 1e-6)
-        return fail(GF_ERR_INVALID, "sensing: " + gfm::go_g(f) + " Hz does not land on the bank's frequency grid "
-                                    "(spacing " + gfm::go_g(bank_df) + " Hz)");
-      if (int rc = check_query(b, r_m[q], (int32_t)bin, err)) return fail(rc, err);
-      break;  // the range check is the same for every bin
-    }
-  }
+  if (int rc = gfbank::impulse_check(b, r_m, n, length, fs_hz)) return rc;
   if (n == 0) return ok();
   const size_t ncoef = (size_t)n * b->bins;
   if (int rc = reserve(b, (size_t)n, ncoef + ((size_t)n * length + 1) / 2 + 1)) return rc;
@@ -413,13 +444,7 @@ int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t le
   }
   HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipEventRecord(b->e0, b->st));
-  dim3 g1((b->bins + 255) / 256, n);
-  k_bank_impulse_coeff<<<g1, 256, 0, b->st>>>(b->q_r, n, b->r_min, b->r_max, b->count, b->bins, b->logmag, b->phase,
-                                              d_sensor, fs_hz, nfft, bank_df, coeff);
-  HIPCHK(hipGetLastError());
-  dim3 g2((length + 255) / 256, n);
-  k_irdft_first<<<g2, 256, 0, b->st>>>(coeff, b->bins, n, nfft, length, trace);
-  HIPCHK(hipGetLastError());
+  HIPCHK(gfbank::impulse_launch(b, b->q_r, n, length, fs_hz, d_sensor, coeff, trace, b->st));
   HIPCHK(hipEventRecord(b->e1, b->st));
   HIPCHK(hipMemcpyAsync(out, trace, sizeof(double) * (size_t)n * length, hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipStreamSynchronize(b->st));

@@ -307,6 +307,13 @@ SIGNATURES = {
     "gf_convolver_reset": (_I, [C.c_void_p]),
     "gf_convolver_process_all": (_I, [C.c_void_p, _PD, _I64, _PD]),
     "gf_convolver_stats": (_I, [C.c_void_p, _PD]),
+    "gf_engine_create": (_I, [C.c_void_p, _I32, _PD, _I32, _D, _I32, _PD, _I32, C.POINTER(C.c_void_p)]),
+    "gf_engine_destroy": (None, [C.c_void_p]),
+    "gf_engine_warm": (_I, [C.c_void_p, _PD, _I32]),
+    "gf_engine_admit": (_I, [C.c_void_p, C.c_char_p, _D, _D, _PD, _I32, _D]),
+    "gf_engine_next": (_I, [C.c_void_p, C.POINTER(C.c_float), _I32]),
+    "gf_engine_info": (_I, [C.c_void_p, _PI64, _PI32, _PI32, _PI32]),
+    "gf_engine_stats": (_I, [C.c_void_p, _PD, _PD, _PI64, _PD, _PI64]),
     "gf_bank_check_range_sampling": (_I, [C.POINTER(GfBankHeader), _D]),
 }
 

@@ -1,0 +1,351 @@
+"""Python face of runtime synthesis in libgeofdtd: the reference's sensing
+engine driven by a bank (internal/sensing/sensing.go with
+sensing.FromBank, propagation.go:79-116) and the walking schedule it
+consumes (internal/source/source.go), over the C ABI.
+
+    Walk, Footfall, Contact, RangeSpan   source.go:39-263 (host logic: which
+                                         contacts begin when, and where)
+    FromBank(bank)                       propagation.go:79-84
+    NewEngineWith(...)                   sensing.go:117-170
+    Engine.Next, ChunkSize, SampleRate, ChunkDuration, StartTime,
+    ActiveVoices, Propagation            sensing.go:224-330
+    RangeQuantum                         sensing.go:49
+
+The engine's state (impulse responses per range quantum, each voice's force
+record) lives in HBM and every chunk is mixed on the GPU for all receivers
+in one launch.  The reference's NewEngineWith takes a resolved config; this
+mirror takes the numbers it reads from one: the sampling rate, the walk's
+duration, the geometry's range (the fallback furthest range), the soil's
+Rayleigh velocity, and the sensor's frequency response.  Errors the Go code
+returns raise fdtd.FdtdError with the same text.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import fdtd
+from .workloads import Stance, _DUTY
+
+RangeQuantum = 0.02
+StanceFraction = _DUTY
+
+
+# ------------------------------------------------------------------ schedule
+
+
+@dataclass
+class Footfall:
+    """source.Footfall (source.go:77-107): a stance resolved to a heading."""
+
+    Stance: Stance
+    Heading: float = 0.0
+
+    def Duration(self) -> float:
+        return self.Stance.Duration
+
+    def ForceAt(self, since):
+        """(fx, fy, fz) in newtons at times since heel strike (scalar or
+        array); zero outside (0, Duration)."""
+        since = np.asarray(since, dtype=np.float64)
+        tau = since / self.Stance.Duration
+        w = self.Stance.body_weight()
+        inside = (since > 0) & (since < self.Stance.Duration)
+        ap = np.where(inside, self.Stance.anterior_posterior_at(tau) * w, 0.0)
+        fz = np.where(inside, self.Stance.profile_at(tau) * w, 0.0)
+        return ap * math.cos(self.Heading), ap * math.sin(self.Heading), fz
+
+
+@dataclass
+class Contact:
+    """source.Contact (source.go:39-58)."""
+
+    ID: str
+    X: float
+    Y: float
+    Start: float
+    Profile: Footfall
+
+    def End(self) -> float:
+        return self.Start + self.Profile.Duration()
+
+    def ForceAt(self, t):
+        return self.Profile.ForceAt(np.asarray(t, dtype=np.float64) - self.Start)
+
+
+@dataclass
+class Walk:
+    """source.Walk (source.go:109-252): two feet alternating along a line."""
+
+    Stance: Stance
+    Speed: float
+    StartX: float = 0.0
+    StartY: float = 0.0
+    Heading: float = 0.0
+    StrideLength: float = 0.0
+    Width: float = 0.0
+    FirstFoot: str = ""
+    SecondFoot: str = ""
+    Until: float = 0.0
+
+    def Validate(self) -> None:
+        if self.Speed <= 0:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID,
+                                 f"source: walking speed must be positive, got {fdtd._go_g(self.Speed)} m/s")
+        if self.Width < 0:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID,
+                                 f"source: stance width must not be negative, got {fdtd._go_g(self.Width)} m")
+        if self.StrideLength < 0:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "source: stride length must not be negative, got "
+                                 f"{fdtd._go_g(self.StrideLength)} m")
+        self.Stance.Validate()
+
+    def StrideOrDefault(self) -> float:
+        if self.StrideLength > 0:
+            return self.StrideLength
+        return self.Speed * self.Stance.Duration / StanceFraction
+
+    def width(self) -> float:
+        return self.Width if self.Width > 0 else 0.12
+
+    def feet(self) -> Tuple[str, str]:
+        return self.FirstFoot or "left_foot", self.SecondFoot or "right_foot"
+
+    def StepPeriod(self) -> float:
+        return self.StrideOrDefault() / 2 / self.Speed
+
+    def ContactAt(self, n: int) -> Contact:
+        step = self.StrideOrDefault() / 2
+        along = float(n) * step
+        side = self.width() / 2
+        if n % 2 == 1:
+            side = -side
+        cos, sin = math.cos(self.Heading), math.sin(self.Heading)
+        first, second = self.feet()
+        return Contact(ID=second if n % 2 == 1 else first, X=self.StartX + along * cos - side * sin,
+                       Y=self.StartY + along * sin + side * cos, Start=float(n) * self.StepPeriod(),
+                       Profile=Footfall(self.Stance, self.Heading))
+
+    def Contacts(self, frm: float, to: float) -> List[Contact]:
+        if to <= frm:
+            return []
+        period = self.StepPeriod()
+        if period <= 0:
+            return []
+        if self.Until > 0 and frm >= self.Until:
+            return []
+        if self.Until > 0 and to > self.Until:
+            to = self.Until
+        lo = max(int(math.ceil(frm / period)), 0)
+        hi = int(math.ceil(to / period))
+        out = []
+        for n in range(lo, hi):
+            c = self.ContactAt(n)
+            if frm <= c.Start < to:
+                out.append(c)
+        return out
+
+
+@dataclass
+class Fixed:
+    """A fixed list of contacts as a schedule (what the reference's tests use)."""
+
+    contacts: List[Contact] = field(default_factory=list)
+
+    def Contacts(self, frm: float, to: float) -> List[Contact]:
+        return [c for c in self.contacts if frm <= c.Start < to]
+
+
+def RangeSpan(s, rx: float, ry: float, frm: float, to: float):
+    """source.RangeSpan (source.go:254-263): (min, max, ok)."""
+    lo, hi, ok = math.inf, 0.0, False
+    for c in s.Contacts(frm, to):
+        r = math.hypot(rx - c.X, ry - c.Y)
+        lo, hi, ok = min(lo, r), max(hi, r), True
+    return lo, hi, ok
+
+
+# -------------------------------------------------------------- propagation
+
+
+class FromBank:
+    """sensing.FromBank (propagation.go:79-116) over a device bank."""
+
+    def __init__(self, b):
+        if b is None:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: bank is nil")
+        self.bank = b
+
+    def TransformSize(self) -> int:
+        return self.bank.header.Samples
+
+    def Describe(self) -> str:
+        h = self.bank.header
+        return (f"bank: {h.Solver}, {h.Count} ranges {fdtd._go_g(h.MinM)}-{fdtd._go_g(h.MaxM)} m, "
+                f"{h.Component}")
+
+
+# ------------------------------------------------------------------- engine
+
+
+class Engine:
+    """sensing.Engine (sensing.go:51-80) for one or many receivers."""
+
+    def __init__(self, handle, prop, schedule, chunk, fs, receivers):
+        self._h = handle
+        self._prop = prop
+        self._schedule = schedule
+        self.chunk = chunk
+        self.fs = fs
+        self.receivers = receivers
+        self._single = receivers.shape[0] == 1
+        self.scheduled = 0.0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            fdtd.load_library().gf_engine_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _info(self):
+        em, act, resp, irl = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
+        fdtd._check(fdtd.load_library().gf_engine_info(self._h, C.byref(em), C.byref(act), C.byref(resp),
+                                                       C.byref(irl)))
+        return em.value, act.value, resp.value, irl.value
+
+    def Propagation(self) -> str:
+        return self._prop.Describe()
+
+    def ChunkSize(self) -> int:
+        return self.chunk
+
+    def SampleRate(self) -> float:
+        return self.fs
+
+    def ChunkDuration(self) -> float:
+        return float(self.chunk) / self.fs
+
+    def StartTime(self) -> float:
+        return float(self._info()[0]) / self.fs
+
+    def ActiveVoices(self) -> int:
+        return self._info()[1]
+
+    def Responses(self) -> int:
+        return self._info()[2]
+
+    def ImpulseLength(self) -> int:
+        return self._info()[3]
+
+    def _warm(self, frm: float, to: float) -> None:
+        cs = self._schedule.Contacts(frm, to)
+        if not cs:
+            return
+        xy = np.ascontiguousarray([[c.X, c.Y] for c in cs], dtype=np.float64)
+        fdtd._check(fdtd.load_library().gf_engine_warm(self._h, fdtd._dptr(xy), len(cs)))
+
+    def _force(self, c: Contact, emitted: int) -> np.ndarray:
+        """The contact's vertical force at the instants the reference's Next
+        evaluates it (t0 + i / fs per chunk, t0 = emitted / fs), from this
+        chunk until the force has surely ended."""
+        last = int(math.ceil(c.End() * self.fs)) + 2
+        n = max(last - emitted, 1)
+        k = np.arange(n)
+        starts = emitted + (k // self.chunk) * self.chunk
+        t = starts / self.fs + (k % self.chunk) / self.fs
+        return np.ascontiguousarray(c.ForceAt(t)[2], dtype=np.float64)
+
+    def Next(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """One chunk of output (volts, float32): [chunk] for one receiver,
+        [receivers][chunk] for several.  out, when given, is filled."""
+        lib = fdtd.load_library()
+        if out is not None and np.shape(out)[-1:] != (self.chunk,):
+            n_out = np.shape(out)[-1] if np.ndim(out) else 0
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, f"sensing: output length {n_out}, want {self.chunk}")
+        emitted = self._info()[0]
+        t0 = float(emitted) / self.fs
+        t1 = t0 + self.ChunkDuration()
+        # admit (sensing.go:295-330): keep the responses warmed ahead, then a
+        # voice for every contact beginning in this chunk.
+        lookahead = 2.0
+        if t1 + lookahead > self.scheduled:
+            self._warm(self.scheduled, t1 + lookahead)
+            self.scheduled = t1 + lookahead
+        for c in self._schedule.Contacts(t0, t1):
+            f = self._force(c, emitted)
+            fdtd._check(lib.gf_engine_admit(self._h, c.ID.encode(), c.X, c.Y, fdtd._dptr(f), f.size, c.End()))
+        buf = np.zeros((self.receivers.shape[0], self.chunk), dtype=np.float32)
+        fdtd._check(lib.gf_engine_next(self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), self.chunk))
+        res = buf[0] if self._single else buf
+        if out is not None:
+            out[...] = res
+        return res
+
+    def Stats(self) -> dict:
+        mix, tot, macs, build, built = C.c_double(), C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        fdtd._check(fdtd.load_library().gf_engine_stats(self._h, C.byref(mix), C.byref(tot), C.byref(macs),
+                                                        C.byref(build), C.byref(built)))
+        return {"mix_ms": mix.value, "mix_ms_total": tot.value, "macs": macs.value, "build_ms": build.value,
+                "built": built.value}
+
+
+def ImpulseLength(schedule, walk_duration: float, geometry_range: float, rayleigh_velocity: float, fs: float,
+                  receivers=((0.0, 0.0),)) -> int:
+    """The length NewEngineWith gives every impulse response
+    (sensing.go:135-140): the furthest travel time plus two seconds of coda,
+    the furthest range over every receiver."""
+    far, any_ = 0.0, False
+    for rx, ry in receivers:
+        _, f, ok = RangeSpan(schedule, rx, ry, 0, walk_duration)
+        if ok:
+            far, any_ = max(far, f), True
+    if not any_:
+        far = geometry_range
+    return int(math.ceil((far / rayleigh_velocity + 2) * fs))
+
+
+def NewEngineWith(schedule, chunk: int, prop: Optional[FromBank], *, sample_rate: float, walk_duration: float,
+                  geometry_range: float, rayleigh_velocity: float,
+                  sensor: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+                  receivers: Sequence[Sequence[float]] = ((0.0, 0.0),)) -> Engine:
+    """sensing.NewEngineWith (sensing.go:117-170) with a bank's propagation.
+
+    sensor maps an array of frequencies (Hz) to the sensor's complex response
+    (None: 1).  receivers are (x, y) positions in metres; the reference's
+    engine listens at the origin."""
+    if chunk <= 0:
+        raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, f"sensing: chunk size must be positive, got {chunk}")
+    if schedule is None:
+        raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a schedule is required")
+    if prop is None:
+        raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a propagation model is required")
+    rec = np.ascontiguousarray(np.atleast_2d(np.asarray(receivers, dtype=np.float64)))
+    ir_len = ImpulseLength(schedule, walk_duration, geometry_range, rayleigh_velocity, sample_rate,
+                           [tuple(r) for r in rec])
+    h = prop.bank.header
+    sens = None
+    if sensor is not None:
+        f = np.arange(h.Bins) * sample_rate / h.Samples
+        g = np.asarray(sensor(f), dtype=np.complex128)
+        sens = np.ascontiguousarray(np.stack([g.real, g.imag], axis=-1))
+    # The longest force record one voice holds: admitted up to a chunk before
+    # its heel strike, sampled until two samples past toe-off.
+    longest = max([c.Profile.Duration() for c in schedule.Contacts(0, walk_duration + 2)] or [0.0])
+    cap = int(math.ceil(longest * sample_rate)) + chunk + 8
+    handle = C.c_void_p()
+    fdtd._check(fdtd.load_library().gf_engine_create(prop.bank._h, rec.shape[0], fdtd._dptr(rec), int(chunk),
+                                                     sample_rate, ir_len, fdtd._dptr(sens) if sens is not None
+                                                     else None, cap, C.byref(handle)))
+    e = Engine(handle, prop, schedule, int(chunk), sample_rate, rec)
+    try:
+        e._warm(0, walk_duration)
+    except fdtd.FdtdError:
+        e.close()
+        raise
+    return e

@@ -71,6 +71,7 @@ def c1(variant: str = "R", steps: int = 10_000) -> Workload:
 _GRAVITY = 9.80665
 _HUMP_WIDTH, _TRANSIENT_PEAK, _TRANSIENT_RISE = 0.11, 0.35, 0.012
 _TAPER, _P1, _P2, _DUTY = 0.098, 0.25, 0.75, 0.60
+_AP_PEAK = 0.20
 _CREST, _CREST_PHASE = 0.63790, 0.20090
 
 
@@ -82,6 +83,36 @@ class Stance:
     SecondPeak: float = 1.12
     MidstanceValley: float = 0.75
     TransientRise: float = 0.0
+    APPeak: float = 0.0
+
+    def Validate(self) -> None:
+        """Stance.Validate (grf.go:145-163) for the fields this mirror carries."""
+        if self.Mass <= 0:
+            raise fd.FdtdError(fd.GF_ERR_INVALID, f"grf: mass must be positive, got {fd._go_g(self.Mass)} kg")
+        if self.Duration <= 0:
+            raise fd.FdtdError(fd.GF_ERR_INVALID,
+                               f"grf: stance duration must be positive, got {fd._go_g(self.Duration)} s")
+        if self.FirstPeak <= 0 or self.SecondPeak <= 0:
+            raise fd.FdtdError(fd.GF_ERR_INVALID, f"grf: peaks must be positive, got {fd._go_g(self.FirstPeak)} "
+                               f"and {fd._go_g(self.SecondPeak)} BW")
+        if self.MidstanceValley <= 0:
+            raise fd.FdtdError(fd.GF_ERR_INVALID,
+                               f"grf: midstance valley must be positive, got {fd._go_g(self.MidstanceValley)} BW")
+        if self.MidstanceValley >= self.FirstPeak or self.MidstanceValley >= self.SecondPeak:
+            raise fd.FdtdError(fd.GF_ERR_INVALID, f"grf: valley {fd._go_g(self.MidstanceValley)} BW must lie below "
+                               f"both peaks ({fd._go_g(self.FirstPeak)}, {fd._go_g(self.SecondPeak)} BW), or the "
+                               "profile is not a double hump")
+
+    def anterior_posterior_at(self, tau):
+        """Stance.AnteriorPosteriorAt (grf.go:264-270): the fore-aft shear in
+        body weights, negative (braking) in the first half of stance."""
+        tau = np.asarray(tau, dtype=np.float64)
+        a = self.APPeak if self.APPeak > 0 else (0.0 if self.APPeak < 0 else _AP_PEAK)
+        r = _TAPER
+        tk = np.where(tau < r, 0.5 * (1 - np.cos(math.pi * tau / r)),
+                      np.where(tau > 1 - r, 0.5 * (1 - np.cos(math.pi * (1 - tau) / r)), 1.0))
+        out = tk * -a * np.sin(2 * math.pi * tau)
+        return np.where((tau <= 0) | (tau >= 1) | (a == 0), 0.0, out)
 
     def _rise(self):
         return self.TransientRise if self.TransientRise > 0 else _TRANSIENT_RISE

@@ -368,6 +368,62 @@ int gf_convolver_process_all(gf_convolver *c, const double *in, int64_t n, doubl
 /* Kernel time (HIP events) of the last Process or ProcessAll. */
 int gf_convolver_stats(const gf_convolver *c, double *kernel_ms);
 
+/* ---- runtime synthesis from a bank (the Go package internal/sensing) ----
+ * sensing.Engine over sensing.FromBank, for n_recv receivers at once (the
+ * reference runs one Engine per receiver, at the origin).  Each contact gets
+ * a voice: its vertical force convolved with the impulse response at its
+ * range, rounded to RangeQuantum = 0.02 m (sensing.go:49, 178); the bank
+ * carries no radial-force response (propagation.go:110-112), so the
+ * reference's second convolver per voice adds exactly zero and is not run.
+ * The schedule (source.Walk and its contacts) stays with the caller, which
+ * warms the responses, admits each contact as its chunk begins and samples
+ * its force; the engine keeps the responses and voices in HBM and mixes a
+ * chunk for every receiver in one launch.  A handle is single-threaded, and
+ * the bank handle must outlive it. */
+typedef struct gf_engine gf_engine;
+
+/* NewEngineWith(res, sch, chunk, FromBank(bank)) (sensing.go:117-170):
+ * receivers at recv_xy [n_recv][2] (m), chunk samples per Next at fs_hz (the
+ * resolved sampling rate), ir_len = ceil((far / c_R + 2) fs) as the Go code
+ * derives it from the schedule's furthest range, capped here at three
+ * quarters of the bank's transform (sensing.go:141-148); sensor (may be
+ * NULL: 1) the sensor's response at the bank's bins, [samples/2+1][2].
+ * max_contact_samples bounds one voice's force record.  Errors: "sensing:
+ * chunk size must be positive, got %d", "sensing: bank is nil". */
+int gf_engine_create(const gf_bank_dev *bank, int32_t n_recv, const double *recv_xy, int32_t chunk, double fs_hz,
+                     int32_t ir_len, const double *sensor, int32_t max_contact_samples, gf_engine **out);
+void gf_engine_destroy(gf_engine *e);
+
+/* Engine.warm (sensing.go:176-184): the responses for contacts at
+ * contact_xy [n][2], for every receiver, built in one batch; the first range
+ * whose Go call would fail sets its error (the bank's range check, the
+ * frequency-grid check). */
+int gf_engine_warm(gf_engine *e, const double *contact_xy, int32_t n);
+
+/* Engine.admit for one contact (sensing.go:295-330): a voice starting at the
+ * next chunk, whose vertical force is force_z[m] at sample emitted + m
+ * (m < n; zero after), running until ceil(end_s fs) + ir_len.  Errors:
+ * "sensing: contact %q at the receiver's own position". */
+int gf_engine_admit(gf_engine *e, const char *id, double x, double y, const double *force_z, int32_t n,
+                    double end_s);
+
+/* Engine.Next (sensing.go:246-289): out [n_recv][chunk] float32, the sum of
+ * the active voices; then retire and advance.  len must equal the chunk
+ * ("sensing: output length %d, want %d"). */
+int gf_engine_next(gf_engine *e, float *out, int32_t len);
+
+/* Samples emitted (StartTime x fs), ActiveVoices, responses cached, and the
+ * impulse-response length in use. */
+int gf_engine_info(const gf_engine *e, int64_t *emitted, int32_t *active_voices, int32_t *responses,
+                   int32_t *ir_len);
+
+/* Kernel time (HIP events) of the last Next's mix and of every mix so far,
+ * the multiply-adds those mixes did (per receiver, output and voice, the taps
+ * where force and response are both defined), the total time of response
+ * builds, and responses built. */
+int gf_engine_stats(const gf_engine *e, double *mix_ms, double *mix_ms_total, int64_t *macs, double *build_ms,
+                    int64_t *built);
+
 /* ---- seeded soil models (batched bank generation) ----------------------
  * Soil model `index` of the stream keyed by `seed`, drawn with Philox4x32-10
  * (Salmon et al., SC'11): 1-3 layers over a half-space, each with Vs

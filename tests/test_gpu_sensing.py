@@ -1,0 +1,220 @@
+"""The device sensing engine (sensing.Engine over FromBank) against the
+oracle's restatement of sensing.Reference, the monolithic sum the
+reference's own tests hold the streaming engine to
+(sensing_test.go:48-77), and the reference's engine properties: voices are
+retired (:174-199), start times follow the sample count (:201-221), a bank
+drives synthesis with a believable geophone signal (bank_test.go:66-121),
+errors carry the Go text.
+
+Tolerance: the reference's own, 1e-7 of the peak (sensing_test.go:70), which
+is the float32 output's rounding; beyond it every sample must equal the
+oracle's sum rounded to float32 within one float32 step (the device sums
+directly in FP64, the oracle through numpy's convolution).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import sensing_oracle as SO
+from conftest import load_package
+from test_bank_lookup_oracle import RATE, SAMPLES, small_bank
+from test_gpu_bank_lookup import device_bank
+
+pkg = load_package()
+fd, bk, fk, se, wl = pkg.fdtd, pkg.bank, pkg.fk, pkg.sensing, pkg.workloads
+
+pytestmark = pytest.mark.gpu
+
+C_R = 186.0  # Rayleigh velocity of the loam test medium (Vs 200, Vp 500), about 0.93 Vs
+
+
+def geophone(f):
+    f = np.asarray(f, dtype=np.float64)
+    return 28.8 * (1j * f / 4.5) ** 2 / (1 + 1j * 0.7 * (f / 4.5) - (f / 4.5) ** 2)
+
+
+def walk_past(range_m=4.0, approach=4.0, speed=1.3):
+    """config.Resolved.WalkPast (config.go:368-381) for a 75 kg walker."""
+    return se.Walk(Stance=wl.Stance(Mass=75), Speed=speed, StartX=-approach, StartY=range_m, Width=0.12,
+                   Until=2 * approach / speed)
+
+
+def oracle_walk(w):
+    return SO.WalkSpec(duration=w.Stance.Duration, speed=w.Speed, start_x=w.StartX, start_y=w.StartY,
+                       heading=w.Heading, stride=w.StrideLength, width=w.Width, until=w.Until)
+
+
+def engine(b, w, chunk, fs=RATE, receivers=((0.0, 0.0),), sensor=geophone):
+    return se.NewEngineWith(w, chunk, se.FromBank(b), sample_rate=fs, walk_duration=w.Until,
+                            geometry_range=w.StartY, rayleigh_velocity=C_R, sensor=sensor, receivers=receivers)
+
+
+def run(e, n):
+    out = []
+    while sum(x.shape[-1] for x in out) < n:
+        out.append(e.Next().copy())
+    return np.concatenate(out, axis=-1)[..., :n]
+
+
+def oracle_reference(ob, rate, w, n, fs, ir_len, rx=0.0, ry=0.0, sensor=geophone):
+    """sensing.Reference on the oracle: the oracle's schedule, each contact's
+    vertical force from the same footfall, the oracle bank's responses."""
+    contacts = SO.walk_contacts(oracle_walk(w), 0, n / fs)
+    foot = se.Footfall(w.Stance, w.Heading)
+    forces = [lambda t, c=c: foot.ForceAt(t - c.start)[2] for c in contacts]
+    s = None if sensor is None else sensor(np.arange(ob.bins) * fs / (2 * (ob.bins - 1)))
+    import oracle as O
+
+    return SO.reference(contacts, forces, n, fs, lambda rq: O.bank_impulse(ob, rate, [rq], ir_len, fs, s)[0], rx, ry)
+
+
+def assert_matches(got, want):
+    scale = np.max(np.abs(want))
+    assert scale > 0
+    err = np.max(np.abs(got.astype(np.float64) - want))
+    assert err <= 1e-7 * scale, err / scale
+    w32 = want.astype(np.float32)
+    step = np.spacing(np.abs(w32)).astype(np.float64)
+    assert np.all(np.abs(got.astype(np.float64) - w32.astype(np.float64)) <= step + 1e-12 * scale)
+    return err / scale
+
+
+def test_streaming_matches_the_reference_across_voices(oracle):
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    ob = oracle.OracleBank(2, 12, data)
+    w = walk_past()
+    n = 8000
+    want = oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4)
+    for chunk in (10, 25, 100):
+        e = engine(b, w, chunk)
+        assert e.ImpulseLength() == 3 * SAMPLES // 4
+        got = run(e, n)
+        rel = assert_matches(got, want)
+        print(f"[sensing] chunk {chunk}: max |engine - reference| = {rel:.2e} of the peak, "
+              f"{e.Responses()} responses", flush=True)
+        e.close()
+    b.close()
+
+
+def test_many_receivers_in_one_engine(oracle):
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    ob = oracle.OracleBank(2, 12, data)
+    w = walk_past()
+    rec = [(0.0, 0.0), (1.5, 0.3), (-2.0, -0.5), (3.0, 1.0)]
+    n = 5000
+    e = engine(b, w, 20, receivers=rec)
+    got = run(e, n)
+    assert got.shape == (4, n)
+    for q, (rx, ry) in enumerate(rec):
+        print(f"[sensing] receiver {q} at ({rx}, {ry})", flush=True)
+        assert_matches(got[q], oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4, rx, ry))
+    # Receiver 0 alone gives the same samples bit for bit.
+    one = engine(b, w, 20)
+    assert np.array_equal(run(one, n), got[0])
+    one.close()
+    e.close()
+    b.close()
+
+
+def test_voices_are_retired_and_the_coda_ends():
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    w = walk_past()
+    e = engine(b, w, 50)
+    most, trail = 0, []
+    total = int(math.ceil((w.Until + 1.5) * RATE / 50))
+    for k in range(total):
+        y = e.Next()
+        most = max(most, e.ActiveVoices())
+        if k * 50 / RATE > w.Until + w.Stance.Duration + e.ImpulseLength() / RATE + 0.05:
+            trail.append(y)
+    assert 0 < most <= 12
+    assert e.ActiveVoices() == 0
+    assert trail and all(np.all(t == 0) for t in trail)
+    e.close()
+    b.close()
+
+
+def test_start_time_tracks_samples_not_calls():
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    e = engine(b, walk_past(), 32)
+    for i in range(20):
+        assert abs(e.StartTime() - i * 32 / RATE) <= 1e-12
+        e.Next()
+    assert abs(e.ChunkDuration() - 32 / RATE) <= 1e-12
+    assert e.ChunkSize() == 32 and e.SampleRate() == RATE
+    assert e.Propagation().startswith("bank: test, 101 ranges 2-12 m")
+    e.close()
+    b.close()
+
+
+def test_errors_carry_the_reference_text():
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    e = engine(b, walk_past(), 20)
+    with pytest.raises(fd.FdtdError) as err:
+        e.Next(np.zeros(7, dtype=np.float32))
+    assert str(err.value) == "sensing: output length 7, want 20"
+    import ctypes as C
+
+    lib = fd.load_library()
+    buf = np.zeros(7, dtype=np.float32)
+    assert lib.gf_engine_next(e._h, buf.ctypes.data_as(C.POINTER(C.c_float)), 7) != 0
+    assert lib.gf_last_error().decode() == "sensing: output length 7, want 20"
+    e.close()
+    # A contact on the receiver (admitted directly: warming it would first ask
+    # the bank for 0.02 m, outside this one).
+    quiet = se.Walk(Stance=wl.Stance(Mass=75), Speed=1.3, StartX=3, StartY=4, Until=0.1)
+    e = se.NewEngineWith(quiet, 20, se.FromBank(b), sample_rate=RATE, walk_duration=0.1, geometry_range=4,
+                         rayleigh_velocity=C_R, receivers=[(5.0, 0.0)])
+    f = np.ones(10)
+    rc = lib.gf_engine_admit(e._h, b"left_foot", 5.0, 0.0, f.ctypes.data_as(C.POINTER(C.c_double)), 10, 0.5)
+    assert rc != 0 and lib.gf_last_error().decode() == 'sensing: contact "left_foot" at the receiver\'s own position'
+    e.close()
+    # Off the bank's frequency grid.
+    with pytest.raises(fd.FdtdError) as err:
+        engine(b, walk_past(), 20, fs=1.5 * RATE)
+    assert str(err.value) == "sensing: 5.859375 Hz does not land on the bank's frequency grid (spacing 3.90625 Hz)"
+    # A walk beyond the bank's ranges: the first contact's quantized range.
+    far = walk_past(range_m=15.0)
+    c0 = far.ContactAt(0)
+    rq = SO.quantized_range(math.hypot(c0.X, c0.Y))
+    with pytest.raises(fd.FdtdError) as err:
+        engine(b, far, 20)
+    assert str(err.value) == f"bank: range {fd._go_g(rq)} m is outside the bank's 2 to 12 m"
+    b.close()
+
+
+def test_a_bank_drives_synthesis(oracle):
+    """bank_test.go:66-121: a bank of layered wavenumber integrals (here filled
+    on the device by fk.FillBank), a walk past at 4 m, a geophone: microvolts
+    to millivolts, and the streamed output equal to the oracle's Reference
+    on the same bank."""
+    medium = fk.Medium(Stack=[fd.Layer(0, 500, 200, 1700, 25)], DefaultQ=30)
+    limit = bk.RangeNyquist(0.87 * 200, 80) / 2
+    count = int(math.ceil((12 - 1) / limit)) + 1
+    h = fd.BankHeader(Solver="test", Medium=[fd.Layer(0, 500, 200, 1700, 25)], SampleRateHz=2000, Samples=1024,
+                      MinM=1, MaxM=12, Count=count,
+                      Component="vertical surface displacement per unit vertical surface point force")
+    bk.CheckRangeSampling(h, 80)
+    z = fk.FillBank(medium, h, 80)
+    b = bk.Bank(h, z)
+    data = np.stack([z.real, z.imag], axis=-1).astype(np.float32)
+    ob = oracle.OracleBank(1, 12, data)
+    w = walk_past()
+    e = engine(b, w, 20, fs=2000.0)
+    assert e.ImpulseLength() == 768
+    n = int(w.Until * 2000)
+    got = run(e, n)
+    peak = float(np.max(np.abs(got)))
+    print(f"[sensing] f-k bank of {count} ranges, walk past at 4 m: peak {peak:.3e} V, "
+          f"{e.Responses()} responses", flush=True)
+    assert 1e-6 <= peak <= 1e-1
+    assert_matches(got, oracle_reference(ob, 2000.0, w, n, 2000.0, 768))
+    e.close()
+    b.close()
