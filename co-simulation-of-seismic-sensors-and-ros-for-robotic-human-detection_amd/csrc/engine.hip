@@ -21,8 +21,9 @@
 //     y(t) = sum_{m} f[m] h[t - a - m]
 // at sample t, which is what the reference's convolver (started at the
 // admission chunk, zero state) produces.  k_engine_mix computes every
-// receiver's chunk in one launch: one workgroup per (receiver, 64 outputs),
-// lanes split over outputs and tap slices, a fixed-order sum of the slices.
+// receiver's chunk in one launch: one 1024-thread workgroup per (receiver, up
+// to 64 outputs), lanes split over outputs and tap slices, the slices added
+// in a fixed order.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -67,20 +68,27 @@ struct MixArgs {
   int64_t first[kMaxVoices];
 };
 
-// One workgroup per (receiver, np outputs); thread = (output t, slice q) with
-// S = 256 / np slices striding the taps.  The slices' partial sums are added
-// in slice order.
-__global__ __launch_bounds__(256) void k_engine_mix(MixArgs a, const double *__restrict__ force, int32_t cap,
-                                                    const double *const *__restrict__ irp, int32_t nrecv,
-                                                    int32_t irlen, int64_t emitted, int32_t chunk, int32_t np,
-                                                    float *__restrict__ out) {
-  __shared__ double part[256];
+// One workgroup of 1024 threads per (receiver, np outputs): thread = (output
+// t, slice q) with S = 1024 / np slices striding every active voice's taps in
+// turn; np is chosen on the host so that the grid fills the chip (64
+// receivers x 20 samples: 5 outputs a workgroup, 204 lanes an output).  The
+// chain is latency-bound (two loads per multiply-add, the responses in L2 or
+// the last-level cache), so lanes, not arithmetic, set its time.  The slices'
+// sums are added in a fixed order: groups of eight, then the groups.
+constexpr int kMixThreads = 1024;
+
+__global__ __launch_bounds__(kMixThreads) void k_engine_mix(MixArgs a, const double *__restrict__ force, int32_t cap,
+                                                            const double *const *__restrict__ irp, int32_t nrecv,
+                                                            int32_t irlen, int64_t emitted, int32_t chunk,
+                                                            int32_t np, float *__restrict__ out) {
+  __shared__ double red[kMixThreads];
+  __shared__ double grp[kMixThreads / 8 + 64];
   const int r = blockIdx.x;
   const int t = threadIdx.x % np, q = threadIdx.x / np;
-  const int S = 256 / np;
+  const int S = kMixThreads / np;
   const int i = blockIdx.y * np + t;
   double acc = 0;
-  if (i < chunk) {
+  if (i < chunk && q < S) {
     for (int v = 0; v < a.nv; v++) {
       const int s = a.slot[v];
       const int64_t T = emitted + i - a.first[v];
@@ -88,15 +96,22 @@ __global__ __launch_bounds__(256) void k_engine_mix(MixArgs a, const double *__r
       const int64_t hi = T < a.n[v] - 1 ? T : a.n[v] - 1;
       const double *f = force + (int64_t)s * cap;
       const double *h = irp[(int64_t)s * nrecv + r];
-#pragma unroll 4
+#pragma unroll 8
       for (int64_t m = lo + q; m <= hi; m += S) acc = fma(f[m], h[T - m], acc);
     }
   }
-  part[threadIdx.x] = acc;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  const int G = (S + 7) / 8;
+  if (q < G && i < chunk) {
+    double g = 0;
+    for (int k = q * 8; k < q * 8 + 8 && k < S; k++) g += red[k * np + t];
+    grp[q * np + t] = g;
+  }
   __syncthreads();
   if (q == 0 && i < chunk) {
     double sum = 0;
-    for (int k = 0; k < S; k++) sum += part[k * np + t];
+    for (int k = 0; k < G; k++) sum += grp[k * np + t];
     out[(int64_t)r * chunk + i] = (float)sum;
   }
 }
@@ -151,6 +166,10 @@ struct gf_engine {
   double *d_force = nullptr;        // [kMaxVoices][cap]
   const double **d_irp = nullptr;   // [kMaxVoices][nrecv]
   float *d_out = nullptr, *pin_out = nullptr;
+  // pinned staging per voice slot, so an admission's copies run
+  // asynchronously: a slot is reused only after a later Next has synchronised
+  double *pin_force = nullptr;       // [kMaxVoices][cap]
+  const double **pin_irp = nullptr;  // [kMaxVoices][nrecv]
   double last_ms = 0, total_ms = 0, build_ms = 0;
   int64_t builds = 0, macs = 0;
 };
@@ -167,6 +186,8 @@ void release(gf_engine *e) {
   hipFree(e->d_irp);
   hipFree(e->d_out);
   if (e->pin_out) hipHostFree(e->pin_out);
+  if (e->pin_force) hipHostFree(e->pin_force);
+  if (e->pin_irp) hipHostFree(e->pin_irp);
   if (e->e0) hipEventDestroy(e->e0);
   if (e->e1) hipEventDestroy(e->e1);
   if (e->st) hipStreamDestroy(e->st);
@@ -243,8 +264,12 @@ int gf_engine_create(const gf_bank_dev *bank, int32_t n_recv, const double *recv
   e->irlen = ir_len > 3 * n / 4 ? 3 * n / 4 : ir_len;
   e->nrecv = n_recv;
   e->cap = max_contact_samples;
-  e->np = 64;
-  while (e->np / 2 >= chunk && e->np > 1) e->np /= 2;
+  // Outputs per workgroup: few enough that the grid covers the 256 CUs
+  // (each output's taps then spread over up to 1024 lanes), at most 64.
+  {
+    const int64_t want = ((int64_t)chunk * n_recv + 255) / 256;
+    e->np = (int32_t)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)chunk, 64}));
+  }
   for (int32_t q = 0; q < n_recv; q++) {
     e->rx.push_back(recv_xy[2 * q]);
     e->ry.push_back(recv_xy[2 * q + 1]);
@@ -263,7 +288,11 @@ int gf_engine_create(const gf_bank_dev *bank, int32_t n_recv, const double *recv
       (he = hipMalloc(&e->d_out, sizeof(float) * (size_t)n_recv * chunk)) != hipSuccess)
     return bail(he, "hipMalloc");
   if ((he = hipHostMalloc((void **)&e->pin_out, sizeof(float) * (size_t)n_recv * chunk, hipHostMallocDefault)) !=
-      hipSuccess)
+          hipSuccess ||
+      (he = hipHostMalloc((void **)&e->pin_force, sizeof(double) * (size_t)kMaxVoices * e->cap,
+                          hipHostMallocDefault)) != hipSuccess ||
+      (he = hipHostMalloc((void **)&e->pin_irp, sizeof(double *) * (size_t)kMaxVoices * n_recv,
+                          hipHostMallocDefault)) != hipSuccess)
     return bail(he, "hipHostMalloc");
   if (sensor) {
     if ((he = hipMalloc(&e->d_sensor, sizeof(double2) * e->bins)) != hipSuccess) return bail(he, "hipMalloc");
@@ -300,13 +329,13 @@ int gf_engine_admit(gf_engine *e, const char *id, double x, double y, const doub
   const double xy[2] = {x, y};
   if (int rc = build_responses(e, xy, 1)) return rc;
   const int32_t slot = e->free_slots.back();
-  std::vector<const double *> ptrs((size_t)e->nrecv);
-  for (int32_t q = 0; q < e->nrecv; q++) ptrs[q] = e->cache[quantum(go_hypot(e->rx[q] - x, e->ry[q] - y))];
-  HIPCHK(hipMemcpyAsync(e->d_force + (size_t)slot * e->cap, force_z, sizeof(double) * n, hipMemcpyHostToDevice,
+  double *pf = e->pin_force + (size_t)slot * e->cap;
+  const double **pp = e->pin_irp + (size_t)slot * e->nrecv;
+  memcpy(pf, force_z, sizeof(double) * n);
+  for (int32_t q = 0; q < e->nrecv; q++) pp[q] = e->cache[quantum(go_hypot(e->rx[q] - x, e->ry[q] - y))];
+  HIPCHK(hipMemcpyAsync(e->d_force + (size_t)slot * e->cap, pf, sizeof(double) * n, hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipMemcpyAsync(e->d_irp + (size_t)slot * e->nrecv, pp, sizeof(double *) * e->nrecv, hipMemcpyHostToDevice,
                         e->st));
-  HIPCHK(hipMemcpyAsync(e->d_irp + (size_t)slot * e->nrecv, ptrs.data(), sizeof(double *) * e->nrecv,
-                        hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipStreamSynchronize(e->st));
   e->free_slots.pop_back();
   // The voice runs until its force has ended and the response it excited has
   // run out (sensing.go:323-328).
@@ -336,8 +365,8 @@ int gf_engine_next(gf_engine *e, float *out, int32_t len) {
   } else {
     HIPCHK(hipEventRecord(e->e0, e->st));
     dim3 grid(e->nrecv, (e->chunk + e->np - 1) / e->np);
-    k_engine_mix<<<grid, 256, 0, e->st>>>(a, e->d_force, e->cap, e->d_irp, e->nrecv, e->irlen, e->emitted, e->chunk,
-                                          e->np, e->d_out);
+    k_engine_mix<<<grid, kMixThreads, 0, e->st>>>(a, e->d_force, e->cap, e->d_irp, e->nrecv, e->irlen, e->emitted,
+                                                  e->chunk, e->np, e->d_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->e1, e->st));
     HIPCHK(hipMemcpyAsync(e->pin_out, e->d_out, sizeof(float) * nout, hipMemcpyDeviceToHost, e->st));
