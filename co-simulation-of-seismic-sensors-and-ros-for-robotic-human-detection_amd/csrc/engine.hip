@@ -74,7 +74,7 @@ struct MixArgs {
 // receivers x 20 samples: 5 outputs a workgroup, 204 lanes an output).  The
 // chain is latency-bound (two loads per multiply-add, the responses in L2 or
 // the last-level cache), so lanes, not arithmetic, set its time.  The slices'
-// sums are added in a fixed order: groups of eight, then the groups.
+// sums are added in a fixed order: groups of 32, then the groups.
 constexpr int kMixThreads = 1024;
 
 __global__ __launch_bounds__(kMixThreads) void k_engine_mix(MixArgs a, const double *__restrict__ force, int32_t cap,
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kMixThreads) void k_engine_mix(MixArgs a, const dou
                                                             int32_t irlen, int64_t emitted, int32_t chunk,
                                                             int32_t np, float *__restrict__ out) {
   __shared__ double red[kMixThreads];
-  __shared__ double grp[kMixThreads / 8 + 64];
+  __shared__ double grp[kMixThreads / 32 + 64];
   const int r = blockIdx.x;
   const int t = threadIdx.x % np, q = threadIdx.x / np;
   const int S = kMixThreads / np;
@@ -102,10 +102,10 @@ __global__ __launch_bounds__(kMixThreads) void k_engine_mix(MixArgs a, const dou
   }
   red[threadIdx.x] = acc;
   __syncthreads();
-  const int G = (S + 7) / 8;
+  const int G = (S + 31) / 32;
   if (q < G && i < chunk) {
     double g = 0;
-    for (int k = q * 8; k < q * 8 + 8 && k < S; k++) g += red[k * np + t];
+    for (int k = q * 32; k < q * 32 + 32 && k < S; k++) g += red[k * np + t];
     grp[q * np + t] = g;
   }
   __syncthreads();

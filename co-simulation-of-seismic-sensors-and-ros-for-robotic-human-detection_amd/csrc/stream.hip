@@ -147,37 +147,51 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
   }
 }
 
-// The latency form for one streaming chunk (n <= 64 outputs): a workgroup
-// per receiver, lanes split as np outputs (n rounded up to a power of two)
-// x S = 256 / np tap slices.  Slice q sums taps q, q + S, q + 2S, ... in order
-// straight from the ring (it sits in L2); the S slice sums are then added in
-// order.  The serial chain per lane is len / S taps instead of len.
+// The latency form for one streaming chunk (n <= 64 outputs): 1024-thread
+// workgroups, each taking np outputs of one receiver, lanes split as np
+// outputs x S = 1024 / np tap slices; np is chosen on the host so that the
+// grid covers the 256 CUs (64 receivers x 20 samples: 5 outputs a workgroup).
+// Slice q sums taps q, q + S, q + 2S, ... in order straight from the ring (it
+// sits in L2); the slice sums are then added in a fixed order (groups of 32,
+// then the groups).  The chain is latency-bound, so the serial length per
+// lane, len / S taps, sets the time.
 //
-// The workgroup first writes its receiver's new chunk into the ring (one
-// launch per chunk, not two); __syncthreads makes those writes visible to
-// the workgroup's own reads.
-__global__ __launch_bounds__(256) void k_fir_chunk(const double *__restrict__ h, int32_t len, double *ring, int64_t M,
-                                                   int64_t s0, const double *__restrict__ in, int32_t n, int32_t np,
-                                                   double *__restrict__ out) {
-  __shared__ double part[256];
+// Every workgroup first writes its receiver's new chunk into the ring (one
+// launch per chunk, not two; workgroups of one receiver write identical
+// values); __syncthreads makes those writes visible to the workgroup's own
+// reads, and no workgroup reads a new sample it has not itself written.
+__global__ __launch_bounds__(1024) void k_fir_chunk(const double *__restrict__ h, int32_t len, double *ring, int64_t M,
+                                                    int64_t s0, const double *__restrict__ in, int32_t n, int32_t np,
+                                                    double *__restrict__ out) {
+  __shared__ double part[1024];
+  __shared__ double grp[1024 / 32 + 64];
   const int r = blockIdx.x;
   const int t = threadIdx.x % np, q = threadIdx.x / np;
-  const int S = 256 / np;
+  const int S = 1024 / np;
+  const int i = blockIdx.y * np + t;
   const double *hr = h + (int64_t)r * len;
   double *xr = ring + (int64_t)r * M;
+  // every workgroup of the receiver writes the same chunk: identical values
   if (threadIdx.x < n) xr[(s0 + threadIdx.x) & (M - 1)] = in[(int64_t)r * n + threadIdx.x];
   __syncthreads();
   double acc = 0;
-  if (t < n) {
+  if (i < n && q < S) {
 #pragma unroll 8
-    for (int j = q; j < len; j += S) acc = fma(hr[j], xr[(s0 + t - j) & (M - 1)], acc);
+    for (int j = q; j < len; j += S) acc = fma(hr[j], xr[(s0 + i - j) & (M - 1)], acc);
   }
   part[threadIdx.x] = acc;
   __syncthreads();
-  if (q == 0 && t < n) {
-    double tot = part[t];
-    for (int k = 1; k < S; k++) tot += part[k * np + t];
-    out[(int64_t)r * n + t] = tot;
+  const int G = (S + 31) / 32;
+  if (q < G && i < n) {
+    double g = 0;
+    for (int k = q * 32; k < q * 32 + 32 && k < S; k++) g += part[k * np + t];
+    grp[q * np + t] = g;
+  }
+  __syncthreads();
+  if (q == 0 && i < n) {
+    double tot = 0;
+    for (int k = 0; k < G; k++) tot += grp[k * np + t];
+    out[(int64_t)r * n + i] = tot;
   }
 }
 
@@ -298,9 +312,11 @@ int gf_convolver_process(gf_convolver *c, const double *in, int32_t len_in, doub
   HIPCHK(hipMemcpyAsync(c->in, c->pin_in, bytes, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipEventRecord(c->e0, c->st));
   if (C <= 64) {
-    int np = 1;
-    while (np < C) np <<= 1;
-    k_fir_chunk<<<c->nrecv, 256, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->pos, c->in, (int32_t)C, np, c->out);
+    // outputs per workgroup: few enough that the grid covers the 256 CUs
+    const int64_t want = (C * c->nrecv + 255) / 256;
+    const int32_t np = (int32_t)std::max<int64_t>(1, std::min<int64_t>(want, C));
+    dim3 g(c->nrecv, (unsigned)((C + np - 1) / np));
+    k_fir_chunk<<<g, 1024, 0, c->st>>>(c->h, c->len, c->x, c->ring, c->pos, c->in, (int32_t)C, np, c->out);
   } else {
     dim3 gw((unsigned)((C + 255) / 256), c->nrecv);
     k_ring_write<<<gw, 256, 0, c->st>>>(c->x, c->ring, c->pos, c->in, C, C);
