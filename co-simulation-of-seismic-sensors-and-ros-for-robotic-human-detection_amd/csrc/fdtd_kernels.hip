@@ -123,9 +123,8 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
     velocity(g, rb, t, c, row_own && own, cc, cur.trr, trr1, cur.tqq, tqq1, cur.trz, trzm, trz_p, tzz_c,
              cur.tzz_dn, cur.vr, cur.vz, vrn, vzn);
     if (row_own && own) {
-      const size_t k = (size_t)t * g.pitch + c;
-      st_stream(&B[kVr][k], vrn);
-      st_stream(&B[kVz][k], vzn);
+      st_stream(at(B[kVr], g, t, c), vrn);
+      st_stream(at(B[kVz], g, t, c), vzn);
       if (do_max) mbits = max(mbits, max(abs_bits(vrn), abs_bits(vzn)));
     }
     const float vrn_m = lane_prev(vrn);
@@ -147,11 +146,10 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
       if (own) {
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
         stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
-        const size_t k = (size_t)s * g.pitch + c;
-        st_stream(&B[kTrr][k], nrr);
-        st_stream(&B[kTqq][k], nqq);
-        st_stream(&B[kTzz][k], nzz);
-        st_stream(&B[kTrz][k], nrz);
+        st_stream(at(B[kTrr], g, s, c), nrr);
+        st_stream(at(B[kTqq], g, s, c), nqq);
+        st_stream(at(B[kTzz], g, s, c), nzz);
+        st_stream(at(B[kTrz], g, s, c), nrz);
         store_mem<NM>(g, s, c, mem_s);
       }
     }

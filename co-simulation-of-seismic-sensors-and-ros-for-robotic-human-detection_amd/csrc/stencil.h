@@ -143,10 +143,24 @@ GF_HD ColConst load_col(const GridDesc &g, int c) {
   return cc;
 }
 
+// Address of element (row t, column c) of a plane of pitch g.pitch.  On the
+// device the row start (less one column) is wave-uniform and lives in
+// scalar registers, and the column is a 32-bit byte offset (c >= -1, so
+// (c + 1) * 4 is non-negative): the access takes the scalar-base +
+// vector-offset form and needs no 64-bit address arithmetic per lane.
+GF_HD GF_GLOBAL float *at(GF_GLOBAL float *base, const GridDesc &g, int t, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  GF_GLOBAL char *row = (GF_GLOBAL char *)(base + ((ptrdiff_t)t * g.pitch - 1));
+  return (GF_GLOBAL float *)(row + (uint32_t)(c + 1) * 4u);
+#else
+  return base + ((ptrdiff_t)t * g.pitch + c);
+#endif
+}
+
 // Field element at (row t, column c); rows -2..nz+1 and columns -1..pitch-1
 // are inside the padded allocation and read as zero outside the grid.
 GF_HD float ld(GF_GLOBAL const float *base, const GridDesc &g, int t, int c) {
-  return base[(ptrdiff_t)t * g.pitch + c];
+  return *at(const_cast<GF_GLOBAL float *>(base), g, t, c);
 }
 
 // tzz with the prescribed surface traction of step n at the axis cell: the
@@ -245,21 +259,21 @@ struct MemVars {
 template <int NM>
 GF_HD void load_mem(const GridDesc &g, int s, int c, MemVars<NM> &m) {
   if (NM == 0) return;
-  const size_t cell = (size_t)s * g.pitch + (size_t)c, ms = (size_t)g.mem_stride;
+  const size_t ms = (size_t)g.mem_stride;
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int q = 0; q < NM; q++) m.r[k][q] = g.mem[(size_t)(k * NM + q) * ms + cell];
+    for (int q = 0; q < NM; q++) m.r[k][q] = *at(g.mem + (size_t)(k * NM + q) * ms, g, s, c);
 }
 
 template <int NM>
 GF_HD void store_mem(const GridDesc &g, int s, int c, const MemVars<NM> &m) {
   if (NM == 0) return;
-  const size_t cell = (size_t)s * g.pitch + (size_t)c, ms = (size_t)g.mem_stride;
+  const size_t ms = (size_t)g.mem_stride;
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int q = 0; q < NM; q++) st_stream(&g.mem[(size_t)(k * NM + q) * ms + cell], m.r[k][q]);
+    for (int q = 0; q < NM; q++) st_stream(at(g.mem + (size_t)(k * NM + q) * ms, g, s, c), m.r[k][q]);
 }
 
 // New stresses at (s, c): Sim.stresses (fdtd.go:219-307) for one cell.
