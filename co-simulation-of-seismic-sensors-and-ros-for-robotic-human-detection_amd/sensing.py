@@ -205,6 +205,7 @@ class Engine:
         self.receivers = receivers
         self._single = receivers.shape[0] == 1
         self.scheduled = 0.0
+        self._records = {}  # (ID, Start, X, Y) -> (admission sample, force record)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -249,6 +250,25 @@ class Engine:
             return
         xy = np.ascontiguousarray([[c.X, c.Y] for c in cs], dtype=np.float64)
         fdtd._check(fdtd.load_library().gf_engine_warm(self._h, fdtd._dptr(xy), len(cs)))
+        # Each contact's force record too: its admission chunk is fixed by
+        # its start, so no chunk has to sample one.
+        for c in cs:
+            key = (c.ID, c.Start, c.X, c.Y)
+            a = self._admission(c)
+            if a is not None and a >= self._info()[0] and key not in self._records:
+                self._records[key] = (a, self._force(c, a))
+
+    def _admission(self, c: Contact) -> Optional[int]:
+        """The first sample of the chunk whose window [t0, t1) holds the
+        contact's start, with the window bounds Next computes."""
+        k0 = int(c.Start * self.fs) // self.chunk
+        for k in (k0 - 1, k0, k0 + 1):
+            if k < 0:
+                continue
+            t0 = float(k * self.chunk) / self.fs
+            if t0 <= c.Start < t0 + self.ChunkDuration():
+                return k * self.chunk
+        return None
 
     def _force(self, c: Contact, emitted: int) -> np.ndarray:
         """The contact's vertical force at the instants the reference's Next
@@ -278,7 +298,9 @@ class Engine:
             self._warm(self.scheduled, t1 + lookahead)
             self.scheduled = t1 + lookahead
         for c in self._schedule.Contacts(t0, t1):
-            f = self._force(c, emitted)
+            a, f = self._records.pop((c.ID, c.Start, c.X, c.Y), (None, None))
+            if a != emitted:
+                f = self._force(c, emitted)
             fdtd._check(lib.gf_engine_admit(self._h, c.ID.encode(), c.X, c.Y, fdtd._dptr(f), f.size, c.End()))
         buf = np.zeros((self.receivers.shape[0], self.chunk), dtype=np.float32)
         fdtd._check(lib.gf_engine_next(self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), self.chunk))
