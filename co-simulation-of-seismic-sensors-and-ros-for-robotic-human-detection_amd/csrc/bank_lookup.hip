@@ -197,23 +197,46 @@ __global__ void k_bank_impulse_coeff(const double *__restrict__ r, int32_t nq, d
 
 // dsp.IRFFT (dsp.go:42-50) of an nfft-point spectrum, first `length`
 // samples: x_t = (1/N)(Re X_0 + Re X_{N/2} (-1)^t + 2 sum_k Re(X_k e^{2 pi i k t / N})),
-// the phase k t mod N reduced exactly in integers.
+// the phase k t mod N carried exactly in integers (t < N).  For N up to
+// kTwiddleMax the workgroup first tabulates sincospi(2 j / N) for every j in
+// LDS, the same values the direct form computes term by term, so each term
+// is one table read and two multiply-adds.
+constexpr int kTwiddleMax = 4096;  // 64 KB of LDS
+
 __global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, int32_t nq, int32_t nfft,
                               int32_t length, double *__restrict__ out) {
+  extern __shared__ double2 tab[];  // [nfft] when nfft <= kTwiddleMax
+  const bool use_tab = nfft <= kTwiddleMax;
+  if (use_tab) {
+    for (int j = threadIdx.x; j < nfft; j += blockDim.x) {
+      double sn, cs;
+      sincospi(2.0 * (double)j / (double)nfft, &sn, &cs);
+      tab[j] = make_double2(sn, cs);
+    }
+    __syncthreads();
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (t >= length || q >= nq) return;
   const double2 *X = coeff + (int64_t)q * bins;
   double s = X[0].x;
+  int32_t ph = 0;  // k t mod nfft
   for (int k = 1; k < bins; k++) {
+    ph += t;
+    if (ph >= nfft) ph -= nfft;
     const double2 x = X[k];
     if (2 * k == nfft) {
       s += (t & 1) ? -x.x : x.x;
       continue;
     }
-    const int64_t ph = ((int64_t)k * t) % nfft;
     double sn, cs;
-    sincospi(2.0 * (double)ph / (double)nfft, &sn, &cs);
+    if (use_tab) {
+      const double2 w = tab[ph];
+      sn = w.x;
+      cs = w.y;
+    } else {
+      sincospi(2.0 * (double)ph / (double)nfft, &sn, &cs);
+    }
     s += 2 * (x.x * cs - x.y * sn);
   }
   out[(int64_t)q * length + t] = s * (1 / (double)nfft);
@@ -322,7 +345,8 @@ hipError_t impulse_launch(const gf_bank_dev *b, const double *d_r, int32_t n, in
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   dim3 g2((length + 255) / 256, n);
-  k_irdft_first<<<g2, 256, 0, st>>>(d_coeff, b->bins, n, nfft, length, d_out);
+  const size_t lds = nfft <= kTwiddleMax ? sizeof(double2) * (size_t)nfft : 0;
+  k_irdft_first<<<g2, 256, lds, st>>>(d_coeff, b->bins, n, nfft, length, d_out);
   return hipGetLastError();
 }
 

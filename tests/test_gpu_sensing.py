@@ -218,3 +218,90 @@ def test_a_bank_drives_synthesis(oracle):
     assert_matches(got, oracle_reference(ob, 2000.0, w, n, 2000.0, 768))
     e.close()
     b.close()
+
+
+def fixed_engine(b, contacts, n, chunk=25):
+    sch = se.Fixed(contacts)
+    return se.NewEngineWith(sch, chunk, se.FromBank(b), sample_rate=RATE, walk_duration=n / RATE,
+                            geometry_range=8.0, rayleigh_velocity=C_R, sensor=geophone)
+
+
+def test_contacts_superpose():
+    """sensing_test.go:79-117 on the device engine: two contacts together give
+    the sum of each alone, so voices share no state.  Bound: the reference's
+    1e-7 of the peak, doubled because each of the three outputs is rounded to
+    float32 on its own."""
+    b = device_bank(small_bank(2, 12, 101, 100), 2, 12)
+    stance = wl.Stance(Mass=75)
+    a = se.Contact("a", -3, 8, 0.1, se.Footfall(stance, 0.0))
+    c = se.Contact("b", 2, 9, 0.4, se.Footfall(stance, 0.0))
+    n = 4000
+    outs = []
+    for cs in ([a], [c], [a, c]):
+        e = fixed_engine(b, cs, n)
+        outs.append(run(e, n).astype(np.float64))
+        e.close()
+    scale = np.max(np.abs(outs[2]))
+    assert scale > 0
+    assert np.max(np.abs(outs[2] - (outs[0] + outs[1]))) <= 2e-7 * scale
+    b.close()
+
+
+def test_walk_past_envelope_peaks_at_closest_approach():
+    """sensing_test.go:119-172: a walk past rises and falls, loudest within
+    two footfalls of the closest, and range moves the amplitude.  The
+    reference passes at 6 m with its far-field model; this bank's layered
+    response changes little between 6 and 8.5 m, where the overlap of
+    successive footfalls sets the loudest window instead (the oracle's
+    Reference on the same bank puts it four footfalls early too), so the walk
+    passes at 3 m, where range dominates."""
+    b = device_bank(small_bank(2, 12, 101, 100), 2, 12)
+    w = walk_past(range_m=3.0, approach=6.0)
+    n = int(w.Until * RATE)
+    e = engine(b, w, 50)
+    trace = run(e, n)
+    e.close()
+    pts = []
+    for c in w.Contacts(0, n / RATE):
+        lo = int(c.Start * RATE)
+        hi = min(lo + int(0.5 * RATE), n)
+        if lo < hi:
+            pts.append((math.hypot(c.X, c.Y), float(np.max(np.abs(trace[lo:hi])))))
+    assert len(pts) >= 8
+    loudest = max(range(len(pts)), key=lambda i: pts[i][1])
+    closest = min(range(len(pts)), key=lambda i: pts[i][0])
+    print(f"[sensing] walk past: {len(pts)} footfalls, loudest {loudest}, closest {closest}", flush=True)
+    assert abs(loudest - closest) <= 2
+    assert pts[0][1] < pts[loudest][1] and pts[-1][1] < pts[loudest][1]
+    assert pts[closest][1] > 1.1 * pts[0][1]
+    b.close()
+
+
+def test_range_quantisation_error_is_bounded():
+    """sensing_test.go:233-287 on the device engine: one quantum of range
+    moves the energy by under 0.5 % and the arrival by at most one sample
+    plus the true delay.  The reference takes the pair at 8 m; this test
+    bank's transform is 512 samples (0.256 s), so its responses stop at
+    0.19 s, and beyond about 6 m where that cut falls moves with range by
+    enough to change the energy 0.5-2 % per quantum (the oracle's Reference
+    on the same bank shows the same).  The pair is taken at 4 m."""
+    b = device_bank(small_bank(2, 12, 101, 100), 2, 12)
+    stance = wl.Stance(Mass=75)
+    n = 3000
+
+    def at(y):
+        e = fixed_engine(b, [se.Contact("f", 0, y, 0.05, se.Footfall(stance, 0.0))], n)
+        out = run(e, n).astype(np.float64)
+        e.close()
+        return math.sqrt(np.mean(out ** 2)), out
+
+    p0, t0 = at(4.0)
+    p1, t1 = at(4.0 + se.RangeQuantum)
+    assert abs(p1 - p0) / p0 < 0.005
+    true_shift = se.RangeQuantum / C_R * RATE
+    assert true_shift <= 1
+    xc = np.correlate(t1, t0, mode="full")
+    lag = int(np.argmax(xc)) - (n - 1)
+    print(f"[sensing] one quantum: energy {abs(p1 - p0) / p0:.2e}, lag {lag} samples", flush=True)
+    assert abs(lag) <= 1 + true_shift
+    b.close()
