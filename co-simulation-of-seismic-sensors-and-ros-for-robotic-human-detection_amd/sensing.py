@@ -9,6 +9,7 @@ consumes (internal/source/source.go), over the C ABI.
     NewEngineWith(...)                   sensing.go:117-170
     Engine.Next, ChunkSize, SampleRate, ChunkDuration, StartTime,
     ActiveVoices, Propagation            sensing.go:224-330
+    Reference(...)                       sensing.go:344-386
     RangeQuantum                         sensing.go:49
 
 The engine's state (impulse responses per range quantum, each voice's force
@@ -29,7 +30,7 @@ from typing import Callable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from . import fdtd
+from . import fdtd, stream
 from .workloads import Stance, _DUTY
 
 RangeQuantum = 0.02
@@ -315,6 +316,69 @@ class Engine:
                                                         C.byref(build), C.byref(built)))
         return {"mix_ms": mix.value, "mix_ms_total": tot.value, "macs": macs.value, "build_ms": build.value,
                 "built": built.value}
+
+
+def _go_hypot(p: float, q: float) -> float:
+    """math.Hypot as Go computes it (the range every quantum is taken from)."""
+    p, q = abs(p), abs(q)
+    if math.isinf(p) or math.isinf(q):
+        return math.inf
+    if p < q:
+        p, q = q, p
+    if p == 0:
+        return 0.0
+    q = q / p
+    return p * math.sqrt(1 + q * q)
+
+
+def _quantized_range(r: float) -> float:
+    """responseFor's range (sensing.go:186-190): r to the nearest
+    RangeQuantum, halves away from zero (math.Round), at least one quantum."""
+    x = r / RangeQuantum
+    k = math.floor(x)
+    if x - k >= 0.5:
+        k += 1
+    return max(float(k) * RangeQuantum, RangeQuantum)
+
+
+def Reference(schedule, n: int, prop: FromBank, *, sample_rate: float, walk_duration: float, geometry_range: float,
+              rayleigh_velocity: float, sensor: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+              receiver: Sequence[float] = (0.0, 0.0)) -> np.ndarray:
+    """sensing.Reference (sensing.go:344-386) on the device: the same signal an
+    engine produces, in one piece.  For every contact beginning in [0, n/fs),
+    its vertical force at i/fs convolved with the impulse response at its
+    quantized range over the whole record (the responses built in one batch,
+    every contact's convolution in one launch), summed in contact order;
+    float64 [n]."""
+    fs = sample_rate
+    if prop is None:
+        raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a propagation model is required")
+    rx, ry = float(receiver[0]), float(receiver[1])
+    transform = prop.TransformSize()
+    ir_len = ImpulseLength(schedule, walk_duration, geometry_range, rayleigh_velocity, fs, [(rx, ry)])
+    ir_len = min(ir_len, 3 * transform // 4)
+    out = np.zeros(n)
+    contacts = schedule.Contacts(0, float(n) / fs)
+    if not contacts or n <= 0:
+        return out
+    rq = [_quantized_range(_go_hypot(rx - c.X, ry - c.Y)) for c in contacts]
+    uniq = list(dict.fromkeys(rq))  # first use first, as the Go loop meets them
+    sens = None
+    if sensor is not None:
+        h = prop.bank.header
+        sens = np.asarray(sensor(np.arange(h.Bins) * fs / h.Samples), dtype=np.complex128)
+    irs = prop.bank.ImpulseResponses(uniq, ir_len, fs, sens)
+    h_rows = np.stack([irs[uniq.index(r)] for r in rq])
+    t = np.arange(n) / fs
+    x = np.stack([np.asarray(c.ForceAt(t)[2], dtype=np.float64) for c in contacts])
+    conv = stream.NewConvolver(h_rows, min(n, 1024))
+    try:
+        y = conv.ProcessAll(x)
+    finally:
+        conv.close()
+    for row in y:
+        out += row
+    return out
 
 
 def ImpulseLength(schedule, walk_duration: float, geometry_range: float, rayleigh_velocity: float, fs: float,

@@ -305,3 +305,28 @@ def test_range_quantisation_error_is_bounded():
     print(f"[sensing] one quantum: energy {abs(p1 - p0) / p0:.2e}, lag {lag} samples", flush=True)
     assert abs(lag) <= 1 + true_shift
     b.close()
+
+
+def test_reference_on_the_device(oracle):
+    """sensing.Reference (sensing.go:344-386) built from device pieces (the
+    bank's impulse responses, one whole-record convolution per contact)
+    against the oracle's restatement, within 1e-12 of the peak (both FP64,
+    direct sums against numpy's); and the streaming engine against it within
+    the reference's 1e-7."""
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    ob = oracle.OracleBank(2, 12, data)
+    w = walk_past()
+    n = 6000
+    kw = dict(sample_rate=RATE, walk_duration=w.Until, geometry_range=w.StartY, rayleigh_velocity=C_R,
+              sensor=geophone)
+    got = se.Reference(w, n, se.FromBank(b), **kw)
+    want = oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) <= 1e-12 * scale
+    e = engine(b, w, 25)
+    assert_matches(run(e, n), got)
+    e.close()
+    off = se.Reference(w, n, se.FromBank(b), receiver=(1.5, 0.3), **kw)
+    assert np.max(np.abs(off - oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4, 1.5, 0.3))) <= 1e-12 * scale
+    b.close()
