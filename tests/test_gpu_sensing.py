@@ -330,3 +330,25 @@ def test_reference_on_the_device(oracle):
     off = se.Reference(w, n, se.FromBank(b), receiver=(1.5, 0.3), **kw)
     assert np.max(np.abs(off - oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4, 1.5, 0.3))) <= 1e-12 * scale
     b.close()
+
+
+def test_mix_kernel_at_every_workgroup_shape(oracle):
+    """k_engine_mix gives each 1,024-thread workgroup np outputs of one
+    receiver, np = clamp(ceil(chunk x receivers / 256), 1, 64): a line of 64
+    receivers at 20 samples (np = 5, the benchmark's shape), 16 receivers at
+    300 samples (np = 19, ragged last block) and 64 at 256 (np = 64); sampled
+    receivers against the oracle's Reference."""
+    data = small_bank(2, 12, 101, 100)
+    b = device_bank(data, 2, 12)
+    ob = oracle.OracleBank(2, 12, data)
+    w = walk_past()
+    n = 3000
+    for R, chunk in ((64, 20), (16, 300), (64, 256)):
+        rec = [(-3.0 + 6.0 * q / (R - 1), -0.5 + 0.01 * q) for q in range(R)]
+        print(f"[sensing] {R} receivers x chunk {chunk}", flush=True)
+        e = engine(b, w, chunk, receivers=rec)
+        got = run(e, n)
+        e.close()
+        for q in range(0, R, max(1, R // 6)):
+            assert_matches(got[q], oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4, *rec[q]))
+    b.close()

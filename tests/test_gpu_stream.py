@@ -94,3 +94,20 @@ def test_process_chunk_by_chunk_at_both_kernel_forms(chunk):
         assert close(got[0], want), (chunk, k)
         assert close(got[1], -0.5 * want), (chunk, k)
     c.close()
+
+
+def test_chunk_kernel_at_every_workgroup_shape():
+    """The latency kernel gives each 1,024-thread workgroup np outputs of one
+    receiver, np = clamp(ceil(chunk x receivers / 256), 1, chunk): 300
+    receivers x 64 samples take np = 64 (16 slices per output), 40 x 48 take
+    np = 8, 3 x 7 take np = 1 (1,024 slices); all must follow the oracle."""
+    rng = np.random.default_rng(9)
+    for R, chunk, L in ((300, 64, 700), (40, 48, 1500), (3, 7, 400)):
+        print(f"[stream] {R} receivers x chunk {chunk}", flush=True)
+        h = np.stack([decaying(L, 50 + r % 97, 0.02 + 0.001 * (r % 31), r) for r in range(R)])
+        x = rng.standard_normal((R, 4 * chunk))
+        c = st.NewConvolver(h, chunk)
+        got = np.concatenate([c.Process(x[:, k * chunk:(k + 1) * chunk]) for k in range(4)], axis=1)
+        for r in range(0, R, max(1, R // 12)):
+            assert close(got[r], S.Monolithic(x[r], h[r])), (R, chunk, r)
+        c.close()
