@@ -17,17 +17,19 @@
 //
 //   k_ring_write  one thread per (receiver, sample): the newest min(n, M)
 //                 samples into the ring (older ones would be overwritten).
-//   k_fir_chunk   a streaming chunk of at most 64 samples: one workgroup per
-//                 receiver, its lanes split over outputs and tap slices, so
-//                 the serial chain is a fraction of the response.
-//   k_fir<RING>   one workgroup per (receiver, 1024 outputs): taps in tiles of
-//                 256 staged in LDS with the 1279 input samples they touch,
-//                 each thread accumulating four consecutive outputs over the
-//                 tile's taps in order from a sliding register window.  RING reads the ring (Process); otherwise a linear
-//                 buffer of the last len - 1 samples followed by the new input
+//   k_fir_chunk   a streaming chunk of at most 64 samples: 1024-thread
+//                 workgroups of a few outputs of one receiver, lanes split
+//                 over outputs and tap slices, so the serial chain is a small
+//                 fraction of the response.
+//   k_fir<RING>   one workgroup per (receiver, 2048 outputs): taps in tiles of
+//                 256 staged in LDS with the 2303 input samples they touch,
+//                 each thread accumulating eight consecutive outputs over the
+//                 tile's taps in order from a sliding register window.
+//                 RING reads the ring (Process); otherwise a linear buffer of
+//                 the last len - 1 samples followed by the new input
 //                 (ProcessAll over a whole record in one launch).
 // Work: len multiply-adds per output sample, FP64; for long records the
-// LDS-staged form runs at the FP64 FMA rate.
+// LDS-staged form runs at about half the FP64 FMA peak.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -87,12 +89,14 @@ __global__ void k_ring_write(double *__restrict__ ring, int64_t M, int64_t s0, c
 // src[r][s mod M]; else x_r(s) = src[r][s - s0 + len - 1] (a linear buffer
 // that starts len - 1 samples before s0), with row stride ld.
 //
-// A workgroup makes kOut = 4 x 256 consecutive outputs; thread u owns outputs
-// 4u..4u+3.  Per tile of kT taps the inputs they meet (kOut + kT - 1) are
-// staged in LDS; the thread keeps a sliding window of four inputs in
+// A workgroup makes kOut = 8 x 256 consecutive outputs; thread u owns outputs
+// 8u..8u+7.  Per tile of kT taps the inputs they meet (kOut + kT - 1) are
+// staged in LDS; the thread keeps a sliding window of eight inputs in
 // registers, so each tap costs one broadcast tap read, one input read and
-// four multiply-adds.  Each output still sums its taps in order.
-constexpr int kR = 4;
+// eight multiply-adds, with no test in the tap loop (it unrolls into 16-byte
+// LDS reads and runs of multiply-adds).  Each output still sums its taps in
+// order.
+constexpr int kR = 8;
 constexpr int kOut = kR * kT;
 
 template <bool RING>
@@ -100,13 +104,17 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
                                             int64_t ld, int64_t M, int64_t s0, int64_t n,
                                             double *__restrict__ out, int64_t ldo) {
   __shared__ double sh[kT];
-  __shared__ double sx[kOut + kT - 1];
+  // sx[1 + m] holds input m; sx[0] is a spare slot, so the window's last
+  // slide (whose value is never used) reads inside the array without a test.
+  __shared__ double sx[1 + kOut + kT - 1];
   const int r = blockIdx.y;
   const int u = threadIdx.x;
   const int64_t t0 = (int64_t)blockIdx.x * kOut;
   const double *hr = h + (int64_t)r * len;
   const double *xr = src + (int64_t)r * ld;
-  double acc[kR] = {0, 0, 0, 0};
+  double acc[kR];
+#pragma unroll
+  for (int i = 0; i < kR; i++) acc[i] = 0;
   for (int j0 = 0; j0 < len; j0 += kT) {
     // Taps j0..j0+kT-1 and the inputs they meet: s = s0 + t0 - j0 - (kT-1) + m.
     const int jt = min(kT, len - j0);
@@ -120,7 +128,7 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
         const int64_t k = s - s0 + len - 1;  // >= 0 for every tap in range
         v = k >= 0 && k < ld ? xr[k] : 0.0;
       }
-      sx[m] = v;
+      sx[1 + m] = v;
     }
     __syncthreads();
     // x(s0 + t0 + t - j) for j = j0 + q sits at sx[t - q + kT - 1]; for the
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
     const int b = kR * u + kT - 1;
     double w[kR];
 #pragma unroll
-    for (int i = 0; i < kR; i++) w[i] = sx[b + i];
+    for (int i = 0; i < kR; i++) w[i] = sx[1 + b + i];
     for (int q = 0; q < jt; q++) {
       const double hq = sh[q];
 #pragma unroll
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(kT) void k_fir(const double *__restrict__ h, int32_
       // Slide the window one tap: w[i] <- sx[b + i - (q + 1)].
 #pragma unroll
       for (int i = kR - 1; i > 0; i--) w[i] = w[i - 1];
-      w[0] = q + 1 < jt ? sx[b - q - 1] : 0.0;
+      w[0] = sx[b - q];
     }
     __syncthreads();
   }
