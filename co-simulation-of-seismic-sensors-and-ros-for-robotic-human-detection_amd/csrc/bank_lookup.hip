@@ -292,14 +292,13 @@ __global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, i
   const double2 *X = coeff + (int64_t)q * bins;
   double s = X[0].x;
   int32_t ph = 0;  // k t mod nfft
-  for (int k = 1; k < bins; k++) {
+  // Bins 1..N/2-1 in order, then the Nyquist bin N/2 (the last, bins - 1),
+  // so the loop carries no test but its bound.
+  const int kn = bins - 1;
+  for (int k = 1; k < kn; k++) {
     ph += t;
     if (ph >= nfft) ph -= nfft;
     const double2 x = X[k];
-    if (2 * k == nfft) {
-      s += (t & 1) ? -x.x : x.x;
-      continue;
-    }
     double sn, cs;
     if (use_tab) {
       const double2 w = tab[ph];
@@ -310,6 +309,7 @@ __global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, i
     }
     s += 2 * (x.x * cs - x.y * sn);
   }
+  if (kn >= 1) s += (t & 1) ? -X[kn].x : X[kn].x;
   out[(int64_t)q * length + t] = s * (1 / (double)nfft);
 }
 
