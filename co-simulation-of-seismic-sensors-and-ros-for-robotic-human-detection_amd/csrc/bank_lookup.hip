@@ -33,6 +33,7 @@
 
 #include "../../include/geofdtd.h"
 #include "model.h"
+#include "fastmath.h"
 
 namespace gfs {
 int hip_fail(hipError_t e, const char *where);
@@ -109,77 +110,6 @@ __global__ void k_bank_prepare(int32_t count, int32_t bins, const float2 *__rest
   }
 }
 
-// e^x and (sin x, cos x) for the interpolated log-magnitude and phase, by
-// Cody and Waite's argument reduction (Software Manual for the Elementary
-// Functions, 1980) with fused multiply-adds, then Taylor series: x = n ln 2 +
-// r with |r| <= ln2 / 2 and e^r to r^13 (remainder below 5e-18 relative);
-// x = q pi/2 + r with pi/2 in three parts, |r| <= pi/4, sin r to r^17 and
-// cos r to r^18 (remainders below 1e-19), the quadrant q mod 4 choosing
-// signs and roles.  Accurate to a few ulp for the magnitudes (|x| below
-// about 1e5) a bank's tables hold, at a fraction of the general routines'
-// cost; they are host-callable so a CPU check can compare them with libm.
-constexpr double kInvLn2 = 1.4426950408889634, kLn2Hi = 0.6931471805599453, kLn2Lo = 2.3190468138462996e-17;
-constexpr double kTwoOverPi = 0.6366197723675814, kPio2A = 1.5707963267948966, kPio2B = 6.123233995736766e-17,
-                 kPio2C = -1.4973849048591698e-33;
-
-__host__ __device__ __forceinline__ double exp_reduced(double x) {
-  if (x < -746.0) return 0.0;
-  const double n = rint(x * kInvLn2);
-  double r = fma(-n, kLn2Hi, x);
-  r = fma(-n, kLn2Lo, r);
-  // 1/k! for k = 13 down to 0
-  double p = 1.6059043836821613e-10;
-  p = fma(p, r, 2.08767569878681e-09);
-  p = fma(p, r, 2.505210838544172e-08);
-  p = fma(p, r, 2.755731922398589e-07);
-  p = fma(p, r, 2.7557319223985893e-06);
-  p = fma(p, r, 2.48015873015873e-05);
-  p = fma(p, r, 0.0001984126984126984);
-  p = fma(p, r, 0.001388888888888889);
-  p = fma(p, r, 0.008333333333333333);
-  p = fma(p, r, 0.041666666666666664);
-  p = fma(p, r, 0.16666666666666666);
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  return ldexp(p, (int)n);
-}
-
-__host__ __device__ __forceinline__ void sincos_reduced(double x, double *sn, double *cs) {
-  const double q = rint(x * kTwoOverPi);
-  double r = fma(-q, kPio2A, x);
-  r = fma(-q, kPio2B, r);
-  r = fma(-q, kPio2C, r);
-  const double r2 = r * r;
-  // sin r = r + r^3 (-1/3! + r^2/5! - ... + r^14/17!)
-  double ps = 2.8114572543455206e-15;
-  ps = fma(ps, r2, -7.647163731819816e-13);
-  ps = fma(ps, r2, 1.6059043836821613e-10);
-  ps = fma(ps, r2, -2.505210838544172e-08);
-  ps = fma(ps, r2, 2.7557319223985893e-06);
-  ps = fma(ps, r2, -0.0001984126984126984);
-  ps = fma(ps, r2, 0.008333333333333333);
-  ps = fma(ps, r2, -0.16666666666666666);
-  const double sr = fma(r * r2, ps, r);
-  // cos r = 1 + r^2 (-1/2! + r^2/4! - ... - r^16/18!)
-  double pc = -1.5619206968586225e-16;
-  pc = fma(pc, r2, 4.779477332387385e-14);
-  pc = fma(pc, r2, -1.1470745597729725e-11);
-  pc = fma(pc, r2, 2.08767569878681e-09);
-  pc = fma(pc, r2, -2.755731922398589e-07);
-  pc = fma(pc, r2, 2.48015873015873e-05);
-  pc = fma(pc, r2, -0.001388888888888889);
-  pc = fma(pc, r2, 0.041666666666666664);
-  pc = fma(pc, r2, -0.5);
-  const double cr = fma(r2, pc, 1.0);
-  switch ((int)((long long)q & 3)) {
-    case 0: *sn = sr; *cs = cr; break;
-    case 1: *sn = cr; *cs = -sr; break;
-    case 2: *sn = -sr; *cs = -cr; break;
-    default: *sn = -cr; *cs = sr; break;
-  }
-}
-
 // Bank.bracket (format.go:295-309) and the interpolation of Bank.Response
 // (interp.go:38-53) for a range already checked to lie in the bank.
 __device__ __forceinline__ double2 lookup(double r, int bin, double r_min, double r_max, int32_t count, int32_t bins,
@@ -204,9 +134,9 @@ __device__ __forceinline__ double2 lookup(double r, int bin, double r_min, doubl
   const double p0 = phase[i0], p1 = phase[i1];
   const double lm = l0 + frac * (l1 - l0);
   const double ph = p0 + frac * (p1 - p0);
-  const double m = exp_reduced(lm);
+  const double m = gff::exp_reduced(lm);
   double s, c;
-  sincos_reduced(ph, &s, &c);
+  gff::sincos_reduced(ph, &s, &c);
   return make_double2(m * c, m * s);
 }
 
