@@ -95,7 +95,10 @@ __global__ __launch_bounds__(kMixThreads) void k_engine_mix(MixArgs a, const dou
       const int64_t lo = T - irlen + 1 > 0 ? T - irlen + 1 : 0;
       const int64_t hi = T < a.n[v] - 1 ? T : a.n[v] - 1;
       const double *f = force + (int64_t)s * cap;
-      const double *h = irp[(int64_t)s * nrecv + r];
+      // the response table holds device addresses: read them as global
+      // memory, not flat
+      const __attribute__((address_space(1))) double *h =
+          (const __attribute__((address_space(1))) double *)irp[(int64_t)s * nrecv + r];
 #pragma unroll 8
       for (int64_t m = lo + q; m <= hi; m += S) acc = fma(f[m], h[T - m], acc);
     }
