@@ -351,6 +351,18 @@ hipError_t impulse_launch(const gf_bank_dev *b, const double *d_r, int32_t n, in
   return hipGetLastError();
 }
 
+// The inverse transform on its own, for spectra built elsewhere (the
+// analytic propagation of the sensing engine): coeff [n][bins] -> out
+// [n][length], nfft = 2 (bins - 1).
+hipError_t irdft_launch(const double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out,
+                        hipStream_t st) {
+  const int32_t nfft = 2 * (bins - 1);
+  dim3 g2((length + 255) / 256, n);
+  const size_t lds = nfft <= kTwiddleMax ? sizeof(double2) * (size_t)nfft : 0;
+  k_irdft_first<<<g2, 256, lds, st>>>(d_coeff, bins, n, nfft, length, d_out);
+  return hipGetLastError();
+}
+
 }  // namespace gfbank
 
 extern "C" {

@@ -193,6 +193,13 @@ class GfSurStats(C.Structure):
     ]
 
 
+class GfHalfspace(C.Structure):
+    """gf_halfspace (soil.HalfSpace)."""
+
+    _fields_ = [("vp_mps", C.c_double), ("vs_mps", C.c_double), ("density_kgm3", C.c_double), ("qs", C.c_double),
+                ("qp", C.c_double)]
+
+
 class GfMedium(C.Structure):
     _fields_ = [
         ("stack", C.POINTER(GfLayer)),
@@ -314,6 +321,11 @@ SIGNATURES = {
     "gf_engine_next": (_I, [C.c_void_p, C.POINTER(C.c_float), _I32]),
     "gf_engine_info": (_I, [C.c_void_p, _PI64, _PI32, _PI32, _PI32]),
     "gf_engine_stats": (_I, [C.c_void_p, _PD, _PD, _PI64, _PD, _PI64]),
+    "gf_rayleigh_velocity": (_I, [C.POINTER(GfHalfspace), _PD]),
+    "gf_engine_create_analytic": (_I, [C.POINTER(GfHalfspace), _D, _I32, _PD, _I32, _D, _I32, _PD, _I32,
+                                       C.POINTER(C.c_void_p)]),
+    "gf_engine_admit_xyz": (_I, [C.c_void_p, C.c_char_p, _D, _D, _PD, _PD, _PD, _I32, _D]),
+    "gf_engine_transform": (_I, [C.c_void_p, _PI32]),
     "gf_bank_check_range_sampling": (_I, [C.POINTER(GfBankHeader), _D]),
 }
 

@@ -6,6 +6,9 @@ consumes (internal/source/source.go), over the C ABI.
     Walk, Footfall, Contact, RangeSpan   source.go:39-263 (host logic: which
                                          contacts begin when, and where)
     FromBank(bank)                       propagation.go:79-84
+    HalfSpace, HalfSpaceGF, Analytic     soil.go:25-163, green.go:120-130,
+                                         propagation.go:38-62
+    NewEngine(...)                       sensing.go:110-113 (analytic)
     NewEngineWith(...)                   sensing.go:117-170
     Engine.Next, ChunkSize, SampleRate, ChunkDuration, StartTime,
     ActiveVoices, Propagation            sensing.go:224-330
@@ -191,6 +194,59 @@ class FromBank:
                 f"{h.Component}")
 
 
+@dataclass
+class HalfSpace:
+    """soil.HalfSpace (soil.go:25-40)."""
+
+    Vp: float
+    Vs: float
+    Density: float
+    Qs: float
+    Qp: float = 0.0
+
+    def to_c(self) -> fdtd.GfHalfspace:
+        return fdtd.GfHalfspace(self.Vp, self.Vs, self.Density, self.Qs, self.Qp)
+
+    def RayleighVelocity(self) -> float:
+        """soil.go:136-163, on the library's bisection; Validate's errors."""
+        out = C.c_double()
+        h = self.to_c()
+        fdtd._check(fdtd.load_library().gf_rayleigh_velocity(C.byref(h), C.byref(out)))
+        return out.value
+
+
+def Loam() -> HalfSpace:
+    """soil.Loam (soil.go:50-52)."""
+    return HalfSpace(500, 200, 1700, 25)
+
+
+@dataclass
+class HalfSpaceGF:
+    """green.HalfSpaceGF (green.go:120-130): RefFreq 0 means 30 Hz."""
+
+    Soil: HalfSpace
+    RefFreq: float = 0.0
+
+
+class Analytic:
+    """sensing.Analytic (propagation.go:38-62): the homogeneous half-space in
+    far-field form, with its radial-force response; the engine builds its
+    responses on the device."""
+
+    def __init__(self, gf: HalfSpaceGF):
+        self.gf = gf
+
+    def TransformSize(self) -> int:
+        return 0
+
+    def Describe(self) -> str:
+        h = self.gf.Soil
+        vp, vs = h.Vp, h.Vs
+        nu = (vp * vp - 2 * vs * vs) / (2 * (vp * vp - vs * vs))
+        return (f"homogeneous half-space, far-field Rayleigh (Vp={vp:.0f} Vs={vs:.0f} rho={h.Density:.0f} "
+                f"Qs={h.Qs:.0f} nu={nu:.3f})")
+
+
 # ------------------------------------------------------------------- engine
 
 
@@ -206,7 +262,8 @@ class Engine:
         self.receivers = receivers
         self._single = receivers.shape[0] == 1
         self.scheduled = 0.0
-        self._records = {}  # (ID, Start, X, Y) -> (admission sample, force record)
+        self._records = {}  # (ID, Start, X, Y) -> (admission sample, force record(s))
+        self._xyz = isinstance(prop, Analytic)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -271,16 +328,20 @@ class Engine:
                 return k * self.chunk
         return None
 
-    def _force(self, c: Contact, emitted: int) -> np.ndarray:
-        """The contact's vertical force at the instants the reference's Next
-        evaluates it (t0 + i / fs per chunk, t0 = emitted / fs), from this
-        chunk until the force has surely ended."""
+    def _force(self, c: Contact, emitted: int):
+        """The contact's force at the instants the reference's Next evaluates
+        it (t0 + i / fs per chunk, t0 = emitted / fs), from this chunk until
+        the force has surely ended: the vertical component, or (fx, fy, fz)
+        for an engine that also carries the radial-force response."""
         last = int(math.ceil(c.End() * self.fs)) + 2
         n = max(last - emitted, 1)
         k = np.arange(n)
         starts = emitted + (k // self.chunk) * self.chunk
         t = starts / self.fs + (k % self.chunk) / self.fs
-        return np.ascontiguousarray(c.ForceAt(t)[2], dtype=np.float64)
+        f = c.ForceAt(t)
+        if self._xyz:
+            return tuple(np.ascontiguousarray(np.broadcast_to(v, t.shape), dtype=np.float64) for v in f)
+        return np.ascontiguousarray(f[2], dtype=np.float64)
 
     def Next(self, out: Optional[np.ndarray] = None) -> np.ndarray:
         """One chunk of output (volts, float32): [chunk] for one receiver,
@@ -302,7 +363,13 @@ class Engine:
             a, f = self._records.pop((c.ID, c.Start, c.X, c.Y), (None, None))
             if a != emitted:
                 f = self._force(c, emitted)
-            fdtd._check(lib.gf_engine_admit(self._h, c.ID.encode(), c.X, c.Y, fdtd._dptr(f), f.size, c.End()))
+            if self._xyz:
+                fx, fy, fz = f
+                fdtd._check(lib.gf_engine_admit_xyz(self._h, c.ID.encode(), c.X, c.Y, fdtd._dptr(fx), fdtd._dptr(fy),
+                                                    fdtd._dptr(fz), fz.size, c.End()))
+            else:
+                fdtd._check(lib.gf_engine_admit(self._h, c.ID.encode(), c.X, c.Y, fdtd._dptr(f), f.size,
+                                                c.End()))
         buf = np.zeros((self.receivers.shape[0], self.chunk), dtype=np.float32)
         fdtd._check(lib.gf_engine_next(self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), self.chunk))
         res = buf[0] if self._single else buf
@@ -414,10 +481,16 @@ def NewEngineWith(schedule, chunk: int, prop: Optional[FromBank], *, sample_rate
     rec = np.ascontiguousarray(np.atleast_2d(np.asarray(receivers, dtype=np.float64)))
     ir_len = ImpulseLength(schedule, walk_duration, geometry_range, rayleigh_velocity, sample_rate,
                            [tuple(r) for r in rec])
-    h = prop.bank.header
+    if isinstance(prop, Analytic):
+        n = 1
+        while n < 4 * ir_len:  # dsp.NextPow2(4 length), the transform the analytic model leaves to the caller
+            n <<= 1
+        bins, samples = n // 2 + 1, n
+    else:
+        bins, samples = prop.bank.header.Bins, prop.bank.header.Samples
     sens = None
     if sensor is not None:
-        f = np.arange(h.Bins) * sample_rate / h.Samples
+        f = np.arange(bins) * sample_rate / samples
         g = np.asarray(sensor(f), dtype=np.complex128)
         sens = np.ascontiguousarray(np.stack([g.real, g.imag], axis=-1))
     # The longest force record one voice holds: admitted up to a chunk before
@@ -425,9 +498,17 @@ def NewEngineWith(schedule, chunk: int, prop: Optional[FromBank], *, sample_rate
     longest = max([c.Profile.Duration() for c in schedule.Contacts(0, walk_duration + 2)] or [0.0])
     cap = int(math.ceil(longest * sample_rate)) + chunk + 8
     handle = C.c_void_p()
-    fdtd._check(fdtd.load_library().gf_engine_create(prop.bank._h, rec.shape[0], fdtd._dptr(rec), int(chunk),
-                                                     sample_rate, ir_len, fdtd._dptr(sens) if sens is not None
-                                                     else None, cap, C.byref(handle)))
+    lib = fdtd.load_library()
+    if isinstance(prop, Analytic):
+        hs = prop.gf.Soil.to_c()
+        fdtd._check(lib.gf_engine_create_analytic(C.byref(hs), prop.gf.RefFreq, rec.shape[0], fdtd._dptr(rec),
+                                                  int(chunk), sample_rate, ir_len,
+                                                  fdtd._dptr(sens) if sens is not None else None, cap,
+                                                  C.byref(handle)))
+    else:
+        fdtd._check(lib.gf_engine_create(prop.bank._h, rec.shape[0], fdtd._dptr(rec), int(chunk), sample_rate,
+                                         ir_len, fdtd._dptr(sens) if sens is not None else None, cap,
+                                         C.byref(handle)))
     e = Engine(handle, prop, schedule, int(chunk), sample_rate, rec)
     try:
         e._warm(0, walk_duration)
@@ -435,3 +516,14 @@ def NewEngineWith(schedule, chunk: int, prop: Optional[FromBank], *, sample_rate
         e.close()
         raise
     return e
+
+
+def NewEngine(schedule, chunk: int, soil: HalfSpace, *, sample_rate: float, walk_duration: float,
+              geometry_range: float, sensor: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+              receivers: Sequence[Sequence[float]] = ((0.0, 0.0),)) -> Engine:
+    """sensing.NewEngine (sensing.go:110-113): the analytic half-space
+    propagation of the resolved soil."""
+    prop = Analytic(HalfSpaceGF(soil))
+    return NewEngineWith(schedule, chunk, prop, sample_rate=sample_rate, walk_duration=walk_duration,
+                         geometry_range=geometry_range, rayleigh_velocity=soil.RayleighVelocity(), sensor=sensor,
+                         receivers=receivers)

@@ -412,6 +412,42 @@ int gf_engine_admit(gf_engine *e, const char *id, double x, double y, const doub
  * ("sensing: output length %d, want %d"). */
 int gf_engine_next(gf_engine *e, float *out, int32_t len);
 
+/* soil.HalfSpace (internal/soil/soil.go:25-40): a homogeneous half-space.
+ * qp 0 means 9/4 qs; it is validated but not otherwise read. */
+typedef struct gf_halfspace {
+  double vp_mps;
+  double vs_mps;
+  double density_kgm3;
+  double qs;
+  double qp;
+} gf_halfspace;
+
+/* soil.HalfSpace.RayleighVelocity (soil.go:136-163): the Rayleigh root by
+ * bisection, after Validate (soil.go:72-89, the same texts). */
+int gf_rayleigh_velocity(const gf_halfspace *h, double *out_mps);
+
+/* NewEngine (sensing.go:110-113): the engine over sensing.Analytic
+ * (propagation.go:38-62), green.HalfSpaceGF{Soil: *h, RefFreq: ref_freq_hz}
+ * (0: 30 Hz), whose responses carry the radial-force component too
+ * (green.go:375-388), so each voice convolves both its vertical force and its
+ * force along the contact-to-receiver line.  The responses are built on the
+ * device at the transform size CausalImpulseResponseAt picks when the
+ * propagation names none, nfft = NextPow2(4 ir_len) (dsp.go:240-242), with no
+ * cap on ir_len; sensor (may be NULL) at that transform's bins,
+ * [nfft/2+1][2].  Voices are admitted with gf_engine_admit_xyz. */
+int gf_engine_create_analytic(const gf_halfspace *h, double ref_freq_hz, int32_t n_recv, const double *recv_xy,
+                              int32_t chunk, double fs_hz, int32_t ir_len, const double *sensor,
+                              int32_t max_contact_samples, gf_engine **out);
+
+/* Engine.admit for one contact of an analytic engine: the contact's force
+ * (fx, fy, fz)[m] at sample emitted + m, m < n. */
+int gf_engine_admit_xyz(gf_engine *e, const char *id, double x, double y, const double *fx, const double *fy,
+                        const double *fz, int32_t n, double end_s);
+
+/* The transform size of the engine's responses (the bank's samples, or the
+ * analytic engine's nfft). */
+int gf_engine_transform(const gf_engine *e, int32_t *nfft);
+
 /* Samples emitted (StartTime x fs), ActiveVoices, responses cached, and the
  * impulse-response length in use. */
 int gf_engine_info(const gf_engine *e, int64_t *emitted, int32_t *active_voices, int32_t *responses,

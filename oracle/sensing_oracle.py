@@ -10,6 +10,12 @@ sum that the reference's own tests hold the streaming engine to.
     reference                                     sensing.go:344-386, with
         FromBank's impulse responses (oracle.bank_impulse) and
         stream.Monolithic (stream_oracle.Monolithic)
+    rayleigh_velocity                             soil.go:136-163
+    eigenfunctions, energy_integral               green.go:58-117
+    velocity_response, radial_force_response      green.go:169-266, 375-388
+    causal_impulse_response                       dsp.go:232-260
+    reference_xyz                                 sensing.go:344-386 with the
+        radial-force response (Analytic propagation)
 
 Forces come in as callables (the contact's vertical force at absolute time
 t), as the reference's Profile interface hands them over; the bank's radial
@@ -22,6 +28,7 @@ tests/test_sensing_oracle.py against the reference's source tests
 
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass
 from typing import Callable, List, Optional
@@ -137,4 +144,136 @@ def reference(contacts: List[OContact], forces: List[Callable[[np.ndarray], np.n
         if rq not in cache:
             cache[rq] = impulse(rq)
         out += stream_oracle.Monolithic(force(t), cache[rq])
+    return out
+
+
+# ---- the analytic propagation: soil.HalfSpace and green.HalfSpaceGF ------
+
+
+@dataclass
+class OHalfSpace:
+    vp: float
+    vs: float
+    rho: float
+    qs: float
+    qp: float = 0.0
+
+
+def rayleigh_velocity(h: OHalfSpace) -> float:
+    """soil.HalfSpace.RayleighVelocity (soil.go:136-163): bisection on
+    (2 - x)^2 - 4 sqrt(1 - x) sqrt(1 - x k^2), x = (c / Vs)^2."""
+    return _rayleigh(h.vp, h.vs)
+
+
+@functools.lru_cache(maxsize=None)
+def _rayleigh(vp: float, vs: float) -> float:
+    k = vs / vp
+    k2 = k * k
+
+    def f(x):
+        return (2 - x) * (2 - x) - 4 * math.sqrt(1 - x) * math.sqrt(1 - x * k2)
+
+    lo, hi = 1e-9, 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid == lo or mid == hi:
+            break
+        if f(mid) < 0:
+            lo = mid
+        else:
+            hi = mid
+    return math.sqrt(0.5 * (lo + hi)) * vs
+
+
+def eigenfunctions(h: OHalfSpace, omega: float):
+    """NewEigenfunctions (green.go:58-80): (k, q, s, xq, xs, zq, zs)."""
+    c = rayleigh_velocity(h)
+    k = omega / c
+    q = k * math.sqrt(1 - (c * c) / (h.vp * h.vp))
+    s = k * math.sqrt(1 - (c * c) / (h.vs * h.vs))
+    k2, s2 = k * k, s * s
+    return k, q, s, k, -2 * k * q * s / (k2 + s2), -q, 2 * k2 * q / (k2 + s2)
+
+
+def energy_integral(e, rho: float) -> float:
+    k, q, s, xq, xs, zq, zs = e
+
+    def sq(a, b):
+        return a * a / (2 * q) + 2 * a * b / (q + s) + b * b / (2 * s)
+
+    return 0.5 * rho * (sq(xq, xs) + sq(zq, zs))
+
+
+def velocity_response(h: OHalfSpace, r: float, f: float, ref: float = 30.0) -> complex:
+    """HalfSpaceGF.VelocityResponse (green.go:218-266) with ComplexWavenumber
+    and PhaseVelocity (green.go:169-198)."""
+    if f <= 0:
+        return 0j
+    cr = rayleigh_velocity(h)
+    gam = math.atan(1 / h.qs) / math.pi
+    c0 = cr * math.cos(math.pi * gam / 2)
+    omega = 2 * math.pi * f
+    kc = (omega / c0 * (f / ref) ** (-gam)) * complex(math.cos(-math.pi * gam / 2), math.sin(-math.pi * gam / 2))
+    c = cr * (f / ref) ** gam
+    e = eigenfunctions(h, omega)
+    i1 = energy_integral(e, h.rho)
+    uz = e[5] + e[6]
+    amp = uz * uz / (8 * c * c * i1) * math.sqrt(2 / (math.pi * kc.real * r))
+    prop = np.exp(complex(0, -math.pi / 4) - 1j * kc * r)
+    return complex(0, omega) * amp * prop
+
+
+def radial_force_response(h: OHalfSpace, r: float, f: float, ref: float = 30.0) -> complex:
+    """HalfSpaceGF.RadialForceResponse (green.go:375-388): i (H(0)/V(0)) times
+    the vertical response."""
+    v = velocity_response(h, r, f, ref)
+    if v == 0:
+        return 0j
+    e = eigenfunctions(h, 2 * math.pi * f)
+    return 1j * ((e[3] + e[4]) / (e[5] + e[6])) * v
+
+
+def causal_impulse_response(n: int, length: int, fs: float, response) -> np.ndarray:
+    """dsp.CausalImpulseResponseAt (dsp.go:232-260): n = 0 picks
+    NextPow2(4 length); taper above 0.8 of Nyquist; numpy's inverse real
+    transform in place of dsp.IRFFT; the first `length` samples."""
+    if n == 0:
+        n = 1
+        while n < 4 * length:
+            n <<= 1
+    nyq = fs / 2
+    frm = 0.8 * nyq
+    coeff = np.zeros(n // 2 + 1, dtype=complex)
+    for k in range(n // 2 + 1):
+        f = k * fs / n
+        v = response(f)
+        if f > frm:
+            v = v * (0.5 * (1 + math.cos(math.pi * (f - frm) / (nyq - frm))))
+        coeff[k] = v
+    return np.fft.irfft(coeff, n)[:length]
+
+
+def reference_xyz(contacts: List[OContact], forces, n: int, fs: float, impulse, rx: float = 0.0,
+                  ry: float = 0.0) -> np.ndarray:
+    """sensing.Reference (sensing.go:353-386) with both components: forces[i]
+    maps times to (fx, fy, fz); impulse(rq) returns (vertical, radial)
+    responses; the radial one is driven by the force along the contact-to-
+    receiver line."""
+    import stream_oracle
+
+    span = float(n) / fs
+    out = np.zeros(n)
+    t = np.arange(n) / fs
+    cache = {}
+    for c, force in zip(contacts, forces):
+        if not (0 <= c.start < span):
+            continue
+        dx, dy = rx - c.x, ry - c.y
+        r = math.hypot(dx, dy)
+        rq = quantized_range(r)
+        if rq not in cache:
+            cache[rq] = impulse(rq)
+        hz, hr = cache[rq]
+        fx, fy, fz = force(t)
+        out += stream_oracle.Monolithic(fz, hz) + stream_oracle.Monolithic(fx * (dx / r) + fy * (dy / r), hr)
     return out

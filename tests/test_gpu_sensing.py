@@ -352,3 +352,49 @@ def test_mix_kernel_at_every_workgroup_shape(oracle):
         for q in range(0, R, max(1, R // 6)):
             assert_matches(got[q], oracle_reference(ob, RATE, w, n, RATE, 3 * SAMPLES // 4, *rec[q]))
     b.close()
+
+
+def test_analytic_engine_matches_the_reference():
+    """sensing.NewEngine's default propagation (sensing.Analytic over
+    green.HalfSpaceGF of loam, propagation.go:38-62) on the device: both
+    response components built there at the transform dsp picks
+    (NextPow2(4 irLen)), each voice driven by its vertical force and by its
+    force along the contact-to-receiver line (sensing.go:264-270), against
+    the oracle's Reference with both components, at the reference's 1e-7 of
+    the peak and within one float32 step; and the radial part is there (the
+    same run without it differs)."""
+    soil = se.Loam()
+    oh = SO.OHalfSpace(500, 200, 1700, 25)
+    fs = 1000.0
+    w = walk_past(range_m=4.0, approach=2.0)
+    n = 4000
+    for rec in ([(0.0, 0.0)], [(0.0, 0.0), (2.0, 1.0)]):
+        e = se.NewEngine(w, 25, soil, sample_rate=fs, walk_duration=w.Until, geometry_range=4.0, sensor=geophone,
+                         receivers=rec)
+        ir_len = e.ImpulseLength()
+        nfft = 1
+        while nfft < 4 * ir_len:
+            nfft <<= 1
+        import ctypes as C
+
+        t_n = C.c_int32()
+        fd._check(fd.load_library().gf_engine_transform(e._h, C.byref(t_n)))
+        assert t_n.value == nfft
+        got = run(e, n)
+        e.close()
+        got = np.atleast_2d(got)
+        contacts = SO.walk_contacts(oracle_walk(w), 0, n / fs)
+        foot = se.Footfall(w.Stance, w.Heading)
+        forces = [lambda t, c=c: foot.ForceAt(t - c.start) for c in contacts]
+
+        def pair(rq):
+            v = SO.causal_impulse_response(0, ir_len, fs, lambda f: SO.velocity_response(oh, rq, f) * geophone(f))
+            r = SO.causal_impulse_response(0, ir_len, fs, lambda f: SO.radial_force_response(oh, rq, f) * geophone(f))
+            return v, r
+
+        for q, (rx, ry) in enumerate(rec):
+            want = SO.reference_xyz(contacts, forces, n, fs, pair, rx, ry)
+            rel = assert_matches(got[q], want)
+            print(f"[sensing] analytic, receiver {q}: {rel:.2e} of the peak", flush=True)
+            vert_only = SO.reference_xyz(contacts, forces, n, fs, lambda rq: (pair(rq)[0], 0 * pair(rq)[1]), rx, ry)
+            assert np.max(np.abs(vert_only - want)) > 1e-3 * np.max(np.abs(want))

@@ -15,7 +15,13 @@ footfall forces, the argument checks) against the oracle.  CPU only.
     contacts outside the span are ignored, and moving the receiver equals
     moving every contact the other way;
   * the heading rotates the shear but not the weight, force is zero outside
-    contact (source_test.go:90-139), and Validate's errors (:215-237).
+    contact (source_test.go:90-139), and Validate's errors (:215-237);
+  * the analytic propagation: the Poisson solid's Rayleigh velocity in closed
+    form, the library's root equal to the oracle's, free-surface tractions
+    vanishing (green_test.go:21-61), the Poisson solid's ellipticity 0.68127
+    at every frequency and scale (:63-84), the radial-force response the
+    ellipticity times a quarter cycle (:503-551), spreading as r^-1/2 with the
+    constant-Q loss (:109-194), soil validation texts (soil.go:72-89).
 """
 
 import math
@@ -249,3 +255,90 @@ def test_engine_arguments_are_checked_before_the_device():
     _, far, _ = SO.range_span(SO.walk_contacts(walker(), 0, 10.0), 0, 0)
     assert se.ImpulseLength(w, 10.0, 10.0, 180.0, 1000.0) == SO.impulse_length(far, 180.0, 1000.0, 0)
     assert se.ImpulseLength(se.Fixed([]), 10.0, 7.5, 180.0, 1000.0) == SO.impulse_length(7.5, 180.0, 1000.0, 0)
+
+
+# ---- the analytic propagation (green.HalfSpaceGF over soil.HalfSpace) -----
+
+SOILS = {"Poisson solid": SO.OHalfSpace(math.sqrt(3) * 200, 200, 1900, 1e9),
+         "dry sand": SO.OHalfSpace(350, 150, 1600, 20), "loam": SO.OHalfSpace(500, 200, 1700, 25),
+         "weathered rock": SO.OHalfSpace(1800, 800, 2300, 80)}
+
+
+def test_rayleigh_velocity_of_a_poisson_solid():
+    """The closed form c_R = Vs sqrt(2 - 2/sqrt 3) (soil.go:62-66's fixture),
+    and the library's bisection equal to the oracle's."""
+    h = SOILS["Poisson solid"]
+    assert abs(SO.rayleigh_velocity(h) / 200 - math.sqrt(2 - 2 / math.sqrt(3))) <= 1e-12
+    for name, o in SOILS.items():
+        got = se.HalfSpace(o.vp, o.vs, o.rho, o.qs).RayleighVelocity()
+        assert got == SO.rayleigh_velocity(o), name
+
+
+def test_free_surface_tractions_vanish():
+    """green_test.go:21-61: the mode shapes satisfy the free-surface conditions."""
+    for name, h in SOILS.items():
+        mu, lam = h.rho * h.vs ** 2, h.rho * (h.vp ** 2 - 2 * h.vs ** 2)
+        for f in (5, 20, 100):
+            k, q, s, xq, xs, zq, zs = SO.eigenfunctions(h, 2 * math.pi * f)
+            xp, zp = -q * xq - s * xs, -q * zq - s * zs
+            v0, h0 = zq + zs, xq + xs
+            assert abs(xp + k * v0) / (abs(xp) + abs(k * v0)) <= 1e-12, (name, f)
+            normal = lam * (-k * h0 + zp) + 2 * mu * zp
+            assert abs(normal) / (abs(lam * k * h0) + abs(2 * mu * zp)) <= 1e-12, (name, f)
+
+
+def test_ellipticity_of_a_poisson_solid():
+    """green_test.go:63-84: 0.68127 at every frequency and medium scale."""
+    h = SOILS["Poisson solid"]
+    for f in (1, 20, 500):
+        e = SO.eigenfunctions(h, 2 * math.pi * f)
+        assert abs(abs((e[3] + e[4]) / (e[5] + e[6])) - 0.68127) <= 1e-4
+    big = SO.OHalfSpace(math.sqrt(3) * 2000, 2000, 2600, 1e9)
+    e1, e2 = SO.eigenfunctions(h, 2 * math.pi * 20), SO.eigenfunctions(big, 2 * math.pi * 20)
+    assert abs(abs((e1[3] + e1[4]) / (e1[5] + e1[6])) - abs((e2[3] + e2[4]) / (e2[5] + e2[6]))) <= 1e-12
+
+
+def test_radial_force_response_is_ellipticity_times_a_quarter_cycle():
+    """green_test.go:503-551."""
+    import cmath
+
+    for name in ("Poisson solid", "dry sand", "loam"):
+        h = SOILS[name]
+        for f in (5, 20, 100):
+            v, r = SO.velocity_response(h, 10, f), SO.radial_force_response(h, 10, f)
+            e = SO.eigenfunctions(h, 2 * math.pi * f)
+            assert abs(abs(r) / abs(v) - abs((e[3] + e[4]) / (e[5] + e[6]))) <= 1e-12
+            d = (cmath.phase(r) - cmath.phase(v)) * 180 / math.pi
+            d = (d + 180) % 360 - 180
+            assert abs(abs(d) - 90) <= 1e-9
+    h = SOILS["Poisson solid"]
+    assert abs(abs(SO.radial_force_response(h, 10, 20)) / abs(SO.velocity_response(h, 10, 20)) - 0.68127) <= 1e-4
+
+
+def test_geometric_spreading_and_attenuation():
+    """green_test.go:109-194: |G| sqrt(r) e^{-Im k r} is range-independent,
+    with Im k = -|k| sin(pi g / 2), g = atan(1/Q) / pi (Kjartansson)."""
+    h = SOILS["loam"]
+    f = 30.0
+    v = [SO.velocity_response(h, r, f) for r in (5.0, 10.0, 20.0, 40.0)]
+    cr = SO.rayleigh_velocity(h)
+    g = math.atan(1 / h.qs) / math.pi
+    mag = 2 * math.pi * f / (cr * math.cos(math.pi * g / 2))  # |k| at the reference frequency
+    kim = -mag * math.sin(math.pi * g / 2)
+    norm = [abs(x) * math.sqrt(r) * math.exp(-kim * r) for x, r in zip(v, (5.0, 10.0, 20.0, 40.0))]
+    assert max(norm) / min(norm) - 1 <= 1e-12
+
+
+def test_soil_validation_texts():
+    """soil.go:72-89, through the library."""
+    cases = [(se.HalfSpace(500, 0, 1700, 25), "soil: Vs must be positive, got 0 m/s"),
+             (se.HalfSpace(0, 200, 1700, 25), "soil: Vp must be positive, got 0 m/s"),
+             (se.HalfSpace(500, 200, 0, 25), "soil: density must be positive, got 0 kg/m^3"),
+             (se.HalfSpace(250, 200, 1700, 25),
+              "soil: Vp=250 m/s must exceed sqrt(2)*Vs=282.842712474619 m/s for a non-negative Poisson ratio"),
+             (se.HalfSpace(500, 200, 1700, 0), "soil: Qs must be positive, got 0"),
+             (se.HalfSpace(500, 200, 1700, 25, -1), "soil: Qp must not be negative, got -1")]
+    for h, message in cases:
+        with pytest.raises(fd.FdtdError) as e:
+            h.RayleighVelocity()
+        assert str(e.value) == message
