@@ -398,3 +398,52 @@ def test_analytic_engine_matches_the_reference():
             print(f"[sensing] analytic, receiver {q}: {rel:.2e} of the peak", flush=True)
             vert_only = SO.reference_xyz(contacts, forces, n, fs, lambda rq: (pair(rq)[0], 0 * pair(rq)[1]), rx, ry)
             assert np.max(np.abs(vert_only - want)) > 1e-3 * np.max(np.abs(want))
+
+
+def analytic_fixed(contacts, n, fs=1000.0, chunk=50):
+    sch = se.Fixed(contacts)
+    return se.NewEngine(sch, chunk, se.Loam(), sample_rate=fs, walk_duration=n / fs, geometry_range=8.0,
+                        sensor=geophone)
+
+
+def test_analytic_walk_past_envelope_peaks_at_closest_approach():
+    """sensing_test.go:119-172 as the reference runs it: the analytic
+    propagation, a walk past at 6 m, 1 kHz, chunks of 50."""
+    w = walk_past(range_m=6.0, approach=6.0)
+    fs = 1000.0
+    n = int(w.Until * fs)
+    e = se.NewEngine(w, 50, se.Loam(), sample_rate=fs, walk_duration=w.Until, geometry_range=6.0, sensor=geophone)
+    trace = run(e, n)
+    e.close()
+    pts = []
+    for c in w.Contacts(0, n / fs):
+        lo = int(c.Start * fs)
+        hi = min(lo + int(0.5 * fs), n)
+        if lo < hi:
+            pts.append((math.hypot(c.X, c.Y), float(np.max(np.abs(trace[lo:hi])))))
+    assert len(pts) >= 8
+    loudest = max(range(len(pts)), key=lambda i: pts[i][1])
+    closest = min(range(len(pts)), key=lambda i: pts[i][0])
+    assert abs(loudest - closest) <= 2
+    assert pts[0][1] < pts[loudest][1] and pts[-1][1] < pts[loudest][1]
+    assert pts[closest][1] > 1.1 * pts[0][1]
+
+
+def test_shear_nulls_abeam_and_contributes_in_line():
+    """sensing_test.go:419-463 on the device engine: one footfall heading +x
+    at 8 m; abeam the shear leaves the energy unchanged (it projects to zero
+    on the line of sight), in line it moves it by 0.1-5 %."""
+    n = 4000
+
+    def energy(x, y, ap):
+        st = wl.Stance(Mass=75, APPeak=ap)
+        e = analytic_fixed([se.Contact("f", x, y, 0.05, se.Footfall(st, 0.0))], n)
+        out = run(e, n).astype(np.float64)
+        e.close()
+        return float(np.sum(out * out))
+
+    abeam_off, abeam_on = energy(0, 8, -1), energy(0, 8, 0.35)
+    assert abs(abeam_on - abeam_off) / abeam_off <= 1e-6
+    inline_off, inline_on = energy(-8, 0, -1), energy(-8, 0, 0.35)
+    rel = abs(inline_on - inline_off) / inline_off
+    assert 1e-3 <= rel <= 0.05, rel
