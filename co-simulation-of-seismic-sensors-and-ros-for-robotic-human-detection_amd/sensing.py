@@ -420,6 +420,8 @@ def Reference(schedule, n: int, prop: FromBank, *, sample_rate: float, walk_dura
     fs = sample_rate
     if prop is None:
         raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a propagation model is required")
+    if not isinstance(prop, FromBank):
+        raise fdtd.FdtdError(fdtd.GF_ERR_ARGUMENT, "geofdtd: Reference on the device takes a bank's propagation")
     rx, ry = float(receiver[0]), float(receiver[1])
     transform = prop.TransformSize()
     ir_len = ImpulseLength(schedule, walk_duration, geometry_range, rayleigh_velocity, fs, [(rx, ry)])
