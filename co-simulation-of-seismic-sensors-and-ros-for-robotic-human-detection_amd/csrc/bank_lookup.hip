@@ -23,6 +23,7 @@
 // zero, cmplx.Rect as r cos + i r sin; log, atan2, exp and sincos are the
 // device library's.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -243,6 +244,26 @@ __global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, i
   out[(int64_t)q * length + t] = s * (1 / (double)nfft);
 }
 
+// Inverse transforms longer than kTwiddleMax go through rocFFT (hipFFT's
+// double-precision complex-to-real transform, O(N log N)) instead of the
+// direct sum: the imaginary parts of bins 0 and N/2 are cleared first (the
+// direct sum and dsp.IRFFT ignore them), and the first `length` samples of
+// each row are kept, scaled by 1/N.
+__global__ void k_real_ends(double2 *__restrict__ coeff, int32_t bins, int32_t n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  coeff[(int64_t)j * bins].y = 0;
+  coeff[(int64_t)j * bins + bins - 1].y = 0;
+}
+
+__global__ void k_take_first(const double *__restrict__ full, int32_t nfft, int32_t length, int32_t n, double scale,
+                             double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (t >= length || j >= n) return;
+  out[(int64_t)j * length + t] = full[(int64_t)j * nfft + t] * scale;
+}
+
 int fail(int code, const std::string &m) { return gfs::api_set_err(code, m); }
 int ok() { return gfs::api_set_err(GF_OK, ""); }
 
@@ -299,6 +320,8 @@ void release(gf_bank_dev *b) {
 
 namespace gfbank {
 
+hipError_t irdft_launch(double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out, hipStream_t st);
+
 int32_t bins(const gf_bank_dev *b) { return b->bins; }
 double rate(const gf_bank_dev *b) { return b->rate; }
 
@@ -345,22 +368,54 @@ hipError_t impulse_launch(const gf_bank_dev *b, const double *d_r, int32_t n, in
                                            d_sensor, fs_hz, nfft, bank_df, d_coeff);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  dim3 g2((length + 255) / 256, n);
-  const size_t lds = nfft <= kTwiddleMax ? sizeof(double2) * (size_t)nfft : 0;
-  k_irdft_first<<<g2, 256, lds, st>>>(d_coeff, b->bins, n, nfft, length, d_out);
-  return hipGetLastError();
+  return irdft_launch(d_coeff, b->bins, n, length, d_out, st);
 }
 
 // The inverse transform on its own, for spectra built elsewhere (the
 // analytic propagation of the sensing engine): coeff [n][bins] -> out
-// [n][length], nfft = 2 (bins - 1).
-hipError_t irdft_launch(const double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out,
-                        hipStream_t st) {
+// [n][length], nfft = 2 (bins - 1).  coeff is scratch (the transform may
+// overwrite it).  Up to kTwiddleMax points, the direct sum with its LDS
+// table; beyond, rocFFT in batches of at most 256 MB of output.
+hipError_t irdft_launch(double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out, hipStream_t st) {
   const int32_t nfft = 2 * (bins - 1);
-  dim3 g2((length + 255) / 256, n);
-  const size_t lds = nfft <= kTwiddleMax ? sizeof(double2) * (size_t)nfft : 0;
-  k_irdft_first<<<g2, 256, lds, st>>>(d_coeff, bins, n, nfft, length, d_out);
-  return hipGetLastError();
+  if (nfft <= kTwiddleMax) {
+    dim3 g2((length + 255) / 256, n);
+    const size_t lds = sizeof(double2) * (size_t)nfft;
+    k_irdft_first<<<g2, 256, lds, st>>>(d_coeff, bins, n, nfft, length, d_out);
+    return hipGetLastError();
+  }
+  hipError_t e;
+  k_real_ends<<<(n + 255) / 256, 256, 0, st>>>(d_coeff, bins, n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int32_t batch = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n, (256LL << 20) / ((int64_t)nfft * 8)));
+  double *full = nullptr;
+  if ((e = hipMalloc(&full, sizeof(double) * (size_t)batch * nfft)) != hipSuccess) return e;
+  hipfftHandle plan = 0;
+  int32_t planned = 0;
+  for (int32_t j0 = 0; j0 < n && e == hipSuccess; j0 += batch) {
+    const int32_t nb = std::min(batch, n - j0);
+    if (nb != planned) {
+      if (planned) hipfftDestroy(plan);
+      planned = 0;
+      if (hipfftPlan1d(&plan, nfft, HIPFFT_Z2D, nb) != HIPFFT_SUCCESS || hipfftSetStream(plan, st) != HIPFFT_SUCCESS) {
+        e = hipErrorUnknown;
+        break;
+      }
+      planned = nb;
+    }
+    if (hipfftExecZ2D(plan, (hipfftDoubleComplex *)(d_coeff + (size_t)j0 * bins), (hipfftDoubleReal *)full) !=
+        HIPFFT_SUCCESS) {
+      e = hipErrorUnknown;
+      break;
+    }
+    dim3 g((length + 255) / 256, nb);
+    k_take_first<<<g, 256, 0, st>>>(full, nfft, length, nb, 1 / (double)nfft, d_out + (size_t)j0 * length);
+    e = hipGetLastError();
+  }
+  const hipError_t s2 = hipStreamSynchronize(st);
+  if (planned) hipfftDestroy(plan);
+  hipFree(full);
+  return e != hipSuccess ? e : s2;
 }
 
 }  // namespace gfbank

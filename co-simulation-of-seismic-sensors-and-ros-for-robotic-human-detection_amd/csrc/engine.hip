@@ -48,8 +48,7 @@ double rate(const gf_bank_dev *b);
 int impulse_check(const gf_bank_dev *b, const double *r_m, int32_t n, int32_t length, double fs_hz);
 hipError_t impulse_launch(const gf_bank_dev *b, const double *d_r, int32_t n, int32_t length, double fs_hz,
                           const double2 *d_sensor, double2 *d_coeff, double *d_out, hipStream_t st);
-hipError_t irdft_launch(const double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out,
-                        hipStream_t st);
+hipError_t irdft_launch(double2 *d_coeff, int32_t bins, int32_t n, int32_t length, double *d_out, hipStream_t st);
 }  // namespace gfbank
 
 #define HIPCHK(expr)                                       \
