@@ -174,3 +174,30 @@ def test_impulse_responses_equal_the_oracle(oracle):
         assert np.max(np.abs(y[q] - want_y)) <= 1e-12 * np.max(np.abs(want_y))
     conv.close()
     dev.close()
+
+
+def test_long_transform_impulse_responses_equal_the_oracle(oracle):
+    """A bank of 8,192 samples: its impulse responses take the rocFFT path
+    (transforms above 4,096 points) and must equal the oracle's direct
+    composition as the shorter ones do."""
+    count, samples = 12, 8192
+    bins = samples // 2 + 1
+    i = np.arange(count)[:, None]
+    k = np.arange(bins)[None, :]
+    # a smooth, range-dependent spectrum with a travel-time phase
+    z = np.exp(-k / 900.0) / (1 + i) * np.exp(-1j * (k * (0.002 + 0.0004 * i)))
+    z[:, 0] = 0
+    data = np.stack([z.real, z.imag], axis=-1).astype(np.float32)
+    h = fd.BankHeader(Solver="test", Medium=[fd.Layer(0, 500, 200, 1700, 25)], SampleRateHz=RATE, Samples=samples,
+                      MinM=2, MaxM=13, Count=count, Component="test")
+    dev = bk.Bank(h, data[..., 0].astype(np.float64) + 1j * data[..., 1].astype(np.float64))
+    ob = oracle.OracleBank(2, 13, data)
+    ranges = [2.0, 4.3, 9.9]
+    f = np.arange(bins) * RATE / samples
+    sensor = 1.0 / (1 + 1j * f / 40.0)
+    for sen in (None, sensor):
+        got = dev.ImpulseResponses(ranges, 5000, RATE, sen)
+        want = oracle.bank_impulse(ob, RATE, ranges, 5000, RATE, sen)
+        for q in range(len(ranges)):
+            assert np.max(np.abs(got[q] - want[q])) <= 1e-12 * np.max(np.abs(want[q])), (q, sen is None)
+    dev.close()
