@@ -554,6 +554,33 @@ def bench_engine(args, fd):
                                                          "whole-record numpy convolution), responses prebuilt"}
         res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     b.close()
+    # The same walk and line through sensing.NewEngine's analytic half-space
+    # (loam): both response components built on the device (32,768-point
+    # transforms at this irLen), each voice convolving its vertical force and
+    # its force along the line to every receiver.
+    t0 = time.perf_counter()
+    e = se.NewEngine(walk, chunk, se.Loam(), sample_rate=fs, walk_duration=walk.Until, geometry_range=4,
+                     sensor=geophone, receivers=rec)
+    create_s = time.perf_counter() - t0
+    alat = []
+    for _ in range(total):
+        t1 = time.perf_counter()
+        e.Next()
+        alat.append(time.perf_counter() - t1)
+    ast = e.Stats()
+    a_ir = e.ImpulseLength()
+    e.close()
+    alat = np.array(alat)
+    res["analytic"] = {
+        "workload": f"the same walk and line, sensing.NewEngine over the loam half-space: {a_ir}-tap vertical and "
+                    "radial-force responses",
+        "value": R * total * chunk / float(np.sum(alat)), "unit": "output samples/s",
+        "create_s": create_s, "build_ms": ast["build_ms"], "responses_built": ast["built"],
+        "chunk_latency_us": {"p50": float(np.percentile(alat, 50) * 1e6),
+                             "p99": float(np.percentile(alat, 99) * 1e6), "max": float(np.max(alat) * 1e6)},
+        "mix_kernel_us_per_chunk": ast["mix_ms_total"] / total * 1e3,
+        "mix_tflops": 2 * ast["macs"] / (ast["mix_ms_total"] / 1e3) / 1e12,
+    }
     return res
 
 
