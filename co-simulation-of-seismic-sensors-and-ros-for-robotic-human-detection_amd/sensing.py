@@ -9,6 +9,7 @@ consumes (internal/source/source.go), over the C ABI.
     HalfSpace, HalfSpaceGF, Analytic     soil.go:25-163, green.go:120-130,
                                          propagation.go:38-62
     NewEngine(...)                       sensing.go:110-113 (analytic)
+    Geophone, SM24                       the sensor model (internal/geophone)
     NewEngineWith(...)                   sensing.go:117-170
     Engine.Next, ChunkSize, SampleRate, ChunkDuration, StartTime,
     ActiveVoices, Propagation            sensing.go:224-330
@@ -172,6 +173,105 @@ def RangeSpan(s, rx: float, ry: float, frm: float, to: float):
         r = math.hypot(rx - c.X, ry - c.Y)
         lo, hi, ok = min(lo, r), max(hi, r), True
     return lo, hi, ok
+
+
+# ------------------------------------------------------------------ sensor
+
+
+@dataclass
+class Coupling:
+    """geophone.Coupling: the spike-to-ground mount as a second-order low-pass;
+    ResonanceFreq 0 is perfect coupling."""
+
+    ResonanceFreq: float = 0.0
+    Damping: float = 0.0
+
+
+def WellPlantedSpike() -> Coupling:
+    return Coupling(200.0, 0.5)
+
+
+@dataclass
+class Geophone:
+    """geophone.Geophone: a moving-coil velocity transducer behind its
+    ground coupling; Response is what the engine multiplies every propagation
+    response by (sensing.go:189-215)."""
+
+    NaturalFreq: float
+    Sensitivity: float
+    CoilResistance: float
+    ShuntResistance: float = 0.0
+    OpenCircuitDamping: float = 0.0
+    MovingMass: float = 0.0
+    Coupling: Coupling = field(default_factory=Coupling)
+
+    def Validate(self) -> None:
+        g = fdtd._go_g
+        checks = [(self.NaturalFreq <= 0, f"geophone: natural frequency must be positive, got {g(self.NaturalFreq)} Hz"),
+                  (self.Sensitivity <= 0, f"geophone: sensitivity must be positive, got {g(self.Sensitivity)} V/(m/s)"),
+                  (self.CoilResistance <= 0,
+                   f"geophone: coil resistance must be positive, got {g(self.CoilResistance)} ohm"),
+                  (self.MovingMass <= 0, f"geophone: moving mass must be positive, got {g(self.MovingMass)} kg"),
+                  (self.OpenCircuitDamping <= 0,
+                   f"geophone: open-circuit damping must be positive, got {g(self.OpenCircuitDamping)}"),
+                  (self.Coupling.ResonanceFreq < 0,
+                   f"geophone: coupling resonance must not be negative, got {g(self.Coupling.ResonanceFreq)} Hz")]
+        for bad, message in checks:
+            if bad:
+                raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, message)
+
+    def _w0(self) -> float:
+        return 2 * math.pi * self.NaturalFreq
+
+    def Damping(self) -> float:
+        """Open-circuit damping plus the electrical damping of the coil and
+        shunt load."""
+        if self.ShuntResistance <= 0:
+            return self.OpenCircuitDamping
+        total = self.CoilResistance + self.ShuntResistance
+        return self.OpenCircuitDamping + self.Sensitivity ** 2 / (2 * self._w0() * self.MovingMass * total)
+
+    def ShuntForDamping(self, want: float) -> float:
+        if want <= self.OpenCircuitDamping:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, f"geophone: damping {fdtd._go_g(want)} is not above the "
+                                 f"open-circuit value {fdtd._go_g(self.OpenCircuitDamping)}; a shunt can only add "
+                                 "damping")
+        total = self.Sensitivity ** 2 / (2 * self._w0() * self.MovingMass * (want - self.OpenCircuitDamping))
+        rs = total - self.CoilResistance
+        if rs <= 0:
+            raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, f"geophone: damping {fdtd._go_g(want)} needs a load of "
+                                 f"{fdtd._go_g(total)} ohm, below the coil's own {fdtd._go_g(self.CoilResistance)} ohm")
+        return rs
+
+    def _division(self) -> float:
+        if self.ShuntResistance <= 0:
+            return 1.0
+        return self.ShuntResistance / (self.CoilResistance + self.ShuntResistance)
+
+    def TransducerResponse(self, f):
+        """G s^2 / (s^2 + 2 zeta w0 s + w0^2), s = i w, volts per (m/s)."""
+        s = 1j * (2 * math.pi * np.asarray(f, dtype=np.float64))
+        w0 = self._w0()
+        return (self.Sensitivity * self._division()) * s * s / (s * s + (2 * self.Damping() * w0) * s + w0 * w0)
+
+    def CouplingResponse(self, f):
+        f = np.asarray(f, dtype=np.float64)
+        if self.Coupling.ResonanceFreq <= 0:
+            return np.ones_like(f, dtype=np.complex128)
+        s = 1j * (2 * math.pi * f)
+        wc = 2 * math.pi * self.Coupling.ResonanceFreq
+        return (wc * wc) / (s * s + (2 * self.Coupling.Damping * wc) * s + wc * wc)
+
+    def Response(self, f):
+        return self.CouplingResponse(f) * self.TransducerResponse(f)
+
+    __call__ = Response
+
+
+def SM24() -> Geophone:
+    """geophone.SM24: the common 4.5 Hz moving-coil geophone, well planted."""
+    return Geophone(NaturalFreq=4.5, Sensitivity=28.8, CoilResistance=375, ShuntResistance=0,
+                    OpenCircuitDamping=0.34, MovingMass=0.011, Coupling=WellPlantedSpike())
 
 
 # -------------------------------------------------------------- propagation
@@ -480,6 +580,8 @@ def NewEngineWith(schedule, chunk: int, prop: Optional[FromBank], *, sample_rate
         raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a schedule is required")
     if prop is None:
         raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, "sensing: a propagation model is required")
+    if isinstance(sensor, Geophone):
+        sensor.Validate()
     rec = np.ascontiguousarray(np.atleast_2d(np.asarray(receivers, dtype=np.float64)))
     ir_len = ImpulseLength(schedule, walk_duration, geometry_range, rayleigh_velocity, sample_rate,
                            [tuple(r) for r in rec])

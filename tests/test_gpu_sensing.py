@@ -368,8 +368,9 @@ def test_analytic_engine_matches_the_reference():
     fs = 1000.0
     w = walk_past(range_m=4.0, approach=2.0)
     n = 4000
+    sm24 = se.SM24()  # the reference's geophone, coupling included
     for rec in ([(0.0, 0.0)], [(0.0, 0.0), (2.0, 1.0)]):
-        e = se.NewEngine(w, 25, soil, sample_rate=fs, walk_duration=w.Until, geometry_range=4.0, sensor=geophone,
+        e = se.NewEngine(w, 25, soil, sample_rate=fs, walk_duration=w.Until, geometry_range=4.0, sensor=sm24,
                          receivers=rec)
         ir_len = e.ImpulseLength()
         nfft = 1
@@ -388,8 +389,10 @@ def test_analytic_engine_matches_the_reference():
         forces = [lambda t, c=c: foot.ForceAt(t - c.start) for c in contacts]
 
         def pair(rq):
-            v = SO.causal_impulse_response(0, ir_len, fs, lambda f: SO.velocity_response(oh, rq, f) * geophone(f))
-            r = SO.causal_impulse_response(0, ir_len, fs, lambda f: SO.radial_force_response(oh, rq, f) * geophone(f))
+            v = SO.causal_impulse_response(0, ir_len, fs,
+                                           lambda f: SO.velocity_response(oh, rq, f) * complex(sm24.Response(f)))
+            r = SO.causal_impulse_response(0, ir_len, fs,
+                                           lambda f: SO.radial_force_response(oh, rq, f) * complex(sm24.Response(f)))
             return v, r
 
         for q, (rx, ry) in enumerate(rec):

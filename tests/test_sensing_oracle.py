@@ -342,3 +342,64 @@ def test_soil_validation_texts():
         with pytest.raises(fd.FdtdError) as e:
             h.RayleighVelocity()
         assert str(e.value) == message
+
+
+# ---- the sensor (geophone.Geophone), evaluated at the response bins --------
+
+
+def isolated():
+    g = se.SM24()
+    g.Coupling = se.Coupling()
+    return g
+
+
+def test_transducer_analytic_limits():
+    """geophone_test.go:27-74: flat above resonance, S / (2 zeta) and +90
+    degrees at resonance, 12 dB per octave below, nothing at DC."""
+    import cmath
+
+    g = isolated()
+    G, zeta = g.Sensitivity, g.Damping()
+    h = complex(g.TransducerResponse(500))
+    assert abs(abs(h) - G) / G <= 1e-3 and abs(cmath.phase(h) * 180 / math.pi) <= 1
+    h0 = complex(g.TransducerResponse(g.NaturalFreq))
+    assert abs(abs(h0) - G / (2 * zeta)) / (G / (2 * zeta)) <= 1e-12
+    assert abs(cmath.phase(h0) * 180 / math.pi - 90) <= 1e-9
+    lo, hi = abs(complex(g.TransducerResponse(0.1))), abs(complex(g.TransducerResponse(0.2)))
+    assert abs(hi / lo - 4) <= 0.05
+    assert abs(lo - G * (0.1 / g.NaturalFreq) ** 2) / (G * (0.1 / g.NaturalFreq) ** 2) <= 5e-3
+    assert abs(complex(g.TransducerResponse(0.0))) == 0
+
+
+def test_shunt_damping_round_trip_and_limits():
+    """geophone_test.go:76-115."""
+    g = isolated()
+    for want in (0.5, 0.6, 0.7, 0.9):
+        loaded = isolated()
+        loaded.ShuntResistance = g.ShuntForDamping(want)
+        assert abs(loaded.Damping() - want) <= 1e-12
+    assert 1000 <= se.SM24().ShuntForDamping(0.7) <= 10000
+    for bad in (se.SM24().OpenCircuitDamping, 0.1):
+        with pytest.raises(fd.FdtdError):
+            se.SM24().ShuntForDamping(bad)
+
+
+def test_coupling_transparent_in_band_and_poor_coupling_distorts():
+    """geophone_test.go:117-146."""
+    g = se.SM24()
+    for f in (1, 10, 50, 100):
+        assert 0.95 <= abs(complex(g.CouplingResponse(f))) <= 1.35
+    assert abs(complex(g.CouplingResponse(2000))) <= 0.05
+    assert abs(complex(isolated().CouplingResponse(10))) == 1
+    g.Coupling = se.Coupling(30, 0.2)
+    assert abs(complex(g.CouplingResponse(30))) >= 2 and abs(complex(g.CouplingResponse(100))) <= 0.2
+
+
+def test_engine_rejects_an_invalid_sensor():
+    """sensing_test.go:289-300: NewEngine validates the sensor first."""
+    bad = se.SM24()
+    bad.CoilResistance = 0
+    with pytest.raises(fd.FdtdError) as e:
+        se.NewEngineWith(product_walker(), 16, object(), sample_rate=1000.0, walk_duration=10.0, geometry_range=10.0,
+                         rayleigh_velocity=180.0, sensor=bad)
+    assert str(e.value) == "geophone: coil resistance must be positive, got 0 ohm"
