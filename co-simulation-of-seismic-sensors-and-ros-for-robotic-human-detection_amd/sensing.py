@@ -9,7 +9,7 @@ consumes (internal/source/source.go), over the C ABI.
     HalfSpace, HalfSpaceGF, Analytic     soil.go:25-163, green.go:120-130,
                                          propagation.go:38-62
     NewEngine(...)                       sensing.go:110-113 (analytic)
-    Geophone, SM24                       the sensor model (internal/geophone)
+    Geophone, SM24                       geophone.go:36-203 (the sensor)
     NewEngineWith(...)                   sensing.go:117-170
     Engine.Next, ChunkSize, SampleRate, ChunkDuration, StartTime,
     ActiveVoices, Propagation            sensing.go:224-330
@@ -180,8 +180,8 @@ def RangeSpan(s, rx: float, ry: float, frm: float, to: float):
 
 @dataclass
 class Coupling:
-    """geophone.Coupling: the spike-to-ground mount as a second-order low-pass;
-    ResonanceFreq 0 is perfect coupling."""
+    """geophone.Coupling (geophone.go:88-97): the spike-to-ground mount as a
+    second-order low-pass; ResonanceFreq 0 is perfect coupling."""
 
     ResonanceFreq: float = 0.0
     Damping: float = 0.0
@@ -193,9 +193,10 @@ def WellPlantedSpike() -> Coupling:
 
 @dataclass
 class Geophone:
-    """geophone.Geophone: a moving-coil velocity transducer behind its
-    ground coupling; Response is what the engine multiplies every propagation
-    response by (sensing.go:189-215)."""
+    """geophone.Geophone (geophone.go:36-62): a moving-coil velocity
+    transducer behind its ground coupling; Response (geophone.go:201-203) is
+    what the engine multiplies every propagation response by
+    (sensing.go:189-215)."""
 
     NaturalFreq: float
     Sensitivity: float
@@ -206,6 +207,7 @@ class Geophone:
     Coupling: Coupling = field(default_factory=Coupling)
 
     def Validate(self) -> None:
+        """geophone.go:104-121."""
         g = fdtd._go_g
         checks = [(self.NaturalFreq <= 0, f"geophone: natural frequency must be positive, got {g(self.NaturalFreq)} Hz"),
                   (self.Sensitivity <= 0, f"geophone: sensitivity must be positive, got {g(self.Sensitivity)} V/(m/s)"),
@@ -224,14 +226,15 @@ class Geophone:
         return 2 * math.pi * self.NaturalFreq
 
     def Damping(self) -> float:
-        """Open-circuit damping plus the electrical damping of the coil and
-        shunt load."""
+        """geophone.go:134-144: open-circuit damping plus the electrical
+        damping of the coil and shunt load."""
         if self.ShuntResistance <= 0:
             return self.OpenCircuitDamping
         total = self.CoilResistance + self.ShuntResistance
         return self.OpenCircuitDamping + self.Sensitivity ** 2 / (2 * self._w0() * self.MovingMass * total)
 
     def ShuntForDamping(self, want: float) -> float:
+        """geophone.go:146-165."""
         if want <= self.OpenCircuitDamping:
             raise fdtd.FdtdError(fdtd.GF_ERR_INVALID, f"geophone: damping {fdtd._go_g(want)} is not above the "
                                  f"open-circuit value {fdtd._go_g(self.OpenCircuitDamping)}; a shunt can only add "
@@ -249,12 +252,14 @@ class Geophone:
         return self.ShuntResistance / (self.CoilResistance + self.ShuntResistance)
 
     def TransducerResponse(self, f):
-        """G s^2 / (s^2 + 2 zeta w0 s + w0^2), s = i w, volts per (m/s)."""
+        """geophone.go:175-186: G s^2 / (s^2 + 2 zeta w0 s + w0^2), s = i w,
+        volts per (m/s)."""
         s = 1j * (2 * math.pi * np.asarray(f, dtype=np.float64))
         w0 = self._w0()
         return (self.Sensitivity * self._division()) * s * s / (s * s + (2 * self.Damping() * w0) * s + w0 * w0)
 
     def CouplingResponse(self, f):
+        """geophone.go:190-199."""
         f = np.asarray(f, dtype=np.float64)
         if self.Coupling.ResonanceFreq <= 0:
             return np.ones_like(f, dtype=np.complex128)
@@ -269,7 +274,8 @@ class Geophone:
 
 
 def SM24() -> Geophone:
-    """geophone.SM24: the common 4.5 Hz moving-coil geophone, well planted."""
+    """geophone.SM24 (geophone.go:64-75): the common 4.5 Hz moving-coil
+    geophone, well planted."""
     return Geophone(NaturalFreq=4.5, Sensitivity=28.8, CoilResistance=375, ShuntResistance=0,
                     OpenCircuitDamping=0.34, MovingMass=0.011, Coupling=WellPlantedSpike())
 
