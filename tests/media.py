@@ -56,6 +56,8 @@ def c1_model(gsls=0):
 def rel_l2(a, b):
     import numpy as np
 
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    # complex spectra keep their imaginary parts
+    a, b = np.asarray(a), np.asarray(b)
+    t = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a, b = a.astype(t), b.astype(t)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
