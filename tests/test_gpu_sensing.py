@@ -450,3 +450,43 @@ def test_shear_nulls_abeam_and_contributes_in_line():
     inline_off, inline_on = energy(-8, 0, -1), energy(-8, 0, 0.35)
     rel = abs(inline_on - inline_off) / inline_off
     assert 1e-3 <= rel <= 0.05, rel
+
+
+def test_surrogate_spectra_drive_synthesis(oracle):
+    """The engine as a consumer of the surrogate's output: the network's
+    spectra for one soil at the bank's 50 ranges (1-50 m, 500 Hz, 512 bins)
+    form a bank in HBM, and a walk past at 4 m through it equals the oracle's
+    Reference over the same numbers.  The network is freshly initialised
+    (seed 0): the check is that its output, in the bank format, is what the
+    engine streams, not what a trained network predicts."""
+    import cosim_amd.surrogate as sur
+    p = 0.5 * (sur.PARAM_RANGES[:, 0] + sur.PARAM_RANGES[:, 1])
+    r = np.linspace(sur.RANGES[0], sur.RANGES[1], int(sur.RANGES[2]))
+    P = np.repeat(p[None, :], len(r), axis=0)
+    net = sur.Network(max_batch=64, seed=0)
+    try:
+        scale, delay = sur.prior(P, r)
+        G = net.Predict(sur.features(P, r), scale, delay)
+    finally:
+        net.close()
+    assert G.shape == (len(r), sur.BINS) and np.all(np.isfinite(G)) and np.any(G)
+    layer = fd.Layer(0, p[8] * p[0], p[0], p[9], p[3])
+    h = fd.BankHeader(Solver="surrogate", Medium=[layer], SampleRateHz=sur.SAMPLE_RATE_HZ, Samples=sur.SAMPLES,
+                      MinM=float(r[0]), MaxM=float(r[-1]), Count=len(r),
+                      Component="vertical surface displacement per unit vertical surface point force")
+    b = bk.Bank(h, G)
+    data = np.stack([G.real, G.imag], axis=-1).astype(np.float32)
+    ob = oracle.OracleBank(float(r[0]), float(r[-1]), data)
+    fs = sur.SAMPLE_RATE_HZ
+    w = walk_past()
+    e = se.NewEngineWith(w, 10, se.FromBank(b), sample_rate=fs, walk_duration=w.Until, geometry_range=w.StartY,
+                         rayleigh_velocity=sur.RAYLEIGH_FRACTION * p[0], sensor=geophone)
+    ir = e.ImpulseLength()
+    assert ir == 3 * sur.SAMPLES // 4
+    n = int(w.Until * fs)
+    got = run(e, n)
+    print(f"[sensing] surrogate bank, walk past at 4 m: {e.Responses()} responses of {ir} taps, "
+          f"peak {float(np.max(np.abs(got))):.3e}", flush=True)
+    assert_matches(got, oracle_reference(ob, fs, w, n, fs, ir))
+    e.close()
+    b.close()
