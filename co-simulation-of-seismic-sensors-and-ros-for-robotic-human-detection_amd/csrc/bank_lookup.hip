@@ -11,17 +11,18 @@
 //                   consecutive bins of one range row, so every step is one
 //                   coalesced row read.
 //   k_bank_points   one thread per (range, bin) query (Bank.Response).
-//   k_bank_spectra  one thread per (query range, bin): the bracket of the
-//                   query range, then two rows of each table read at
-//                   consecutive bins and one complex value written.  Queries
+//   k_bank_spectra  a workgroup per query range (more when the queries are
+//                   few): the bracket of the query range, then for each bin
+//                   two rows of each table read at consecutive bins and one
+//                   complex value written.  Queries
 //                   run in range order, so table rows are fetched once per
 //                   L2 rather than once per query.
 // A value reads 32 B of tables (mostly from L2 once queries run in range
-// order) and writes 16 B; the FP64 exp and sincos per value bound it.
+// order) and writes 16 B; the writes bound it.
 //
 // Arithmetic is Go's: cmplx.Abs as math.Hypot, math.Round half away from
-// zero, cmplx.Rect as r cos + i r sin; log, atan2, exp and sincos are the
-// device library's.
+// zero, cmplx.Rect as r cos + i r sin; log and atan2 are the device
+// library's, exp and sincos on lookup those of fastmath.h.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include <math.h>
@@ -111,34 +112,48 @@ __global__ void k_bank_prepare(int32_t count, int32_t bins, const float2 *__rest
   }
 }
 
-// Bank.bracket (format.go:295-309) and the interpolation of Bank.Response
-// (interp.go:38-53) for a range already checked to lie in the bank.
-__device__ __forceinline__ double2 lookup(double r, int bin, double r_min, double r_max, int32_t count, int32_t bins,
-                                          const double *__restrict__ logmag, const double *__restrict__ phase) {
-  int lo = 0;
-  double frac = 0;
+// Bank.bracket (format.go:295-309) for a range already checked to lie in the
+// bank: the lower row and the fraction towards the next.
+struct Bracket {
+  int lo;
+  double frac;
+};
+
+__device__ __forceinline__ Bracket bracket(double r, double r_min, double r_max, int32_t count) {
+  Bracket b{0, 0.0};
   if (count > 1) {
     const double spacing = (r_max - r_min) / (double)(count - 1);
     const double x = (r - r_min) / spacing;
-    lo = (int)floor(x);
-    if (lo >= count - 1) {
-      lo = count - 2;
-      frac = 1;
+    b.lo = (int)floor(x);
+    if (b.lo >= count - 1) {
+      b.lo = count - 2;
+      b.frac = 1;
     } else {
-      frac = x - (double)lo;
+      b.frac = x - (double)b.lo;
     }
   }
-  const int64_t i0 = (int64_t)lo * bins + bin;
-  const int64_t i1 = count == 1 ? i0 : (int64_t)(lo + 1) * bins + bin;
+  return b;
+}
+
+// The interpolation of Bank.Response (interp.go:38-53) at one bin.
+__device__ __forceinline__ double2 interpolate(const Bracket &b, int bin, int32_t count, int32_t bins,
+                                               const double *__restrict__ logmag, const double *__restrict__ phase) {
+  const int64_t i0 = (int64_t)b.lo * bins + bin;
+  const int64_t i1 = count == 1 ? i0 : (int64_t)(b.lo + 1) * bins + bin;
   const double l0 = logmag[i0], l1 = logmag[i1];
   if (l0 == kNegInf || l1 == kNegInf) return make_double2(0, 0);
   const double p0 = phase[i0], p1 = phase[i1];
-  const double lm = l0 + frac * (l1 - l0);
-  const double ph = p0 + frac * (p1 - p0);
+  const double lm = l0 + b.frac * (l1 - l0);
+  const double ph = p0 + b.frac * (p1 - p0);
   const double m = gff::exp_reduced(lm);
   double s, c;
   gff::sincos_reduced(ph, &s, &c);
   return make_double2(m * c, m * s);
+}
+
+__device__ __forceinline__ double2 lookup(double r, int bin, double r_min, double r_max, int32_t count, int32_t bins,
+                                          const double *__restrict__ logmag, const double *__restrict__ phase) {
+  return interpolate(bracket(r, r_min, r_max, count), bin, count, bins, logmag, phase);
 }
 
 __global__ void k_bank_points(const double *__restrict__ r, const int32_t *__restrict__ bin, int64_t n,
@@ -153,14 +168,19 @@ __global__ void k_bank_points(const double *__restrict__ r, const int32_t *__res
 // Queries arrive sorted by range (slot q holds caller query dest[q]), so
 // neighbouring workgroups read the same two table rows and each XCD's L2
 // fetches a row pair once for all the queries that bracket into it.
+// A thread walks the bins of its query with a stride of the workgroups
+// along x times 256, so the bracket and the coefficients held in scalar
+// registers are set up once for several values.
 __global__ void k_bank_spectra(const double *__restrict__ r, const int32_t *__restrict__ dest, int32_t nq,
                                double r_min, double r_max, int32_t count, int32_t bins,
                                const double *__restrict__ logmag, const double *__restrict__ phase,
                                double2 *__restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
-  if (k >= bins || q >= nq) return;
-  out[(int64_t)dest[q] * bins + k] = lookup(r[q], k, r_min, r_max, count, bins, logmag, phase);
+  if (q >= nq) return;
+  const Bracket b = bracket(r[q], r_min, r_max, count);
+  double2 *row = out + (int64_t)dest[q] * bins;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < bins; k += gridDim.x * blockDim.x)
+    row[k] = interpolate(b, k, count, bins, logmag, phase);
 }
 
 // The spectra CausalImpulseResponseAt inverts (dsp.go:245-258) for the
@@ -504,7 +524,10 @@ int gf_bank_dev_spectra(gf_bank_dev *b, const double *r_m, int32_t n, double *ou
   HIPCHK(hipMemcpyAsync(b->q_r, sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipMemcpyAsync(b->q_bin, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipEventRecord(b->e0, b->st));
-  dim3 grid((b->bins + 255) / 256, n);
+  // Workgroups along the bins: enough for about 8,192 workgroups in all (32
+  // waves on each of the 256 CUs), one when the queries alone provide them.
+  const int32_t per_q = (b->bins + 255) / 256;
+  dim3 grid((unsigned)std::max(1, std::min(per_q, (8192 + n - 1) / n)), n);
   k_bank_spectra<<<grid, 256, 0, b->st>>>(b->q_r, b->q_bin, n, b->r_min, b->r_max, b->count, b->bins, b->logmag,
                                           b->phase, b->out);
   HIPCHK(hipGetLastError());
