@@ -403,7 +403,9 @@ int gf_engine_warm(gf_engine *e, const double *contact_xy, int32_t n);
 /* Engine.admit for one contact (sensing.go:295-330): a voice starting at the
  * next chunk, whose vertical force is force_z[m] at sample emitted + m
  * (m < n; zero after), running until ceil(end_s fs) + ir_len.  Errors:
- * "sensing: contact %q at the receiver's own position". */
+ * "sensing: contact %q at the receiver's own position"; at most 64 voices
+ * live at once (the reference has no bound; a walker keeps about four):
+ * "geofdtd: more than 64 voices at once", the engine left as it was. */
 int gf_engine_admit(gf_engine *e, const char *id, double x, double y, const double *force_z, int32_t n,
                     double end_s);
 

@@ -226,6 +226,29 @@ def fixed_engine(b, contacts, n, chunk=25):
                             geometry_range=8.0, rayleigh_velocity=C_R, sensor=geophone)
 
 
+def test_voice_slots_are_bounded_and_refusal_is_clean():
+    """The device engine holds at most 64 live voices (a walker keeps about
+    four); the 65th admission is refused with a stated error, and the engine
+    goes on mixing the 64 it holds."""
+    import ctypes as C
+
+    lib = fd.load_library()
+    b = device_bank(small_bank(2, 12, 101, 100), 2, 12)
+    e = engine(b, walk_past(), 20)
+    f = np.ones(40)
+    fp = f.ctypes.data_as(C.POINTER(C.c_double))
+    for i in range(64):
+        assert lib.gf_engine_admit(e._h, b"c%d" % i, 3.0 + 0.1 * i, 4.0, fp, 40, 0.5) == 0, lib.gf_last_error()
+    assert lib.gf_engine_admit(e._h, b"c64", 9.5, 4.0, fp, 40, 0.5) != 0
+    assert lib.gf_last_error().decode() == "geofdtd: more than 64 voices at once"
+    out = np.zeros((20, 20), dtype=np.float32)  # 0.2 s: past the arrivals at 5-10 m
+    for row in out:
+        assert lib.gf_engine_next(e._h, row.ctypes.data_as(C.POINTER(C.c_float)), 20) == 0, lib.gf_last_error()
+    assert np.all(np.isfinite(out)) and np.max(np.abs(out)) > 0
+    e.close()
+    b.close()
+
+
 def test_contacts_superpose():
     """sensing_test.go:79-117 on the device engine: two contacts together give
     the sum of each alone, so voices share no state.  Bound: the reference's
