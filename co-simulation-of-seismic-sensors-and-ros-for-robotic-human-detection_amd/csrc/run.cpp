@@ -64,6 +64,7 @@ void free_group(Group &G) {
   if (G.d_nbase) hipFree(G.d_nbase);
   if (G.d_tile_model) hipFree(G.d_tile_model);
   G.d_tile_model = nullptr;
+  G.tile_model_cap = 0;
   G.exec = nullptr;
   G.graph = nullptr;
   G.d_descs = nullptr;

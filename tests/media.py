@@ -39,20 +39,6 @@ def rayleigh_velocity(vp, vs):
     return math.sqrt(0.5 * (lo + hi)) * vs
 
 
-def c1_model(gsls=0):
-    """Config 1: homogeneous Vp 400, Vs 200, rho 1800 on a 520 x 270 grid at
-    h = 0.1 m with a 20-cell C-PML.  gsls=0 is the reference physics (one SLS,
-    Q = 30 at 30 Hz, Qp = Qs); gsls=3 is three mechanisms over 5-150 Hz with
-    Qp 60, Qs 30."""
-    if gsls:
-        stack = [fd.Layer(0, 400, 200, 1800, 30, 60)]
-        relax = fd.SLS()
-    else:
-        stack = [fd.Layer(0, 400, 200, 1800, 30)]
-        relax = fd.Fit(30, 30) if False else None
-    return stack, relax
-
-
 def rel_l2(a, b):
     import numpy as np
 

@@ -1,0 +1,46 @@
+"""The library's host code on the CPU under AddressSanitizer.
+
+tests/hipstub builds csrc/run.cpp, api.cpp, tune.cpp, model.cpp, sampler.cpp
+and bank.cpp against a recording stand-in for the HIP runtime and for the
+kernel launchers (g++ -fsanitize=address, no device code).  Device memory is
+host heap memory, so every copy, fill and free the host code issues is
+checked by ASan against the allocation it names; the stub also keeps each
+stream's unfinished operations and fails on a synchronous copy or another
+stream's operation that touches memory one of them still uses, on anything
+but kernels recorded into a captured graph, and on a step launch (direct or
+replayed) whose descriptors, tile-to-model table, field planes, C-PML strips,
+memory variables, tables, trace slot or probe slot are not live allocations
+of the sizes the device layout gives them.
+
+The scenarios drive the batched driver as the config-3 bank generator runs
+it: cache-sized chunks with the tile-height tuner re-planning the group
+between trial launches (the path a profiled run once ended in a host fault
+on), all models advanced together, mixed kernel variants with failing shots
+and a bank generation, and the step API.
+"""
+
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HERE = os.path.join(ROOT, "tests", "hipstub")
+
+
+@pytest.fixture(scope="module")
+def driver():
+    subprocess.run(["make", "-s", "-C", HERE], check=True, capture_output=True)
+    return os.path.join(HERE, "lib", "batch_driver")
+
+
+@pytest.mark.parametrize("scenario", ["chunked", "together", "mixed", "sim"])
+def test_host_path_is_clean_under_asan(driver, scenario):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0")
+    env.pop("GEOFDTD_TILE_ROWS", None)
+    p = subprocess.run([driver, scenario], capture_output=True, text=True, timeout=300, env=env)
+    print(p.stdout + p.stderr[-3000:])
+    assert p.returncode == 0, p.stderr[-3000:]
+    assert f"batch_driver {scenario}: clean" in p.stdout
+    assert ", 0 live," in p.stdout  # every device allocation was freed
