@@ -633,9 +633,21 @@ void orc_sim_column(const orc_sim *s, double r, int32_t *col, double *snapped) {
   *snapped = (double)i * s->dr;
 }
 
+/* Each row of a pass of velocities or stresses writes only its own cells,
+ * so the passes may be split over threads by row (the parallel
+ * build, liboracle_fdtd_omp.so, used to write long golden runs); every value
+ * is computed by the same expression either way, so both builds give the
+ * same bits. */
+#ifdef _OPENMP
+#define ORC_ROWS _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_ROWS
+#endif
+
 static void velocities(orc_sim *s) {
   int nr = s->nr, nz = s->nz;
   double dr = s->dr, dz = s->dz, dt = s->dt;
+  ORC_ROWS
   for (int j = 0; j < nz - 1; j++) {
     int zp = j >= s->pZint.first;
     size_t row = (size_t)j * nr;
@@ -654,6 +666,7 @@ static void velocities(orc_sim *s) {
       s->vr[k] += dt / rho * (dtrr + dtrz + hoop);
     }
   }
+  ORC_ROWS
   for (int j = 0; j < nz - 1; j++) {
     int zh = j >= s->pZhalf.first;
     size_t row = (size_t)j * nr;
@@ -695,6 +708,7 @@ static void stresses(orc_sim *s, double dTdt) {
   int nr = s->nr, nz = s->nz, nm = s->nmech;
   double dr = s->dr, dz = s->dz, dt = s->dt;
   size_t n = (size_t)nr * nz;
+  ORC_ROWS
   for (int j = 0; j < nz; j++) {
     int zp = j >= s->pZint.first;
     size_t row = (size_t)j * nr;
@@ -756,6 +770,7 @@ static void stresses(orc_sim *s, double dTdt) {
       }
     }
   }
+  ORC_ROWS
   for (int j = 0; j < nz - 1; j++) {
     int zh = j >= s->pZhalf.first;
     size_t row = (size_t)j * nr;

@@ -37,6 +37,19 @@ _lib_fma = None
 LIB_FMA_PATH = os.path.join(HERE, "lib", "liboracle_fdtd_fma.so")
 
 
+LIB_OMP_PATH = os.path.join(HERE, "lib", "liboracle_fdtd_omp.so")
+
+
+def use_row_threads() -> None:
+    """Route every later call through the build whose time-step passes are
+    split over OpenMP threads by row (OMP_NUM_THREADS); it computes the same
+    bits as the serial build."""
+    global _lib
+    if not os.path.exists(LIB_OMP_PATH):
+        build()
+    _lib = _load(LIB_OMP_PATH)
+
+
 def lib(fma: bool = False):
     """The oracle library; fma=True: the same code built with fused
     multiply-adds (for measuring sensitivity to per-operation rounding)."""

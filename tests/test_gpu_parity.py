@@ -81,6 +81,36 @@ def test_c2_layered_grf_prefix(fdtd, oracle, variant):
     compare(f"c2-{variant} prefix", [t.Vertical for t in got.Traces], want.traces)
 
 
+def test_default_pml_tuning_frequency(fdtd, oracle):
+    """Sim.dominantFreq's default branch (internal/fdtd/fdtd.go:105-115):
+    with DominantFreq left at 0 the C-PML alpha is tuned to the slowest
+    relaxed shear velocity over 20 h.  The GPU's default run equals its run
+    with that frequency given explicitly, bit for bit; it matches the
+    oracle's default run; and it differs from a run tuned elsewhere, so the
+    frequency reaches the boundary's profiles."""
+    h = 0.15
+    base = dict(Stack=loam(), MaxRange=6, Depth=6, Spacing=h)
+    m0 = fdtd.Model(**base)
+    m_explicit = fdtd.Model(**base, DominantFreq=200.0 / (20 * h))
+    m_other = fdtd.Model(**base, DominantFreq=15)
+    dt = m0.Grid()[0]
+    steps = int(0.12 / dt)
+    force = fdtd.Ricker(1000, 30, dt, steps)
+    ranges = [1.0, 3.0, 5.0, 5.8]
+    got = fdtd.Run(fdtd.Shot(m0, force, ranges, steps))
+    same = fdtd.Run(fdtd.Shot(m_explicit, force, ranges, steps))
+    other = fdtd.Run(fdtd.Shot(m_other, force, ranges, steps))
+    g = np.array([t.Vertical for t in got.Traces])
+    np.testing.assert_array_equal(g, np.array([t.Vertical for t in same.Traces]))
+    moved = max(rel_l2(g[k], other.Traces[k].Vertical) for k in range(len(ranges)))
+    assert moved > 1e-5, moved
+    compare("default C-PML tuning", g, oracle.run(m0, force, ranges, steps).traces)
+    # and with attenuation (the relaxed velocity of the slowest layer)
+    ml = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=h)
+    got = fdtd.Run(fdtd.Shot(ml, force, ranges, steps))
+    compare("default C-PML tuning, lossy", [t.Vertical for t in got.Traces], oracle.run(ml, force, ranges, steps).traces)
+
+
 # ----------------------------------------------- reference known answers
 
 
