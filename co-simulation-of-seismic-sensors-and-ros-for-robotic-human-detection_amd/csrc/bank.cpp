@@ -25,7 +25,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -295,40 +297,47 @@ extern "C" int gf_bank_header_json(const gf_bank_header *h, char *out, int32_t c
   return gfs::api_set_err(GF_OK, "");
 }
 
-extern "C" int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *spec, float *spectra,
-                                double *snapped, gf_result *results) {
-  if (!shots || !spec || !spectra || n_shots <= 0)
-    return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad bank arguments");
+namespace {
+
+int check_spec(const gf_bank_spec &spec) {
   gf_bank_header probe_h;
   memset(&probe_h, 0, sizeof probe_h);
-  probe_h.sample_rate_hz = spec->sample_rate_hz;
-  probe_h.samples = spec->samples;
-  probe_h.r_min_m = spec->r_min_m;
-  probe_h.r_max_m = spec->r_max_m;
-  probe_h.count = spec->count;
+  probe_h.sample_rate_hz = spec.sample_rate_hz;
+  probe_h.samples = spec.samples;
+  probe_h.r_min_m = spec.r_min_m;
+  probe_h.r_max_m = spec.r_max_m;
+  probe_h.count = spec.count;
   probe_h.component = "x";
   std::string err;
   int rc = validate_header(probe_h, err);
-  if (rc) return gfs::api_set_err(rc, err);
-  const int bins = spec->samples / 2 + 1;
-  int top = (int)(spec->max_freq_hz / (spec->sample_rate_hz / spec->samples));
+  return rc ? gfs::api_set_err(rc, err) : GF_OK;
+}
+
+std::vector<std::vector<double>> spec_ranges(const gf_bank_spec &spec, int32_t n_shots) {
+  std::vector<std::vector<double>> ranges(n_shots);
+  for (int i = 0; i < n_shots; i++)
+    for (int r = 0; r < spec.count; r++) ranges[i].push_back(range_at(spec, r));
+  return ranges;
+}
+
+// The bank spectra of a finished batch whose receivers sit on the spec's
+// range grid: each shot's traces, still in HBM, reduced to G(f) on the device.
+int bank_reduce(gfb::Batch &B, const gf_bank_spec &spec, float *spectra, double *snapped, gf_result *results) {
+  const int n_shots = (int)B.sims.size();
+  const int bins = spec.samples / 2 + 1;
+  int top = (int)(spec.max_freq_hz / (spec.sample_rate_hz / spec.samples));
   if (top >= bins) top = bins - 1;
   const int nb = std::max(top, 0);  // bins 1..top
   std::vector<double> freqs(std::max(nb, 1));
-  for (int k = 1; k <= top; k++) freqs[k - 1] = (double)k * spec->sample_rate_hz / spec->samples;
-
-  std::vector<std::vector<double>> ranges(n_shots);
-  for (int i = 0; i < n_shots; i++)
-    for (int r = 0; r < spec->count; r++) ranges[i].push_back(range_at(*spec, r));
-  gfb::Batch B;
-  B.setup(shots, n_shots, ranges);
-  rc = B.run();
-  if (rc) return rc;
-  memset(spectra, 0, sizeof(float) * 2 * (size_t)n_shots * spec->count * bins);
-
+  for (int k = 1; k <= top; k++) freqs[k - 1] = (double)k * spec.sample_rate_hz / spec.samples;
+  memset(spectra, 0, sizeof(float) * 2 * (size_t)n_shots * spec.count * bins);
+  int rc = GF_OK;
   double *d_freq = nullptr;
   HIPCHK(hipMalloc(&d_freq, sizeof(double) * freqs.size()));
-  HIPCHK(hipMemcpy(d_freq, freqs.data(), sizeof(double) * freqs.size(), hipMemcpyHostToDevice));
+  if (hipMemcpy(d_freq, freqs.data(), sizeof(double) * freqs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d_freq);
+    return gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: copying the bank's frequencies failed");
+  }
   std::vector<double> g;
   for (int i = 0; i < n_shots; i++) {
     if (!B.sims[i]) continue;
@@ -337,16 +346,16 @@ extern "C" int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_
     if (results) results[i].peak = peak;
     if (!B.sims[i]) continue;
     gfs::Sim &s = *B.sims[i];
-    for (int r = 0; r < spec->count && snapped; r++) snapped[(size_t)i * spec->count + r] = B.snapped[i][r];
+    for (int r = 0; r < spec.count && snapped; r++) snapped[(size_t)i * spec.count + r] = B.snapped[i][r];
     if (nb <= 0) continue;
     const double dt = s.R.dt;
-    if ((rc = reduce_to_g(s.d_traces, s.g.trace_cap, spec->count, B.steps[i], dt, 0.5 * dt, s.force, d_freq, freqs,
+    if ((rc = reduce_to_g(s.d_traces, s.g.trace_cap, spec.count, B.steps[i], dt, 0.5 * dt, s.force, d_freq, freqs,
                           nb, B.stream, g))) {
       B.fail_shot(i, rc, gf_last_error());
       continue;
     }
-    float *out = spectra + (size_t)2 * i * spec->count * bins;
-    for (int r = 0; r < spec->count; r++)
+    float *out = spectra + (size_t)2 * i * spec.count * bins;
+    for (int r = 0; r < spec.count; r++)
       for (int k = 1; k <= top; k++) {
         out[2 * ((size_t)r * bins + k)] = (float)g[2 * ((size_t)r * nb + k - 1)];
         out[2 * ((size_t)r * bins + k) + 1] = (float)g[2 * ((size_t)r * nb + k - 1) + 1];
@@ -368,3 +377,82 @@ extern "C" int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_
   if (B.first_err) return gfs::api_set_err(B.first_err, B.first_msg);
   return gfs::api_set_err(GF_OK, "");
 }
+
+}  // namespace
+
+extern "C" int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *spec, float *spectra,
+                                double *snapped, gf_result *results) {
+  if (!shots || !spec || !spectra || n_shots <= 0)
+    return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad bank arguments");
+  int rc = check_spec(*spec);
+  if (rc) return rc;
+  gfb::Batch B;
+  B.setup(shots, n_shots, spec_ranges(*spec, n_shots));
+  rc = B.run();
+  if (rc) return rc;
+  return bank_reduce(B, *spec, spectra, snapped, results);
+}
+
+// ---- the batch as a handle (gf_batch_*) -------------------------------------
+
+struct gf_batch {
+  gfb::Batch B;
+  bool bank = false;
+  gf_bank_spec spec;
+};
+
+extern "C" int gf_batch_create(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *spec, gf_batch **out) {
+  if (!shots || !out || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
+  *out = nullptr;
+  std::vector<std::vector<double>> ranges(n_shots);
+  if (spec) {
+    int rc = check_spec(*spec);
+    if (rc) return rc;
+    ranges = spec_ranges(*spec, n_shots);
+  } else {
+    for (int i = 0; i < n_shots; i++) {
+      if (shots[i].n_ranges > 0 && !shots[i].ranges)
+        return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null ranges or traces in shot");
+      if (shots[i].n_ranges > 0) ranges[i].assign(shots[i].ranges, shots[i].ranges + shots[i].n_ranges);
+    }
+  }
+  std::unique_ptr<gf_batch> b(new gf_batch());
+  b->bank = spec != nullptr;
+  if (spec) b->spec = *spec;
+  b->B.setup(shots, n_shots, ranges);
+  int rc = b->B.plan();
+  if (rc) return rc;
+  *out = b.release();
+  return gfs::api_set_err(GF_OK, "");
+}
+
+extern "C" int gf_batch_advance(gf_batch *b, int32_t num, int32_t den) {
+  if (!b) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null batch");
+  int rc = b->B.advance(num, den);
+  return rc ? rc : gfs::api_set_err(GF_OK, "");
+}
+
+extern "C" int gf_batch_reset(gf_batch *b) {
+  if (!b) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null batch");
+  int rc = b->B.reset();
+  return rc ? rc : gfs::api_set_err(GF_OK, "");
+}
+
+extern "C" int gf_batch_results(gf_batch *b, gf_result *results, int32_t *statuses) {
+  if (!b || !results) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  if (!b->B.finished()) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: the batch has not run to its end");
+  for (size_t i = 0; i < b->B.sims.size(); i++)
+    if (!b->B.cols[i].empty() && !results[i].traces)
+      return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null ranges or traces in shot");
+  gfb::describe_results(b->B, results);
+  return gfb::finish_results(b->B, results, statuses);
+}
+
+extern "C" int gf_batch_spectra(gf_batch *b, float *spectra, double *snapped, gf_result *results) {
+  if (!b || !spectra) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
+  if (!b->bank) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: the batch was created without a bank spec");
+  if (!b->B.finished()) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: the batch has not run to its end");
+  return bank_reduce(b->B, b->spec, spectra, snapped, results);
+}
+
+extern "C" void gf_batch_destroy(gf_batch *b) { delete b; }

@@ -10,6 +10,33 @@
 
 namespace gfb {
 
+// The launch plan of one set of models advanced together: one kernel variant
+// (attenuation form), one tile height, models ordered longest run first so
+// the running models are always a prefix of the tile list.
+struct Group {
+  int nmech = 0;
+  bool scalar = true;
+  int th = 0;                        // tile rows shared by every model of the group
+  std::vector<int> order;            // shot indices, longest run first
+  std::vector<int64_t> prefix;       // tiles of the first k models
+  gfk::GridDesc *d_descs = nullptr;
+  uint16_t *d_tile_model = nullptr;  // model of each tile (groups of 2+)
+  int64_t tile_model_cap = 0;
+  int32_t *d_nbase = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_models = -1;             // active-model count the graph was captured for
+};
+
+// A group whose combined state fits the last-level cache, with its progress:
+// every model of a chunk is at step `done` (or finished).
+struct Chunk {
+  Group G;
+  int variant = 0;
+  int64_t done = 0;
+  int64_t max_steps = 0;
+};
+
 struct Batch {
   std::vector<std::unique_ptr<gfs::Sim>> sims;  // null for shots that failed set-up
   std::vector<int32_t> st, steps;
@@ -18,19 +45,36 @@ struct Batch {
   std::vector<std::vector<double>> snapped;
   int first_err = GF_OK;
   std::string first_msg;
-  hipStream_t stream = nullptr;  // the stream the batch ran on
-  int tile_rows = 0;             // tile height of the last group run
-  double kernel_ms = 0;          // step-loop device time (HIP events)
-  int64_t launches = 0;          // step kernels launched
-  double bytes = 0;              // algorithmic HBM bytes of all steps run
+  hipStream_t stream = nullptr;  // the stream the batch runs on
+  int tile_rows = 0;             // tile height of the last group planned
+  double kernel_ms = 0;          // step-loop device time since the last reset (HIP events)
+  int64_t launches = 0;          // step kernels launched since the last reset
+  double bytes = 0;              // algorithmic HBM bytes of the steps run since the last reset
+  std::vector<Chunk> chunks;
+  bool planned = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
 
+  ~Batch();
   void setup(const gf_shot *shots, int32_t n, const std::vector<std::vector<double>> &ranges);
-  int run();
+  // Group the set-up models into cache-sized chunks, pick each variant's tile
+  // height (tuning it on long runs) and upload the launch tables.  Once.
+  int plan();
+  // Advance every chunk to ceil(longest run x num / den) steps (num == den:
+  // to the end).  Chunks run one after another, each in graph blocks.
+  int advance(int64_t num, int64_t den);
+  // Every model back to step 0 (zero state, cleared probes), timing cleared.
+  int reset();
+  bool finished() const;
+  int run() { return advance(1, 1); }
   void probes(int i, double *peak);
   int fail_shot(int i, int code, const std::string &m);
 };
 
 // Statistics of the calling thread's last batch (gf_batch_stats).
 void record_stats(const Batch &b);
+// Per-shot dt, t0, cells and snapped ranges of a set-up batch.
+void describe_results(const Batch &B, gf_result *results);
+// Probes, traces and statuses of a finished batch, as gf_run_batch returns them.
+int finish_results(Batch &B, gf_result *results, int32_t *statuses);
 
 }  // namespace gfb

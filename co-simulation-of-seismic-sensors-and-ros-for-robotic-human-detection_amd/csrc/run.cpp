@@ -39,23 +39,9 @@ int api_set_err(int code, const std::string &msg);
   } while (0)
 
 using gfs::Sim;
+using gfb::Group;
 
 namespace {
-
-struct Group {
-  int nmech = 0;
-  bool scalar = true;
-  int th = 0;  // tile rows shared by every model of the group
-  std::vector<int> order;        // shot indices, longest run first
-  std::vector<int64_t> prefix;   // tiles of the first k models
-  gfk::GridDesc *d_descs = nullptr;
-  uint16_t *d_tile_model = nullptr;  // model of each tile (batches of 2+)
-  int64_t tile_model_cap = 0;
-  int32_t *d_nbase = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_models = -1;  // active-model count the graph was captured for
-};
 
 void free_group(Group &G) {
   if (G.exec) hipGraphExecDestroy(G.exec);
@@ -239,9 +225,25 @@ double cache_budget_bytes() {
   return mb * 1e6;
 }
 
-// All set-up shots: one launch per step over the tiles of every running
-// model of a chunk, in graph blocks, chunk after chunk.
-int Batch::run() {
+Batch::~Batch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (Chunk &C : chunks) free_group(C.G);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+}
+
+bool Batch::finished() const {
+  for (const Chunk &C : chunks)
+    if (C.done < C.max_steps) return false;
+  return true;
+}
+
+// Chunks: models of one kernel variant sorted by step count (the running set
+// is a prefix), cut where the cache budget would be exceeded; each variant's
+// tile height picked on its first chunk; every chunk's tables uploaded.
+int Batch::plan() {
+  if (planned) return GF_OK;
+  planned = true;
   const int n = (int)sims.size();
   std::vector<Group> variants;
   for (int i = 0; i < n; i++) {
@@ -260,22 +262,16 @@ int Batch::run() {
     it->order.push_back(i);
   }
   if (variants.empty()) return GF_OK;
-  int rc = GF_OK;
   stream = sims[variants[0].order[0]]->stream;
   for (auto &sp : sims)
-    if (sp && !rc) {
+    if (sp) {
       hipError_t e = hipStreamSynchronize(sp->stream);
-      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch set-up");
+      if (e != hipSuccess) return gfs::hip_fail(e, "batch set-up");
     }
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event set-up failed");
 
-  // chunks: models sorted by step count (the running set is a prefix), cut
-  // where the cache budget would be exceeded
   const double budget = cache_budget_bytes();
-  struct Chunk {
-    Group G;
-    int variant;
-  };
-  std::vector<Chunk> chunks;
   std::vector<int64_t> variant_cell_steps(variants.size(), 0), variant_cells(variants.size(), 0);
   for (size_t v = 0; v < variants.size(); v++) {
     Group &V = variants[v];
@@ -284,7 +280,8 @@ int Batch::run() {
     for (int i : V.order) {
       const double b = state_bytes(*sims[i]);
       if (chunks.empty() || chunks.back().variant != (int)v || (budget > 0 && acc + b > budget)) {
-        chunks.push_back(Chunk{Group(), (int)v});
+        chunks.emplace_back();
+        chunks.back().variant = (int)v;
         chunks.back().G.nmech = V.nmech;
         chunks.back().G.scalar = V.scalar;
         acc = 0;
@@ -296,26 +293,18 @@ int Batch::run() {
     }
   }
 
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-    rc = gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event set-up failed");
-  bytes = 0;
-  for (int i = 0; i < n; i++)
-    if (sims[i]) bytes += gfs::step_bytes(sims[i]->R) * steps[i];
-  launches = 0;
-  double total_ms = 0;
   std::vector<int> variant_th(variants.size(), 0);  // tile height chosen for each variant
-
   for (Chunk &C : chunks) {
-    if (rc) break;
     Group &G = C.G;
+    C.max_steps = steps[G.order.front()];
     int64_t cols_rows = 0;
     for (int i : G.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
     int th = variant_th[C.variant];
-    if ((rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows), stream))) break;
+    int rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows), stream);
+    if (rc) return rc;
     if (!th) {
       // Long runs pay for a few trial steps at each tile height, once per
-      // variant, on its first chunk.
+      // variant, on its first chunk; the trial steps are then undone.
       const char *tune_env = getenv("GEOFDTD_TUNE");
       const bool want_tune = !getenv("GEOFDTD_TILE_ROWS") &&
                              (tune_env ? atoi(tune_env) != 0 : variant_cell_steps[C.variant] >= 5000000000LL);
@@ -329,74 +318,100 @@ int Batch::run() {
         };
         int best = gft::cached(key);
         if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
-        if (best && (rc = plan_group(G, sims, best, stream))) break;
+        if (best && (rc = plan_group(G, sims, best, stream))) return rc;
         for (int i : G.order)
-          if ((rc = sims[i]->reset_state())) break;
-        if (rc) break;
+          if ((rc = sims[i]->reset_state())) return rc;
       }
       variant_th[C.variant] = G.th;
     }
     tile_rows = G.th;
+  }
+  for (auto &sp : sims)
+    if (sp) {
+      hipError_t e = hipStreamSynchronize(sp->stream);
+      if (e != hipSuccess) return gfs::hip_fail(e, "batch set-up");
+    }
+  return GF_OK;
+}
+
+int Batch::advance(int64_t num, int64_t den) {
+  if (den <= 0 || num < 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch fraction");
+  int rc = plan();
+  if (rc) return rc;
+  for (Chunk &C : chunks) {
+    Group &G = C.G;
+    const int64_t target = num >= den ? C.max_steps : std::min(C.max_steps, (C.max_steps * num + den - 1) / den);
+    if (target <= C.done) continue;
+    // algorithmic bytes of the real steps this call runs
     for (int i : G.order) {
-      hipError_t e = hipStreamSynchronize(sims[i]->stream);
-      if (e != hipSuccess) {
-        rc = gfs::hip_fail(e, "batch set-up");
-        break;
-      }
+      const int64_t a = std::min<int64_t>(C.done, steps[i]), b = std::min<int64_t>(target, steps[i]);
+      bytes += gfs::step_bytes(sims[i]->R) * (double)(b - a);
     }
-    if (rc) break;
-    const int64_t max_steps = steps[G.order.front()];
-    if (hipEventRecord(e0, stream) != hipSuccess) {
-      rc = gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event record failed");
-      break;
-    }
-    for (int64_t k = 0; k < max_steps && !rc;) {
-      const bool graph_block = k + gfs::kGraphSteps <= max_steps;
+    if (hipEventRecord(e0, stream) != hipSuccess) return gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event record failed");
+    int64_t k = C.done;
+    while (k < target) {
       const int act = active_models(G, steps, k);
       if (act == 0) break;
+      // A graph block may carry models past their last step (their tiles
+      // return at once) but never past this call's target.
+      const bool graph_block = k + gfs::kGraphSteps <= target;
       launches += graph_block ? gfs::kGraphSteps : 1;
       if (graph_block) {
         // Re-capture when the running set has shrunk by a tenth of its tiles.
         if (!G.exec || G.graph_models < act ||
             (G.prefix[act] < G.prefix[G.graph_models] - G.prefix[G.graph_models] / 10)) {
-          if ((rc = capture(G, act, stream))) break;
+          if ((rc = capture(G, act, stream))) return rc;
         }
         hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)k, stream);
         if (e == hipSuccess) e = hipGraphLaunch(G.exec, stream);
-        if (e != hipSuccess) {
-          rc = gfs::hip_fail(e, "graph launch");
-          break;
-        }
+        if (e != hipSuccess) return gfs::hip_fail(e, "graph launch");
+        k += gfs::kGraphSteps;
       } else {
-        hipError_t e =
-            gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[act], (int32_t)k, nullptr, 1,
-                             stream);
-        if (e != hipSuccess) {
-          rc = gfs::hip_fail(e, "step launch");
-          break;
-        }
+        hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[act], (int32_t)k,
+                                        nullptr, 1, stream);
+        if (e != hipSuccess) return gfs::hip_fail(e, "step launch");
+        k += 1;
       }
-      k += graph_block ? gfs::kGraphSteps : 1;
     }
-    if (!rc) {
-      hipError_t e = hipEventRecord(e1, stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(stream);
-      float ms = 0;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) rc = gfs::hip_fail(e, "batch run");
-      total_ms += ms;
-    }
-    free_group(G);
+    C.done = target;
+    hipError_t e = hipEventRecord(e1, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) return gfs::hip_fail(e, "batch run");
+    kernel_ms += ms;
+    // a finished chunk's launch tables and graph are no longer needed
+    if (C.done >= C.max_steps) free_group(G);
   }
-  kernel_ms = total_ms;
-  // an error path can leave launches in flight: let them finish before any
-  // graph or table of theirs is freed
-  (void)hipStreamSynchronize(stream);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  for (Chunk &C : chunks) free_group(C.G);
   record_stats(*this);
-  return rc;
+  return GF_OK;
+}
+
+int Batch::reset() {
+  int rc = plan();
+  if (rc) return rc;
+  for (const Chunk &C : chunks)
+    for (int i : C.G.order)
+      if (!sims[i]) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: a batch with a failed shot cannot be reset");
+  for (Chunk &C : chunks) {
+    if (C.done == 0) continue;
+    if (C.done >= C.max_steps) {
+      // its tables were released when it finished: plan it again
+      if ((rc = plan_group(C.G, sims, C.G.th, stream))) return rc;
+    }
+    for (int i : C.G.order)
+      if ((rc = sims[i]->reset_state()) || (rc = sims[i]->reset_max_slots(steps[i]))) return rc;
+    C.done = 0;
+  }
+  for (auto &sp : sims)
+    if (sp) {
+      hipError_t e = hipStreamSynchronize(sp->stream);
+      if (e != hipSuccess) return gfs::hip_fail(e, "batch reset");
+    }
+  kernel_ms = 0;
+  launches = 0;
+  bytes = 0;
+  return GF_OK;
 }
 
 thread_local gf_batch_stats_t g_stats;
@@ -444,17 +459,11 @@ extern "C" int gf_batch_stats(gf_batch_stats_t *out) {
   return gfs::api_set_err(GF_OK, "");
 }
 
-extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
-  if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
-  std::vector<std::vector<double>> ranges(n_shots);
-  for (int i = 0; i < n_shots; i++) {
-    if (shots[i].n_ranges > 0 && (!shots[i].ranges || !results[i].traces))
-      return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null ranges or traces in shot");
-    if (shots[i].n_ranges > 0) ranges[i].assign(shots[i].ranges, shots[i].ranges + shots[i].n_ranges);
-  }
-  gfb::Batch B;
-  B.setup(shots, n_shots, ranges);
-  for (int i = 0; i < n_shots; i++) {
+namespace gfb {
+
+// Per-shot geometry of a set-up batch into the caller's results.
+void describe_results(const Batch &B, gf_result *results) {
+  for (size_t i = 0; i < B.sims.size(); i++) {
     if (!B.sims[i]) continue;
     results[i].dt = B.sims[i]->R.dt;
     results[i].t0 = 0.5 * B.sims[i]->R.dt;
@@ -464,9 +473,13 @@ extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *re
     for (size_t r = 0; r < B.snapped[i].size(); r++)
       if (results[i].snapped) results[i].snapped[r] = B.snapped[i][r];
   }
-  int rc = B.run();
-  if (rc) return rc;
-  for (int i = 0; i < n_shots; i++) {
+}
+
+// What fdtd.Run returns for each shot of a finished batch (run.go:104-127):
+// the divergence probes, then the traces widened to float64.
+int finish_results(Batch &B, gf_result *results, int32_t *statuses) {
+  const int n = (int)B.sims.size();
+  for (int i = 0; i < n; i++) {
     if (!B.sims[i]) continue;
     B.probes(i, &results[i].peak);
     if (!B.sims[i]) continue;
@@ -481,11 +494,29 @@ extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *re
       for (size_t k = 0; k < tmp.size(); k++) results[i].traces[k] = tmp[k];
     }
   }
-  for (int i = 0; i < n_shots; i++) {
+  for (int i = 0; i < n; i++) {
     if (statuses) statuses[i] = B.st[i];
     results[i].status = B.st[i];
     snprintf(results[i].error, sizeof results[i].error, "%s", B.msg[i].c_str());
   }
   if (B.first_err) return gfs::api_set_err(B.first_err, B.first_msg);
   return gfs::api_set_err(GF_OK, "");
+}
+
+}  // namespace gfb
+
+extern "C" int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results, int32_t *statuses) {
+  if (!shots || !results || n_shots <= 0) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: bad batch arguments");
+  std::vector<std::vector<double>> ranges(n_shots);
+  for (int i = 0; i < n_shots; i++) {
+    if (shots[i].n_ranges > 0 && (!shots[i].ranges || !results[i].traces))
+      return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: null ranges or traces in shot");
+    if (shots[i].n_ranges > 0) ranges[i].assign(shots[i].ranges, shots[i].ranges + shots[i].n_ranges);
+  }
+  gfb::Batch B;
+  B.setup(shots, n_shots, ranges);
+  gfb::describe_results(B, results);
+  int rc = B.run();
+  if (rc) return rc;
+  return gfb::finish_results(B, results, statuses);
 }

@@ -272,6 +272,12 @@ SIGNATURES = {
                               C.POINTER(GfResult)]),
     "gf_soil_sample": (_I, [C.c_uint64, C.c_uint32, C.POINTER(GfSoil)]),
     "gf_batch_stats": (_I, [C.POINTER(GfBatchStats)]),
+    "gf_batch_create": (_I, [C.POINTER(GfShot), _I32, C.POINTER(GfBankSpec), C.POINTER(C.c_void_p)]),
+    "gf_batch_advance": (_I, [C.c_void_p, _I32, _I32]),
+    "gf_batch_reset": (_I, [C.c_void_p]),
+    "gf_batch_results": (_I, [C.c_void_p, C.POINTER(GfResult), _PI32]),
+    "gf_batch_spectra": (_I, [C.c_void_p, C.POINTER(C.c_float), _PD, C.POINTER(GfResult)]),
+    "gf_batch_destroy": (None, [C.c_void_p]),
     "gf_spectrum": (_I, [C.POINTER(C.c_float), _I32, _I32, _D, _D, _PD, _I64, _PD, _I32, _PD]),
     "gf_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gf_soil_sample_layered": (_I, [C.c_uint64, C.c_uint32, C.POINTER(GfSoil), _PD]),
@@ -838,6 +844,98 @@ def GenerateBanks(shots: Sequence[Shot], spec: BankSpec):
         else:
             results.append(Result(cres[i].dt, cres[i].t0, [], (cres[i].nr, cres[i].nz), cres[i].peak))
     return out[..., 0] + 1j * out[..., 1], snapped, results
+
+
+class Batch:
+    """Many shots as one handle, run in pieces: the bank generator's job loop
+    (cmd/geobank/main.go:128-167) split into set-up (constructor: every shot
+    through fdtd.New's checks, launches planned and tuned), Advance(num, den)
+    (every chunk of models to that fraction of its longest run), Reset() and,
+    once the batch has run to its end, Results() (what RunBatch returns) or
+    Spectra() (what GenerateBanks returns, when built with a BankSpec)."""
+
+    def __init__(self, shots: Sequence[Shot], spec: Optional["BankSpec"] = None):
+        lib = load_library()
+        self._shots = list(shots)
+        self._spec = spec
+        n = len(self._shots)
+        cshots = (GfShot * n)()
+        self._keep = []
+        for i, sh in enumerate(self._shots):
+            c, k = _shot_c(Shot(sh.Model, sh.Force, [] if spec else sh.Ranges, sh.Steps))
+            cshots[i] = c
+            self._keep.append(k)
+        self._cs = None if spec is None else GfBankSpec(spec.SampleRateHz, spec.Samples, spec.MaxFreqHz, spec.MinM,
+                                                        spec.MaxM, spec.Count)
+        h = C.c_void_p()
+        _check(lib.gf_batch_create(cshots, n, C.byref(self._cs) if self._cs else None, C.byref(h)))
+        self._h = h
+
+    def Advance(self, num: int = 1, den: int = 1) -> None:
+        _check(load_library().gf_batch_advance(self._h, num, den))
+
+    def Reset(self) -> None:
+        _check(load_library().gf_batch_reset(self._h))
+
+    def Stats(self) -> dict:
+        """Work and device time since the last Reset (the thread's last batch)."""
+        return BatchStats()
+
+    def _results(self, cres, status):
+        out: List[object] = []
+        for i in range(len(self._shots)):
+            if status[i] != GF_OK:
+                out.append(FdtdError(int(status[i]), cres[i].error.decode()))
+            else:
+                out.append(Result(cres[i].dt, cres[i].t0, [], (cres[i].nr, cres[i].nz), cres[i].peak))
+        return out
+
+    def Results(self) -> List[object]:
+        n = len(self._shots)
+        cres = (GfResult * n)()
+        bufs = []
+        for i, sh in enumerate(self._shots):
+            tr = np.zeros((max(len(sh.Ranges), 1), max(sh.Steps, 1)), dtype=np.float64)
+            sn = np.zeros(max(len(sh.Ranges), 1), dtype=np.float64)
+            bufs.append((tr, sn))
+            cres[i].traces = _dptr(tr)
+            cres[i].snapped = _dptr(sn)
+        statuses = np.zeros(n, dtype=np.int32)
+        rc = load_library().gf_batch_results(self._h, cres, statuses.ctypes.data_as(_PI32))
+        if rc != GF_OK and not statuses.any():
+            _check(rc)
+        out = self._results(cres, statuses)
+        for i, sh in enumerate(self._shots):
+            if isinstance(out[i], Result):
+                tr, sn = bufs[i]
+                out[i].Traces = [Trace(float(sn[k]), tr[k, : sh.Steps].copy()) for k in range(len(sh.Ranges))]
+        return out
+
+    def Spectra(self):
+        spec = self._spec
+        if spec is None:
+            raise FdtdError(GF_ERR_ARGUMENT, "geofdtd: the batch was created without a bank spec")
+        n = len(self._shots)
+        bins = spec.Samples // 2 + 1
+        out = np.zeros((n, spec.Count, bins, 2), dtype=np.float32)
+        snapped = np.zeros((n, spec.Count), dtype=np.float64)
+        cres = (GfResult * n)()
+        rc = load_library().gf_batch_spectra(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), _dptr(snapped), cres)
+        status = [cres[i].status for i in range(n)]
+        if rc != GF_OK and not any(status):
+            _check(rc)
+        return out[..., 0] + 1j * out[..., 1], snapped, self._results(cres, status)
+
+    def close(self) -> None:
+        if self._h:
+            load_library().gf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def Spectrum(traces: np.ndarray, dt: float, t0: float, force: Sequence[float], freqs: Sequence[float]) -> np.ndarray:

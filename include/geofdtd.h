@@ -277,6 +277,34 @@ typedef struct gf_bank_spec {
 int gf_bank_generate(const gf_shot *shots, int32_t n_shots, const gf_bank_spec *spec, float *spectra,
                      double *snapped, gf_result *results);
 
+/* The same batch as a handle, so a caller can run it in pieces: the bank
+ * generator's job loop (cmd/geobank/main.go:128-167, one fdtd.Run per soil
+ * in a worker pool) split into set-up, advance and read-back.  No single
+ * reference function.
+ *   gf_batch_create  sets every shot up (fdtd.New's checks and errors per
+ *                    shot, as gf_run_batch) and plans the launches: models
+ *                    grouped by kernel variant into chunks whose state fits
+ *                    the last-level cache, tile heights tuned on long runs.
+ *                    With spec != NULL the receivers sit on the bank's range
+ *                    grid (shot ranges ignored) and gf_batch_spectra applies.
+ *   gf_batch_advance advances every chunk to ceil(its longest run x num /
+ *                    den) steps (num >= den: to the end), chunk after chunk;
+ *                    gf_batch_stats then reports the work since the last
+ *                    reset.
+ *   gf_batch_reset   returns every model to step 0 (zero state, cleared
+ *                    divergence probes) to run the batch again.
+ *   gf_batch_results / gf_batch_spectra  once the batch has run to its end:
+ *                    what gf_run_batch / gf_bank_generate return. */
+typedef struct gf_batch gf_batch;
+int gf_batch_create(const gf_shot *shots, int32_t n_shots,
+                    const gf_bank_spec *spec, gf_batch **out);
+int gf_batch_advance(gf_batch *b, int32_t num, int32_t den);
+int gf_batch_reset(gf_batch *b);
+int gf_batch_results(gf_batch *b, gf_result *results, int32_t *statuses);
+int gf_batch_spectra(gf_batch *b, float *spectra, double *snapped,
+                     gf_result *results);
+void gf_batch_destroy(gf_batch *b);
+
 /* The reduction gf_bank_generate applies, on caller traces: for each of
  * `rows` traces of n samples at t0 + k dt (float32, row-major) and the force
  * series that drove them (F[k] at k dt), G(f) = V(f) / (i w W(f)) at each of

@@ -7,6 +7,8 @@
 //   batch_driver mixed     scalar SLS, GSLS and elastic models with failing
 //                          shots in one batch, then a bank generation
 //   batch_driver sim       the step API: create, drive, step, probe, reuse
+//   batch_driver handle    the batch as a handle: advanced in pieces, reset
+//                          half-way and after its end, run again, read back
 //
 // Every HIP call the library makes is checked by the stub (see there); this
 // program checks what comes back through the ABI.  It prints one summary line
@@ -177,6 +179,56 @@ int scenario_sim() {
   return 0;
 }
 
+int scenario_handle() {
+  setenv("GEOFDTD_CACHE_MB", "50", 1);
+  setenv("GEOFDTD_TUNE", "1", 1);
+  std::vector<Shot> v(9);
+  for (int i = 0; i < 9; i++) c3_shot(v[i], (uint32_t)(60 + i), 180 + 53 * i, i % 3 == 2 ? 'R' : 'X');
+  std::vector<gf_shot> sh = shots_of(v);
+  gf_batch *b = nullptr;
+  expect(gf_batch_create(sh.data(), (int32_t)sh.size(), nullptr, &b) == GF_OK && b, "gf_batch_create");
+  if (!b) return 1;
+  std::vector<gf_result> res(v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    memset(&res[i], 0, sizeof res[i]);
+    res[i].traces = v[i].traces.data();
+    res[i].snapped = v[i].snapped.data();
+  }
+  std::vector<int32_t> st(v.size());
+  expect(gf_batch_advance(b, 1, 3) == GF_OK, "advance to a third");
+  expect(gf_batch_results(b, res.data(), st.data()) != GF_OK, "results refused before the end");
+  expect(gf_batch_reset(b) == GF_OK, "reset half-way");
+  for (int k = 1; k <= 7; k++) expect(gf_batch_advance(b, k, 7) == GF_OK, "advance in sevenths");
+  expect(gf_batch_advance(b, 1, 1) == GF_OK, "advance at the end is a no-op");
+  expect(gf_batch_results(b, res.data(), st.data()) == GF_OK, "results");
+  expect(gf_batch_reset(b) == GF_OK, "reset after the end");
+  expect(gf_batch_advance(b, 1, 1) == GF_OK, "second run");
+  expect(gf_batch_results(b, res.data(), st.data()) == GF_OK, "results of the second run");
+  gf_batch_destroy(b);
+  // with a bank spec
+  gf_bank_spec spec;
+  memset(&spec, 0, sizeof spec);
+  spec.sample_rate_hz = 500;
+  spec.samples = 256;
+  spec.max_freq_hz = 100;
+  spec.r_min_m = 1;
+  spec.r_max_m = 20;
+  spec.count = 8;
+  for (auto &s : sh) s.n_ranges = 0, s.ranges = nullptr;
+  expect(gf_batch_create(sh.data(), (int32_t)sh.size(), &spec, &b) == GF_OK && b, "gf_batch_create with a bank");
+  if (!b) return 1;
+  const int bins = spec.samples / 2 + 1;
+  std::vector<float> spectra((size_t)v.size() * spec.count * bins * 2);
+  std::vector<double> snapped((size_t)v.size() * spec.count);
+  for (int k = 1; k <= 4; k++) expect(gf_batch_advance(b, k, 4) == GF_OK, "bank batch in quarters");
+  expect(gf_batch_spectra(b, spectra.data(), snapped.data(), res.data()) == GF_OK, "bank spectra");
+  expect(gf_batch_reset(b) == GF_OK, "bank batch reset");
+  expect(gf_batch_advance(b, 1, 1) == GF_OK, "bank batch again");
+  expect(gf_batch_spectra(b, spectra.data(), snapped.data(), res.data()) == GF_OK, "bank spectra again");
+  gf_batch_destroy(b);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -185,6 +237,7 @@ int main(int argc, char **argv) {
   else if (what == "together") scenario_batch(false);
   else if (what == "mixed") scenario_mixed();
   else if (what == "sim") scenario_sim();
+  else if (what == "handle") scenario_handle();
   else {
     fprintf(stderr, "unknown scenario %s\n", what.c_str());
     return 2;
