@@ -403,23 +403,41 @@ class GateReport:
     gate_seconds: float
     bound: float
     guarded_soils: List[int] = field(default_factory=list)
+    escalated: bool = False               # the whole request was answered by the FDTD
+    fdtd_soils: List[int] = field(default_factory=list)   # soils answered by their FDTD solution
+
+    def vouch_bound(self, confidence: float = 0.95) -> float:
+        """With no sampled soil failing, the largest fraction of failing
+        soils in the request that the sample rules out at `confidence`:
+        a request in which a fraction p of the soils fail passes k sampled
+        soils unnoticed with probability (1 - p)^k."""
+        k = len(self.sampled_soils)
+        return 1.0 if k == 0 else 1.0 - (1.0 - confidence) ** (1.0 / k)
 
 
 def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
           gate_fraction: float = 0.01, bound: float = 0.02, seed: int = 0,
           solver: Optional[Callable] = None, predictions: Optional[np.ndarray] = None,
-          norm_limit: Optional[float] = None):
+          norm_limit: Optional[float] = None, escalate: bool = True):
     """Answer every (soil, range) query of `soils` ([S] soils with their 10
     numbers `params` [S][10] and receiver ranges [S][R]).
 
     The network predicts all S*R spectra; then ceil(gate_fraction * S) soils,
     drawn with `seed`, are re-solved by the batched FDTD (`solver(soils)` ->
-    (spectra [k][R][512], ...), default `solve`), and every sampled soil with a
-    query whose relative L2 against its FDTD solution exceeds `bound` is
-    answered by that solution instead.  With `norm_limit` (see norm_bound),
-    every soil with a prediction whose normalized norm exceeds it is re-solved
-    and answered by the FDTD as well.  `predictions` ([S*R][512]) replaces the
-    network's output, for tests of the gate itself.
+    (spectra [k][R][512], ...), default `solve`).  Every sampled soil with a
+    query whose relative L2 against its FDTD solution (both limited to
+    BAND_HZ) exceeds `bound` is answered by that solution.  With `escalate`
+    (the default) one such soil is enough to show that the network is out of
+    its bound on this request, so every soil of the request is then answered
+    by its FDTD solution; otherwise the unsampled soils keep their
+    predictions.  With `norm_limit` (see norm_bound), every soil with a
+    prediction whose normalized norm exceeds it is re-solved and answered by
+    the FDTD as well.  `predictions` ([S*R][512]) replaces the network's
+    output, for tests of the gate itself.
+
+    What the gate guarantees: a query answered by the network belongs to a
+    request in which no sampled soil exceeded the bound; at 95 % confidence
+    fewer than GateReport.vouch_bound() of that request's soils do.
 
     Returns (spectra [S][R][512] complex64, GateReport)."""
     S, R = ranges.shape
@@ -439,26 +457,32 @@ def serve(net: Network, params: np.ndarray, ranges: np.ndarray, soils: Sequence,
     sampled = sorted(rng.choice(S, size=k, replace=False).tolist()) if k > 0 else []
     guarded: List[int] = []
     if norm_limit is not None:
-        p = np.repeat(np.asarray(params), R, axis=0)
-        scale, _ = prior(p, ranges.reshape(-1))
+        scale, _ = prior(p, r)
         nrm = (np.linalg.norm(pred.reshape(S * R, BINS), axis=1) / scale).reshape(S, R)
         guarded = sorted(set(np.flatnonzero(np.any(nrm > norm_limit, axis=1)).tolist()) - set(sampled))
+    solve_ = solver or solve
     failed: List[int] = []
+    fdtd: List[int] = []
     errs = np.zeros((0,))
+    known = {}                     # soil -> its FDTD solution, band-limited
     resolve = sampled + guarded
     if resolve:
-        truth = (solver or solve)([soils[i] for i in resolve])[0]
-        if predictions is None:
-            truth = band_limit(truth)
-        k = len(sampled)
-        if k:
-            e = rel_l2(out[sampled].reshape(k * R, BINS), truth[:k].reshape(k * R, BINS)).reshape(k, R)
+        truth = band_limit(solve_([soils[i] for i in resolve])[0])
+        known = {s_: truth[j] for j, s_ in enumerate(resolve)}
+        if sampled:
+            kk = len(sampled)
+            e = rel_l2(band_limit(out[sampled]).reshape(kk * R, BINS), truth[:kk].reshape(kk * R, BINS)).reshape(kk, R)
             errs = e.reshape(-1)
-            for j, s in enumerate(sampled):
-                if np.any(e[j] > bound):
-                    failed.append(s)
-                    out[s] = truth[j]
-        for j, s in enumerate(guarded):
-            out[s] = truth[k + j]
+            failed = [s_ for j, s_ in enumerate(sampled) if np.any(e[j] > bound)]
+        fdtd = sorted(set(failed) | set(guarded))
+    escalated = bool(escalate and failed)
+    if escalated:
+        rest = [s_ for s_ in range(S) if s_ not in known]
+        if rest:
+            truth = band_limit(solve_([soils[i] for i in rest])[0])
+            known.update({s_: truth[j] for j, s_ in enumerate(rest)})
+        fdtd = list(range(S))
+    for s_ in fdtd:
+        out[s_] = known[s_]
     t2 = time.perf_counter()
-    return out, GateReport(S * R, S, sampled, failed, errs, t1 - t0, t2 - t1, bound, guarded)
+    return out, GateReport(S * R, S, sampled, failed, errs, t1 - t0, t2 - t1, bound, guarded, escalated, fdtd)

@@ -172,14 +172,11 @@ def test_layered_family_is_deterministic_and_inside_its_box():
     assert fd.SampleSoil(7, 0).Stack[0].Vs != fd.SampleLayeredSoil(7, 0)[0].Stack[0].Vs
 
 
-def test_gate_falls_back_for_injected_bad_predictions():
-    """The error gate with the network's output replaced: sampled soils whose
-    predictions are off by more than the bound get their FDTD solution, the
-    others keep the prediction.  The solver is a stand-in returning known
-    truth, so this runs without a GPU."""
+def _gate_case():
     S_, R = 50, 4
     rng = np.random.default_rng(3)
     truth = (rng.normal(size=(S_, R, S.BINS)) + 1j * rng.normal(size=(S_, R, S.BINS))).astype(np.complex64)
+    truth = S.band_limit(truth)
     pred = truth.copy()
     bad = set(range(0, S_, 3))
     for s in bad:
@@ -192,13 +189,23 @@ def test_gate_falls_back_for_injected_bad_predictions():
         calls.append(list(soils))
         return truth[np.asarray(soils)], None, 0.0, {}
 
+    return S_, R, truth, pred, bad, calls, solver
+
+
+def test_gate_falls_back_for_injected_bad_predictions():
+    """The error gate with the network's output replaced and escalation off:
+    sampled soils whose predictions are off by more than the bound get their
+    FDTD solution, the others keep the prediction.  The solver is a stand-in
+    returning known truth, so this runs without a GPU."""
+    S_, R, truth, pred, bad, calls, solver = _gate_case()
     soils = list(range(S_))
     params = np.zeros((S_, 10))
     ranges = np.ones((S_, R))
     out, rep = S.serve(None, params, ranges, soils, gate_fraction=0.5, bound=0.02, seed=4, solver=solver,
-                       predictions=pred.reshape(S_ * R, S.BINS))
+                       predictions=pred.reshape(S_ * R, S.BINS), escalate=False)
     assert len(rep.sampled_soils) == 25 and len(calls) == 1 and calls[0] == rep.sampled_soils
     assert rep.failed_soils == sorted(s for s in rep.sampled_soils if s in bad)
+    assert not rep.escalated and rep.fdtd_soils == rep.failed_soils
     for s in range(S_):
         if s in rep.failed_soils:
             np.testing.assert_array_equal(out[s], truth[s])
@@ -207,8 +214,37 @@ def test_gate_falls_back_for_injected_bad_predictions():
     assert rep.sampled_errors.shape == (25 * R,)
     # a gate fraction of 1% samples ceil(0.01 S) soils
     _, rep1 = S.serve(None, params, ranges, soils, gate_fraction=0.01, solver=solver,
-                      predictions=pred.reshape(S_ * R, S.BINS))
+                      predictions=pred.reshape(S_ * R, S.BINS), escalate=False)
     assert len(rep1.sampled_soils) == 1
+
+
+def test_gate_escalates_a_request_the_sample_shows_out_of_bound():
+    """With escalation (the default), one failing sampled soil sends the whole
+    request to the FDTD: every soil is answered by its solution, in one more
+    solver call over the soils not yet solved."""
+    S_, R, truth, pred, bad, calls, solver = _gate_case()
+    soils = list(range(S_))
+    out, rep = S.serve(None, np.zeros((S_, 10)), np.ones((S_, R)), soils, gate_fraction=0.5, bound=0.02, seed=4,
+                       solver=solver, predictions=pred.reshape(S_ * R, S.BINS))
+    assert rep.failed_soils and rep.escalated and rep.fdtd_soils == soils
+    assert len(calls) == 2 and sorted(calls[0] + calls[1]) == soils and not set(calls[0]) & set(calls[1])
+    np.testing.assert_array_equal(out, truth)
+
+
+def test_gate_serves_predictions_when_the_sample_passes():
+    """Predictions within the bound on every sampled soil: no escalation, the
+    network's answers are served, and the report states what the sample
+    vouches for (1 - 0.05^(1/k) of the soils at 95 % confidence)."""
+    S_, R, truth, pred, bad, calls, solver = _gate_case()
+    for s in bad:
+        pred[s] = truth[s] * 1.01
+    soils = list(range(S_))
+    out, rep = S.serve(None, np.zeros((S_, 10)), np.ones((S_, R)), soils, gate_fraction=0.2, bound=0.02, seed=5,
+                       solver=solver, predictions=pred.reshape(S_ * R, S.BINS))
+    assert not rep.failed_soils and not rep.escalated and rep.fdtd_soils == []
+    np.testing.assert_array_equal(out, pred)
+    assert len(calls) == 1 and rep.sampled_errors.max() < 0.02
+    assert rep.vouch_bound() == pytest.approx(1 - 0.05 ** (1 / 10))
 
 
 def test_polar_targets_reconstruct_the_spectrum_and_the_loss():
@@ -269,6 +305,7 @@ def test_magnitude_guard_sends_blown_up_soils_to_the_fdtd():
     scale, _ = S.prior(np.repeat(params, R, axis=0), ranges.reshape(-1))
     truth = (rng.normal(size=(S_, R, S.BINS)) + 1j * rng.normal(size=(S_, R, S.BINS))).astype(np.complex64)
     truth *= scale.reshape(S_, R, 1).astype(np.float32)        # normalized norm ~ sqrt(2 * 512)
+    truth = S.band_limit(truth)
     pred = truth.copy()
     blown = [5, 17, 33]
     for s in blown:
