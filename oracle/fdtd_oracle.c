@@ -804,6 +804,35 @@ void orc_sim_step(orc_sim *s) {
   s->step++;
 }
 
+/* Checkpoint of the evolving state (the six fields, the eight C-PML psi
+ * arrays, the memory variables and the step count) to `path` (save != 0) or
+ * back from it, so a run of hours can be continued after an interruption.
+ * The file is raw doubles in a fixed order, for the same model only. */
+int orc_sim_checkpoint(orc_sim *s, const char *path, int save) {
+  FILE *f = fopen(path, save ? "wb" : "rb");
+  if (!f) return fail(GF_ERR_ARGUMENT, "oracle: cannot open checkpoint %s", path);
+  size_t n = (size_t)s->nr * s->nz;
+  size_t nm = s->rrr ? (size_t)s->nmech * n : 0;
+  double *arrs[] = {s->vr, s->vz, s->trr, s->tqq, s->tzz, s->trz, s->psiTrrR, s->psiTrzZ, s->psiTrzR, s->psiTzzZ,
+                    s->psiVrR, s->psiVzZ, s->psiVzR, s->psiVrZ, s->rrr, s->rqq, s->rzz, s->rrz};
+  int64_t head[3] = {s->step, (int64_t)n, (int64_t)nm};
+  int ok = 1;
+  if (save) {
+    ok = fwrite(head, sizeof head, 1, f) == 1;
+  } else {
+    int64_t h[3];
+    ok = fread(h, sizeof h, 1, f) == 1 && h[1] == head[1] && h[2] == head[2];
+    if (ok) s->step = (int)h[0];
+  }
+  for (size_t a = 0; ok && a < sizeof arrs / sizeof arrs[0]; a++) {
+    size_t len = a < 14 ? n : nm;
+    if (!len || !arrs[a]) continue;
+    ok = save ? fwrite(arrs[a], sizeof(double), len, f) == len : fread(arrs[a], sizeof(double), len, f) == len;
+  }
+  ok = (fclose(f) == 0) && ok;
+  return ok ? GF_OK : fail(GF_ERR_ARGUMENT, "oracle: checkpoint %s is incomplete or of another model", path);
+}
+
 double orc_sim_surface_velocity(const orc_sim *s, int i) {
   double dvr, vrr;
   if (i == 0) {

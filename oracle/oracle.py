@@ -98,6 +98,7 @@ def _load(path):
         "orc_sim_drive": (None, [C.c_void_p, _PD, _I64]),
         "orc_sim_column": (None, [C.c_void_p, _D, _PI32, _PD]),
         "orc_sim_step": (None, [C.c_void_p]),
+        "orc_sim_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
         "orc_sim_surface_velocity": (_D, [C.c_void_p, C.c_int]),
         "orc_sim_radial_velocity": (_D, [C.c_void_p, C.c_int]),
         "orc_sim_max_speed": (_D, [C.c_void_p]),
@@ -260,6 +261,10 @@ class OracleSim:
 
     def Step(self):
         lib().orc_sim_step(self._h)
+
+    def checkpoint(self, path: str, save: bool) -> None:
+        """Write (save) or restore the evolving state and step count."""
+        _check(lib().orc_sim_checkpoint(self._h, path.encode(), 1 if save else 0))
 
     def SurfaceVelocity(self, i):
         return lib().orc_sim_surface_velocity(self._h, i)

@@ -12,6 +12,11 @@ the kept samples measures the same error) in float64, the oracle's peak per
 trace at full rate, and the late-window diagnostics.
 
     python tests/golden/make_full_traces.py c2-R c2-X c3
+
+Config-2 shots run step by step through the oracle's Sim (the same calls, in
+the same order, as its Run) and checkpoint their state every few thousand
+steps into GOLDEN_CHECKPOINTS (default /tmp/golden_checkpoints), so an
+interrupted run continues where it stopped.
 """
 
 import json
@@ -42,12 +47,58 @@ def late_window(traces, dt, start_s):
     return (late / full).tolist()
 
 
-def write(name, w, ranges, decimate, late_start):
+class _Run:
+    def __init__(self, traces, snapped, peak, dt, cells):
+        self.traces, self.snapped, self.peak, self.dt, self.cells = traces, snapped, peak, dt, cells
+
+
+def run_resumable(name, w, ranges, block=2000):
+    """fdtd.Run's loop (run.go:104-126) over the oracle's Sim: step, record
+    the receivers, probe MaxSpeed every 256 steps; state checkpointed every
+    `block` steps."""
+    import oracle as O
+
+    ck = os.environ.get("GOLDEN_CHECKPOINTS", "/tmp/golden_checkpoints")
+    os.makedirs(ck, exist_ok=True)
+    state, meta = os.path.join(ck, f"{name}.state"), os.path.join(ck, f"{name}.npz")
+    o = O.OracleSim(w.model)
+    o.Drive(w.force)
+    cols, snapped = zip(*[o.Column(r) for r in ranges])
+    steps = w.steps
+    traces = np.zeros((len(ranges), steps))
+    peak, start = 0.0, 0
+    if os.path.exists(state) and os.path.exists(meta):
+        z = np.load(meta)
+        start = int(z["done"])
+        traces[:, :start] = z["traces"]
+        peak = float(z["peak"])
+        o.checkpoint(state, save=False)
+        print(f"{name}: resuming at step {start}", flush=True)
+    t = time.time()
+    for n in range(start, steps):
+        o.Step()
+        for i, c in enumerate(cols):
+            traces[i, n] = o.SurfaceVelocity(c)
+        if n % 256 == 0:
+            p = o.MaxSpeed()
+            peak = p if (p > peak or p != p) else peak
+            if not np.isfinite(p):
+                raise SystemExit(f"{name}: diverged at step {n} of {steps}")
+        if (n + 1) % block == 0 and n + 1 < steps:
+            o.checkpoint(state + ".tmp", save=True)
+            np.savez(meta + ".tmp.npz", traces=traces[:, :n + 1], peak=peak, done=n + 1)
+            os.replace(state + ".tmp", state)
+            os.replace(meta + ".tmp.npz", meta)
+            print(f"{name}: step {n + 1} of {steps} ({time.time() - t:.0f} s this sitting)", flush=True)
+    return _Run(traces, np.asarray(snapped), peak, o.Dt(), o.Cells())
+
+
+def write(name, w, ranges, decimate, late_start, resumable=False):
     import oracle as O
 
     O.use_row_threads()
     t = time.time()
-    r = O.run(w.model, w.force, ranges, w.steps)
+    r = run_resumable(name, w, ranges) if resumable else O.run(w.model, w.force, ranges, w.steps)
     el = time.time() - t
     tr = r.traces
     np.savez_compressed(
@@ -87,7 +138,7 @@ def main(which):
     os.makedirs(OUT, exist_ok=True)
     for name in which:
         if name in ("c2-R", "c2-X"):
-            write(name, W.c2(name[-1]), C2_RANGES, 4, 1.5)
+            write(name, W.c2(name[-1]), C2_RANGES, 4, 1.5, resumable=True)
         elif name == "c3":
             for i in C3_SOILS:
                 w = W.c3(n_models=1, first=i)[0]
