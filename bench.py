@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark: Green's-function traces per second of the MI355X FDTD solver.
 
-One benchmark "step" is one complete shot of the workload: every time step of
-the FDTD run over one soil model, recording every receiver.  With the default
-config-2 workload that is 80,000 time steps of a 2048 x 1024 grid and 100
-surface traces.  Inputs (model tables, source series, receiver columns) are
-resident in HBM before the timed region; each timed shot resets the fields on
-the device and runs to the end, leaving its traces in HBM.
+The default workload is config 3, batched bank generation: 256 seeded soil
+models (600-2048 x 300-1024 cells, GSLS attenuation, 1 s each, 50 receivers)
+advanced together and reduced on the device to bank spectra.  The batch is
+set up before the timed region (every model's state, tables, source series
+and receiver columns resident in HBM, launches planned and tuned); one
+benchmark "step" advances every model by 1/steps of its run, so the timed
+steps generate the whole batch once, the last step ending with the spectra.
 
 Printed (one JSON line on stdout):
-  value       traces/s over the timed shots (wall clock, bracketed by device syncs)
+  value       traces/s over the timed steps (wall clock; every step ends with
+              a device synchronization)
   roofline    the step kernel's algorithmic HBM bytes per launch over its
-              average launch time (HIP events on the solver's stream), against
-              MI355X's 8 TB/s; `traffic` is the rocprofv3-measured HBM bytes
-              per launch when a committed profile summary provides it
+              average launch time (HIP events around each chunk's step loop
+              on the solver's stream), against MI355X's 8 TB/s; `traffic` is
+              the rocprofv3-measured HBM bytes per launch when a committed
+              profile summary provides it
   cpu_baseline the FP64 CPU restatement of the reference scheme on the same
               workload, a bounded sample of time steps on every worker thread,
-              scaled to whole shots
+              scaled by cell-steps
+  accuracy    whole-shot traces of two of the batch's soils against the FP64
+              oracle's committed traces
+  also        config 2 (2048 x 1024, 80,000 steps) as single shots; fk,
+              bank_lookup, stream, engine: the adjacent components
 """
 
 from __future__ import annotations
@@ -33,6 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "co-simulation-of-seismic-sensors-and-ros-for-robotic-human-detection_amd")
 METRIC = "GF traces/s: batched FDTD vs surrogate at stated L2 err; stencil HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+SUR_WEIGHTS = os.path.join(PKG_DIR, "weights", "surrogate_layered.npz")
 
 
 def log(msg: str) -> None:
@@ -141,8 +149,39 @@ def cpu_baseline(w, gpu_traces, sample_steps: int, threads: int) -> tuple:
     return base, acc
 
 
-def bench_c3(args, fd) -> int:
-    """Config 3: batched bank generation over seeded soils (one job)."""
+C3_GOLDEN = ("c3-X-204", "c3-X-153")
+
+
+def c3_accuracy(fd) -> dict:
+    """The two config-3 soils with the fewest cell-steps, batched on the GPU
+    over their whole second, against the FP64 oracle's committed traces
+    (tests/golden/full_traces, ten receivers each)."""
+    import numpy as np
+
+    import cosim_amd.workloads as W
+
+    gold = [np.load(os.path.join(ROOT, "tests", "golden", "full_traces", f"{n}.npz")) for n in C3_GOLDEN]
+    ws = [W.c3(n_models=1, first=int(n.rsplit("-", 1)[1]))[0] for n in C3_GOLDEN]
+    shots = [fd.Shot(w.model, w.force, [float(r) for r in z["ranges"]], w.steps) for w, z in zip(ws, gold)]
+    errs = []
+    for g, z in zip(fd.RunBatch(shots), gold):
+        d = int(z["decimate"])
+        for k, t in enumerate(g.Traces):
+            a, b = t.Vertical[::d], z["traces"][k]
+            errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    return {"rel_l2_max": max(errs), "rel_l2_median": float(np.median(errs)), "traces_compared": len(errs),
+            "window_steps": "whole shots (6,934 and 7,025 steps)",
+            "against": "FP64 CPU restatement of the reference scheme, soils 204 and 153 of the batch"}
+
+
+def bench_c3(args, fd) -> dict:
+    """Config 3: batched bank generation over seeded soils.  The batch is set
+    up once (every model's arrays and tables in HBM, launches planned and
+    tuned) outside the timed region; a bench step advances every chunk of
+    models by 1/steps of its longest run, so `steps` steps generate the whole
+    batch of banks, the last one ending with the device reduction of every
+    trace to its bank spectrum.  Warm-up steps run the first fractions of the
+    batch, which is then reset to step 0."""
     import numpy as np
 
     import cosim_amd.workloads as W
@@ -150,54 +189,66 @@ def bench_c3(args, fd) -> int:
     ws = W.c3(n_models=args.c3_models, seconds=args.c3_seconds)
     spec = W.c3_bank_spec()
     shots = [w.shot() for w in ws]
-    cells = [w.model.Grid()[1] * w.model.Grid()[2] for w in ws]
+    grids = [w.model.Grid() for w in ws]
+    cells = [g[1] * g[2] for g in grids]
     cell_steps = float(sum(c * w.steps for c, w in zip(cells, ws)))
     log(f"c3: {len(ws)} models, {sum(cells) / 1e6:.1f} M cells, {cell_steps:.3g} cell-steps")
     t0 = time.perf_counter()
-    spectra, snapped, res = fd.GenerateBanks(shots, spec)
+    batch = fd.Batch(shots, spec)
+    log(f"set-up and launch planning {time.perf_counter() - t0:.1f} s")
+    K, Wm = max(args.steps, 1), max(args.warmup, 0)
+    for i in range(min(Wm, K)):
+        batch.Advance(i + 1, K)
+        log(f"warmup step {i + 1}/{Wm}")
+    batch.Reset()
+    t0 = time.perf_counter()
+    for i in range(K):
+        batch.Advance(i + 1, K)
+        if i + 1 == K:
+            spectra, snapped, res = batch.Spectra()
+        log(f"step {i + 1}/{K}: {time.perf_counter() - t0:.1f} s")
     wall = time.perf_counter() - t0
     st = fd.BatchStats()
+    batch.close()
     bad = [r for r in res if isinstance(r, fd.FdtdError)]
     if bad:
         raise SystemExit(f"{len(bad)} models failed: {bad[0]}")
     if not np.all(np.isfinite(spectra)):
         raise SystemExit("non-finite spectra")
     traces = len(ws) * spec.Count
-    achieved = st["algorithmic_bytes"] / (st["kernel_ms"] / 1e3) / 1e9
+    launch_s = st["kernel_ms"] / 1e3 / max(st["step_launches"], 1)
     per_launch = st["algorithmic_bytes"] / max(st["step_launches"], 1)
-    # the committed counter profile was taken with every model advanced
-    # together; with cache-sized chunks (the default) the HBM bytes are lower
-    # and not measured, so no traffic figure is claimed for them
+    achieved = per_launch / launch_s / 1e9
     lockstep = float(os.environ.get("GEOFDTD_CACHE_MB", "160")) <= 0
-    ratio = traffic_ratio_from_profiles("c3") if lockstep else None
+    ratio = traffic_ratio_from_profiles("c3" if lockstep else "c3_chunked")
     out = {
         "metric": METRIC,
         "value": traces / wall,
         "unit": "traces/s",
         "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
-        "ms_per_step": wall * 1e3,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"c3: {len(ws)} seeded soils (Philox seed 0), 600-2048 x 300-1024 cells, "
-                        f"{args.c3_seconds:g} s each, {spec.Count} receivers, banks of {spec.Samples // 2 + 1} bins",
+            "workload": f"c3: batched bank generation, {len(ws)} seeded soils (Philox seed 0), 600-2048 x 300-1024 "
+                        f"cells, {args.c3_seconds:g} s each, {spec.Count} receivers, banks of "
+                        f"{spec.Samples // 2 + 1} bins; a step advances every model by 1/{K} of its run",
             "models": len(ws), "cells": sum(cells), "cell_steps": cell_steps,
             "physics": "GSLS, 3 mechanisms, per-layer Qs, Qp = 2 Qs",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None if ratio is None else ratio * per_launch,
-            "kernel": "k_step over the running models of a chunk (one launch per step)" if not lockstep
-            else "k_step over every model still running (one launch per step)",
+            "kernel": "k_step over every model of a chunk still running (one launch per step, graph blocks of 100)"
+            if not lockstep else "k_step over every model still running (one launch per step)",
             "tile_rows": st["tile_rows"], "algorithmic_bytes": st["algorithmic_bytes"],
             "step_loop_ms": st["kernel_ms"], "step_launches": st["step_launches"],
-            "avg_launch_us": st["kernel_ms"] * 1e3 / max(st["step_launches"], 1),
-            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_us": launch_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
         },
         "cells_per_s": cell_steps / wall,
     }
@@ -219,8 +270,11 @@ def bench_c3(args, fd) -> int:
             "cell_steps_per_s": rate,
         }
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
-    print(json.dumps(out), flush=True)
-    return 0
+    else:
+        out["cpu_baseline"] = None
+    if os.path.exists(os.path.join(ROOT, "tests", "golden", "full_traces", f"{C3_GOLDEN[0]}.npz")):
+        out["accuracy"] = c3_accuracy(fd)
+    return out
 
 
 FP64_PEAK_TFLOPS = 78.6
@@ -680,6 +734,17 @@ def bench_surrogate(args, fd) -> int:
         },
         "cpu_baseline": None,
     }
+    if args.sur_save:
+        S.save_weights(args.sur_save, net, limit, {
+            "trained_by": "bench.py --workload surrogate (this project's BF16 MFMA forward/backward kernels and Adam)",
+            "command": " ".join(sys.argv),
+            "train_soils": args.sur_soils, "train_soils_passing_qc": int(qc.sum()), "train_steps": args.sur_steps,
+            "train_batch": 4096, "lr": args.sur_lr, "held_out_first": 1_000_000, "held_out_soils": n_held,
+            "held_out_queries": args.sur_queries,
+            "held_out_rel_l2_p50": float(np.median(err)), "held_out_rel_l2_p99": float(np.quantile(err, 0.99)),
+            "held_out_fraction_within_2pct": float(np.mean(err <= 0.02)),
+        })
+        log(f"weights saved to {args.sur_save}")
     if not args.no_cpu:
         # the FP64 CPU oracle's forward pass on the trained weights, on a
         # sample of the same queries (numpy matmuls on the host's cores)
@@ -701,6 +766,67 @@ def bench_surrogate(args, fd) -> int:
     net.close()
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_served(args, fd) -> dict:
+    """Config 5: --sur-queries held-out (soil, range) queries of the layered
+    family served by the network with the committed weights, at inference
+    batch 8,192, behind the 1 % FDTD error gate (escalating the request to the
+    batched FDTD when a sampled soil exceeds 2 %).  Timed from the queries to
+    the served spectra, FDTD re-solves included."""
+    import numpy as np
+
+    import cosim_amd.surrogate as S
+
+    R = S.RANGES[2]
+    n_soils = -(-args.sur_queries // R)
+    net, limit, meta = S.load_weights(args.sur_weights)
+    held = S.soil_set(0, meta.get("held_out_first", 1_000_000), n_soils)
+    ranges = S.receiver_ranges(held.soils)
+    x = S.features(np.repeat(held.params, R, axis=0), ranges.reshape(-1))
+    sc, dl = S.prior(np.repeat(held.params, R, axis=0), ranges.reshape(-1))
+    for _ in range(2):
+        net.Predict(x, sc, dl)
+    inf = net.Stats()
+    inf_rate = inf["forward_queries"] / (inf["forward_ms"] / 1e3)
+    t0 = time.perf_counter()
+    served, rep = S.serve(net, held.params, ranges, held.soils, gate_fraction=0.01, bound=0.02, seed=3,
+                          norm_limit=limit)
+    wall = time.perf_counter() - t0
+    net.close()
+    queries = n_soils * R
+    achieved = inf["flops_per_query"] * inf_rate / 1e12
+    known = sorted(set(rep.sampled_soils) | set(rep.fdtd_soils))
+    out = {
+        "workload": f"config 5: {queries} held-out queries ({n_soils} soils x {R} ranges), BF16 inference batch "
+                    f"8192, 1% of soils re-solved by the batched FP32 FDTD, the request escalated to the FDTD when "
+                    f"a sampled soil exceeds 2% relative L2",
+        "value": queries / wall, "unit": "traces/s", "seconds": wall,
+        "inference_traces_per_s": inf_rate,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "surrogate forward (BF16 MFMA GEMMs, FP32 accumulation)",
+                     "flops_per_query": inf["flops_per_query"]},
+        "gate": {"sampled_soils": len(rep.sampled_soils), "failed_soils": len(rep.failed_soils),
+                 "escalated": rep.escalated, "fdtd_soils": len(rep.fdtd_soils),
+                 "magnitude_guard_soils": len(rep.guarded_soils),
+                 "sampled_rel_l2_p50": float(np.median(rep.sampled_errors)) if len(rep.sampled_errors) else None,
+                 "sampled_rel_l2_max": float(np.max(rep.sampled_errors)) if len(rep.sampled_errors) else None,
+                 "vouch_bound_95": rep.vouch_bound() if not rep.escalated else None,
+                 "surrogate_seconds": rep.surrogate_seconds, "fdtd_seconds": rep.gate_seconds},
+        "served_from_network": queries - R * len(rep.fdtd_soils),
+        "weights": {k: meta[k] for k in ("train_soils", "train_steps", "held_out_rel_l2_p50",
+                                        "held_out_rel_l2_p99", "held_out_fraction_within_2pct") if k in meta},
+    }
+    if known:
+        # served answers against the FDTD where the run solved the soil
+        truth_soils = S.solve([held.soils[i] for i in known[:min(len(known), 16)]])[0]
+        e = S.rel_l2(served[known[:len(truth_soils)]].reshape(-1, S.BINS),
+                     S.band_limit(truth_soils).reshape(-1, S.BINS))
+        out["served_rel_l2_vs_fdtd"] = {"p50": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
+                                        "queries": int(len(e)),
+                                        "over": "the first 16 soils the gate solved (sampled or escalated)"}
+    return out
 
 
 def surrogate_speed(args, S) -> int:
@@ -784,19 +910,83 @@ def ceiling(args) -> int:
     return 0
 
 
+def measure_shot(args, fd, name):
+    """One single-shot workload (c1/c2): --warmup untimed and --steps timed whole shots."""
+    import numpy as np
+
+    w = workload(name)
+    if args.time_steps:
+        w.steps = args.time_steps
+    dt, nr, nz = w.model.Grid()
+    log(f"workload {w.name}: {nr}x{nz}, {w.steps} steps, {len(w.ranges)} receivers")
+    sim = fd.New(w.model)
+    sim.Drive(w.force)
+    cols = [sim.Column(r)[0] for r in w.ranges]
+    sim.SetReceivers(cols, w.steps)
+    use_graph = not args.no_graph
+    step_bytes = sim.StepBytes()
+    env_rows = os.environ.get("GEOFDTD_TILE_ROWS")
+    tile_rows = int(env_rows) if env_rows else sim.Autotune(10)
+    if env_rows:
+        sim.SetTileRows(tile_rows)
+    log(f"tile rows {tile_rows}")
+    for i in range(args.warmup):
+        sim.Reset()
+        sim.Advance(w.steps, use_graph)
+        sim.SyncTiming()
+        log(f"warmup shot {i + 1}/{args.warmup} done")
+    sim.SyncTiming()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sim.Reset()
+        sim.Advance(w.steps, use_graph)
+    kernel_ms, launches = sim.SyncTiming()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} shots in {wall:.3f} s; {launches} step launches in {kernel_ms:.1f} ms of device time")
+    traces = sim.ReadTraces(len(cols))
+    sim.close()
+    if not np.all(np.isfinite(traces)):
+        raise SystemExit("non-finite traces")
+    avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = step_bytes / avg_launch_s / 1e9
+    roof = {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic_from_profiles(w.name),
+        "kernel": "k_step (fused velocity+stress+memory-variable step)",
+        "tile_rows": tile_rows,
+        "algorithmic_bytes_per_launch": step_bytes,
+        "avg_launch_us": avg_launch_s * 1e6,
+    }
+    res = {
+        "value": len(w.ranges) * args.steps / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": describe(w, nr, nz, dt),
+        "roofline": roof,
+        "cells_per_s": nr * nz * w.steps * args.steps / wall,
+    }
+    return w, traces, res
+
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed shots")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
-    ap.add_argument("--workload", default="c2-X",
-                    help="c2-X (default: config 2 as stated, per-layer Q), c2-R (the reference's one-SLS physics), "
-                         "c1-R, c1-X, c3 (batched bank generation), surrogate (train on FDTD banks, serve held-out "
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed steps: c3, each advances the whole batch by 1/steps of its run; c1/c2, whole shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed steps")
+    ap.add_argument("--workload", default="c3",
+                    help="c3 (default: batched bank generation, 256 seeded soils), c2-X (config 2 as stated, "
+                         "per-layer Q), c2-R (the reference's one-SLS physics), c1-R, c1-X, surrogate (train on FDTD banks, serve held-out "
                          "queries behind the FDTD error gate), fk (one bank by wavenumber integration), bank-lookup "
                          "(batched interpolation in a resident bank), stream (chunked convolution), engine (runtime "
                          "synthesis of a walk past a geophone line)")
-    ap.add_argument("--also", default="c2-R", help="second workload measured the same way and reported under "
-                    "'also' (empty to skip)")
+    ap.add_argument("--also", default="c2-X", help="single-shot workload measured over --also-shots shots and "
+                    "reported under 'also' (empty to skip)")
+    ap.add_argument("--also-shots", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of HIP graphs")
     ap.add_argument("--no-fk", dest="fk", action="store_false",
                     help="skip the f-k bank line reported under 'fk' (the wavenumber-integration bank filler)")
@@ -810,6 +1000,10 @@ def main() -> int:
     ap.add_argument("--sur-queries", type=int, default=65536, help="surrogate: held-out queries served")
     ap.add_argument("--sur-steps", type=int, default=60000, help="surrogate: Adam steps (batch 4096)")
     ap.add_argument("--sur-lr", type=float, default=2e-3, help="surrogate: peak learning rate")
+    ap.add_argument("--sur-save", default="", help="surrogate: write the trained weights to this .npz")
+    ap.add_argument("--sur-weights", default=SUR_WEIGHTS,
+                    help="weights the served line of the default run loads (written by --workload surrogate "
+                         "--sur-save)")
     ap.add_argument("--sur-speed-only", action="store_true",
                     help="surrogate: time the kernels on synthetic inputs only (no FDTD data, no accuracy)")
     ap.add_argument("--c3-seconds", type=float, default=1.0, help="config 3: simulated seconds per model")
@@ -847,7 +1041,24 @@ def main() -> int:
     import numpy as np
 
     if args.workload == "c3":
-        return bench_c3(args, fd)
+        out = bench_c3(args, fd)
+        if args.also:
+            shots_args = argparse.Namespace(**vars(args))
+            shots_args.steps, shots_args.warmup = args.also_shots, 1
+            _, _, also = measure_shot(shots_args, fd, args.also)
+            out["also"] = {"workload": args.also, **also}
+        if args.sur_weights and os.path.exists(args.sur_weights):
+            out["surrogate"] = bench_served(args, fd)
+            out["surrogate"]["served_vs_batched_fdtd_per_trace"] = out["surrogate"]["value"] / out["value"]
+            out["surrogate"]["inference_vs_batched_fdtd_per_trace"] = (
+                out["surrogate"]["inference_traces_per_s"] / out["value"])
+        if args.fk:
+            out["fk"] = bench_fk(args, fd, standalone=False)
+            out["bank_lookup"] = bench_bank_lookup(args, fd)
+            out["stream"] = bench_stream(args, fd)
+            out["engine"] = bench_engine(args, fd)
+        print(json.dumps(out), flush=True)
+        return 0
     if args.workload == "surrogate":
         return bench_surrogate(args, fd)
     if args.workload == "fk":
@@ -869,64 +1080,7 @@ def main() -> int:
                           **res}), flush=True)
         return 0
 
-    def measure(name):
-        w = workload(name)
-        if args.time_steps:
-            w.steps = args.time_steps
-        dt, nr, nz = w.model.Grid()
-        log(f"workload {w.name}: {nr}x{nz}, {w.steps} steps, {len(w.ranges)} receivers")
-        sim = fd.New(w.model)
-        sim.Drive(w.force)
-        cols = [sim.Column(r)[0] for r in w.ranges]
-        sim.SetReceivers(cols, w.steps)
-        use_graph = not args.no_graph
-        step_bytes = sim.StepBytes()
-        env_rows = os.environ.get("GEOFDTD_TILE_ROWS")
-        tile_rows = int(env_rows) if env_rows else sim.Autotune(10)
-        if env_rows:
-            sim.SetTileRows(tile_rows)
-        log(f"tile rows {tile_rows}")
-        for i in range(args.warmup):
-            sim.Reset()
-            sim.Advance(w.steps, use_graph)
-            sim.SyncTiming()
-            log(f"warmup shot {i + 1}/{args.warmup} done")
-        sim.SyncTiming()
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            sim.Reset()
-            sim.Advance(w.steps, use_graph)
-        kernel_ms, launches = sim.SyncTiming()
-        wall = time.perf_counter() - t0
-        log(f"timed {args.steps} shots in {wall:.3f} s; {launches} step launches in {kernel_ms:.1f} ms of device time")
-        traces = sim.ReadTraces(len(cols))
-        sim.close()
-        if not np.all(np.isfinite(traces)):
-            raise SystemExit("non-finite traces")
-        avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
-        achieved = step_bytes / avg_launch_s / 1e9
-        roof = {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(w.name),
-            "kernel": "k_step (fused velocity+stress+memory-variable step)",
-            "tile_rows": tile_rows,
-            "algorithmic_bytes_per_launch": step_bytes,
-            "avg_launch_us": avg_launch_s * 1e6,
-        }
-        res = {
-            "value": len(w.ranges) * args.steps / wall,
-            "ms_per_step": wall * 1e3 / args.steps,
-            "config": describe(w, nr, nz, dt),
-            "roofline": roof,
-            "cells_per_s": nr * nz * w.steps * args.steps / wall,
-        }
-        return w, traces, res
-
-    w, traces, main_res = measure(args.workload)
+    w, traces, main_res = measure_shot(args, fd, args.workload)
     out = {
         "metric": METRIC,
         "value": main_res["value"],
@@ -955,7 +1109,7 @@ def main() -> int:
     else:
         out["cpu_baseline"] = None
     if args.also:
-        _, _, also = measure(args.also)
+        _, _, also = measure_shot(args, fd, args.also)
         out["also"] = {"workload": args.also, **also}
     if args.fk:
         out["fk"] = bench_fk(args, fd, standalone=False)

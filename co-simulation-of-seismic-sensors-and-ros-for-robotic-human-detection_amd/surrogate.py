@@ -315,6 +315,42 @@ def shots_for(soils: Sequence, seconds: float = SECONDS) -> List[fd.Shot]:
     return out
 
 
+def receiver_ranges(soils: Sequence) -> np.ndarray:
+    """[S][R]: the ranges the bank grid's receivers sit at in each soil's
+    grid, as the batch snaps them (Sim.Column, internal/fdtd/fdtd.go:142-146:
+    column round(r / h), half away from zero, clamped to [0, nr - 2]), without
+    solving anything."""
+    i = np.arange(RANGES[2], dtype=np.float64)
+    grid = RANGES[0] + (RANGES[1] - RANGES[0]) * i / (RANGES[2] - 1)   # the bank's range grid, as bank.cpp forms it
+    out = np.empty((len(soils), len(grid)))
+    for i, s in enumerate(soils):
+        nr = shots_for([s], seconds=1e-3)[0].Model.Grid()[1]
+        c = np.clip(np.floor(grid / s.Spacing + 0.5), 0, nr - 2)
+        out[i] = c * s.Spacing
+    return out
+
+
+def save_weights(path: str, net: "Network", norm_limit: float, meta: dict) -> None:
+    """The network's FP32 master weights, its magnitude-guard limit and a JSON
+    record of how they were trained and how they score."""
+    import json
+    import os
+
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez_compressed(path, params=net.Params(), norm_limit=np.float64(norm_limit),
+                        meta=np.array(json.dumps(meta)))
+
+
+def load_weights(path: str, max_batch: int = 8192):
+    """(Network with the saved weights, norm_limit, meta dict)."""
+    import json
+
+    z = np.load(path, allow_pickle=False)
+    net = Network(max_batch=max_batch, seed=0)
+    net.SetParams(z["params"])
+    return net, float(z["norm_limit"]), json.loads(str(z["meta"]))
+
+
 def solve(soils: Sequence, seconds: float = SECONDS, chunk: int = 256,
           progress: Optional[Callable[[int, int, float], None]] = None):
     """Batched FDTD banks for soils, `chunk` soils per batch: (spectra

@@ -241,3 +241,78 @@ def test_trained_network_gains_over_fdtd_only(S):
     assert np.median(after) <= 0.6 * np.median(base)
     assert np.median(after) < np.median(untrained)
     assert fdtd_per_trace / sur_per_trace >= 100
+
+
+@pytest.fixture(scope="module")
+def held_out(S):
+    """165 held-out soils of the layered family (8,250 queries: one inference
+    batch of 8,192 and a ragged one) with their FDTD banks."""
+    return S.solve_set(S.soil_set(0, 2_000_000, 165), seconds=1.0)
+
+
+def test_receiver_ranges_equal_the_batch_snapping(S, held_out):
+    """The host's snapped receiver ranges (what a query names before any
+    solve) equal the ranges the batched FDTD recorded at."""
+    np.testing.assert_array_equal(S.receiver_ranges(held_out.soils), held_out.ranges)
+
+
+def test_served_request_with_degraded_weights_falls_back_to_the_fdtd(S, held_out):
+    """Config 5's service on the device with the real batched-FDTD re-solve:
+    an untrained network (degraded weights) fails the 1 % sample, so the
+    request escalates and every soil is answered by its FDTD solution; what
+    is served then equals the soils' FDTD banks (batch = single, bit for
+    bit), so its error against the FDTD is zero at p50 and p99."""
+    h = held_out
+    net = S.Network(max_batch=8192, seed=3)
+    try:
+        served, rep = S.serve(net, h.params, h.ranges, h.soils, gate_fraction=0.01, bound=0.02, seed=1)
+        st = net.Stats()
+    finally:
+        net.close()
+    truth = S.band_limit(h.spectra)
+    err = S.rel_l2(served.reshape(-1, S.BINS), truth.reshape(-1, S.BINS))
+    print(f"degraded weights: {len(rep.sampled_soils)} sampled, {len(rep.failed_soils)} failed (sampled rel-L2 "
+          f"max {rep.sampled_errors.max():.2f}), escalated {rep.escalated}, {len(rep.fdtd_soils)} of {rep.soils} "
+          f"soils from the FDTD; served rel-L2 p50 {np.median(err):.1e} p99 {np.quantile(err, 0.99):.1e}; "
+          f"inference {st['forward_queries']} queries", flush=True)
+    assert len(rep.sampled_soils) == 2 and rep.failed_soils and rep.escalated
+    assert rep.fdtd_soils == list(range(165))
+    assert st["forward_queries"] >= 8250
+    np.testing.assert_array_equal(served, truth)
+
+
+def test_served_request_within_the_bound_keeps_the_predictions(S, held_out):
+    """The same request with predictions within the bound (the FDTD banks
+    perturbed by 1 % in amplitude and 0.005 rad in phase, standing in for a
+    network inside its bound): the sampled soils are re-solved by the
+    batched FDTD and pass, nothing escalates, and the predictions are served."""
+    h = held_out
+    rng = np.random.default_rng(4)
+    truth = S.band_limit(h.spectra)
+    pert = (1 + 0.01 * rng.uniform(-1, 1, truth.shape)) * np.exp(0.005j * rng.normal(size=truth.shape))
+    pred = (truth * pert).astype(np.complex64)
+    served, rep = S.serve(None, h.params, h.ranges, h.soils, gate_fraction=0.01, bound=0.02, seed=1,
+                          predictions=pred.reshape(-1, S.BINS))
+    print(f"within bound: sampled {rep.sampled_soils}, rel-L2 max {rep.sampled_errors.max():.4f}, "
+          f"vouches for < {rep.vouch_bound():.2f} failing soils at 95 %", flush=True)
+    assert not rep.failed_soils and not rep.escalated and rep.fdtd_soils == []
+    assert 0.004 < rep.sampled_errors.max() < 0.02
+    np.testing.assert_array_equal(served, pred)
+
+
+def test_fused_inference_equals_the_per_layer_gemms(S, monkeypatch):
+    """The fused encoder kernel (seven layers in one launch, activations in
+    LDS, fragment-packed weights) gives the per-layer GEMMs' bits: the same
+    MFMA k-step order, the same bias + GELU epilogue and BF16 rounding.
+    8,200 queries: a full batch of 8,192 and a ragged one of 8."""
+    rng = np.random.default_rng(21)
+    net = S.Network(max_batch=8192, seed=7)
+    try:
+        x = rng.uniform(-1, 1, (8200, S.FEATURES)).astype(np.float32)
+        fused = net.Forward(x)
+        monkeypatch.setenv("GEOFDTD_SUR_UNFUSED", "1")
+        per_layer = net.Forward(x)
+    finally:
+        net.close()
+    print(f"fused vs per-layer: max |diff| {np.max(np.abs(fused - per_layer)):.3e}", flush=True)
+    np.testing.assert_array_equal(fused, per_layer)
