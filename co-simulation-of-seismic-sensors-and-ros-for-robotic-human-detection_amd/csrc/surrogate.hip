@@ -527,4 +527,136 @@ hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale,
   return hipGetLastError();
 }
 
+// ------------------------------------------------------- fused inference
+
+__global__ void k_pack_fragments(const uint16_t *__restrict__ w, int32_t n, int32_t k, uint4 *__restrict__ out) {
+  const int ks_n = k / 16;
+  const int64_t total = (int64_t)(n / 32) * ks_n * 64;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(i & 63);
+    const int64_t t = i >> 6;
+    const int ks = (int)(t % ks_n), nt = (int)(t / ks_n);
+    const int col = nt * 32 + (lane & 31), kk = ks * 16 + (lane >> 5) * 8;
+    out[i] = *(const uint4 *)(w + (int64_t)col * k + kk);
+  }
+}
+
+hipError_t launch_pack_fragments(const uint16_t *w, int32_t n, int32_t k, uint16_t *out, hipStream_t st) {
+  if (n % 32 || k % 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pack_fragments, grid_for((int64_t)n / 32 * (k / 16) * 64), dim3(256), 0, st, w, n, k,
+                     (uint4 *)out);
+  return hipGetLastError();
+}
+
+constexpr int kFusedRows = 32;        // queries per workgroup: one MFMA row tile
+constexpr int kFusedWaves = 8;        // wavefronts per workgroup, splitting the output columns
+constexpr int kFusedPitch = 512 + 8;  // LDS row pitch of the activations (BF16 elements)
+
+// One layer of the fused chain: out = GELU(in W^T + bias) for the block's 32
+// rows.  Each wavefront owns N / 32 / 8 consecutive 32-column tiles, taken
+// two at a time; per k-step it reads its A fragment (row lane%32, k-slice
+// 8*(lane/32)) from LDS and each tile's packed B fragment from global memory.
+// The B fragments come in chunks of 8 k-steps, double-buffered in registers:
+// the next chunk's loads are in flight while this chunk's MFMAs run.
+template <int KS, int KC, int TN>
+__device__ __forceinline__ void fetch_chunk(uint4 (&buf)[TN][KC], const uint4 *__restrict__ wp, int nt0, int c,
+                                            int lane) {
+#pragma unroll
+  for (int t = 0; t < TN; t++)
+#pragma unroll
+    for (int s = 0; s < KC; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * KS + c * KC + s) * 64 + lane];
+}
+
+template <int KC, int TN>
+__device__ __forceinline__ void mfma_chunk(f32x16 (&acc)[TN], const uint4 (&buf)[TN][KC], const uint16_t *act_in,
+                                           int c, int lane) {
+#pragma unroll
+  for (int s = 0; s < KC; s++) {
+    const bf16x8 fa = *(const bf16x8 *)(act_in + (lane & 31) * kFusedPitch + (c * KC + s) * 16 + (lane >> 5) * 8);
+#pragma unroll
+    for (int t = 0; t < TN; t++) {
+      bf16x8 fb;
+      __builtin_memcpy(&fb, &buf[t][s], 16);
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc[t], 0, 0, 0);
+    }
+  }
+}
+
+template <int K, int N, bool GLOBAL_OUT>
+__device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *act_out, const uint4 *__restrict__ wp,
+                                            const float *__restrict__ bias, uint16_t *__restrict__ gout, int m0, int M,
+                                            int lane, int wave) {
+  constexpr int KS = K / 16, NT = N / 32, TPW = NT / kFusedWaves, TN = 2;
+  constexpr int KC = KS < 8 ? KS : 8, NC = KS / KC;
+  static_assert(TPW % TN == 0 && KS % KC == 0 && (NC == 1 || NC % 2 == 0), "whole tile pairs and chunk pairs");
+#pragma unroll 1
+  for (int g = 0; g < TPW; g += TN) {
+    const int nt0 = wave * TPW + g;
+    f32x16 acc[TN];
+#pragma unroll
+    for (int t = 0; t < TN; t++)
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
+    uint4 b0[TN][KC], b1[TN][KC];
+    fetch_chunk<KS, KC, TN>(b0, wp, nt0, 0, lane);
+#pragma unroll 1
+    for (int c = 0; c < NC; c += 2) {
+      if (c + 1 < NC) fetch_chunk<KS, KC, TN>(b1, wp, nt0, c + 1, lane);
+      mfma_chunk<KC, TN>(acc, b0, act_in, c, lane);
+      if (c + 1 < NC) {
+        if (c + 2 < NC) fetch_chunk<KS, KC, TN>(b0, wp, nt0, c + 2, lane);
+        mfma_chunk<KC, TN>(acc, b1, act_in, c + 1, lane);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TN; t++) {
+      const int col = (nt0 + t) * 32 + (lane & 31);
+      const float bv = bias[col];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
+        const uint16_t y = to_bf16(gelu(acc[t][i] + bv));
+        if (GLOBAL_OUT) {
+          if (m0 + row < M) gout[(int64_t)(m0 + row) * N + col] = y;
+        } else {
+          act_out[row * kFusedPitch + col] = y;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[2][kFusedRows * kFusedPitch];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * kFusedRows;
+  // the block's features (16 BF16 per row; rows past M read as zero)
+  for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
+    const int row = v >> 1, half = v & 1;
+    uint4 val = make_uint4(0, 0, 0, 0);
+    if (m0 + row < a.M) val = *(const uint4 *)(a.x0 + (int64_t)(m0 + row) * 16 + half * 8);
+    *(uint4 *)(&act[0][row * kFusedPitch + half * 8]) = val;
+  }
+  __syncthreads();
+  fused_layer<16, 512, false>(act[0], act[1], (const uint4 *)a.wp[0], a.bias[0], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[1], a.bias[1], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 512, false>(act[0], act[1], (const uint4 *)a.wp[2], a.bias[2], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[3], a.bias[3], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 512, false>(act[0], act[1], (const uint4 *)a.wp[4], a.bias[4], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[5], a.bias[5], nullptr, m0, a.M, lane, wave);
+  __syncthreads();
+  fused_layer<512, 2048, true>(act[0], nullptr, (const uint4 *)a.wp[6], a.bias[6], a.out, m0, a.M, lane, wave);
+}
+
+hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st) {
+  if (a.M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_encoder_fused, dim3((a.M + kFusedRows - 1) / kFusedRows), dim3(kFusedWaves * 64), 0, st, a);
+  return hipGetLastError();
+}
+
 }  // namespace gfsk

@@ -46,6 +46,7 @@ namespace {
 constexpr int kFeatures = 15;
 constexpr int kInPad = 16;
 constexpr int kOut = 1024;  // 512 bins x (re, im)
+constexpr int kFusedLayers = 7;  // encoder (6) + the decoder's dense layer, fused at inference
 
 struct Layer {
   int k, n, gelu, rows, bias_mod;  // rows: GEMM rows per query
@@ -114,6 +115,11 @@ struct gf_sur {
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;
   uint16_t *pbt = nullptr;  // each layer's BF16 weights transposed, [K][N]
+  // the first kFusedLayers layers' BF16 weights in MFMA fragment order, for
+  // the fused inference kernel; rebuilt after the weights change
+  uint16_t *wpk = nullptr;
+  int64_t wpk_off[8] = {};
+  bool packed = false;
   int64_t adam_t = 0;
   // activations: X[l] is the BF16 input of layer l (X[L] the last GELU output
   // is not needed); Z[l] the FP32 pre-activation (the output for the last)
@@ -131,7 +137,7 @@ struct gf_sur {
 
   ~gf_sur() {
     hipStreamSynchronize(st);
-    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)dF[0], (void *)dF[1],
+    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)wpk, (void *)dF[0], (void *)dF[1],
                     (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)feat, (void *)tgt, (void *)wgt,
                     (void *)loss, (void *)scale, (void *)delay, (void *)outp})
       if (q) hipFree(q);
@@ -160,6 +166,12 @@ struct gf_sur {
     HIPCHK(dalloc(&v, P));
     HIPCHK(dalloc(&pb, P));
     HIPCHK(dalloc(&pbt, P));
+    int64_t wo = 0;
+    for (int l = 0; l < kFusedLayers; l++) {
+      wpk_off[l] = wo;
+      wo += (int64_t)L[l].n * L[l].k;
+    }
+    HIPCHK(dalloc(&wpk, wo));
     const int64_t B = max_batch;
     X.assign(L.size(), nullptr);
     Z.assign(L.size(), nullptr);
@@ -232,13 +244,46 @@ struct gf_sur {
   // pbt[layer] = pb[layer]^T, the B operand of the input-gradient GEMMs
   int transpose_weights() {
     for (const Layer &y : L) HIPCHK(gfsk::launch_transpose_bf16(pb + y.w_off, y.n, y.k, pbt + y.w_off, st));
+    packed = false;
     return GF_OK;
+  }
+
+  // Inference: the encoder and the decoder's dense layer in one launch
+  // (gfsk::launch_encoder_fused), then the two transposed convolutions.
+  // GEOFDTD_SUR_UNFUSED=1 runs every layer as its own GEMM instead.
+  static bool fused_enabled() {
+    const char *e = getenv("GEOFDTD_SUR_UNFUSED");
+    return !(e && atoi(e) != 0);
+  }
+
+  int forward_fused(int B) {
+    if (!packed) {
+      for (int l = 0; l < kFusedLayers; l++)
+        HIPCHK(gfsk::launch_pack_fragments(pb + L[l].w_off, L[l].n, L[l].k, wpk + wpk_off[l], st));
+      packed = true;
+    }
+    gfsk::FusedArgs a;
+    a.x0 = X[0];
+    a.M = B;
+    for (int l = 0; l < kFusedLayers; l++) {
+      a.wp[l] = wpk + wpk_off[l];
+      a.bias[l] = p + L[l].b_off;
+    }
+    a.out = X[kFusedLayers];
+    HIPCHK(gfsk::launch_encoder_fused(a, st));
+    return forward_from(kFusedLayers, B, false);
   }
 
   // Forward over B queries whose padded BF16 features are in X[0].  keep:
   // store pre-activations for a backward pass.  Output: Z.back() [B][1024].
   int forward(int B, bool keep) {
-    for (size_t l = 0; l < L.size(); l++) {
+    if (!keep && fused_enabled()) return forward_fused(B);
+    return forward_from(0, B, keep);
+  }
+
+  // Layers first.. as one GEMM each.
+  int forward_from(size_t first, int B, bool keep) {
+    for (size_t l = first; l < L.size(); l++) {
       const Layer &y = L[l];
       gfsk::GemmArgs a;
       a.M = B * y.rows;

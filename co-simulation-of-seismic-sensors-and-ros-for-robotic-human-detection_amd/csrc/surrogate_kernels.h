@@ -58,4 +58,26 @@ hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_c
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
                               int32_t k_lo, int32_t k_hi, int32_t polar, float *out, int32_t rows, hipStream_t st);
 
+// Inference without per-layer launches.  A layer's BF16 weights W[N][K]
+// (N a multiple of 32, K of 16) packed in MFMA fragment order: 16-byte
+// element (nt * K/16 + ks) * 64 + lane holds W[nt*32 + lane%32][ks*16 +
+// 8*(lane/32) .. +7], so a wavefront's B operand for one 32-column tile and
+// one 16-deep k-step is one contiguous 1 KB load.
+hipError_t launch_pack_fragments(const uint16_t *w, int32_t n, int32_t k, uint16_t *out, hipStream_t st);
+
+// The encoder and the decoder's dense layer in one kernel: for each block of
+// 32 queries, x0 [M][16] (BF16 features) -> GELU(x W0^T + b0) -> five
+// 512-wide GELU layers -> GELU(h W6^T + b6) [M][2048] into out (BF16).
+// The block's activations stay in LDS between layers; the packed weights
+// stream from L2.  Same arithmetic as the per-layer GEMMs (k-steps in order,
+// bias and GELU in FP32, one BF16 rounding per output).
+struct FusedArgs {
+  const uint16_t *x0 = nullptr;
+  int32_t M = 0;
+  const uint16_t *wp[7] = {};
+  const float *bias[7] = {};
+  uint16_t *out = nullptr;
+};
+hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st);
+
 }  // namespace gfsk
