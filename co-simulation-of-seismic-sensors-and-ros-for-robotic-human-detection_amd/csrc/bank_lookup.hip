@@ -14,9 +14,9 @@
 //   k_bank_spectra  a workgroup per query range (more when the queries are
 //                   few): the bracket of the query range, then for each bin
 //                   two rows of each table read at consecutive bins and one
-//                   complex value written.  Queries
-//                   run in range order, so table rows are fetched once per
-//                   L2 rather than once per query.
+//                   complex value written.  Queries run in range order and
+//                   each XCD takes one contiguous stretch of them, so a table
+//                   row is fetched by about one L2 rather than by every one.
 // A value reads 32 B of tables (mostly from L2 once queries run in range
 // order) and writes 16 B; the writes bound it.
 //
@@ -24,12 +24,13 @@
 // zero, cmplx.Rect as r cos + i r sin; log and atan2 are the device
 // library's, exp and sincos on lookup those of fastmath.h.
 #include <hip/hip_runtime.h>
-#include <hipfft/hipfft.h>
 #include <math.h>
+#include <rocfft/rocfft.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,7 @@ struct gf_bank_dev {
   double *q_r = nullptr;
   int32_t *q_bin = nullptr;
   double2 *out = nullptr;
+  double2 *sensor = nullptr;  // a caller's sensor response per bin (impulse responses)
   size_t q_cap = 0, out_cap = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -171,15 +173,27 @@ __global__ void k_bank_points(const double *__restrict__ r, const int32_t *__res
 // A thread walks the bins of its query with a stride of the workgroups
 // along x times 256, so the bracket and the coefficients held in scalar
 // registers are set up once for several values.
+// Workgroups are dealt to the 8 XCDs round-robin by launch order.  The
+// workgroup that launches b-th takes the b-th of each XCD's contiguous run of
+// (bin slice, query) pairs, so an XCD serves one stretch of the range-sorted
+// queries and its L2 fetches about 1/8 of the tables instead of all of them.
+__device__ __forceinline__ int xcd_run(int b, int nb) {
+  const int x = b & 7, j = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + j;
+}
+
 __global__ void k_bank_spectra(const double *__restrict__ r, const int32_t *__restrict__ dest, int32_t nq,
                                double r_min, double r_max, int32_t count, int32_t bins,
                                const double *__restrict__ logmag, const double *__restrict__ phase,
                                double2 *__restrict__ out) {
-  const int q = blockIdx.y;
+  const int nb = (int)(gridDim.x * gridDim.y);
+  const int w = xcd_run((int)(blockIdx.x + blockIdx.y * gridDim.x), nb);
+  const int q = w / (int)gridDim.x, bx = w % (int)gridDim.x;
   if (q >= nq) return;
   const Bracket b = bracket(r[q], r_min, r_max, count);
   double2 *row = out + (int64_t)dest[q] * bins;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < bins; k += gridDim.x * blockDim.x)
+  for (int k = bx * blockDim.x + threadIdx.x; k < bins; k += gridDim.x * blockDim.x)
     row[k] = interpolate(b, k, count, bins, logmag, phase);
 }
 
@@ -264,8 +278,8 @@ __global__ void k_irdft_first(const double2 *__restrict__ coeff, int32_t bins, i
   out[(int64_t)q * length + t] = s * (1 / (double)nfft);
 }
 
-// Inverse transforms longer than kTwiddleMax go through rocFFT (hipFFT's
-// double-precision complex-to-real transform, O(N log N)) instead of the
+// Inverse transforms longer than kTwiddleMax go through rocFFT (its
+// double-precision real inverse transform, O(N log N)) instead of the
 // direct sum: the imaginary parts of bins 0 and N/2 are cleared first (the
 // direct sum and dsp.IRFFT ignore them), and the first `length` samples of
 // each row are kept, scaled by 1/N.
@@ -330,6 +344,7 @@ void release(gf_bank_dev *b) {
   hipFree(b->q_r);
   hipFree(b->q_bin);
   hipFree(b->out);
+  hipFree(b->sensor);
   if (b->e0) hipEventDestroy(b->e0);
   if (b->e1) hipEventDestroy(b->e1);
   if (b->st) hipStreamDestroy(b->st);
@@ -408,23 +423,48 @@ hipError_t irdft_launch(double2 *d_coeff, int32_t bins, int32_t n, int32_t lengt
   k_real_ends<<<(n + 255) / 256, 256, 0, st>>>(d_coeff, bins, n);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int32_t batch = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n, (256LL << 20) / ((int64_t)nfft * 8)));
+  static std::once_flag setup;
+  std::call_once(setup, [] { rocfft_setup(); });
   double *full = nullptr;
+  void *work = nullptr;
+  size_t work_bytes = 0;
   if ((e = hipMalloc(&full, sizeof(double) * (size_t)batch * nfft)) != hipSuccess) return e;
-  hipfftHandle plan = 0;
+  rocfft_plan plan = nullptr;
+  rocfft_execution_info info = nullptr;
   int32_t planned = 0;
+  const size_t len = (size_t)nfft;
+  if (rocfft_execution_info_create(&info) != rocfft_status_success ||
+      rocfft_execution_info_set_stream(info, st) != rocfft_status_success)
+    e = hipErrorUnknown;
   for (int32_t j0 = 0; j0 < n && e == hipSuccess; j0 += batch) {
     const int32_t nb = std::min(batch, n - j0);
     if (nb != planned) {
-      if (planned) hipfftDestroy(plan);
+      if (planned) rocfft_plan_destroy(plan);
       planned = 0;
-      if (hipfftPlan1d(&plan, nfft, HIPFFT_Z2D, nb) != HIPFFT_SUCCESS || hipfftSetStream(plan, st) != HIPFFT_SUCCESS) {
+      size_t need = 0;
+      if (rocfft_plan_create(&plan, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                             rocfft_precision_double, 1, &len, (size_t)nb, nullptr) != rocfft_status_success ||
+          rocfft_plan_get_work_buffer_size(plan, &need) != rocfft_status_success) {
         e = hipErrorUnknown;
         break;
       }
       planned = nb;
+      if (need > work_bytes) {
+        // the previous transform may still read the old work buffer
+        if (work && (e = hipStreamSynchronize(st)) != hipSuccess) break;
+        hipFree(work);
+        work = nullptr;
+        if ((e = hipMalloc(&work, need)) != hipSuccess) break;
+        work_bytes = need;
+      }
+      if (need && rocfft_execution_info_set_work_buffer(info, work, work_bytes) != rocfft_status_success) {
+        e = hipErrorUnknown;
+        break;
+      }
     }
-    if (hipfftExecZ2D(plan, (hipfftDoubleComplex *)(d_coeff + (size_t)j0 * bins), (hipfftDoubleReal *)full) !=
-        HIPFFT_SUCCESS) {
+    void *in[1] = {d_coeff + (size_t)j0 * bins};
+    void *out[1] = {full};
+    if (rocfft_execute(plan, in, out, info) != rocfft_status_success) {
       e = hipErrorUnknown;
       break;
     }
@@ -433,7 +473,9 @@ hipError_t irdft_launch(double2 *d_coeff, int32_t bins, int32_t n, int32_t lengt
     e = hipGetLastError();
   }
   const hipError_t s2 = hipStreamSynchronize(st);
-  if (planned) hipfftDestroy(plan);
+  if (planned) rocfft_plan_destroy(plan);
+  if (info) rocfft_execution_info_destroy(info);
+  hipFree(work);
   hipFree(full);
   return e != hipSuccess ? e : s2;
 }
@@ -466,7 +508,8 @@ int gf_bank_dev_open(const gf_bank_header *h, const float *payload, gf_bank_dev 
     return bail(e, "hipEventCreate");
   if ((e = hipMalloc(&b->data, sizeof(float2) * n)) != hipSuccess ||
       (e = hipMalloc(&b->logmag, sizeof(double) * n)) != hipSuccess ||
-      (e = hipMalloc(&b->phase, sizeof(double) * n)) != hipSuccess)
+      (e = hipMalloc(&b->phase, sizeof(double) * n)) != hipSuccess ||
+      (e = hipMalloc(&b->sensor, sizeof(double2) * b->bins)) != hipSuccess)
     return bail(e, "hipMalloc");
   if ((e = hipMemcpyAsync(b->data, payload, sizeof(float2) * n, hipMemcpyHostToDevice, b->st)) != hipSuccess)
     return bail(e, "hipMemcpyAsync");
@@ -554,7 +597,7 @@ int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t le
   double *trace = (double *)(b->out + ncoef);
   double2 *d_sensor = nullptr;
   if (sensor) {
-    HIPCHK(hipMalloc(&d_sensor, sizeof(double2) * b->bins));
+    d_sensor = b->sensor;
     HIPCHK(hipMemcpyAsync(d_sensor, sensor, sizeof(double2) * b->bins, hipMemcpyHostToDevice, b->st));
   }
   HIPCHK(hipMemcpyAsync(b->q_r, r_m, sizeof(double) * n, hipMemcpyHostToDevice, b->st));
@@ -563,7 +606,6 @@ int gf_bank_dev_impulse(gf_bank_dev *b, const double *r_m, int32_t n, int32_t le
   HIPCHK(hipEventRecord(b->e1, b->st));
   HIPCHK(hipMemcpyAsync(out, trace, sizeof(double) * (size_t)n * length, hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
-  if (d_sensor) HIPCHK(hipFree(d_sensor));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
   b->last_ms = ms;

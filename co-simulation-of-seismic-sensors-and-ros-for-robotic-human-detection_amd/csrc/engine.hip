@@ -690,7 +690,13 @@ int gf_engine_next(gf_engine *e, float *out, int32_t len) {
       }
     e->macs += taps * e->nrecv * (e->radial ? 2 : 1);
   }
-  // retire (sensing.go:332-341)
+  // retire (sensing.go:332-341).  A slot goes back on the free list only once
+  // the stream has finished the admission copies out of its pinned staging:
+  // after a mix the synchronization above covers them; when nothing mixed
+  // (every live voice starts later, or a voice retires unheard) wait here.
+  bool retiring = false;
+  for (const auto &v : e->voices) retiring |= e->emitted > v.silent;
+  if (retiring && a.nv == 0) HIPCHK(hipStreamSynchronize(e->st));
   std::vector<gf_engine::Voice> kept;
   for (const auto &v : e->voices) {
     if (e->emitted <= v.silent)
