@@ -297,6 +297,8 @@ SIGNATURES = {
     "gf_sur_stats": (_I, [C.c_void_p, C.POINTER(GfSurStats)]),
     "gf_gemm_bf16": (_I, [_I32, _I32, _I32, C.POINTER(C.c_uint16), _I64, _I32, C.POINTER(C.c_uint16), _I64, _I32,
                           _I32, C.POINTER(C.c_float)]),
+    "gf_gemm_bf16_noalloc": (_I, [_I32, _I32, _I32, C.c_void_p, _I64, _I32, C.c_void_p, _I64, _I32, _I32, C.c_void_p,
+                                  C.c_void_p]),
     "gf_fk_surface_response": (_I, [C.POINTER(GfMedium), _PD, _PD, _I64, _PD]),
     "gf_fk_static_coefficient": (_I, [C.POINTER(GfMedium), _D, _D, _PD]),
     "gf_fk_grid_for": (_I, [C.POINTER(GfMedium), _PD, _I32, _D, GfIntegration, C.POINTER(GfFkGrid)]),

@@ -610,6 +610,18 @@ int gf_sur_stats(const gf_sur *h, gf_sur_stats_t *out);
 int gf_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda, int32_t a_trans, const uint16_t *B,
                  int64_t ldb, int32_t b_nk, int32_t splits, float *C);
 
+/* gf_gemm_bf16 for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * reference counterpart.  A, B and C are device pointers (A and B 16-byte
+ * aligned), the work is enqueued on `stream` (a hipStream_t; NULL for the
+ * null stream) and the call returns without waiting.  The split-K partial
+ * sums use a file-scope device array of 4 Mi floats (16 MiB) that each GPU
+ * the library runs on holds until the process ends; a call needing more
+ * (ceil(K / k_split) * M * N floats) is refused with GF_ERR_ARGUMENT.  Calls
+ * that overlap on one GPU share that array and race.  Allocates nothing. */
+int gf_gemm_bf16_noalloc(int32_t M, int32_t N, int32_t K, const uint16_t *A, int64_t lda, int32_t a_trans,
+                         const uint16_t *B, int64_t ldb, int32_t b_nk, int32_t splits, float *C, void *stream);
+
 /* Philox4x32-10 block (counter, key) -> 4 words, for checking the sampler. */
 void gf_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
 
