@@ -124,23 +124,28 @@ int Sim::set_tile_rows(int th) {
   return upload_desc();
 }
 
-// Time each candidate tile height on this model, keep the fastest, and put
-// the model back to step 0 with zero state.
+// Time each candidate launch shape (tile height, tiles per workgroup) on
+// this model, keep the fastest, and put the model back to step 0 with zero
+// state.  *chosen receives the tile height.
 int Sim::autotune(int trial_steps, int *chosen) {
   const int64_t s0 = step;
-  gft::Key key{g.lossy ? g.nmech : 0, g.scalar, 1, (int64_t)log2((double)R.nr * R.nz)};
+  gft::Key key{0, g.lossy ? g.nmech : 0, g.scalar, 1, (int64_t)log2((double)R.nr * R.nz)};
   int err = GF_OK;
-  auto prepare = [&](int th) {
-    err = set_tile_rows(th);
+  auto prepare = [&](int code) {
+    waves = gft::Shape::of(code).waves;
+    err = set_tile_rows(gft::Shape::of(code).th);
     return err == GF_OK;
   };
-  auto launch = [&](int th, int n) {
-    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, th, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
-                            (int32_t)(s0 + n), nullptr, 0, stream);
+  auto launch = [&](int code, int n) {
+    const gft::Shape sh = gft::Shape::of(code);
+    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, sh.th, sh.waves, d_desc, nullptr,
+                            (int64_t)g.tiles_r * g.tiles_z, (int32_t)(s0 + n), nullptr, 0, stream);
   };
-  int best = gft::cached(key);
-  if (!best) best = gft::tune(key, stream, trial_steps, prepare, launch, nullptr);
-  if (!best) return err ? err : set_err(GF_ERR_DEVICE, "geofdtd: autotune found no working tile height");
+  int code = gft::cached(key);
+  if (!code) code = gft::tune(key, stream, trial_steps, gft::shape_candidates(), prepare, launch, nullptr, nullptr);
+  if (!code) return err ? err : set_err(GF_ERR_DEVICE, "geofdtd: autotune found no working tile height");
+  const int best = gft::Shape::of(code).th;
+  waves = gft::Shape::of(code).waves;
   int rc = set_tile_rows(best);
   if (rc) return rc;
   if ((rc = reset_state())) return rc;
@@ -316,7 +321,7 @@ int Sim::reset_max_slots(int64_t steps) {
 }
 
 int Sim::launch_one(int64_t n, bool probe) {
-  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
+  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, waves, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
                           (int32_t)n, nullptr, probe ? 1 : 0, stream));
   return GF_OK;
 }
@@ -365,7 +370,7 @@ int Sim::ensure_graph(bool probe) {
   graph = nullptr;
   HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, d_desc, nullptr,
+    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, waves, d_desc, nullptr,
                                     (int64_t)g.tiles_r * g.tiles_z, k, nbase_dev, probe ? 1 : 0, stream);
     if (e != hipSuccess) {
       hipGraph_t tmp;

@@ -17,10 +17,12 @@ struct Group {
   int nmech = 0;
   bool scalar = true;
   int th = 0;                        // tile rows shared by every model of the group
+  int waves = 4;                     // tiles (wavefronts) per workgroup
   std::vector<int> order;            // shot indices, longest run first
   std::vector<int64_t> prefix;       // tiles of the first k models
   gfk::GridDesc *d_descs = nullptr;
-  uint16_t *d_tile_model = nullptr;  // model of each tile (groups of 2+)
+  int64_t descs_cap = 0;
+  uint16_t *d_tile_model = nullptr;  // model of each tile
   int64_t tile_model_cap = 0;
   int32_t *d_nbase = nullptr;
   hipGraph_t graph = nullptr;
@@ -47,6 +49,8 @@ struct Batch {
   std::string first_msg;
   hipStream_t stream = nullptr;  // the stream the batch runs on
   int tile_rows = 0;             // tile height of the last group planned
+  int waves = 0;                 // tiles per workgroup of the last group planned
+  double budget_mb = 0;          // chunk budget the batch was cut with (0: all together)
   double kernel_ms = 0;          // step-loop device time since the last reset (HIP events)
   int64_t launches = 0;          // step kernels launched since the last reset
   double bytes = 0;              // algorithmic HBM bytes of the steps run since the last reset

@@ -67,8 +67,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + min(x, r) + j;
 }
 
-// One wavefront = one tile; four tiles per 256-thread workgroup.  Tiles are
-// numbered model by model (GridDesc::tile_base), row-major within a model.
+// One wavefront = one tile; one, two or four tiles per workgroup (the
+// launch's choice).  Tiles are numbered model by model (GridDesc::tile_base),
+// row-major within a model.
 //
 // TH > 0: the tile is TH rows tall (every model of the launch has g.th == TH)
 // and every load of the tile -- TH + 2 rows of the six fields and TH rows of
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   // Wave-uniform by construction; readfirstlane lets the compiler keep the
   // descriptor reads scalar.
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int tile = __builtin_amdgcn_readfirstlane(blk * 4 + (int)(threadIdx.x >> 6));
+  const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
 
   // A batch's tiles are numbered model by model; the host-built table gives
   // each tile's model in one load (a single model needs none).
@@ -256,38 +257,40 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
 }
 
 template <int NM, bool SC, int TH>
-static hipError_t launch_step_t(const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
+static hipError_t launch_step_t(int waves, const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
                                 const int32_t *nbase, int32_t probe, hipStream_t st) {
-  unsigned blocks = (unsigned)((tiles + 3) / 4);
-  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(256), 0, st, d, tm, n_off, nbase, probe);
+  unsigned blocks = (unsigned)((tiles + waves - 1) / waves);
+  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(64 * waves), 0, st, d, tm, n_off, nbase, probe);
   return hipGetLastError();
 }
 
 template <int NM, bool SC>
-static hipError_t launch_step_th(int th, const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
-                                 const int32_t *nbase, int32_t probe, hipStream_t st) {
+static hipError_t launch_step_th(int th, int waves, const GridDesc *d, const uint16_t *tm, int64_t tiles,
+                                 int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
   switch (th) {
-    case 1: return launch_step_t<NM, SC, 1>(d, tm, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_t<NM, SC, 2>(d, tm, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_t<NM, SC, 4>(d, tm, tiles, n_off, nbase, probe, st);
-    case 8: return launch_step_t<NM, SC, 8>(d, tm, tiles, n_off, nbase, probe, st);
-    default: return launch_step_t<NM, SC, 0>(d, tm, tiles, n_off, nbase, probe, st);
+    case 1: return launch_step_t<NM, SC, 1>(waves, d, tm, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_t<NM, SC, 2>(waves, d, tm, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_t<NM, SC, 4>(waves, d, tm, tiles, n_off, nbase, probe, st);
+    case 8: return launch_step_t<NM, SC, 8>(waves, d, tm, tiles, n_off, nbase, probe, st);
+    default: return launch_step_t<NM, SC, 0>(waves, d, tm, tiles, n_off, nbase, probe, st);
   }
 }
 
 // `th` is the tile height every model of the launch was planned with; 1, 2,
 // 4 and 8 select the fully unrolled kernels, anything else the row march.
-// `tile_model` maps each tile to its model (null: every tile is model 0).
-hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, const uint16_t *tile_model, int64_t tiles,
-                       int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
+// `waves` tiles per workgroup (1, 2 or 4).  `tile_model` maps each tile to
+// its model (null: every tile is model 0); it is padded to whole groups of 4.
+hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
+                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
   if (tiles <= 0) return hipSuccess;
-  if (nmech == 0) return launch_step_th<0, true>(th, d, tile_model, tiles, n_off, nbase, probe, st);
-  if (scalar) return launch_step_th<1, true>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+  if (waves != 1 && waves != 2 && waves != 4) return hipErrorInvalidValue;
+  if (nmech == 0) return launch_step_th<0, true>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
+  if (scalar) return launch_step_th<1, true>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
   switch (nmech) {
-    case 1: return launch_step_th<1, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_th<2, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 3: return launch_step_th<3, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_th<4, false>(th, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 1: return launch_step_th<1, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 2: return launch_step_th<2, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 3: return launch_step_th<3, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
+    case 4: return launch_step_th<4, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
     default: return hipErrorInvalidValue;
   }
 }

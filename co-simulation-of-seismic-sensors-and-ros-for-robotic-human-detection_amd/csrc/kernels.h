@@ -15,8 +15,9 @@ hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st);
 hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st);
 hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers, float *rowc, float *colc,
                                hipStream_t st);
-hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, const uint16_t *tile_model, int64_t tiles,
-                       int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st);
+// waves: wavefronts (tiles) per workgroup, 1, 2 or 4.
+hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
+                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st);
 hipError_t launch_dft_f32(const float *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,
                           const double *freqs, int32_t nbins, double *out, hipStream_t st);
 hipError_t launch_dft_f64(const double *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,

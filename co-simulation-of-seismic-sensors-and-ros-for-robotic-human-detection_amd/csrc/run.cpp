@@ -51,6 +51,7 @@ void free_group(Group &G) {
   if (G.d_tile_model) hipFree(G.d_tile_model);
   G.d_tile_model = nullptr;
   G.tile_model_cap = 0;
+  G.descs_cap = 0;
   G.exec = nullptr;
   G.graph = nullptr;
   G.d_descs = nullptr;
@@ -91,12 +92,18 @@ int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th, hipStr
     G.prefix.push_back(G.prefix.back() + (int64_t)d.tiles_r * d.tiles_z);
   }
   if (G.prefix.back() > (int64_t)INT32_MAX) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: batch has too many tiles");
-  if (!G.d_descs) {
+  if ((int64_t)descs.size() > G.descs_cap) {
+    if (G.d_descs) (void)hipFree(G.d_descs);
+    G.d_descs = nullptr;
     HIPCHK(hipMalloc(&G.d_descs, sizeof(gfk::GridDesc) * descs.size()));
-    HIPCHK(hipMalloc(&G.d_nbase, 64));
+    G.descs_cap = (int64_t)descs.size();
   }
+  if (!G.d_nbase) HIPCHK(hipMalloc(&G.d_nbase, 64));
   HIPCHK(hipMemcpy(G.d_descs, descs.data(), sizeof(gfk::GridDesc) * descs.size(), hipMemcpyHostToDevice));
-  if (descs.size() > 1) {
+  {
+    // Built for every plan, one model included: a group re-planned with
+    // fewer models (the budget tuner's trial chunks) must not keep the table
+    // of an earlier plan.
     if (descs.size() > 65535) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: too many models in one batch");
     // Padded to whole workgroups of 4 tiles: the launch's last workgroup
     // can hold up to three tiles past the end, which read the last model and
@@ -124,7 +131,8 @@ int capture(Group &G, int models, hipStream_t st) {
   if (rc) return rc;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < gfs::kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[models], k, G.d_nbase, 1, st);
+    hipError_t e =
+        gfk::launch_step(G.nmech, G.scalar, G.th, G.waves, G.d_descs, G.d_tile_model, G.prefix[models], k, G.d_nbase, 1, st);
     if (e != hipSuccess) {
       hipGraph_t tmp = nullptr;
       hipStreamEndCapture(st, &tmp);
@@ -214,16 +222,12 @@ void Batch::setup(const gf_shot *shots, int32_t n, const std::vector<std::vector
 double state_bytes(const Sim &s) { return gfs::step_bytes(s.R) / 2; }
 
 // Models of one kernel variant are advanced in chunks whose combined state fits
-// the GPU's last-level cache (256 MB Infinity Cache): a chunk runs all of its
-// steps before the next one starts, so its arrays are served from cache step
-// after step instead of streaming every model's state from HBM on every step.
-// Soils are independent, so the order changes nothing in any result.
-// GEOFDTD_CACHE_MB sets the budget; 0 advances every model together.
-double cache_budget_bytes() {
-  const char *e = getenv("GEOFDTD_CACHE_MB");
-  const double mb = e ? atof(e) : 160.0;
-  return mb * 1e6;
-}
+// the GPU's last-level cache (256 MB Infinity Cache): a chunk runs its steps
+// while its arrays are served from cache step after step instead of streaming
+// every model's state from HBM on every step.  Soils are independent, so the
+// order changes nothing in any result.  The budget per chunk is tuned on long
+// runs (candidates 64-224 MB); GEOFDTD_CACHE_MB fixes it, 0 advancing every
+// model together.
 
 Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
@@ -239,8 +243,9 @@ bool Batch::finished() const {
 }
 
 // Chunks: models of one kernel variant sorted by step count (the running set
-// is a prefix), cut where the cache budget would be exceeded; each variant's
-// tile height picked on its first chunk; every chunk's tables uploaded.
+// is a prefix), cut where the chunk budget would be exceeded; each variant's
+// launch shape (tile height, tiles per workgroup) and chunk budget picked by
+// the tuner on long runs; every chunk's tables uploaded.
 int Batch::plan() {
   if (planned) return GF_OK;
   planned = true;
@@ -271,60 +276,110 @@ int Batch::plan() {
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event set-up failed");
 
-  const double budget = cache_budget_bytes();
-  std::vector<int64_t> variant_cell_steps(variants.size(), 0), variant_cells(variants.size(), 0);
+  const char *budget_env = getenv("GEOFDTD_CACHE_MB");
+  const char *rows_env = getenv("GEOFDTD_TILE_ROWS");
+  const char *tune_env = getenv("GEOFDTD_TUNE");
+  int rc = GF_OK;
   for (size_t v = 0; v < variants.size(); v++) {
     Group &V = variants[v];
     std::stable_sort(V.order.begin(), V.order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    int64_t cells = 0, cell_steps = 0;
+    for (int i : V.order) {
+      cells += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
+      cell_steps += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
+    }
+    const bool want_tune = tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL;
+    const int64_t bucket = (int64_t)log2((double)std::max<int64_t>(1, cells));
+    // the models of the first chunk under a budget (all of them for 0)
+    auto prefix_for = [&](double mb) {
+      std::vector<int> out;
+      double acc = 0;
+      for (int i : V.order) {
+        const double b = state_bytes(*sims[i]);
+        if (!out.empty() && mb > 0 && acc + b > mb * 1e6) break;
+        out.push_back(i);
+        acc += b;
+      }
+      return out;
+    };
+    double mb = budget_env ? atof(budget_env) : 160.0;
+    Group T;  // trial group: the first chunk
+    T.nmech = V.nmech;
+    T.scalar = V.scalar;
+    T.order = prefix_for(mb);
+    int64_t cols_rows = 0;
+    for (int i : T.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
+    gft::Shape shape{gfs::choose_tile_rows_cells(cols_rows), 4};
+    std::vector<int> touched;
+    if (want_tune && !rows_env) {
+      gft::Key key{0, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
+      auto prepare = [&](int code) {
+        T.waves = gft::Shape::of(code).waves;
+        return plan_group(T, sims, gft::Shape::of(code).th, stream) == GF_OK;
+      };
+      auto launch = [&](int code, int k) {
+        const gft::Shape sh = gft::Shape::of(code);
+        return gfk::launch_step(T.nmech, T.scalar, sh.th, sh.waves, T.d_descs, T.d_tile_model, T.prefix.back(), k,
+                                nullptr, 0, stream);
+      };
+      int code = gft::cached(key);
+      if (!code) code = gft::tune(key, stream, 8, gft::shape_candidates(), prepare, launch, nullptr, nullptr);
+      if (code) shape = gft::Shape::of(code);
+      touched = T.order;
+    }
+    if (want_tune && !budget_env) {
+      // chunk budget: the first chunk under each candidate, timed at the
+      // chosen shape; the rate is its algorithmic bytes per second
+      gft::Key key{1, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
+      std::vector<int> cands(std::begin(gft::kBudgetsMB), std::end(gft::kBudgetsMB));
+      auto prepare = [&](int b) {
+        T.order = prefix_for(b);
+        if (T.order.size() > touched.size()) touched = T.order;
+        T.waves = shape.waves;
+        return plan_group(T, sims, shape.th, stream) == GF_OK;
+      };
+      auto launch = [&](int, int k) {
+        return gfk::launch_step(T.nmech, T.scalar, T.th, T.waves, T.d_descs, T.d_tile_model, T.prefix.back(), k,
+                                nullptr, 0, stream);
+      };
+      auto work = [&](int b) {
+        double bytes = 0;
+        for (int i : prefix_for(b)) bytes += gfs::step_bytes(sims[i]->R);
+        return bytes;
+      };
+      int best = gft::cached(key);
+      if (!best) best = gft::tune(key, stream, 24, cands, prepare, launch, work, nullptr);
+      if (best) mb = best;
+    }
+    if ((rc = drop_graph(T, stream))) return rc;
+    free_group(T);
+    for (int i : touched)
+      if ((rc = sims[i]->reset_state())) return rc;
+    // cut the variant into chunks under the budget
     double acc = 0;
+    bool first = true;
     for (int i : V.order) {
       const double b = state_bytes(*sims[i]);
-      if (chunks.empty() || chunks.back().variant != (int)v || (budget > 0 && acc + b > budget)) {
+      if (first || (mb > 0 && acc + b > mb * 1e6)) {
         chunks.emplace_back();
         chunks.back().variant = (int)v;
         chunks.back().G.nmech = V.nmech;
         chunks.back().G.scalar = V.scalar;
+        chunks.back().G.waves = shape.waves;
         acc = 0;
+        first = false;
       }
       chunks.back().G.order.push_back(i);
       acc += b;
-      variant_cells[v] += (int64_t)sims[i]->g.nr * sims[i]->g.nz;
-      variant_cell_steps[v] += (int64_t)sims[i]->g.nr * sims[i]->g.nz * steps[i];
     }
-  }
-
-  std::vector<int> variant_th(variants.size(), 0);  // tile height chosen for each variant
-  for (Chunk &C : chunks) {
-    Group &G = C.G;
-    C.max_steps = steps[G.order.front()];
-    int64_t cols_rows = 0;
-    for (int i : G.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
-    int th = variant_th[C.variant];
-    int rc = plan_group(G, sims, th ? th : gfs::choose_tile_rows_cells(cols_rows), stream);
-    if (rc) return rc;
-    if (!th) {
-      // Long runs pay for a few trial steps at each tile height, once per
-      // variant, on its first chunk; the trial steps are then undone.
-      const char *tune_env = getenv("GEOFDTD_TUNE");
-      const bool want_tune = !getenv("GEOFDTD_TILE_ROWS") &&
-                             (tune_env ? atoi(tune_env) != 0 : variant_cell_steps[C.variant] >= 5000000000LL);
-      if (want_tune) {
-        gft::Key key{G.nmech, G.scalar ? 1 : 0, (int)G.order.size(),
-                     (int64_t)log2((double)std::max<int64_t>(1, variant_cells[C.variant]))};
-        auto prepare = [&](int t) { return plan_group(G, sims, t, stream) == GF_OK; };
-        auto launch = [&](int t, int k) {
-          return gfk::launch_step(G.nmech, G.scalar, t, G.d_descs, G.d_tile_model, G.prefix.back(), k, nullptr, 0,
-                                  stream);
-        };
-        int best = gft::cached(key);
-        if (!best) best = gft::tune(key, stream, 8, prepare, launch, nullptr);
-        if (best && (rc = plan_group(G, sims, best, stream))) return rc;
-        for (int i : G.order)
-          if ((rc = sims[i]->reset_state())) return rc;
-      }
-      variant_th[C.variant] = G.th;
+    for (Chunk &C : chunks) {
+      if (C.variant != (int)v) continue;
+      C.max_steps = steps[C.G.order.front()];
+      if ((rc = plan_group(C.G, sims, shape.th, stream))) return rc;
     }
-    tile_rows = G.th;
+    tile_rows = shape.th;
+    waves = shape.waves;
+    budget_mb = mb;
   }
   for (auto &sp : sims)
     if (sp) {
@@ -367,8 +422,8 @@ int Batch::advance(int64_t num, int64_t den) {
         if (e != hipSuccess) return gfs::hip_fail(e, "graph launch");
         k += gfs::kGraphSteps;
       } else {
-        hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.d_descs, G.d_tile_model, G.prefix[act], (int32_t)k,
-                                        nullptr, 1, stream);
+        hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.waves, G.d_descs, G.d_tile_model, G.prefix[act],
+                                        (int32_t)k, nullptr, 1, stream);
         if (e != hipSuccess) return gfs::hip_fail(e, "step launch");
         k += 1;
       }
@@ -421,6 +476,8 @@ void record_stats(const Batch &b) {
   g_stats.step_launches = b.launches;
   g_stats.algorithmic_bytes = b.bytes;
   g_stats.tile_rows = b.tile_rows;
+  g_stats.waves = b.waves;
+  g_stats.budget_mb = b.budget_mb;
   int models = 0;
   for (auto &s : b.sims) models += s ? 1 : 0;
   g_stats.models = models;

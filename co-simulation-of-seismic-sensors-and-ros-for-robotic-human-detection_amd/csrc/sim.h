@@ -44,6 +44,7 @@ inline double step_bytes(const gfm::Resolved &R) {
 struct Sim {
   gfm::Resolved R;
   gfk::GridDesc g;  // host copy of the device descriptor
+  int waves = 4;    // tiles (wavefronts) per step-kernel workgroup
   gfk::GridDesc *d_desc = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

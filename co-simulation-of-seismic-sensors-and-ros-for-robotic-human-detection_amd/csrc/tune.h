@@ -1,36 +1,54 @@
-// tune.h — the tile-shape autotuner.
+// tune.h — the launch-shape autotuner.
 //
 // The step kernel's speed depends on the tile height mostly through how many
 // loads each SIMD keeps in flight, which depends on the grid sizes, the
-// number of models in a launch and the attenuation variant (register use).
-// Rather than model that, the tuner times the real launch: a few steps at
-// each candidate height on the caller's own models, keeping the fastest.
-// Results are cached in-process by (variant, models, grid-size bucket).
+// number of models in a launch and the attenuation variant (register use);
+// the tiles per workgroup change how the waves of a CU are scheduled; and a
+// batch's chunk size decides whether its state stays in the last-level cache.
+// Rather than model that, the tuner times the real launches: a few steps of
+// each candidate on the caller's own models, keeping the fastest.  Results
+// are cached in-process by (what is tuned, variant, models, grid-size bucket).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #include <functional>
+#include <vector>
 
 namespace gft {
 
-// Candidate tile heights: the fully unrolled kernels.
-constexpr int kCandidates[] = {1, 2, 4, 8};
+// A launch shape: tile height (the fully unrolled kernels 1, 2, 4, 8) and
+// tiles (wavefronts) per workgroup (1, 2, 4), packed in one int.
+struct Shape {
+  int th = 1, waves = 4;
+  int code() const { return th | (waves << 8); }
+  static Shape of(int code) { return Shape{code & 0xff, code >> 8}; }
+};
+
+// Every candidate shape, tile heights outermost.
+std::vector<int> shape_candidates();
+
+// Candidate chunk budgets of a batch, MB of state per chunk.
+constexpr int kBudgetsMB[] = {64, 96, 128, 160, 192, 224};
 
 struct Key {
+  int what;  // 0: launch shape, 1: chunk budget
   int nmech, scalar, models;
   int64_t cells_bucket;  // log2 of the total cell count
   bool operator<(const Key &o) const;
 };
 
-// prepare(th) re-plans the launch for tile height th (returns false to skip
-// the candidate); launch(th, n) enqueues step n on `stream`.  Returns the
-// fastest th (0 if every candidate failed).  `trial_steps` steps per
-// candidate, after one untimed warm-up step.  The state is left advanced and
-// the launch planned for the last candidate: callers re-plan and reset.
-int tune(const Key &key, hipStream_t stream, int trial_steps, const std::function<bool(int)> &prepare,
-         const std::function<hipError_t(int, int)> &launch, double *best_us);
+// prepare(c) re-plans the launch for candidate c (returns false to skip it);
+// launch(c, n) enqueues step n on `stream`; work(c) is the algorithmic bytes
+// of one step under c (null: every candidate does the same work, so time
+// decides).  Returns the best candidate by bytes per second (0 if every
+// candidate failed).  `trial_steps` steps per candidate, after one untimed
+// warm-up step.  The state is left advanced and the launch planned for the
+// last candidate: callers re-plan and reset.
+int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<int> &candidates,
+         const std::function<bool(int)> &prepare, const std::function<hipError_t(int, int)> &launch,
+         const std::function<double(int)> &work, double *best_us);
 
 // Cached result for a key, or 0.
 int cached(const Key &key);

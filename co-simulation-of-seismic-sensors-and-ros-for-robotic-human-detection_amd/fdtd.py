@@ -109,6 +109,9 @@ class GfBatchStats(C.Structure):
         ("algorithmic_bytes", C.c_double),
         ("tile_rows", C.c_int32),
         ("models", C.c_int32),
+        ("waves", C.c_int32),
+        ("reserved", C.c_int32),
+        ("budget_mb", C.c_double),
     ]
 
 
@@ -635,7 +638,7 @@ def BatchStats() -> dict:
     s = GfBatchStats()
     _check(load_library().gf_batch_stats(C.byref(s)))
     return {"kernel_ms": s.kernel_ms, "step_launches": s.step_launches, "algorithmic_bytes": s.algorithmic_bytes,
-            "tile_rows": s.tile_rows, "models": s.models}
+            "tile_rows": s.tile_rows, "models": s.models, "waves": s.waves, "budget_mb": s.budget_mb}
 
 
 def _shot_c(sh: Shot):

@@ -185,16 +185,21 @@ int gf_run(const gf_shot *shot, gf_result *res);
 int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results,
                  int32_t *statuses);
 
-/* Timing of the calling thread's last gf_run_batch or gf_bank_generate: the
- * device time of its step loop (HIP events around it on the solver's
- * stream), the step kernels launched, the algorithmic HBM bytes of every step
- * of every model, the tile height and the models run. */
+/* Timing of the calling thread's last gf_run_batch, gf_bank_generate or
+ * gf_batch_advance (since the batch's last reset): the device time of its
+ * step loop (HIP events around it on the solver's stream), the step kernels
+ * launched, the algorithmic HBM bytes of every step of every model, the
+ * launch shape (tile height, tiles per workgroup), the models run and the
+ * chunk budget in MB (0: every model advanced together). */
 typedef struct gf_batch_stats_t {
   double kernel_ms;
   int64_t step_launches;
   double algorithmic_bytes;
   int32_t tile_rows;
   int32_t models;
+  int32_t waves;
+  int32_t reserved;
+  double budget_mb;
 } gf_batch_stats_t;
 
 int gf_batch_stats(gf_batch_stats_t *out);

@@ -7,6 +7,8 @@
 //   batch_driver mixed     scalar SLS, GSLS and elastic models with failing
 //                          shots in one batch, then a bank generation
 //   batch_driver sim       the step API: create, drive, step, probe, reuse
+//   batch_driver tuned     chunk budget and launch shape both tuned (the
+//                          trial chunks re-planned between trial launches)
 //   batch_driver handle    the batch as a handle: advanced in pieces, reset
 //                          half-way and after its end, run again, read back
 //
@@ -111,6 +113,22 @@ int scenario_batch(bool chunked) {
   int64_t want = 0;
   for (auto &s : v) want += s.steps;
   (void)want;
+  return 0;
+}
+
+int scenario_tuned() {
+  unsetenv("GEOFDTD_CACHE_MB");
+  unsetenv("GEOFDTD_TILE_ROWS");
+  setenv("GEOFDTD_TUNE", "1", 1);
+  std::vector<Shot> v(14);
+  for (int i = 0; i < 14; i++) c3_shot(v[i], (uint32_t)(80 + i), 140 + 41 * i, i % 4 == 3 ? 'R' : 'X');
+  std::vector<int32_t> st;
+  run_batch(v, st);
+  for (size_t i = 0; i < v.size(); i++) expect(st[i] == GF_OK, "tuned batch shot runs");
+  gf_batch_stats_t bs;
+  expect(gf_batch_stats(&bs) == GF_OK && bs.budget_mb >= 64 && bs.budget_mb <= 224 &&
+             (bs.waves == 1 || bs.waves == 2 || bs.waves == 4),
+         "tuned budget and shape reported");
   return 0;
 }
 
@@ -238,6 +256,7 @@ int main(int argc, char **argv) {
   else if (what == "mixed") scenario_mixed();
   else if (what == "sim") scenario_sim();
   else if (what == "handle") scenario_handle();
+  else if (what == "tuned") scenario_tuned();
   else {
     fprintf(stderr, "unknown scenario %s\n", what.c_str());
     return 2;

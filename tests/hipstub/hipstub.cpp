@@ -99,7 +99,7 @@ long g_seq = 0;
 // A step launch as the kernel would see it.
 struct Launch {
   int kind = 0;  // 0 step, 1 set_base
-  int nmech = 0, th = 0;
+  int nmech = 0, th = 0, waves = 4;
   bool scalar = false;
   const gfk::GridDesc *d = nullptr;
   const uint16_t *tm = nullptr;
@@ -213,7 +213,7 @@ void run_launch(ihipStream_t *s, const Launch &L) {
     reads.insert(need(L.nbase, 4, "graph base"));
     base = *L.nbase;
   }
-  const int64_t padded = (L.tiles + 3) / 4 * 4;
+  const int64_t padded = (L.tiles + L.waves - 1) / L.waves * L.waves;
   std::vector<int64_t> seen(ndesc, 0);
   if (L.tm) {
     reads.insert(need(L.tm, sizeof(uint16_t) * padded, "tile-to-model table"));
@@ -516,14 +516,16 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
   return hipSuccess;
 }
 
-hipError_t launch_step(int nmech, bool scalar, int th, const GridDesc *d, const uint16_t *tile_model, int64_t tiles,
-                       int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
+hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
+                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   if (tiles <= 0) return hipSuccess;
+  if (waves != 1 && waves != 2 && waves != 4) fail("launch with %d tiles per workgroup", waves);
   Launch L;
   L.nmech = nmech;
   L.scalar = scalar;
   L.th = th;
+  L.waves = waves;
   L.d = d;
   L.tm = tile_model;
   L.tiles = tiles;

@@ -15,7 +15,8 @@ of the sizes the device layout gives them.
 The scenarios drive the batched driver as the config-3 bank generator runs
 it: cache-sized chunks with the tile-height tuner re-planning the group
 between trial launches (the path a profiled run once ended in a host fault
-on), all models advanced together, mixed kernel variants with failing shots
+on), the chunk budget tuned over trial chunks re-planned between trial
+launches, all models advanced together, mixed kernel variants with failing shots
 and a bank generation, the step API, and the batch as a handle advanced in
 pieces, reset half-way and after its end, and run again.
 """
@@ -36,7 +37,7 @@ def driver():
     return os.path.join(HERE, "lib", "batch_driver")
 
 
-@pytest.mark.parametrize("scenario", ["chunked", "together", "mixed", "sim", "handle"])
+@pytest.mark.parametrize("scenario", ["chunked", "together", "mixed", "sim", "handle", "tuned"])
 def test_host_path_is_clean_under_asan(driver, scenario):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0")
     env.pop("GEOFDTD_TILE_ROWS", None)
