@@ -599,12 +599,18 @@ __device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *ac
       for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
     uint4 b0[TN][KC], b1[TN][KC];
     fetch_chunk<KS, KC, TN>(b0, wp, nt0, 0, lane);
-#pragma unroll 1
+    // Fully unrolled, so the waits before each chunk's MFMAs count only the
+    // loads of that chunk; the scheduling barriers keep every load of the
+    // next chunk issued before this chunk's MFMAs (the scheduler would
+    // otherwise sink them next to their uses, leaving few loads in flight).
+#pragma unroll
     for (int c = 0; c < NC; c += 2) {
       if (c + 1 < NC) fetch_chunk<KS, KC, TN>(b1, wp, nt0, c + 1, lane);
+      __builtin_amdgcn_sched_barrier(0);
       mfma_chunk<KC, TN>(acc, b0, act_in, c, lane);
       if (c + 1 < NC) {
         if (c + 2 < NC) fetch_chunk<KS, KC, TN>(b0, wp, nt0, c + 2, lane);
+        __builtin_amdgcn_sched_barrier(0);
         mfma_chunk<KC, TN>(acc, b1, act_in, c + 1, lane);
       }
     }
@@ -626,10 +632,90 @@ __device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *ac
   }
 }
 
+// Layers 1-6 as one software pipeline over "chunks": 8 k-steps of a pair of
+// 32-column tiles.  Layers 1-5 (512 x 512) are 4 chunks per wavefront (one
+// tile pair, K = 512); layer 6 (512 -> 2048) is 16 (four tile pairs).  The
+// weights do not depend on the activations, so the B fragments of chunk C+1
+// are loaded while chunk C's MFMAs run even across a tile pair's epilogue or
+// a layer's barrier: the load stream never drains until the last chunk.
+constexpr int kPipeChunks = 5 * 4 + 16;
+
+template <int C>
+struct PipeChunk {
+  static constexpr int L = C < 20 ? 1 + C / 4 : 6;        // layer
+  static constexpr int G = C < 20 ? 0 : (C - 20) / 4;     // tile pair within the wavefront's tiles
+  static constexpr int KC = C % 4;                        // chunk of 8 k-steps within K = 512
+  static constexpr int N = L == 6 ? 2048 : 512;
+  static constexpr int TPW = N / 32 / kFusedWaves;        // tiles per wavefront
+  static constexpr bool kLastOfPair = KC == 3;
+  static constexpr bool kLastOfLayer = kLastOfPair && (L < 6 || G == TPW / 2 - 1);
+};
+
+template <int C>
+__device__ __forceinline__ void pipe_fetch(uint4 (&buf)[2][8], const FusedArgs &a, int wave, int lane) {
+  using P = PipeChunk<C>;
+  const uint4 *__restrict__ wp = (const uint4 *)a.wp[P::L];
+  const int nt0 = wave * P::TPW + P::G * 2;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int s = 0; s < 8; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * 32 + P::KC * 8 + s) * 64 + lane];
+}
+
+template <int C>
+__device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f32x16 (&acc)[2], float (&bias)[2],
+                                         uint16_t (*act)[kFusedRows * kFusedPitch], const FusedArgs &a, int m0,
+                                         int wave, int lane) {
+  using P = PipeChunk<C>;
+  if constexpr (C + 1 < kPipeChunks) {
+    if constexpr ((C + 1) & 1) pipe_fetch<C + 1>(b1, a, wave, lane);
+    else pipe_fetch<C + 1>(b0, a, wave, lane);
+  }
+  if constexpr (P::KC == 0) {
+    // the pair's biases, loaded now so that the epilogue's wait on them does
+    // not also wait for the chunks prefetched after them
+    const int nt0 = wave * P::TPW + P::G * 2;
+#pragma unroll
+    for (int t = 0; t < 2; t++) bias[t] = a.bias[P::L][(nt0 + t) * 32 + (lane & 31)];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (P::KC == 0) {
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
+  }
+  if constexpr (C & 1) mfma_chunk<8, 2>(acc, b1, act[P::L & 1], P::KC, lane);
+  else mfma_chunk<8, 2>(acc, b0, act[P::L & 1], P::KC, lane);
+  if constexpr (P::kLastOfPair) {
+    const int nt0 = wave * P::TPW + P::G * 2;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int col = (nt0 + t) * 32 + (lane & 31);
+      const float bv = bias[t];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
+        const uint16_t y = to_bf16(gelu(acc[t][i] + bv));
+        if constexpr (P::L == 6) {
+          if (m0 + row < a.M) a.out[(int64_t)(m0 + row) * P::N + col] = y;
+        } else {
+          act[(P::L + 1) & 1][row * kFusedPitch + col] = y;
+        }
+      }
+    }
+    if constexpr (P::kLastOfLayer && P::L < 6) __syncthreads();
+  }
+  if constexpr (C + 1 < kPipeChunks) pipe_run<C + 1>(b0, b1, acc, bias, act, a, m0, wave, lane);
+}
+
 __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFusedRows * kFusedPitch];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * kFusedRows;
+  // the first chunk of layer 1 is in flight while layer 0 runs
+  uint4 b0[2][8], b1[2][8];
+  pipe_fetch<0>(b0, a, wave, lane);
   // the block's features (16 BF16 per row; rows past M read as zero)
   for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
     const int row = v >> 1, half = v & 1;
@@ -640,17 +726,9 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
   __syncthreads();
   fused_layer<16, 512, false>(act[0], act[1], (const uint4 *)a.wp[0], a.bias[0], nullptr, m0, a.M, lane, wave);
   __syncthreads();
-  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[1], a.bias[1], nullptr, m0, a.M, lane, wave);
-  __syncthreads();
-  fused_layer<512, 512, false>(act[0], act[1], (const uint4 *)a.wp[2], a.bias[2], nullptr, m0, a.M, lane, wave);
-  __syncthreads();
-  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[3], a.bias[3], nullptr, m0, a.M, lane, wave);
-  __syncthreads();
-  fused_layer<512, 512, false>(act[0], act[1], (const uint4 *)a.wp[4], a.bias[4], nullptr, m0, a.M, lane, wave);
-  __syncthreads();
-  fused_layer<512, 512, false>(act[1], act[0], (const uint4 *)a.wp[5], a.bias[5], nullptr, m0, a.M, lane, wave);
-  __syncthreads();
-  fused_layer<512, 2048, true>(act[0], nullptr, (const uint4 *)a.wp[6], a.bias[6], a.out, m0, a.M, lane, wave);
+  f32x16 acc[2];
+  float bias[2];
+  pipe_run<0>(b0, b1, acc, bias, act, a, m0, wave, lane);
 }
 
 hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st) {
