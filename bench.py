@@ -246,7 +246,8 @@ def bench_c3(args, fd) -> dict:
             "frac": achieved / HBM_PEAK_GBS, "traffic": None if ratio is None else ratio * per_launch,
             "kernel": "k_step over every model of a chunk still running (one launch per step, graph blocks of 100)"
             if not lockstep else "k_step over every model still running (one launch per step)",
-            "tile_rows": st["tile_rows"], "algorithmic_bytes": st["algorithmic_bytes"],
+            "tile_rows": st["tile_rows"], "tiles_per_workgroup": st["waves"], "chunk_budget_mb": st["budget_mb"],
+            "algorithmic_bytes": st["algorithmic_bytes"],
             "step_loop_ms": st["kernel_ms"], "step_launches": st["step_launches"],
             "avg_launch_us": launch_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
         },

@@ -142,7 +142,9 @@ int Sim::autotune(int trial_steps, int *chosen) {
                             (int64_t)g.tiles_r * g.tiles_z, (int32_t)(s0 + n), nullptr, 0, stream);
   };
   int code = gft::cached(key);
-  if (!code) code = gft::tune(key, stream, trial_steps, gft::shape_candidates(), prepare, launch, nullptr, nullptr);
+  const int preferred = gft::Shape{choose_tile_rows(R.nr, R.nz), 4}.code();
+  if (!code)
+    code = gft::tune(key, stream, trial_steps, gft::shape_candidates(), preferred, prepare, launch, nullptr, nullptr);
   if (!code) return err ? err : set_err(GF_ERR_DEVICE, "geofdtd: autotune found no working tile height");
   const int best = gft::Shape::of(code).th;
   waves = gft::Shape::of(code).waves;

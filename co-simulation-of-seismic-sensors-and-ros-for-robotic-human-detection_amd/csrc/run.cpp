@@ -290,11 +290,13 @@ int Batch::plan() {
     }
     const bool want_tune = tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL;
     const int64_t bucket = (int64_t)log2((double)std::max<int64_t>(1, cells));
-    // the models of the first chunk under a budget (all of them for 0)
-    auto prefix_for = [&](double mb) {
+    // the models of a chunk under a budget (all of them for 0), cut from
+    // position `from` of the longest-first order
+    auto prefix_for = [&](double mb, size_t from = 0) {
       std::vector<int> out;
       double acc = 0;
-      for (int i : V.order) {
+      for (size_t p = from; p < V.order.size(); p++) {
+        const int i = V.order[p];
         const double b = state_bytes(*sims[i]);
         if (!out.empty() && mb > 0 && acc + b > mb * 1e6) break;
         out.push_back(i);
@@ -323,18 +325,23 @@ int Batch::plan() {
                                 nullptr, 0, stream);
       };
       int code = gft::cached(key);
-      if (!code) code = gft::tune(key, stream, 8, gft::shape_candidates(), prepare, launch, nullptr, nullptr);
+      if (!code)
+        code = gft::tune(key, stream, 8, gft::shape_candidates(), shape.code(), prepare, launch, nullptr, nullptr);
       if (code) shape = gft::Shape::of(code);
       touched = T.order;
     }
     if (want_tune && !budget_env) {
-      // chunk budget: the first chunk under each candidate, timed at the
-      // chosen shape; the rate is its algorithmic bytes per second
+      // chunk budget: the chunk each candidate cuts from the median model on
+      // (the first chunk often holds a single large model whatever the
+      // budget), timed at the chosen shape; the rate is its algorithmic
+      // bytes per second
       gft::Key key{1, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
       std::vector<int> cands(std::begin(gft::kBudgetsMB), std::end(gft::kBudgetsMB));
+      const size_t mid = V.order.size() / 2;
       auto prepare = [&](int b) {
-        T.order = prefix_for(b);
-        if (T.order.size() > touched.size()) touched = T.order;
+        T.order = prefix_for(b, mid);
+        for (int i : T.order)
+          if (std::find(touched.begin(), touched.end(), i) == touched.end()) touched.push_back(i);
         T.waves = shape.waves;
         return plan_group(T, sims, shape.th, stream) == GF_OK;
       };
@@ -344,11 +351,11 @@ int Batch::plan() {
       };
       auto work = [&](int b) {
         double bytes = 0;
-        for (int i : prefix_for(b)) bytes += gfs::step_bytes(sims[i]->R);
+        for (int i : prefix_for(b, mid)) bytes += gfs::step_bytes(sims[i]->R);
         return bytes;
       };
       int best = gft::cached(key);
-      if (!best) best = gft::tune(key, stream, 24, cands, prepare, launch, work, nullptr);
+      if (!best) best = gft::tune(key, stream, 48, cands, (int)mb, prepare, launch, work, nullptr);
       if (best) mb = best;
     }
     if ((rc = drop_graph(T, stream))) return rc;

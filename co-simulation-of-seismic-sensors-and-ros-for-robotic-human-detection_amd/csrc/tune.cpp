@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -31,7 +32,7 @@ int cached(const Key &key) {
   return it == g_cache.end() ? 0 : it->second;
 }
 
-int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<int> &candidates,
+int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<int> &candidates, int preferred,
          const std::function<bool(int)> &prepare, const std::function<hipError_t(int, int)> &launch,
          const std::function<double(int)> &work, double *best_us) {
   if (int c = cached(key)) return c;
@@ -41,25 +42,43 @@ int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<
     (void)hipEventDestroy(e0);
     return 0;
   }
-  int best = 0, n = 0;
-  double best_rate = 0, best_ms = 0;
-  for (int c : candidates) {
-    if (!prepare(c)) continue;
-    if (launch(c, n++) != hipSuccess) continue;  // warm-up (code load, caches)
-    bool ok = hipEventRecord(e0, stream) == hipSuccess;
-    for (int k = 0; ok && k < trial_steps; k++) ok = launch(c, n++) == hipSuccess;
-    ok = ok && hipEventRecord(e1, stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-    float ms = 0;
-    if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0) continue;
-    const double rate = (work ? work(c) : 1.0) / ms;
-    if (rate > best_rate) {
-      best_rate = rate;
-      best_ms = ms;
-      best = c;
+  const size_t nc = candidates.size();
+  std::vector<std::vector<float>> ms(nc);
+  int n = 0;
+  for (int round = 0; round < kRounds; round++)
+    for (size_t i = 0; i < nc; i++) {
+      const int c = candidates[i];
+      if (!prepare(c)) continue;
+      if (launch(c, n++) != hipSuccess) continue;  // warm-up (code load, caches)
+      bool ok = hipEventRecord(e0, stream) == hipSuccess;
+      for (int k = 0; ok && k < trial_steps; k++) ok = launch(c, n++) == hipSuccess;
+      ok = ok && hipEventRecord(e1, stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float t = 0;
+      if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0) ms[i].push_back(t);
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  int best = 0;
+  double best_rate = 0, best_ms = 0, preferred_rate = 0, preferred_ms = 0;
+  for (size_t i = 0; i < nc; i++) {
+    if (ms[i].size() != (size_t)kRounds) continue;  // failed in some round
+    std::sort(ms[i].begin(), ms[i].end());
+    const double med = ms[i][kRounds / 2];
+    const double rate = (work ? work(candidates[i]) : 1.0) / med;
+    if (candidates[i] == preferred) {
+      preferred_rate = rate;
+      preferred_ms = med;
+    }
+    if (rate > best_rate) {
+      best_rate = rate;
+      best_ms = med;
+      best = candidates[i];
+    }
+  }
+  if (preferred_rate > 0 && best_rate <= preferred_rate * (1.0 + kMargin)) {
+    best = preferred;
+    best_ms = preferred_ms;
+  }
   if (best) {
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[key] = best;

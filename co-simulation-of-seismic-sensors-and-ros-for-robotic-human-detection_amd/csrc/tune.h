@@ -30,7 +30,7 @@ struct Shape {
 std::vector<int> shape_candidates();
 
 // Candidate chunk budgets of a batch, MB of state per chunk.
-constexpr int kBudgetsMB[] = {64, 96, 128, 160, 192, 224};
+constexpr int kBudgetsMB[] = {96, 128, 160, 192, 224};
 
 struct Key {
   int what;  // 0: launch shape, 1: chunk budget
@@ -42,11 +42,17 @@ struct Key {
 // prepare(c) re-plans the launch for candidate c (returns false to skip it);
 // launch(c, n) enqueues step n on `stream`; work(c) is the algorithmic bytes
 // of one step under c (null: every candidate does the same work, so time
-// decides).  Returns the best candidate by bytes per second (0 if every
-// candidate failed).  `trial_steps` steps per candidate, after one untimed
-// warm-up step.  The state is left advanced and the launch planned for the
-// last candidate: callers re-plan and reset.
-int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<int> &candidates,
+// decides).  The candidates are timed in kRounds interleaved rounds of
+// `trial_steps` steps (after one untimed warm-up step each time), and each is
+// scored by its median round.  Returns the best candidate by bytes per second,
+// except that `preferred` (the untuned choice) is kept unless another beats
+// it by more than kMargin: trial timings vary by a few per cent, and a choice
+// made on noise can cost more over a long run than it saves.  0 if every
+// candidate failed.  The state is left advanced and the launch planned for
+// the last candidate: callers re-plan and reset.
+constexpr int kRounds = 3;
+constexpr double kMargin = 0.03;
+int tune(const Key &key, hipStream_t stream, int trial_steps, const std::vector<int> &candidates, int preferred,
          const std::function<bool(int)> &prepare, const std::function<hipError_t(int, int)> &launch,
          const std::function<double(int)> &work, double *best_us);
 

@@ -126,7 +126,7 @@ int scenario_tuned() {
   run_batch(v, st);
   for (size_t i = 0; i < v.size(); i++) expect(st[i] == GF_OK, "tuned batch shot runs");
   gf_batch_stats_t bs;
-  expect(gf_batch_stats(&bs) == GF_OK && bs.budget_mb >= 64 && bs.budget_mb <= 224 &&
+  expect(gf_batch_stats(&bs) == GF_OK && bs.budget_mb >= 96 && bs.budget_mb <= 224 &&
              (bs.waves == 1 || bs.waves == 2 || bs.waves == 4),
          "tuned budget and shape reported");
   return 0;

@@ -122,3 +122,56 @@ def test_config3_soils_match_the_oracle(fdtd, oracle):
         from test_gpu_parity import compare
 
         compare(w.name, [t.Vertical for t in g.Traces], want.traces)
+
+
+def test_batch_handle_in_slices_equals_one_call(fdtd):
+    """gf_batch_* in slices (a third, two thirds, the rest), then reset and
+    run again whole: every pass returns gf_run_batch's traces bit for bit, and
+    results are refused before the batch has run to its end."""
+    import cosim_amd.workloads as W
+
+    ws = W.c3(n_models=2, first=3)
+    ranges = [1.0, 2.0, 5.0, 10.0, 20.0]
+    shots = [fdtd.Shot(w.model, w.force, ranges, 400 + 77 * i) for i, w in enumerate(ws)]
+    want = fdtd.RunBatch(shots)
+    b = fdtd.Batch(shots)
+    try:
+        b.Advance(1, 3)
+        with pytest.raises(fdtd.FdtdError):
+            b.Results()
+        b.Advance(2, 3)
+        b.Advance(1, 1)
+        st = b.Stats()
+        assert st["models"] == 2 and st["step_launches"] > 0 and st["algorithmic_bytes"] > 0
+        first = b.Results()
+        b.Reset()
+        b.Advance(1, 1)
+        again = b.Results()
+    finally:
+        b.close()
+    for w, g, a, r in zip(ws, first, again, want):
+        for tg, ta, tr in zip(g.Traces, a.Traces, r.Traces):
+            np.testing.assert_array_equal(tg.Vertical, tr.Vertical)
+            np.testing.assert_array_equal(ta.Vertical, tr.Vertical)
+        assert g.Dt == r.Dt and g.Peak == r.Peak
+    print(f"batch handle: {len(ws)} soils, slices and a rerun equal one gf_run_batch call")
+
+
+def test_batch_handle_spectra_equal_bank_generation(fdtd):
+    spec = fdtd.BankSpec(SampleRateHz=500.0, Samples=256, MaxFreqHz=100.0, MinM=1.0, MaxM=5.0, Count=5)
+    shots = []
+    for m in small_models(fdtd):
+        dt = m.Grid()[0]
+        steps = int(0.2 / dt)
+        shots.append(fdtd.Shot(m, fdtd.Ricker(1000, 40, dt, steps), [], steps))
+    want, want_snapped, _ = fdtd.GenerateBanks(shots, spec)
+    b = fdtd.Batch(shots, spec)
+    try:
+        b.Advance(1, 2)
+        b.Advance(1, 1)
+        got, snapped, results = b.Spectra()
+    finally:
+        b.close()
+    assert all(isinstance(r, fdtd.Result) for r in results)
+    np.testing.assert_array_equal(snapped, want_snapped)
+    np.testing.assert_array_equal(got, want)
