@@ -84,19 +84,24 @@ def describe(w, nr, nz, dt) -> dict:
     }
 
 
-def traffic_from_profiles(workload_name: str):
+def kernel_name(span: int) -> str:
+    """The step kernel of `span` time steps per launch."""
+    return "k_step2" if span == 2 else "k_step"
+
+
+def traffic_from_profiles(workload_name: str, span: int = 1):
     """HBM bytes per step launch measured by rocprofv3 --pmc, if committed."""
-    path = os.path.join(ROOT, "profiles", f"k_step_{workload_name}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel_name(span)}_{workload_name}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
-def traffic_ratio_from_profiles(workload_name: str):
+def traffic_ratio_from_profiles(workload_name: str, span: int = 1):
     """Measured HBM bytes over algorithmic bytes for a workload whose launches
     differ in size (config 3: the set of running models shrinks), if committed."""
-    path = os.path.join(ROOT, "profiles", f"k_step_{workload_name}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel_name(span)}_{workload_name}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -216,11 +221,16 @@ def bench_c3(args, fd) -> dict:
     if not np.all(np.isfinite(spectra)):
         raise SystemExit("non-finite spectra")
     traces = len(ws) * spec.Count
+    span = st["steps_per_launch"]
     launch_s = st["kernel_ms"] / 1e3 / max(st["step_launches"], 1)
-    per_launch = st["algorithmic_bytes"] / max(st["step_launches"], 1)
+    # algorithmic bytes of one launch: every running model's state read and
+    # written once (a two-step launch moves it once for two steps)
+    per_launch = st["pass_bytes"] / max(st["step_launches"], 1)
     achieved = per_launch / launch_s / 1e9
     lockstep = float(os.environ.get("GEOFDTD_CACHE_MB", "160")) <= 0
-    ratio = traffic_ratio_from_profiles("c3" if lockstep else "c3_chunked")
+    ratio = traffic_ratio_from_profiles("c3" if lockstep else "c3_chunked", span)
+    kern = kernel_name(span)
+    what = "one step per launch" if span == 1 else "two steps per launch, the state read and written once per launch"
     out = {
         "metric": METRIC,
         "value": traces / wall,
@@ -244,12 +254,15 @@ def bench_c3(args, fd) -> dict:
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None if ratio is None else ratio * per_launch,
-            "kernel": "k_step over every model of a chunk still running (one launch per step, graph blocks of 100)"
-            if not lockstep else "k_step over every model still running (one launch per step)",
-            "tile_rows": st["tile_rows"], "tiles_per_workgroup": st["waves"], "chunk_budget_mb": st["budget_mb"],
-            "algorithmic_bytes": st["algorithmic_bytes"],
+            "kernel": f"{kern} over every model of a chunk still running ({what}, graph blocks of 100 steps)"
+            if not lockstep else f"{kern} over every model still running ({what})",
+            "steps_per_launch": span, "tile_rows": st["tile_rows"], "tiles_per_workgroup": st["waves"],
+            "chunk_budget_mb": st["budget_mb"], "algorithmic_bytes": st["pass_bytes"],
             "step_loop_ms": st["kernel_ms"], "step_launches": st["step_launches"],
             "avg_launch_us": launch_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
+            # each step's state read and written once, over the same time: the
+            # rate a one-step-per-launch kernel would need to match
+            "step_equivalent_gbs": st["algorithmic_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
         },
         "cells_per_s": cell_steps / wall,
     }
@@ -934,7 +947,9 @@ def measure_shot(args, fd, name):
     tile_rows = int(env_rows) if env_rows else sim.Autotune(10)
     if env_rows:
         sim.SetTileRows(tile_rows)
-    log(f"tile rows {tile_rows}")
+    shape = sim.LaunchShape()
+    span = shape["steps_per_launch"]
+    log(f"launch shape {shape}")
     for i in range(args.warmup):
         sim.Reset()
         sim.Advance(w.steps, use_graph)
@@ -953,6 +968,7 @@ def measure_shot(args, fd, name):
     if not np.all(np.isfinite(traces)):
         raise SystemExit("non-finite traces")
     avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
+    # one launch reads and writes the state once, whatever its steps
     achieved = step_bytes / avg_launch_s / 1e9
     roof = {
         "bound": "hbm",
@@ -960,11 +976,15 @@ def measure_shot(args, fd, name):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": traffic_from_profiles(w.name),
-        "kernel": "k_step (fused velocity+stress+memory-variable step)",
-        "tile_rows": tile_rows,
+        "traffic": traffic_from_profiles(w.name, span),
+        "kernel": f"{kernel_name(span)} (fused velocity+stress+memory-variable step"
+                  + (")" if span == 1 else "s, two per launch)"),
+        "steps_per_launch": span,
+        "tile_rows": shape["tile_rows"],
+        "tiles_per_workgroup": shape["waves"],
         "algorithmic_bytes_per_launch": step_bytes,
         "avg_launch_us": avg_launch_s * 1e6,
+        "step_equivalent_gbs": step_bytes * w.steps * args.steps / (kernel_ms / 1e3) / 1e9,
     }
     res = {
         "value": len(w.ranges) * args.steps / wall,

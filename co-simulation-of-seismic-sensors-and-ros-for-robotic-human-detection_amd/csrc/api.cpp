@@ -72,14 +72,31 @@ int check_device(std::string &why) {
   return GF_OK;
 }
 
-// Tile height when nothing better is known: the tallest of the unrolled
-// variants (8, 4, 2, 1 rows) that still puts about a dozen waves on each of
-// the 1024 SIMDs, since the step is bound by how many loads are in flight.
-// GEOFDTD_TILE_ROWS overrides.
-int choose_tile_rows_cells(int64_t tiles_r_times_nz) {
+// Time steps per launch when nothing better is known: one, unless
+// GEOFDTD_STEPS_PER_LAUNCH asks for the two-step kernel.
+int choose_span() {
+  if (const char *e = getenv("GEOFDTD_STEPS_PER_LAUNCH")) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2) return v;
+  }
+  return 1;
+}
+
+// Tile height when nothing better is known.  One-step kernel: the tallest of
+// the unrolled variants (8, 4, 2, 1 rows) that still puts about a dozen waves
+// on each of the 1024 SIMDs, since the step is bound by how many loads are
+// in flight.  Two-step kernel: the tallest of 32, 16, 8 rows that gives four
+// waves per SIMD (its march holds a row of loads in flight; shorter tiles
+// repeat more of the six halo rows).  GEOFDTD_TILE_ROWS overrides.
+int choose_tile_rows_cells(int span, int64_t tiles_r_times_nz) {
   if (const char *e = getenv("GEOFDTD_TILE_ROWS")) {
     int v = atoi(e);
     if (v >= 1 && v <= 4096) return v;
+  }
+  if (span == 2) {
+    for (int th = 32; th > 8; th /= 2)
+      if (tiles_r_times_nz / th >= 1024 * 4) return th;
+    return 8;
   }
   const int64_t want_waves = 1024 * 12;
   for (int th = 8; th > 1; th /= 2)
@@ -87,9 +104,9 @@ int choose_tile_rows_cells(int64_t tiles_r_times_nz) {
   return 1;
 }
 
-int choose_tile_rows(int nr, int nz) {
-  const int tiles_r = (nr + gfk::kOwn - 1) / gfk::kOwn;
-  return choose_tile_rows_cells((int64_t)tiles_r * nz);
+int choose_tile_rows(int span, int nr, int nz) {
+  const int tiles_r = (nr + gfl::tile_cols(span) - 1) / gfl::tile_cols(span);
+  return choose_tile_rows_cells(span, (int64_t)tiles_r * nz);
 }
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -111,12 +128,13 @@ Sim::~Sim() {
   if (stream) hipStreamDestroy(stream);
 }
 
-int Sim::set_tile_rows(int th) {
+int Sim::set_launch(int sp, int th) {
   if (th < 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: tile rows must be positive");
+  if (sp != 1 && sp != 2) return set_err(GF_ERR_ARGUMENT, "geofdtd: steps per launch must be 1 or 2");
   // launches in flight read the graph and the descriptor about to change
   HIPCHK(hipStreamSynchronize(stream));
-  g.th = th;
-  g.tiles_z = (g.nz + th - 1) / th;
+  span = sp;
+  gfl::set_tiling(g, span, th);
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   exec = nullptr;
@@ -132,23 +150,34 @@ int Sim::autotune(int trial_steps, int *chosen) {
   gft::Key key{0, g.lossy ? g.nmech : 0, g.scalar, 1, (int64_t)log2((double)R.nr * R.nz)};
   int err = GF_OK;
   auto prepare = [&](int code) {
-    waves = gft::Shape::of(code).waves;
-    err = set_tile_rows(gft::Shape::of(code).th);
+    const gft::Shape sh = gft::Shape::of(code);
+    waves = sh.waves;
+    err = set_launch(sh.span, sh.th);
     return err == GF_OK;
   };
-  auto launch = [&](int code, int n) {
-    const gft::Shape sh = gft::Shape::of(code);
-    return gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, sh.th, sh.waves, d_desc, nullptr,
-                            (int64_t)g.tiles_r * g.tiles_z, (int32_t)(s0 + n), nullptr, 0, stream);
+  // trial state is discarded, but each trial reads the buffers the one
+  // before wrote
+  int tb = buf, tm = mbuf;
+  auto launch = [&](int code, int k) {
+    gfk::StepLaunch L = step_launch();
+    L.n_off = (int32_t)(s0 + (int64_t)k * gft::Shape::of(code).span);
+    L.rb_off = tb;
+    L.mb_off = tm;
+    L.count = L.span;
+    tb ^= 1;
+    if (L.span == 2) tm ^= 1;
+    return gfk::launch_step(L, stream);
   };
+  auto work = [&](int code) { return gft::Shape::of(code).span * step_bytes(R); };
   int code = gft::cached(key);
-  const int preferred = gft::Shape{choose_tile_rows(R.nr, R.nz), 4}.code();
+  const int sp0 = choose_span();
+  const int preferred = gft::Shape{choose_tile_rows(sp0, R.nr, R.nz), 4, sp0}.code();
   if (!code)
-    code = gft::tune(key, stream, trial_steps, gft::shape_candidates(), preferred, prepare, launch, nullptr, nullptr);
+    code = gft::tune(key, stream, trial_steps, gft::shape_candidates(), preferred, prepare, launch, work, nullptr);
   if (!code) return err ? err : set_err(GF_ERR_DEVICE, "geofdtd: autotune found no working tile height");
   const int best = gft::Shape::of(code).th;
   waves = gft::Shape::of(code).waves;
-  int rc = set_tile_rows(best);
+  int rc = set_launch(gft::Shape::of(code).span, best);
   if (rc) return rc;
   if ((rc = reset_state())) return rc;
   if (d_traces && trace_floats) HIPCHK(gfk::launch_fill_f32(d_traces, trace_floats, 0.f, stream));
@@ -163,6 +192,8 @@ int Sim::autotune(int trial_steps, int *chosen) {
 int Sim::reset_state() {
   HIPCHK(gfk::launch_fill_f32((float *)arena, (int64_t)(state_bytes / sizeof(float)), 0.f, stream));
   step = 0;
+  buf = 0;
+  mbuf = 0;
   return GF_OK;
 }
 
@@ -183,7 +214,8 @@ int Sim::init(const gf_model &m, int32_t steps_limit) {
 
   memset(&g, 0, sizeof g);
   gfl::Layout L;
-  gfl::plan(R, choose_tile_rows(R.nr, R.nz), g, L);
+  span = choose_span();
+  gfl::plan(R, span, choose_tile_rows(span, R.nr, R.nz), g, L);
   g.tile_base = 0;
   g.steps = steps_limit;
   const size_t rs = L.rs, zs = L.zs;
@@ -198,8 +230,8 @@ int Sim::init(const gf_model &m, int32_t steps_limit) {
     return parts.size() - 1;
   };
   size_t p_fields = add(12 * L.field_floats);
-  size_t p_vr = add(2 * 2 * rs), p_vz = add(2 * 2 * zs), p_sr = add(2 * rs), p_sz = add(2 * zs);
-  size_t p_mem = add(std::max<size_t>(L.mem_floats, 1)), p_rowc = add(L.rowc_floats), p_colc = add(L.colc_floats);
+  size_t p_vr = add(2 * 2 * rs), p_vz = add(2 * 2 * zs), p_sr = add(2 * 2 * rs), p_sz = add(2 * 2 * zs);
+  size_t p_mem = add(std::max<size_t>(2 * L.mem_floats, 1)), p_rowc = add(L.rowc_floats), p_colc = add(L.colc_floats);
   size_t total = 0;
   for (auto &p : parts) {
     p.off = total;
@@ -219,11 +251,12 @@ int Sim::init(const gf_model &m, int32_t steps_limit) {
       g.vpsi_r[b][k] = at(p_vr) + (size_t)(b * 2 + k) * rs;
       g.vpsi_z[b][k] = at(p_vz) + (size_t)(b * 2 + k) * zs;
     }
-  for (int k = 0; k < 2; k++) {
-    g.spsi_r[k] = at(p_sr) + (size_t)k * rs;
-    g.spsi_z[k] = at(p_sz) + (size_t)k * zs;
-  }
-  g.mem = R.lossy ? at(p_mem) : nullptr;
+  for (int b = 0; b < 2; b++)
+    for (int k = 0; k < 2; k++) {
+      g.spsi_r[b][k] = at(p_sr) + (size_t)(b * 2 + k) * rs;
+      g.spsi_z[b][k] = at(p_sz) + (size_t)(b * 2 + k) * zs;
+    }
+  for (int b = 0; b < 2; b++) g.mem[b] = R.lossy ? at(p_mem) + (size_t)b * L.mem_floats : nullptr;
   rowc = at(p_rowc);
   colc = at(p_colc);
   g.rowc = rowc;
@@ -322,9 +355,29 @@ int Sim::reset_max_slots(int64_t steps) {
   return upload_desc();
 }
 
-int Sim::launch_one(int64_t n, bool probe) {
-  HIPCHK(gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, waves, d_desc, nullptr, (int64_t)g.tiles_r * g.tiles_z,
-                          (int32_t)n, nullptr, probe ? 1 : 0, stream));
+gfk::StepLaunch Sim::step_launch() const {
+  gfk::StepLaunch L;
+  L.nmech = g.lossy ? g.nmech : 0;
+  L.scalar = g.scalar != 0;
+  L.span = span;
+  L.th = g.th;
+  L.waves = waves;
+  L.d = d_desc;
+  L.tiles = (int64_t)g.tiles_r * g.tiles_z;
+  return L;
+}
+
+// One launch of `count` steps from step n, reading the current buffer.
+int Sim::launch_one(int64_t n, int count, bool probe) {
+  gfk::StepLaunch L = step_launch();
+  L.n_off = (int32_t)n;
+  L.rb_off = buf;
+  L.mb_off = mbuf;
+  L.probe = probe ? 1 : 0;
+  L.count = count;
+  HIPCHK(gfk::launch_step(L, stream));
+  buf ^= 1;
+  if (span == 2) mbuf ^= 1;
   return GF_OK;
 }
 
@@ -340,18 +393,21 @@ int Sim::advance(int64_t n, bool use_graph, bool probe) {
   if (use_graph && n >= kGraphSteps) {
     int rc = ensure_graph(probe);
     if (rc) return rc;
+    // a block is an even number of launches: the current buffers are the
+    // same after it
     while (n - done >= kGraphSteps) {
       const int64_t n0 = step + done;
-      HIPCHK(gfk::launch_set_base(nbase_dev, (int32_t)n0, stream));
+      HIPCHK(gfk::launch_set_base(nbase_dev, (int32_t)n0, buf, mbuf, stream));
       HIPCHK(hipGraphLaunch(exec, stream));
       done += kGraphSteps;
-      launches += kGraphSteps;
+      launches += kGraphSteps / span;
     }
   }
-  for (; done < n; done++) {
-    const int64_t k = step + done;
-    int rc = launch_one(k, probe);
+  while (done < n) {
+    const int count = (int)std::min<int64_t>(span, n - done);
+    int rc = launch_one(step + done, count, probe);
     if (rc) return rc;
+    done += count;
     launches++;
   }
   step += n;
@@ -359,9 +415,11 @@ int Sim::advance(int64_t n, bool use_graph, bool probe) {
   return GF_OK;
 }
 
-// A graph of kGraphSteps step kernels; node k runs step *nbase + k, where
-// nbase is set by a one-thread kernel before each replay (kernel arguments
-// of a captured node are fixed, the step number is not).
+// A graph of kGraphSteps steps (kGraphSteps / span launches); node k runs
+// step nbase[0] + k * span from wavefield buffer nbase[1] ^ (k & 1) and
+// memory-variable buffer nbase[2] (^ (k & 1) for two-step launches), where
+// nbase is set by a one-thread kernel before each replay (kernel arguments of
+// a captured node are fixed, the step number and the buffers are not).
 int Sim::ensure_graph(bool probe) {
   if (exec && graph_probe == probe) return GF_OK;
   // the old graph may still be running: wait before destroying it
@@ -371,9 +429,15 @@ int Sim::ensure_graph(bool probe) {
   exec = nullptr;
   graph = nullptr;
   HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kGraphSteps; k++) {
-    hipError_t e = gfk::launch_step(g.lossy ? g.nmech : 0, g.scalar != 0, g.th, waves, d_desc, nullptr,
-                                    (int64_t)g.tiles_r * g.tiles_z, k, nbase_dev, probe ? 1 : 0, stream);
+  for (int k = 0; k < kGraphSteps / span; k++) {
+    gfk::StepLaunch L = step_launch();
+    L.n_off = k * span;
+    L.rb_off = k & 1;
+    L.mb_off = span == 2 ? (k & 1) : 0;
+    L.nbase = nbase_dev;
+    L.probe = probe ? 1 : 0;
+    L.count = span;
+    hipError_t e = gfk::launch_step(L, stream);
     if (e != hipSuccess) {
       hipGraph_t tmp;
       hipStreamEndCapture(stream, &tmp);
@@ -468,6 +532,22 @@ int gf_sim_set_tile_rows(gf_sim *sp, int32_t tile_rows) {
   return rc ? rc : ok();
 }
 
+int gf_sim_set_launch(gf_sim *sp, int32_t steps_per_launch, int32_t tile_rows) {
+  Sim *s = reinterpret_cast<Sim *>(sp);
+  if (!s) return set_err(GF_ERR_ARGUMENT, "geofdtd: null handle");
+  int rc = s->set_launch(steps_per_launch, tile_rows);
+  return rc ? rc : ok();
+}
+
+int gf_sim_launch_shape(const gf_sim *sp, int32_t *steps_per_launch, int32_t *tile_rows, int32_t *waves) {
+  const Sim *s = reinterpret_cast<const Sim *>(sp);
+  if (!s) return set_err(GF_ERR_ARGUMENT, "geofdtd: null handle");
+  if (steps_per_launch) *steps_per_launch = s->span;
+  if (tile_rows) *tile_rows = s->g.th;
+  if (waves) *waves = s->waves;
+  return ok();
+}
+
 int gf_sim_create(const gf_model *m, gf_sim **out) {
   if (!m || !out) return set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
   *out = nullptr;
@@ -544,7 +624,7 @@ int gf_sim_surface_velocity(gf_sim *sp, int32_t col, double *out) {
   Sim *s = reinterpret_cast<Sim *>(sp);
   if (!s || !out) return set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
   if (col < 0 || col > s->R.nr - 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: column out of range");
-  HIPCHK(gfk::launch_surface_velocity(s->d_desc, (int32_t)(s->step & 1), col, (float *)s->d_scratch + 4, s->stream));
+  HIPCHK(gfk::launch_surface_velocity(s->d_desc, (int32_t)s->buf, col, (float *)s->d_scratch + 4, s->stream));
   HIPCHK(hipMemcpyAsync(s->h_scratch, (float *)s->d_scratch + 4, 2 * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -556,7 +636,7 @@ int gf_sim_radial_velocity(gf_sim *sp, int32_t col, double *out) {
   Sim *s = reinterpret_cast<Sim *>(sp);
   if (!s || !out) return set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
   if (col < 0 || col > s->R.nr - 1) return set_err(GF_ERR_ARGUMENT, "geofdtd: column out of range");
-  HIPCHK(gfk::launch_surface_velocity(s->d_desc, (int32_t)(s->step & 1), col, (float *)s->d_scratch + 4, s->stream));
+  HIPCHK(gfk::launch_surface_velocity(s->d_desc, (int32_t)s->buf, col, (float *)s->d_scratch + 4, s->stream));
   HIPCHK(hipMemcpyAsync(s->h_scratch, (float *)s->d_scratch + 4, 2 * sizeof(float), hipMemcpyDeviceToHost,
                         s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -569,7 +649,7 @@ int gf_sim_max_speed(gf_sim *sp, double *out) {
   if (!s || !out) return set_err(GF_ERR_ARGUMENT, "geofdtd: null argument");
   uint32_t *slot = (uint32_t *)s->d_scratch + 8;
   HIPCHK(gfk::launch_fill_u32(slot, 1, 0u, s->stream));
-  HIPCHK(gfk::launch_max_speed(s->d_desc, (int32_t)(s->step & 1), slot, (int64_t)s->R.nz * s->g.pitch, s->stream));
+  HIPCHK(gfk::launch_max_speed(s->d_desc, (int32_t)s->buf, slot, (int64_t)s->R.nz * s->g.pitch, s->stream));
   HIPCHK(hipMemcpyAsync(s->h_scratch, slot, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   *out = gfs::bits_to_speed(((uint32_t *)s->h_scratch)[0]);

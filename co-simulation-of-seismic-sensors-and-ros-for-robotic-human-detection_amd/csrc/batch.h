@@ -16,6 +16,7 @@ namespace gfb {
 struct Group {
   int nmech = 0;
   bool scalar = true;
+  int span = 1;                      // time steps per launch (1 or 2)
   int th = 0;                        // tile rows shared by every model of the group
   int waves = 4;                     // tiles (wavefronts) per workgroup
   std::vector<int> order;            // shot indices, longest run first
@@ -37,6 +38,8 @@ struct Chunk {
   int variant = 0;
   int64_t done = 0;
   int64_t max_steps = 0;
+  int buf = 0;   // the buffer holding the chunk's current wavefields
+  int mbuf = 0;  // the buffer holding its memory variables and stress psi
 };
 
 struct Batch {
@@ -50,10 +53,12 @@ struct Batch {
   hipStream_t stream = nullptr;  // the stream the batch runs on
   int tile_rows = 0;             // tile height of the last group planned
   int waves = 0;                 // tiles per workgroup of the last group planned
+  int span = 0;                  // time steps per launch of the last group planned
   double budget_mb = 0;          // chunk budget the batch was cut with (0: all together)
   double kernel_ms = 0;          // step-loop device time since the last reset (HIP events)
   int64_t launches = 0;          // step kernels launched since the last reset
   double bytes = 0;              // algorithmic HBM bytes of the steps run since the last reset
+  double pass_bytes = 0;         // the same per launch: each launch reads and writes the state once
   std::vector<Chunk> chunks;
   bool planned = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;

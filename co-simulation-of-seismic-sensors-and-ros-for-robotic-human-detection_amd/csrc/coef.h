@@ -70,10 +70,10 @@ GF_HD double harmonic_mean(double a, double b) {
   return 2 * a * b / (a + b);
 }
 
-// Row j of the [kRowN + 2 nmech][nz] table.
+// Row j of the [nz][kRowW] table.
 GF_HD void build_row(const TableParams &P, const LayerParams *L, int j, float *rowc) {
 #pragma clang fp contract(off)
-  const size_t nz = (size_t)P.nz;
+  float *r = rowc + (size_t)j * kRowW;
   const int k = j + 1 < P.nz - 1 ? j + 1 : P.nz - 1;
   const LayerParams &lj = L[layer_index(P, L, (double)j * P.h)];
   const LayerParams &lk = L[layer_index(P, L, (double)k * P.h)];
@@ -81,23 +81,23 @@ GF_HD void build_row(const TableParams &P, const LayerParams *L, int j, float *r
   const double rhoz = 0.5 * (lj.rho + lk.rho);
   const double muz = harmonic_mean(lj.ms, lk.ms);
   const double l2m = lam + 2 * mu;
-  rowc[kRowCvr * nz + j] = (float)(P.dt / rho);
-  rowc[kRowCvz * nz + j] = (float)(P.dt / rhoz);
-  rowc[kRowLam * nz + j] = (float)lam;
-  rowc[kRowMu * nz + j] = (float)mu;
-  rowc[kRowL2m * nz + j] = (float)l2m;
-  rowc[kRowIl2m * nz + j] = (float)(1.0 / l2m);
-  rowc[kRowMuz * nz + j] = (float)muz;
+  r[kRowCvr] = (float)(P.dt / rho);
+  r[kRowCvz] = (float)(P.dt / rhoz);
+  r[kRowLam] = (float)lam;
+  r[kRowMu] = (float)mu;
+  r[kRowL2m] = (float)l2m;
+  r[kRowIl2m] = (float)(1.0 / l2m);
+  r[kRowMuz] = (float)muz;
   double a, b;
   pml_coef(j, P.h, 0.0, P.depth, P.pml_thickness, P.d0, P.amax, P.dt, a, b);
-  rowc[kRowAzi * nz + j] = (float)a;
-  rowc[kRowBzi * nz + j] = (float)b;
+  r[kRowAzi] = (float)a;
+  r[kRowBzi] = (float)b;
   pml_coef(j, P.h, 0.5, P.depth, P.pml_thickness, P.d0, P.amax, P.dt, a, b);
-  rowc[kRowAzh * nz + j] = (float)a;
-  rowc[kRowBzh * nz + j] = (float)b;
+  r[kRowAzh] = (float)a;
+  r[kRowBzh] = (float)b;
   for (int q = 0; q < P.nmech_table; q++) {
-    rowc[(kRowN + q) * nz + j] = (float)lj.drive_p[q];
-    rowc[(kRowN + P.nmech_table + q) * nz + j] = (float)lj.drive_s[q];
+    r[kRowN + q] = (float)lj.drive_p[q];
+    r[kRowN + P.nmech_table + q] = (float)lj.drive_s[q];
   }
 }
 

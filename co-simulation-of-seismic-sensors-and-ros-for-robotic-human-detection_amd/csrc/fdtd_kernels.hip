@@ -14,8 +14,11 @@
 
 #include <algorithm>
 
+#include <type_traits>
+
 #include "coef.h"
 #include "kernels.h"
+#include "march2.h"
 #include "stencil.h"
 
 namespace gfk {
@@ -32,8 +35,14 @@ __global__ void k_fill_f32(float *p, int64_t n, float v) {
     p[i] = v;
 }
 
-__global__ void k_set_base(int32_t *p, int32_t v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+// A captured graph's base step and the buffers that hold its first step's
+// state (wavefields, memory variables).
+__global__ void k_set_base(int32_t *p, int32_t v, int32_t buf, int32_t mbuf) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    p[0] = v;
+    p[1] = buf;
+    p[2] = mbuf;
+  }
 }
 
 // --------------------------------------------------------- coefficient build
@@ -76,9 +85,14 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // memory variables -- is issued before any arithmetic, so a wave pays one
 // memory latency per step instead of one per row.  TH == 0: the tile height
 // is g.th at run time and loads run one row ahead of the arithmetic.
+//
+// The step is n_off (+ nbase[0] when replayed from a graph); the wavefields
+// are read from buffer rb_off (^ nbase[1]) and written to the other, the
+// memory variables updated in place in buffer mb_off (^ nbase[2]).
 template <int NM, bool SC, int TH>
 __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
-                                              int32_t n_off, const int32_t *__restrict__ nbase, int32_t probe) {
+                                              int32_t n_off, int32_t rb_off, int32_t mb_off,
+                                              const int32_t *__restrict__ nbase, int32_t probe) {
   const int lane = threadIdx.x & (kLanes - 1);
   // Wave-uniform by construction; readfirstlane lets the compiler keep the
   // descriptor reads scalar.
@@ -102,7 +116,8 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   const int j1 = min(j0 + th, g.nz);
   const int c = i0 - 1 + lane;
   const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
-  const int rb = n & 1, wb = rb ^ 1;
+  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1, wb = rb ^ 1;
+  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1;
   const ColConst cc = load_col(g, c);
   GF_GLOBAL float *const *A = g.f[rb];
   GF_GLOBAL float *const *B = g.f[wb];
@@ -146,12 +161,12 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
       const float vzn_p1 = lane_next(vzn_p);  // new vz(s, c+1)
       if (own) {
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
-        stress<NM, SC>(g, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
+        stress<NM, SC>(g, mb, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
         st_stream(at(B[kTrr], g, s, c), nrr);
         st_stream(at(B[kTqq], g, s, c), nqq);
         st_stream(at(B[kTzz], g, s, c), nzz);
         st_stream(at(B[kTrz], g, s, c), nrz);
-        store_mem<NM>(g, s, c, mem_s);
+        store_mem<NM>(g, mb, s, c, mem_s);
       }
     }
     trr_p = cur.trr;
@@ -175,7 +190,7 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
 #pragma unroll
     for (int k = 0; k < TH; k++) {
       const int t = j0 + k;
-      if (own && t < j1) load_mem<NM>(g, t, c, M[k + 1]);
+      if (own && t < j1) load_mem<NM>(g, mb, t, c, M[k + 1]);
     }
 #pragma unroll
     for (int k = 0; k < TH + 2; k++) {
@@ -192,7 +207,7 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
       RowLoad nxt;
       if (t < j1) nxt = load_row(A, g, n, t + 1, c);
       MemVars<NM> mem_c;
-      if (own && t >= j0 && t < j1) load_mem<NM>(g, t, c, mem_c);
+      if (own && t >= j0 && t < j1) load_mem<NM>(g, mb, t, c, mem_c);
       row(t, cur, mem_p);
       mem_p = mem_c;
       cur = nxt;
@@ -202,6 +217,182 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   if (do_max) {
     for (int off = 32; off >= 1; off >>= 1) mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, off, kLanes));
     if (lane == 0) atomicMax(&g.maxbits[n >> 8], mbits);
+  }
+}
+
+// Two time steps per launch (march2.h): the tile is kOwn2 columns by g.th
+// rows; stage 1 (step n) marches rows j0-3 .. j1+2 from buffers rb (fields)
+// and mb (memory variables, stress psi), stage 2 (step n+1) follows two rows
+// behind and stores rows [j0, j1) to buffers rb ^ 1 and mb ^ 1.  With count == 1, or when step n is a model's last, stage 1's
+// owned rows are stored instead (one step).  Loads run one march row ahead;
+// every per-row array is indexed by the march row modulo 3 at compile time,
+// so the rows in flight stay in registers without copies.
+template <int NM, bool SC>
+__global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
+                                               int32_t n_off, int32_t rb_off, int32_t mb_off,
+                                               const int32_t *__restrict__ nbase, int32_t probe, int32_t count) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
+  const GridDesc &g = gds[m];
+  const int n = n_off + (nbase ? nbase[0] : 0);
+  if (n >= g.steps) return;
+  const int lt = tile - g.tile_base;
+  if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
+  // A model whose run ends after step n takes one step.
+  const bool two = count >= 2 && n + 1 < g.steps;
+  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1, wb = rb ^ 1;
+  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1, mw = mb ^ 1;
+
+  const int th = g.th;
+  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
+  const int i0 = tr * kOwn2, j0 = tzi * th;
+  const int j1 = min(j0 + th, g.nz);
+  const int c = i0 - 2 + lane;
+  const bool own = lane >= 2 && lane < 2 + kOwn2 && c < g.nr;
+  const ColConst cc = load_col(g, c);
+  GF_GLOBAL float *const *A = g.f[rb];
+  GF_GLOBAL float *const *B = g.f[wb];
+  // MaxSpeed every 256 steps; at most one of the two steps is a multiple.
+  const bool max1 = probe && (n & 255) == 0;
+  const bool max2 = two && probe && ((n + 1) & 255) == 0;
+  uint32_t mbits = 0;
+  // Receivers sit on row 0; the tile that owns it records them.
+  const bool rec = j0 == 0 && g.nrec > 0;
+  auto record = [&](int64_t step, float sv) {
+    const int64_t slot = step - g.rec_base;
+    if (slot >= 0 && slot < g.trace_cap) {
+      for (int r = 0; r < g.nrec; r++) {
+        const int col = g.rec_col[r];
+        if (own && col == c) g.traces[(size_t)r * g.trace_cap + slot] = sv;
+      }
+    }
+  };
+
+  const int t0 = j0 - 3, t_end = j1 + 2;
+  Stage S1 = stage_start(ld(A[kTrz], g, j0 - 4, c), ld_tzz(A[kTzz], g, n, j0 - 3, c));
+  Stage S2 = stage_start(0.f, 0.f);
+  // stage 1's inputs (loaded a row ahead) and outputs, by march row mod 3
+  RowLoad IN[3];
+  VPsi IVP[3];
+  SPsi ISP[3];
+  MemVars<NM> IMEM[3];
+  VelOut QV[3];       // velocity half of row t
+  StrOut<NM> QS[3];   // stress half of row t - 1
+
+  auto fetch = [&](auto P, int t) {
+    constexpr int p = decltype(P)::value;
+    IN[p] = load_row(A, g, n, t, c);
+    IVP[p] = VPsi{0.f, 0.f, 0.f, 0.f};
+    if (vel_cell(g, t, c)) IVP[p] = load_vpsi(g, rb, t, c);
+    const int s = t - 1;
+    ISP[p] = SPsi{0.f, 0.f, 0.f, 0.f};
+    IMEM[p] = MemVars<NM>{};
+    if (s >= j0 - 2 && grid_cell(g, s, c)) {
+      ISP[p] = load_spsi(g, mb, s, c);
+      load_mem<NM>(g, mb, s, c, IMEM[p]);
+    }
+  };
+
+  auto body = [&](auto P, int t) {
+    constexpr int p = decltype(P)::value;
+    constexpr int p1 = (p + 1) % 3, p2 = (p + 2) % 3;
+    if (t + 1 <= t_end) fetch(std::integral_constant<int, p1>(), t + 1);
+    const RowLoad &in = IN[p];
+
+    // stage 1: velocities of row t
+    VelOut &v1 = QV[p];
+    v1.vp = IVP[p];
+    stage_velocity(g, t, c, cc, S1, in, lane_next(in.trr), lane_next(in.tqq), lane_prev(in.trz), v1.vp, v1.vr,
+                   v1.vz);
+    const bool vown = own && t >= j0 && t < j1;
+    if (vown) {
+      if (!two) {
+        st_stream(at(B[kVr], g, t, c), v1.vr);
+        st_stream(at(B[kVz], g, t, c), v1.vz);
+        if (vel_cell(g, t, c)) store_vpsi(g, wb, t, c, v1.vp);
+      }
+      if (max1) mbits = max(mbits, max(abs_bits(v1.vr), abs_bits(v1.vz)));
+    }
+    if (rec && t == 0) record(n, surface_velocity(g, c, cc, v1.vr, lane_prev(v1.vr), v1.vz));
+
+    // stage 1: stresses of row t - 1
+    const int s = t - 1;
+    if (s >= j0 - 2) {
+      const float vrn_pm = lane_prev(S1.vrn_p), vzn_p1 = lane_next(S1.vzn_p);
+      StrOut<NM> &o1 = QS[p];
+      o1.sp = ISP[p];
+      o1.mem = IMEM[p];
+      stage_stress<NM, SC>(g, n, s, c, cc, S1, v1.vr, vrn_pm, vzn_p1, o1.st, o1.sp, o1.mem);
+      if (!two && own && s >= j0 && s < j1) {
+        st_stream(at(B[kTrr], g, s, c), o1.st[0]);
+        st_stream(at(B[kTqq], g, s, c), o1.st[1]);
+        st_stream(at(B[kTzz], g, s, c), o1.st[2]);
+        st_stream(at(B[kTrz], g, s, c), o1.st[3]);
+        store_spsi(g, mw, s, c, o1.sp);
+        store_mem<NM>(g, mw, s, c, o1.mem);
+      }
+    }
+    stage_carry(S1, in, v1.vr, v1.vz);
+    if (!two) return;
+
+    // stage 2 on row u = t - 2: its velocity half was formed at march row
+    // t - 2 (slot p1), its stresses at t - 1 (slot p2), the stresses of row
+    // u + 1 just now (slot p) and those of row u - 1 at t - 2 (slot p1).
+    const int u = t - 2;
+    if (u < j0 - 1) return;
+    const VelOut &vu = QV[p1];
+    const StrOut<NM> &su = QS[p2];
+    if (u == j0 - 1) S2 = stage_start(QS[p1].st[3], tzz_for_step(g, n + 1, u, c, su.st[2]));
+    RowLoad in2;
+    in2.trr = su.st[0];
+    in2.tqq = su.st[1];
+    in2.trz = su.st[3];
+    in2.tzz_dn = tzz_for_step(g, n + 1, u + 1, c, QS[p].st[2]);
+    in2.vr = vu.vr;
+    in2.vz = vu.vz;
+    VPsi vp2 = vu.vp;
+    float vr2, vz2;
+    stage_velocity(g, u, c, cc, S2, in2, lane_next(in2.trr), lane_next(in2.tqq), lane_prev(in2.trz), vp2, vr2, vz2);
+    if (own && u >= j0 && u < j1) {
+      st_stream(at(B[kVr], g, u, c), vr2);
+      st_stream(at(B[kVz], g, u, c), vz2);
+      if (vel_cell(g, u, c)) store_vpsi(g, wb, u, c, vp2);
+      if (max2) mbits = max(mbits, max(abs_bits(vr2), abs_bits(vz2)));
+    }
+    if (rec && u == 0) record((int64_t)n + 1, surface_velocity(g, c, cc, vr2, lane_prev(vr2), vz2));
+    const int s2 = u - 1;
+    if (s2 >= j0) {
+      const float vrn_pm = lane_prev(S2.vrn_p), vzn_p1 = lane_next(S2.vzn_p);
+      SPsi sp2 = QS[p1].sp;
+      MemVars<NM> mem2 = QS[p1].mem;
+      float o2[4];
+      stage_stress<NM, SC>(g, n + 1, s2, c, cc, S2, vr2, vrn_pm, vzn_p1, o2, sp2, mem2);
+      if (own) {
+        st_stream(at(B[kTrr], g, s2, c), o2[0]);
+        st_stream(at(B[kTqq], g, s2, c), o2[1]);
+        st_stream(at(B[kTzz], g, s2, c), o2[2]);
+        st_stream(at(B[kTrz], g, s2, c), o2[3]);
+        store_spsi(g, mw, s2, c, sp2);
+        store_mem<NM>(g, mw, s2, c, mem2);
+      }
+    }
+    stage_carry(S2, in2, vr2, vz2);
+  };
+
+  fetch(std::integral_constant<int, 0>(), t0);
+  for (int t = t0; t <= t_end; t += 3) {
+    body(std::integral_constant<int, 0>(), t);
+    if (t + 1 > t_end) break;
+    body(std::integral_constant<int, 1>(), t + 1);
+    if (t + 2 > t_end) break;
+    body(std::integral_constant<int, 2>(), t + 2);
+  }
+
+  if (max1 || max2) {
+    for (int off = 32; off >= 1; off >>= 1) mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, off, kLanes));
+    if (lane == 0) atomicMax(&g.maxbits[(max1 ? n : n + 1) >> 8], mbits);
   }
 }
 
@@ -244,8 +435,8 @@ hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, st, p, v);
+hipError_t launch_set_base(int32_t *p, int32_t v, int32_t buf, int32_t mbuf, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, st, p, v, buf, mbuf);
   return hipGetLastError();
 }
 
@@ -257,40 +448,42 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
 }
 
 template <int NM, bool SC, int TH>
-static hipError_t launch_step_t(int waves, const GridDesc *d, const uint16_t *tm, int64_t tiles, int32_t n_off,
-                                const int32_t *nbase, int32_t probe, hipStream_t st) {
-  unsigned blocks = (unsigned)((tiles + waves - 1) / waves);
-  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(64 * waves), 0, st, d, tm, n_off, nbase, probe);
+static hipError_t launch_step_t(const StepLaunch &L, hipStream_t st) {
+  unsigned blocks = (unsigned)((L.tiles + L.waves - 1) / L.waves);
+  hipLaunchKernelGGL((k_step<NM, SC, TH>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
+                     L.rb_off, L.mb_off, L.nbase, L.probe);
   return hipGetLastError();
 }
 
 template <int NM, bool SC>
-static hipError_t launch_step_th(int th, int waves, const GridDesc *d, const uint16_t *tm, int64_t tiles,
-                                 int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
-  switch (th) {
-    case 1: return launch_step_t<NM, SC, 1>(waves, d, tm, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_t<NM, SC, 2>(waves, d, tm, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_t<NM, SC, 4>(waves, d, tm, tiles, n_off, nbase, probe, st);
-    case 8: return launch_step_t<NM, SC, 8>(waves, d, tm, tiles, n_off, nbase, probe, st);
-    default: return launch_step_t<NM, SC, 0>(waves, d, tm, tiles, n_off, nbase, probe, st);
+static hipError_t launch_step_v(const StepLaunch &L, hipStream_t st) {
+  if (L.span == 2) {
+    unsigned blocks = (unsigned)((L.tiles + L.waves - 1) / L.waves);
+    hipLaunchKernelGGL((k_step2<NM, SC>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
+                       L.rb_off, L.mb_off, L.nbase, L.probe, L.count);
+    return hipGetLastError();
+  }
+  switch (L.th) {
+    case 1: return launch_step_t<NM, SC, 1>(L, st);
+    case 2: return launch_step_t<NM, SC, 2>(L, st);
+    case 4: return launch_step_t<NM, SC, 4>(L, st);
+    case 8: return launch_step_t<NM, SC, 8>(L, st);
+    default: return launch_step_t<NM, SC, 0>(L, st);
   }
 }
 
-// `th` is the tile height every model of the launch was planned with; 1, 2,
-// 4 and 8 select the fully unrolled kernels, anything else the row march.
-// `waves` tiles per workgroup (1, 2 or 4).  `tile_model` maps each tile to
-// its model (null: every tile is model 0); it is padded to whole groups of 4.
-hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
-                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
-  if (tiles <= 0) return hipSuccess;
-  if (waves != 1 && waves != 2 && waves != 4) return hipErrorInvalidValue;
-  if (nmech == 0) return launch_step_th<0, true>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
-  if (scalar) return launch_step_th<1, true>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
-  switch (nmech) {
-    case 1: return launch_step_th<1, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 2: return launch_step_th<2, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 3: return launch_step_th<3, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
-    case 4: return launch_step_th<4, false>(th, waves, d, tile_model, tiles, n_off, nbase, probe, st);
+hipError_t launch_step(const StepLaunch &L, hipStream_t st) {
+  if (L.tiles <= 0) return hipSuccess;
+  if (L.waves != 1 && L.waves != 2 && L.waves != 4) return hipErrorInvalidValue;
+  if (L.span != 1 && L.span != 2) return hipErrorInvalidValue;
+  if (L.count < 1 || L.count > L.span) return hipErrorInvalidValue;
+  if (L.nmech == 0) return launch_step_v<0, true>(L, st);
+  if (L.scalar) return launch_step_v<1, true>(L, st);
+  switch (L.nmech) {
+    case 1: return launch_step_v<1, false>(L, st);
+    case 2: return launch_step_v<2, false>(L, st);
+    case 3: return launch_step_v<3, false>(L, st);
+    case 4: return launch_step_v<4, false>(L, st);
     default: return hipErrorInvalidValue;
   }
 }

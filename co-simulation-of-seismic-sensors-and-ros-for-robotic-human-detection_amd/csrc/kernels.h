@@ -12,12 +12,35 @@ namespace gfk {
 
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st);
-hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st);
+// Writes a captured graph's base step v and the buffers holding its state
+// (wavefields buf, memory variables mbuf): p[0..2].
+hipError_t launch_set_base(int32_t *p, int32_t v, int32_t buf, int32_t mbuf, hipStream_t st);
 hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers, float *rowc, float *colc,
                                hipStream_t st);
-// waves: wavefronts (tiles) per workgroup, 1, 2 or 4.
-hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
-                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st);
+// One launch of a step kernel over `tiles` tiles of the descriptors d.
+//   span 1: k_step, one time step; tiles of kOwn columns by th rows (th 1, 2,
+//           4, 8: fully unrolled kernels; anything else: the row march).
+//   span 2: k_step2, `count` (1 or 2) time steps; tiles of kOwn2 columns by
+//           th rows.
+// Every descriptor's tiling (tiles_r, tiles_z, th, tile_base) must be the
+// one of the kernel launched.  The step is n_off (+ nbase[0]); the wavefields
+// are read from buffer rb_off (^ nbase[1]) and written to the other one; the
+// memory variables and stress psi are in buffer mb_off (^ nbase[2]), updated
+// in place by span 1 and written to the other buffer by span 2.  waves:
+// wavefronts (tiles) per workgroup, 1, 2 or 4.  tile_model maps each tile to
+// its model (null: every tile is model 0), padded to whole groups of 4.
+struct StepLaunch {
+  int nmech = 0;
+  bool scalar = true;
+  int span = 1, th = 1, waves = 4;
+  const GridDesc *d = nullptr;
+  const uint16_t *tile_model = nullptr;
+  int64_t tiles = 0;
+  int32_t n_off = 0, rb_off = 0, mb_off = 0;
+  const int32_t *nbase = nullptr;
+  int32_t probe = 0, count = 1;
+};
+hipError_t launch_step(const StepLaunch &L, hipStream_t st);
 hipError_t launch_dft_f32(const float *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,
                           const double *freqs, int32_t nbins, double *out, hipStream_t st);
 hipError_t launch_dft_f64(const double *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,

@@ -22,21 +22,34 @@ struct Layout {
   size_t field_origin = 0;   // offset of element (0, 0) inside it
   size_t rs = 0;             // floats of one r-strip psi array
   size_t zs = 0;             // floats of one z-strip psi array
-  size_t mem_floats = 0;     // all memory variables (4 components x nmech planes)
+  size_t mem_floats = 0;     // one buffer of memory variables (4 components x nmech planes)
   size_t rowc_floats = 0;
   size_t colc_floats = 0;
   int nmech_table = 0;       // GSLS drive rows in the row table
 };
 
+// Owned columns of a tile of the step kernel of `span` time steps.
+inline int tile_cols(int span) { return span == 2 ? gfk::kOwn2 : gfk::kOwn; }
+
+// The descriptor's tiling for the kernel of `span` steps and `th` rows.
+inline void set_tiling(gfk::GridDesc &g, int span, int th) {
+  g.th = th;
+  g.tiles_r = (g.nr + tile_cols(span) - 1) / tile_cols(span);
+  g.tiles_z = (g.nz + th - 1) / th;
+}
+
 // Tile rows for a grid (see choose_tile_rows in api.cpp for the heuristic).
-inline void plan(const gfm::Resolved &R, int tile_rows, gfk::GridDesc &g, Layout &L) {
+inline void plan(const gfm::Resolved &R, int span, int tile_rows, gfk::GridDesc &g, Layout &L) {
   g.nr = R.nr;
   g.nz = R.nz;
-  g.th = tile_rows;
-  g.tiles_r = (R.nr + gfk::kOwn - 1) / gfk::kOwn;
-  g.tiles_z = (R.nz + g.th - 1) / g.th;
-  // Columns up to tiles_r*kOwn are read by halo lanes; keep them inside the row.
-  g.pitch = (int)(((size_t)g.tiles_r * gfk::kOwn + 2 + 63) / 64 * 64);
+  set_tiling(g, span, tile_rows);
+  // Halo lanes read columns up to tiles_r*kOwn (one-step tiles) or
+  // tiles_r*kOwn2 + 1 (two-step tiles), and columns -2 and -1, which are the
+  // last two of the row above: keep all of them inside a row and past nr, so
+  // they read zeros, whichever kernel runs.
+  const size_t w1 = (size_t)((R.nr + gfk::kOwn - 1) / gfk::kOwn) * gfk::kOwn;
+  const size_t w2 = (size_t)((R.nr + gfk::kOwn2 - 1) / gfk::kOwn2) * gfk::kOwn2;
+  g.pitch = (int)((std::max(w1, w2) + 2 + 63) / 64 * 64);
   g.first_rint = R.first_rint;
   g.first_rhalf = R.first_rhalf;
   g.first_zint = R.first_zint;
@@ -64,7 +77,7 @@ inline void plan(const gfm::Resolved &R, int tile_rows, gfk::GridDesc &g, Layout
   L.rs = std::max<size_t>(1, nz * (size_t)g.rs_pitch);
   L.zs = std::max<size_t>(1, nzs * pitch);
   L.nmech_table = R.scalar ? 0 : R.nmech;
-  L.rowc_floats = (size_t)(gfk::kRowN + 2 * L.nmech_table) * nz;
+  L.rowc_floats = (size_t)gfk::kRowW * nz;
   L.colc_floats = (size_t)gfk::kColN * pitch;
   L.mem_floats = R.lossy ? (size_t)4 * R.nmech * nz * pitch : 0;
 }

@@ -25,6 +25,7 @@
 #include "kernels.h"
 #include "model.h"
 #include "batch.h"
+#include "layout.h"
 #include "sim.h"
 #include "tune.h"
 
@@ -79,14 +80,14 @@ int drop_graph(Group &G, hipStream_t st) {
   return GF_OK;
 }
 
-int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th, hipStream_t st) {
+int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int span, int th, hipStream_t st) {
+  G.span = span;
   G.th = th;
   std::vector<gfk::GridDesc> descs;
   G.prefix.assign(1, 0);
   for (int i : G.order) {
     gfk::GridDesc d = sims[i]->g;
-    d.th = th;
-    d.tiles_z = (d.nz + th - 1) / th;
+    gfl::set_tiling(d, span, th);
     d.tile_base = (int32_t)G.prefix.back();
     descs.push_back(d);
     G.prefix.push_back(G.prefix.back() + (int64_t)d.tiles_r * d.tiles_z);
@@ -126,13 +127,37 @@ int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int th, hipStr
   return GF_OK;
 }
 
+// A launch of the group's first `models` models.
+gfk::StepLaunch group_launch(const Group &G, int models) {
+  gfk::StepLaunch L;
+  L.nmech = G.nmech;
+  L.scalar = G.scalar;
+  L.span = G.span;
+  L.th = G.th;
+  L.waves = G.waves;
+  L.d = G.d_descs;
+  L.tile_model = G.d_tile_model;
+  L.tiles = G.prefix[models];
+  L.count = G.span;
+  return L;
+}
+
+// kGraphSteps steps (kGraphSteps / span launches, an even number) of the
+// group's first `models` models; node k runs step nbase[0] + k * span from
+// wavefield buffer nbase[1] ^ (k & 1) and memory-variable buffer nbase[2]
+// (^ (k & 1) for two-step launches).
 int capture(Group &G, int models, hipStream_t st) {
   int rc = drop_graph(G, st);
   if (rc) return rc;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < gfs::kGraphSteps; k++) {
-    hipError_t e =
-        gfk::launch_step(G.nmech, G.scalar, G.th, G.waves, G.d_descs, G.d_tile_model, G.prefix[models], k, G.d_nbase, 1, st);
+  for (int k = 0; k < gfs::kGraphSteps / G.span; k++) {
+    gfk::StepLaunch L = group_launch(G, models);
+    L.n_off = k * G.span;
+    L.rb_off = k & 1;
+    L.mb_off = G.span == 2 ? (k & 1) : 0;
+    L.nbase = G.d_nbase;
+    L.probe = 1;
+    hipError_t e = gfk::launch_step(L, st);
     if (e != hipSuccess) {
       hipGraph_t tmp = nullptr;
       hipStreamEndCapture(st, &tmp);
@@ -309,24 +334,50 @@ int Batch::plan() {
     T.nmech = V.nmech;
     T.scalar = V.scalar;
     T.order = prefix_for(mb);
+    const int span0 = gfs::choose_span();
     int64_t cols_rows = 0;
-    for (int i : T.order) cols_rows += (int64_t)sims[i]->g.tiles_r * sims[i]->g.nz;
-    gft::Shape shape{gfs::choose_tile_rows_cells(cols_rows), 4};
+    for (int i : T.order)
+      cols_rows += (int64_t)((sims[i]->g.nr + gfl::tile_cols(span0) - 1) / gfl::tile_cols(span0)) * sims[i]->g.nz;
+    gft::Shape shape{gfs::choose_tile_rows_cells(span0, cols_rows), 4, span0};
     std::vector<int> touched;
+    // Trial launches read the buffers the launch before wrote.  Each
+    // candidate starts from cleared state (the trial state is discarded
+    // afterwards), so its first launch may read either buffer.
+    auto clear = [&](const std::vector<int> &models) {
+      for (int i : models) {
+        if (int e = sims[i]->reset_state()) return e;
+        // the fill ran on the model's own stream; the trials run on the batch's
+        hipError_t e = hipStreamSynchronize(sims[i]->stream);
+        if (e != hipSuccess) return gfs::hip_fail(e, "trial reset");
+      }
+      return (int)GF_OK;
+    };
+    int tb = 0, tm = 0;
+    auto trial_launch = [&](int k) {
+      gfk::StepLaunch L = group_launch(T, (int)T.order.size());
+      L.n_off = k * T.span;
+      L.rb_off = tb;
+      L.mb_off = tm;
+      tb ^= 1;
+      if (T.span == 2) tm ^= 1;
+      return gfk::launch_step(L, stream);
+    };
+    auto trial_bytes = [&](const std::vector<int> &models, int span) {
+      double bytes = 0;
+      for (int i : models) bytes += gfs::step_bytes(sims[i]->R);
+      return bytes * span;
+    };
     if (want_tune && !rows_env) {
       gft::Key key{0, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
       auto prepare = [&](int code) {
-        T.waves = gft::Shape::of(code).waves;
-        return plan_group(T, sims, gft::Shape::of(code).th, stream) == GF_OK;
-      };
-      auto launch = [&](int code, int k) {
         const gft::Shape sh = gft::Shape::of(code);
-        return gfk::launch_step(T.nmech, T.scalar, sh.th, sh.waves, T.d_descs, T.d_tile_model, T.prefix.back(), k,
-                                nullptr, 0, stream);
+        T.waves = sh.waves;
+        return plan_group(T, sims, sh.span, sh.th, stream) == GF_OK && clear(T.order) == GF_OK;
       };
+      auto launch = [&](int, int k) { return trial_launch(k); };
+      auto work = [&](int code) { return trial_bytes(T.order, gft::Shape::of(code).span); };
       int code = gft::cached(key);
-      if (!code)
-        code = gft::tune(key, stream, 8, gft::shape_candidates(), shape.code(), prepare, launch, nullptr, nullptr);
+      if (!code) code = gft::tune(key, stream, 8, gft::shape_candidates(), shape.code(), prepare, launch, work, nullptr);
       if (code) shape = gft::Shape::of(code);
       touched = T.order;
     }
@@ -343,17 +394,10 @@ int Batch::plan() {
         for (int i : T.order)
           if (std::find(touched.begin(), touched.end(), i) == touched.end()) touched.push_back(i);
         T.waves = shape.waves;
-        return plan_group(T, sims, shape.th, stream) == GF_OK;
+        return plan_group(T, sims, shape.span, shape.th, stream) == GF_OK && clear(T.order) == GF_OK;
       };
-      auto launch = [&](int, int k) {
-        return gfk::launch_step(T.nmech, T.scalar, T.th, T.waves, T.d_descs, T.d_tile_model, T.prefix.back(), k,
-                                nullptr, 0, stream);
-      };
-      auto work = [&](int b) {
-        double bytes = 0;
-        for (int i : prefix_for(b, mid)) bytes += gfs::step_bytes(sims[i]->R);
-        return bytes;
-      };
+      auto launch = [&](int, int k) { return trial_launch(k); };
+      auto work = [&](int b) { return trial_bytes(prefix_for(b, mid), shape.span); };
       int best = gft::cached(key);
       if (!best) best = gft::tune(key, stream, 48, cands, (int)mb, prepare, launch, work, nullptr);
       if (best) mb = best;
@@ -382,10 +426,11 @@ int Batch::plan() {
     for (Chunk &C : chunks) {
       if (C.variant != (int)v) continue;
       C.max_steps = steps[C.G.order.front()];
-      if ((rc = plan_group(C.G, sims, shape.th, stream))) return rc;
+      if ((rc = plan_group(C.G, sims, shape.span, shape.th, stream))) return rc;
     }
     tile_rows = shape.th;
     waves = shape.waves;
+    span = shape.span;
     budget_mb = mb;
   }
   for (auto &sp : sims)
@@ -408,6 +453,8 @@ int Batch::advance(int64_t num, int64_t den) {
     for (int i : G.order) {
       const int64_t a = std::min<int64_t>(C.done, steps[i]), b = std::min<int64_t>(target, steps[i]);
       bytes += gfs::step_bytes(sims[i]->R) * (double)(b - a);
+      // launches start at C.done and take G.span steps (the last maybe fewer)
+      pass_bytes += gfs::step_bytes(sims[i]->R) * (double)((b - a + G.span - 1) / G.span);
     }
     if (hipEventRecord(e0, stream) != hipSuccess) return gfs::api_set_err(GF_ERR_DEVICE, "geofdtd: event record failed");
     int64_t k = C.done;
@@ -417,22 +464,31 @@ int Batch::advance(int64_t num, int64_t den) {
       // A graph block may carry models past their last step (their tiles
       // return at once) but never past this call's target.
       const bool graph_block = k + gfs::kGraphSteps <= target;
-      launches += graph_block ? gfs::kGraphSteps : 1;
       if (graph_block) {
         // Re-capture when the running set has shrunk by a tenth of its tiles.
         if (!G.exec || G.graph_models < act ||
             (G.prefix[act] < G.prefix[G.graph_models] - G.prefix[G.graph_models] / 10)) {
           if ((rc = capture(G, act, stream))) return rc;
         }
-        hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)k, stream);
+        hipError_t e = gfk::launch_set_base(G.d_nbase, (int32_t)k, C.buf, C.mbuf, stream);
         if (e == hipSuccess) e = hipGraphLaunch(G.exec, stream);
         if (e != hipSuccess) return gfs::hip_fail(e, "graph launch");
+        // an even number of launches: the chunk's buffers are the same after
         k += gfs::kGraphSteps;
+        launches += gfs::kGraphSteps / G.span;
       } else {
-        hipError_t e = gfk::launch_step(G.nmech, G.scalar, G.th, G.waves, G.d_descs, G.d_tile_model, G.prefix[act],
-                                        (int32_t)k, nullptr, 1, stream);
+        gfk::StepLaunch L = group_launch(G, act);
+        L.n_off = (int32_t)k;
+        L.rb_off = C.buf;
+        L.mb_off = C.mbuf;
+        L.probe = 1;
+        L.count = (int)std::min<int64_t>(G.span, target - k);
+        hipError_t e = gfk::launch_step(L, stream);
         if (e != hipSuccess) return gfs::hip_fail(e, "step launch");
-        k += 1;
+        C.buf ^= 1;
+        if (G.span == 2) C.mbuf ^= 1;
+        k += L.count;
+        launches += 1;
       }
     }
     C.done = target;
@@ -459,11 +515,13 @@ int Batch::reset() {
     if (C.done == 0) continue;
     if (C.done >= C.max_steps) {
       // its tables were released when it finished: plan it again
-      if ((rc = plan_group(C.G, sims, C.G.th, stream))) return rc;
+      if ((rc = plan_group(C.G, sims, C.G.span, C.G.th, stream))) return rc;
     }
     for (int i : C.G.order)
       if ((rc = sims[i]->reset_state()) || (rc = sims[i]->reset_max_slots(steps[i]))) return rc;
     C.done = 0;
+    C.buf = 0;
+    C.mbuf = 0;
   }
   for (auto &sp : sims)
     if (sp) {
@@ -473,6 +531,7 @@ int Batch::reset() {
   kernel_ms = 0;
   launches = 0;
   bytes = 0;
+  pass_bytes = 0;
   return GF_OK;
 }
 
@@ -482,8 +541,10 @@ void record_stats(const Batch &b) {
   g_stats.kernel_ms = b.kernel_ms;
   g_stats.step_launches = b.launches;
   g_stats.algorithmic_bytes = b.bytes;
+  g_stats.pass_bytes = b.pass_bytes;
   g_stats.tile_rows = b.tile_rows;
   g_stats.waves = b.waves;
+  g_stats.steps_per_launch = b.span;
   g_stats.budget_mb = b.budget_mb;
   int models = 0;
   for (auto &s : b.sims) models += s ? 1 : 0;

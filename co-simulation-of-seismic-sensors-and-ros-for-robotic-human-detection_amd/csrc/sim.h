@@ -19,8 +19,12 @@ constexpr int kGraphSteps = 100;
 
 int hip_fail(hipError_t e, const char *where);
 int check_device(std::string &why);
-int choose_tile_rows(int nr, int nz);
-int choose_tile_rows_cells(int64_t tiles_r_times_nz);
+// Launch shape when nothing better is known: time steps per launch (1 or 2,
+// GEOFDTD_STEPS_PER_LAUNCH overrides) and the tile height for it
+// (GEOFDTD_TILE_ROWS overrides), from the grid columns and rows to cover.
+int choose_span();
+int choose_tile_rows(int span, int nr, int nz);
+int choose_tile_rows_cells(int span, int64_t tiles_r_times_nz);
 
 // Ordered |v| bits back to a speed; NaN bits stay NaN (math.Max semantics).
 inline double bits_to_speed(uint32_t b) {
@@ -45,6 +49,9 @@ struct Sim {
   gfm::Resolved R;
   gfk::GridDesc g;  // host copy of the device descriptor
   int waves = 4;    // tiles (wavefronts) per step-kernel workgroup
+  int span = 1;     // time steps per step-kernel launch (1 or 2)
+  int buf = 0;      // the buffer holding the current wavefields
+  int mbuf = 0;     // the buffer holding the memory variables and stress psi
   gfk::GridDesc *d_desc = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -78,10 +85,12 @@ struct Sim {
   int set_force(const double *force, int64_t n);
   int set_receivers(const int32_t *cols, int32_t ncols, int64_t cap, int64_t rec_base);
   int reset_max_slots(int64_t steps);
-  int launch_one(int64_t n, bool probe);
+  gfk::StepLaunch step_launch() const;
+  int launch_one(int64_t n, int count, bool probe);
   int advance(int64_t n, bool use_graph, bool probe);
   int ensure_graph(bool probe);
-  int set_tile_rows(int th);
+  int set_launch(int span, int th);
+  int set_tile_rows(int th) { return set_launch(span, th); }
   int autotune(int trial_steps, int *chosen);
 };
 

@@ -14,8 +14,14 @@
 // Because halo lanes and rows read neighbours' old values while neighbours
 // write their new ones, everything a halo recompute reads is double-buffered:
 // the six wavefield arrays and the four C-PML psi arrays of the velocity
-// derivatives.  Memory variables and the psi arrays of the stress
-// derivatives are touched only by the owning lane and are updated in place.
+// derivatives are read from buffer rb and written to buffer rb ^ 1 by every
+// launch.  The memory variables and the psi arrays of the stress derivatives
+// are touched only by the owning lane in a one-step launch, which updates
+// them in place in buffer mb; the two-step kernel of march2.h recomputes a
+// neighbour's stresses in its halo, so it reads them from buffer mb and
+// writes buffer mb ^ 1.  (Doubling them for every launch would grow a
+// one-step launch's working set by half and push config-2 grids out of the
+// 256 MB last-level cache.)  The caller tracks both buffer indices.
 //
 // The functions below are the per-lane phases of that march; the kernel
 // (fdtd_kernels.hip) strings them together with cross-lane shuffles.  They
@@ -47,12 +53,16 @@ namespace gfk {
 
 constexpr int kLanes = 64;   // CDNA wavefront
 constexpr int kOwn = 62;     // owned columns per tile (two halo lanes)
+constexpr int kOwn2 = 60;    // owned columns per tile of the two-step kernel
 constexpr int kMaxMech = 8;  // GSLS mechanisms
-constexpr int kGuardRows = 2;   // zero rows above and below every field
+constexpr int kGuardRows = 4;   // zero rows above and below every field
 constexpr int kGuardCols = 64;  // zero floats before row -kGuardRows
 
-// Per-row coefficient table, float32, [kRow*][nz] then the GSLS drives
-// [kRowN + q][nz] (P) and [kRowN + nmech + q][nz] (S).
+// Per-row coefficient table, float32, one row of kRowW floats per grid row:
+// [t][kRow*], then the GSLS drives at kRowN + q (P) and kRowN + nmech + q
+// (S).  A row's coefficients are contiguous, so a wavefront reads them with
+// a few wide scalar loads from one address per row.
+constexpr int kRowW = 32;  // >= kRowN + 2 * kMaxMech
 enum RowCoef {
   kRowCvr = 0,   // dt / rho            (vr update, fdtd.go:186)
   kRowCvz,       // dt / rhoZ           (vz update, fdtd.go:214)
@@ -97,10 +107,10 @@ struct GridDesc {
   GF_GLOBAL float *f[2][6];        // [buffer][vr, vz, trr, tqq, tzz, trz] at (0,0)
   GF_GLOBAL float *vpsi_r[2][2];   // [buffer][psiTrrR (r half), psiTrzR (r int)]
   GF_GLOBAL float *vpsi_z[2][2];   // [buffer][psiTrzZ (z int), psiTzzZ (z half)]
-  GF_GLOBAL float *spsi_r[2];      // psiVrR (r int), psiVzR (r half); in place
-  GF_GLOBAL float *spsi_z[2];      // psiVzZ (z int), psiVrZ (z half); in place
-  GF_GLOBAL float *mem;            // [4 components][nmech][nz][pitch]
-  GF_GLOBAL const float *rowc;     // [kRowN + 2 nmech][nz]
+  GF_GLOBAL float *spsi_r[2][2];   // [buffer][psiVrR (r int), psiVzR (r half)]
+  GF_GLOBAL float *spsi_z[2][2];   // [buffer][psiVzZ (z int), psiVrZ (z half)]
+  GF_GLOBAL float *mem[2];         // [buffer] -> [4 components][nmech][nz][pitch]
+  GF_GLOBAL const float *rowc;     // [nz][kRowW]
   GF_GLOBAL const float *colc;     // [kColN][pitch]
   GF_GLOBAL const float *src_tzz;  // -F[n] / A          [nsrc]
   GF_GLOBAL const float *src_dtdt; // (F[n+1]-F[n]) / dt  [nsrc]
@@ -144,21 +154,25 @@ GF_HD ColConst load_col(const GridDesc &g, int c) {
 }
 
 // Address of element (row t, column c) of a plane of pitch g.pitch.  On the
-// device the row start (less one column) is wave-uniform and lives in
-// scalar registers, and the column is a 32-bit byte offset (c >= -1, so
-// (c + 1) * 4 is non-negative): the access takes the scalar-base +
-// vector-offset form and needs no 64-bit address arithmetic per lane.
+// device the plane's origin (row -kGuardRows, column -2) is wave-uniform and
+// does not change along a march, so it stays in scalar registers, and row
+// and column go into one 32-bit byte offset (t >= -kGuardRows, c >= -2: the
+// offset is non-negative), the same for every plane: the access takes the
+// scalar-base + vector-offset form and a march row costs one offset update,
+// not 64-bit scalar address arithmetic per plane.
 GF_HD GF_GLOBAL float *at(GF_GLOBAL float *base, const GridDesc &g, int t, int c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  GF_GLOBAL char *row = (GF_GLOBAL char *)(base + ((ptrdiff_t)t * g.pitch - 1));
-  return (GF_GLOBAL float *)(row + (uint32_t)(c + 1) * 4u);
+  GF_GLOBAL char *org = (GF_GLOBAL char *)(base - ((ptrdiff_t)kGuardRows * g.pitch + 2));
+  return (GF_GLOBAL float *)(org + (uint32_t)((t + kGuardRows) * g.pitch + c + 2) * 4u);
 #else
   return base + ((ptrdiff_t)t * g.pitch + c);
 #endif
 }
 
-// Field element at (row t, column c); rows -2..nz+1 and columns -1..pitch-1
-// are inside the padded allocation and read as zero outside the grid.
+// Field element at (row t, column c); rows -4..nz+3 and columns -2..pitch-1
+// are inside the padded allocation and read as zero outside the grid
+// (columns -2 and -1 of a row are the last two of the row above, which lie
+// past the grid's last column, nr <= pitch - 2).
 GF_HD float ld(GF_GLOBAL const float *base, const GridDesc &g, int t, int c) {
   return *at(const_cast<GF_GLOBAL float *>(base), g, t, c);
 }
@@ -187,58 +201,143 @@ GF_HD RowLoad load_row(GF_GLOBAL float *const *A, const GridDesc &g, int n, int 
   return L;
 }
 
+// The C-PML auxiliaries one cell's velocity update reads and advances
+// (zero, and untouched, outside their strips).
+struct VPsi {
+  float trr_r;  // psiTrrR (r half)
+  float trz_r;  // psiTrzR (r int)
+  float trz_z;  // psiTrzZ (z int)
+  float tzz_z;  // psiTzzZ (z half)
+};
+
+// The C-PML auxiliaries of one cell's stress update.
+struct SPsi {
+  float vr_r;  // psiVrR (r int)
+  float vz_r;  // psiVzR (r half)
+  float vz_z;  // psiVzZ (z int)
+  float vr_z;  // psiVrZ (z half)
+};
+
+// Cells whose velocities the reference updates (fdtd.go:158-160), and cells
+// of the grid (whose stresses it updates, fdtd.go:219-221).
+GF_HD bool vel_cell(const GridDesc &g, int t, int c) { return t >= 0 && t <= g.nz - 2 && c >= 0 && c <= g.nr - 2; }
+GF_HD bool grid_cell(const GridDesc &g, int s, int c) { return s >= 0 && s < g.nz && c >= 0 && c < g.nr; }
+
+// Byte offsets into an r strip (every row, columns from r0) and a z strip
+// (rows from z0, every column), for cells inside the strip; 32-bit, like
+// at()'s, so an access is a scalar base plus a vector offset.
+GF_HD uint32_t rs_at(const GridDesc &g, int t, int c) { return (uint32_t)(t * g.rs_pitch + (c - g.r0)) * 4u; }
+GF_HD uint32_t zs_at(const GridDesc &g, int t, int c) { return (uint32_t)((t - g.z0) * g.pitch + c) * 4u; }
+GF_HD GF_GLOBAL float &strip(GF_GLOBAL float *base, uint32_t off) {
+  return *(GF_GLOBAL float *)((GF_GLOBAL char *)base + off);
+}
+
+// A row of the coefficient table.
+GF_HD GF_GLOBAL const float *row_coef(const GridDesc &g, int t) { return g.rowc + (size_t)t * kRowW; }
+
+// Which auxiliaries exist at a velocity cell and at a stress cell (the
+// strips, and for psiVzZ, psiVzR, psiVrZ the rows and columns whose update
+// uses them).
+GF_HD bool vp_trr_r(const GridDesc &g, int, int c) { return c >= g.first_rhalf; }
+GF_HD bool vp_trz_r(const GridDesc &g, int, int c) { return c != 0 && c >= g.first_rint; }
+GF_HD bool vp_trz_z(const GridDesc &g, int t, int) { return t >= g.first_zint; }
+GF_HD bool vp_tzz_z(const GridDesc &g, int t, int) { return t >= g.first_zhalf; }
+GF_HD bool sp_vr_r(const GridDesc &g, int, int c) { return c >= g.first_rint; }
+GF_HD bool sp_vz_z(const GridDesc &g, int s, int) { return s > 0 && s >= g.first_zint; }
+GF_HD bool sp_vz_r(const GridDesc &g, int s, int c) { return s <= g.nz - 2 && c <= g.nr - 2 && c >= g.first_rhalf; }
+GF_HD bool sp_vr_z(const GridDesc &g, int s, int c) { return s <= g.nz - 2 && c <= g.nr - 2 && s >= g.first_zhalf; }
+
+// Velocity auxiliaries of cell (t, c), a velocity cell, from buffer b.
+GF_HD VPsi load_vpsi(const GridDesc &g, int b, int t, int c) {
+  VPsi p = {0.f, 0.f, 0.f, 0.f};
+  if (vp_trr_r(g, t, c)) p.trr_r = strip(g.vpsi_r[b][0], rs_at(g, t, c));
+  if (vp_trz_r(g, t, c)) p.trz_r = strip(g.vpsi_r[b][1], rs_at(g, t, c));
+  if (vp_trz_z(g, t, c)) p.trz_z = strip(g.vpsi_z[b][0], zs_at(g, t, c));
+  if (vp_tzz_z(g, t, c)) p.tzz_z = strip(g.vpsi_z[b][1], zs_at(g, t, c));
+  return p;
+}
+
+GF_HD void store_vpsi(const GridDesc &g, int b, int t, int c, const VPsi &p) {
+  if (vp_trr_r(g, t, c)) strip(g.vpsi_r[b][0], rs_at(g, t, c)) = p.trr_r;
+  if (vp_trz_r(g, t, c)) strip(g.vpsi_r[b][1], rs_at(g, t, c)) = p.trz_r;
+  if (vp_trz_z(g, t, c)) strip(g.vpsi_z[b][0], zs_at(g, t, c)) = p.trz_z;
+  if (vp_tzz_z(g, t, c)) strip(g.vpsi_z[b][1], zs_at(g, t, c)) = p.tzz_z;
+}
+
+// Stress auxiliaries of grid cell (s, c) from buffer b.
+GF_HD SPsi load_spsi(const GridDesc &g, int b, int s, int c) {
+  SPsi p = {0.f, 0.f, 0.f, 0.f};
+  if (sp_vr_r(g, s, c)) p.vr_r = strip(g.spsi_r[b][0], rs_at(g, s, c));
+  if (sp_vz_r(g, s, c)) p.vz_r = strip(g.spsi_r[b][1], rs_at(g, s, c));
+  if (sp_vz_z(g, s, c)) p.vz_z = strip(g.spsi_z[b][0], zs_at(g, s, c));
+  if (sp_vr_z(g, s, c)) p.vr_z = strip(g.spsi_z[b][1], zs_at(g, s, c));
+  return p;
+}
+
+GF_HD void store_spsi(const GridDesc &g, int b, int s, int c, const SPsi &p) {
+  if (sp_vr_r(g, s, c)) st_stream(&strip(g.spsi_r[b][0], rs_at(g, s, c)), p.vr_r);
+  if (sp_vz_r(g, s, c)) st_stream(&strip(g.spsi_r[b][1], rs_at(g, s, c)), p.vz_r);
+  if (sp_vz_z(g, s, c)) st_stream(&strip(g.spsi_z[b][0], zs_at(g, s, c)), p.vz_z);
+  if (sp_vr_z(g, s, c)) st_stream(&strip(g.spsi_z[b][1], zs_at(g, s, c)), p.vr_z);
+}
+
 // New vr and vz at (t, c): Sim.velocities (fdtd.go:158-217) for one cell.
 // Inputs are the old stresses around the cell (tzz_dn = tzz(t+1, c),
-// trz_up = trz(t-1, c), *1 = column c+1, trzm = trz(t, c-1)).  Velocity psi
-// are read from buffer rb and, when `write`, stored to buffer rb ^ 1.
-GF_HD void velocity(const GridDesc &g, int rb, int t, int c, bool write, const ColConst &cc, float trr,
-                    float trr1, float tqq, float tqq1, float trz, float trzm, float trz_up, float tzz,
-                    float tzz_dn, float vr_old, float vz_old, float &vr_new, float &vz_new) {
+// trz_up = trz(t-1, c), *1 = column c+1, trzm = trz(t, c-1)); psi carries the
+// cell's velocity auxiliaries in and their advanced values out.  Outside the
+// updated cells the velocities are returned unchanged.
+GF_HD void velocity_core(const GridDesc &g, int t, int c, const ColConst &cc, float trr, float trr1, float tqq,
+                         float tqq1, float trz, float trzm, float trz_up, float tzz, float tzz_dn, float vr_old,
+                         float vz_old, VPsi &psi, float &vr_new, float &vz_new) {
   vr_new = vr_old;
   vz_new = vz_old;
-  if (t < 0 || t > g.nz - 2 || c < 0 || c > g.nr - 2) return;
+  if (!vel_cell(g, t, c)) return;
   const float invh = g.invh;
-  const size_t nz = (size_t)g.nz;
-  const int wb = rb ^ 1;
+  GF_GLOBAL const float *rc = row_coef(g, t);
 
   float dtrr = (trr1 - trr) * invh;
-  if (c >= g.first_rhalf) {
-    size_t k = (size_t)t * g.rs_pitch + (size_t)(c - g.r0);
-    float psi = cc.brh * g.vpsi_r[rb][0][k] + cc.arh * dtrr;
-    if (write) g.vpsi_r[wb][0][k] = psi;
-    dtrr += psi;
+  if (vp_trr_r(g, t, c)) {
+    psi.trr_r = cc.brh * psi.trr_r + cc.arh * dtrr;
+    dtrr += psi.trr_r;
   }
   float dtrz = t == 0 ? 2.f * trz * invh : (trz - trz_up) * invh;
-  if (t >= g.first_zint) {
-    size_t k = (size_t)(t - g.z0) * g.pitch + (size_t)c;
-    float psi = g.rowc[kRowBzi * nz + t] * g.vpsi_z[rb][0][k] + g.rowc[kRowAzi * nz + t] * dtrz;
-    if (write) g.vpsi_z[wb][0][k] = psi;
-    dtrz += psi;
+  if (vp_trz_z(g, t, c)) {
+    psi.trz_z = rc[kRowBzi] * psi.trz_z + rc[kRowAzi] * dtrz;
+    dtrz += psi.trz_z;
   }
   float hoop = 0.5f * ((trr - tqq) + (trr1 - tqq1)) * cc.irh;
-  vr_new = vr_old + g.rowc[kRowCvr * nz + t] * (dtrr + dtrz + hoop);
+  vr_new = vr_old + rc[kRowCvr] * (dtrr + dtrz + hoop);
 
   float dtzz = (tzz_dn - tzz) * invh;
-  if (t >= g.first_zhalf) {
-    size_t k = (size_t)(t - g.z0) * g.pitch + (size_t)c;
-    float psi = g.rowc[kRowBzh * nz + t] * g.vpsi_z[rb][1][k] + g.rowc[kRowAzh * nz + t] * dtzz;
-    if (write) g.vpsi_z[wb][1][k] = psi;
-    dtzz += psi;
+  if (vp_tzz_z(g, t, c)) {
+    psi.tzz_z = rc[kRowBzh] * psi.tzz_z + rc[kRowAzh] * dtzz;
+    dtzz += psi.tzz_z;
   }
   float radial;
   if (c == 0) {
     radial = 4.f * trz * invh;
   } else {
     float d = (trz - trzm) * invh;
-    if (c >= g.first_rint) {
-      size_t k = (size_t)t * g.rs_pitch + (size_t)(c - g.r0);
-      float psi = cc.bri * g.vpsi_r[rb][1][k] + cc.ari * d;
-      if (write) g.vpsi_r[wb][1][k] = psi;
-      d += psi;
+    if (vp_trz_r(g, t, c)) {
+      psi.trz_r = cc.bri * psi.trz_r + cc.ari * d;
+      d += psi.trz_r;
     }
     radial = d + 0.5f * (trz + trzm) * cc.iri;
   }
-  vz_new = vz_old + g.rowc[kRowCvz * nz + t] * (dtzz + radial);
+  vz_new = vz_old + rc[kRowCvz] * (dtzz + radial);
+}
+
+// One-step form: the auxiliaries are read from buffer rb and, when `write`,
+// the advanced ones stored to buffer rb ^ 1.
+GF_HD void velocity(const GridDesc &g, int rb, int t, int c, bool write, const ColConst &cc, float trr,
+                    float trr1, float tqq, float tqq1, float trz, float trzm, float trz_up, float tzz,
+                    float tzz_dn, float vr_old, float vz_old, float &vr_new, float &vz_new) {
+  VPsi psi = {0.f, 0.f, 0.f, 0.f};
+  const bool cell = vel_cell(g, t, c);
+  if (cell) psi = load_vpsi(g, rb, t, c);
+  velocity_core(g, t, c, cc, trr, trr1, tqq, tqq1, trz, trzm, trz_up, tzz, tzz_dn, vr_old, vz_old, psi, vr_new,
+                vz_new);
+  if (cell && write) store_vpsi(g, rb ^ 1, t, c, psi);
 }
 
 // Crank-Nicolson memory variable (fdtd.go:315-320), drive already formed.
@@ -257,39 +356,39 @@ struct MemVars {
 };
 
 template <int NM>
-GF_HD void load_mem(const GridDesc &g, int s, int c, MemVars<NM> &m) {
+GF_HD void load_mem(const GridDesc &g, int b, int s, int c, MemVars<NM> &m) {
   if (NM == 0) return;
   const size_t ms = (size_t)g.mem_stride;
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int q = 0; q < NM; q++) m.r[k][q] = *at(g.mem + (size_t)(k * NM + q) * ms, g, s, c);
+    for (int q = 0; q < NM; q++) m.r[k][q] = *at(g.mem[b] + (size_t)(k * NM + q) * ms, g, s, c);
 }
 
 template <int NM>
-GF_HD void store_mem(const GridDesc &g, int s, int c, const MemVars<NM> &m) {
+GF_HD void store_mem(const GridDesc &g, int b, int s, int c, const MemVars<NM> &m) {
   if (NM == 0) return;
   const size_t ms = (size_t)g.mem_stride;
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int q = 0; q < NM; q++) st_stream(at(g.mem + (size_t)(k * NM + q) * ms, g, s, c), m.r[k][q]);
+    for (int q = 0; q < NM; q++) st_stream(at(g.mem[b] + (size_t)(k * NM + q) * ms, g, s, c), m.r[k][q]);
 }
 
-// New stresses at (s, c): Sim.stresses (fdtd.go:219-307) for one cell.
-// vr/vz are this step's new velocities: vrm = vr(s, c-1), vr_dn = vr(s+1, c),
-// vz1 = vz(s, c+1), vz_up = vz(s-1, c).  trr..trz carry the old values in and
-// the new ones out, mem the memory variables.  NM = 0 is elastic; SC selects
-// the reference's scalar relaxation (NM = 1, one drive for P and S alike),
+// New stresses at grid cell (s, c): Sim.stresses (fdtd.go:219-307) for one
+// cell.  vr/vz are this step's new velocities: vrm = vr(s, c-1),
+// vr_dn = vr(s+1, c), vz1 = vz(s, c+1), vz_up = vz(s-1, c).  trr..trz carry
+// the old values in and the new ones out, psi and mem the cell's C-PML
+// auxiliaries and memory variables.  NM = 0 is elastic; SC selects the
+// reference's scalar relaxation (NM = 1, one drive for P and S alike),
 // otherwise separate P and S drives per mechanism.
 template <int NM, bool SC>
-GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, float vr, float vrm, float vr_dn,
-                  float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz,
-                  MemVars<NM> &mem) {
+GF_HD void stress_core(const GridDesc &g, int n, int s, int c, const ColConst &cc, float vr, float vrm, float vr_dn,
+                       float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz, SPsi &psi,
+                       MemVars<NM> &mem) {
   const float invh = g.invh, dt = g.dt;
-  const size_t nz = (size_t)g.nz;
-  GF_GLOBAL const float *rc = g.rowc;
-  const float lam = rc[kRowLam * nz + s], l2m = rc[kRowL2m * nz + s];
+  GF_GLOBAL const float *rc = row_coef(g, s);
+  const float lam = rc[kRowLam], l2m = rc[kRowL2m];
 
   float dvr, vrr;
   if (c == 0) {
@@ -299,24 +398,20 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     dvr = (vr - vrm) * invh;
     vrr = 0.5f * (vr + vrm) * cc.iri;
   }
-  if (c >= g.first_rint) {
-    GF_GLOBAL float *p = g.spsi_r[0] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
-    float psi = cc.bri * *p + cc.ari * dvr;
-    *p = psi;
-    dvr += psi;
+  if (sp_vr_r(g, s, c)) {
+    psi.vr_r = cc.bri * psi.vr_r + cc.ari * dvr;
+    dvr += psi.vr_r;
   }
   float dvz;
   if (s == 0) {
     // F6: the applied rate enters unscaled by -1/A, exactly as fdtd.go:251-256.
     float applied = (c == 0 && n < g.nsrc) ? g.src_dtdt[n] : 0.f;
-    dvz = (applied - lam * (dvr + vrr)) * rc[kRowIl2m * nz + s];
+    dvz = (applied - lam * (dvr + vrr)) * rc[kRowIl2m];
   } else {
     dvz = (vz - vz_up) * invh;
-    if (s >= g.first_zint) {
-      GF_GLOBAL float *p = g.spsi_z[0] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
-      float psi = rc[kRowBzi * nz + s] * *p + rc[kRowAzi * nz + s] * dvz;
-      *p = psi;
-      dvz += psi;
+    if (sp_vz_z(g, s, c)) {
+      psi.vz_z = rc[kRowBzi] * psi.vz_z + rc[kRowAzi] * dvz;
+      dvz += psi.vz_z;
     }
   }
   const float err = l2m * dvr + lam * (vrr + dvz);
@@ -329,12 +424,12 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     trr += dt * (err + relax_cn(g.mem_decay, mem.r[0][0], g.mem_drive * err));
     tqq += dt * (eqq + relax_cn(g.mem_decay, mem.r[1][0], g.mem_drive * eqq));
   } else {
-    const float mu2 = 2.f * rc[kRowMu * nz + s];
+    const float mu2 = 2.f * rc[kRowMu];
     const float pb = l2m * (dvr + vrr + dvz);
     float sr = 0.f, sq = 0.f;
 #pragma unroll
     for (int q = 0; q < NM; q++) {
-      const float dP = rc[(kRowN + q) * nz + s], dS = rc[(kRowN + NM + q) * nz + s];
+      const float dP = rc[(kRowN + q)], dS = rc[(kRowN + NM + q)];
       sr += relax_cn(g.decay[q], mem.r[0][q], dP * pb - dS * (mu2 * (vrr + dvz)));
       sq += relax_cn(g.decay[q], mem.r[1][q], dP * pb - dS * (mu2 * (dvr + dvz)));
     }
@@ -349,12 +444,12 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     } else if (SC) {
       tzz += dt * (ezz + relax_cn(g.mem_decay, mem.r[2][0], g.mem_drive * ezz));
     } else {
-      const float mu2 = 2.f * rc[kRowMu * nz + s];
+      const float mu2 = 2.f * rc[kRowMu];
       const float pb = l2m * (dvr + vrr + dvz);
       float sz = 0.f;
 #pragma unroll
       for (int q = 0; q < NM; q++) {
-        const float dP = rc[(kRowN + q) * nz + s], dS = rc[(kRowN + NM + q) * nz + s];
+        const float dP = rc[(kRowN + q)], dS = rc[(kRowN + NM + q)];
         sz += relax_cn(g.decay[q], mem.r[2][q], dP * pb - dS * (mu2 * (dvr + vrr)));
       }
       tzz += dt * (ezz + sz);
@@ -365,20 +460,16 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
 
   if (s <= g.nz - 2 && c <= g.nr - 2) {
     float dvzdr = (vz1 - vz) * invh;
-    if (c >= g.first_rhalf) {
-      GF_GLOBAL float *p = g.spsi_r[1] + (size_t)s * g.rs_pitch + (size_t)(c - g.r0);
-      float psi = cc.brh * *p + cc.arh * dvzdr;
-      *p = psi;
-      dvzdr += psi;
+    if (sp_vz_r(g, s, c)) {
+      psi.vz_r = cc.brh * psi.vz_r + cc.arh * dvzdr;
+      dvzdr += psi.vz_r;
     }
     float dvrdz = (vr_dn - vr) * invh;
-    if (s >= g.first_zhalf) {
-      GF_GLOBAL float *p = g.spsi_z[1] + (size_t)(s - g.z0) * g.pitch + (size_t)c;
-      float psi = rc[kRowBzh * nz + s] * *p + rc[kRowAzh * nz + s] * dvrdz;
-      *p = psi;
-      dvrdz += psi;
+    if (sp_vr_z(g, s, c)) {
+      psi.vr_z = rc[kRowBzh] * psi.vr_z + rc[kRowAzh] * dvrdz;
+      dvrdz += psi.vr_z;
     }
-    const float e = rc[kRowMuz * nz + s] * (dvzdr + dvrdz);
+    const float e = rc[kRowMuz] * (dvzdr + dvrdz);
     if (NM == 0) {
       trz += dt * e;
     } else if (SC) {
@@ -386,10 +477,22 @@ GF_HD void stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, fl
     } else {
       float sz = 0.f;
 #pragma unroll
-      for (int q = 0; q < NM; q++) sz += relax_cn(g.decay[q], mem.r[3][q], rc[(kRowN + NM + q) * nz + s] * e);
+      for (int q = 0; q < NM; q++) sz += relax_cn(g.decay[q], mem.r[3][q], rc[(kRowN + NM + q)] * e);
       trz += dt * (e + sz);
     }
   }
+}
+
+// One-step form: auxiliaries updated in place in buffer mb; the memory
+// variables come in (loaded ahead by the caller) and go out advanced, for
+// the caller to store.
+template <int NM, bool SC>
+GF_HD void stress(const GridDesc &g, int mb, int n, int s, int c, const ColConst &cc, float vr, float vrm,
+                  float vr_dn, float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz,
+                  MemVars<NM> &mem) {
+  SPsi psi = load_spsi(g, mb, s, c);
+  stress_core<NM, SC>(g, n, s, c, cc, vr, vrm, vr_dn, vz, vz1, vz_up, trr, tqq, tzz, trz, psi, mem);
+  store_spsi(g, mb, s, c, psi);
 }
 
 // Sim.SurfaceVelocity (fdtd.go:330-342) from this step's row-0 velocities.

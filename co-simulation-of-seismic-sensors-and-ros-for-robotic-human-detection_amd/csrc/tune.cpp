@@ -19,7 +19,9 @@ bool Key::operator<(const Key &o) const {
 std::vector<int> shape_candidates() {
   std::vector<int> out;
   for (int th : {1, 2, 4, 8})
-    for (int w : {4, 2, 1}) out.push_back(Shape{th, w}.code());
+    for (int w : {4, 2, 1}) out.push_back(Shape{th, w, 1}.code());
+  for (int th : {8, 16, 32})
+    for (int w : {4, 2, 1}) out.push_back(Shape{th, w, 2}.code());
   return out;
 }
 

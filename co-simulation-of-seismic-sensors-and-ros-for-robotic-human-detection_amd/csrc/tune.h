@@ -18,15 +18,19 @@
 
 namespace gft {
 
-// A launch shape: tile height (the fully unrolled kernels 1, 2, 4, 8) and
-// tiles (wavefronts) per workgroup (1, 2, 4), packed in one int.
+// A launch shape: time steps per launch (span 1: the one-step kernel, whose
+// tile heights 1, 2, 4, 8 are fully unrolled; span 2: the two-step kernel,
+// any tile height), tile height and tiles (wavefronts) per workgroup (1, 2,
+// 4), packed in one int.
 struct Shape {
-  int th = 1, waves = 4;
-  int code() const { return th | (waves << 8); }
-  static Shape of(int code) { return Shape{code & 0xff, code >> 8}; }
+  int th = 1, waves = 4, span = 1;
+  int code() const { return th | (waves << 8) | (span << 12); }
+  static Shape of(int code) { return Shape{code & 0xff, (code >> 8) & 0xf, (code >> 12) & 0xf}; }
 };
 
-// Every candidate shape, tile heights outermost.
+// Every candidate shape: the one-step kernel's, then the two-step kernel's
+// (tile heights 8, 16, 32: its halo is three rows above and below, so short
+// tiles repeat much of their work).
 std::vector<int> shape_candidates();
 
 // Candidate chunk budgets of a batch, MB of state per chunk.
@@ -40,9 +44,9 @@ struct Key {
 };
 
 // prepare(c) re-plans the launch for candidate c (returns false to skip it);
-// launch(c, n) enqueues step n on `stream`; work(c) is the algorithmic bytes
-// of one step under c (null: every candidate does the same work, so time
-// decides).  The candidates are timed in kRounds interleaved rounds of
+// launch(c, k) enqueues the k-th trial launch on `stream` (reading buffer
+// k & 1); work(c) is the algorithmic bytes of one launch under c (null:
+// every candidate does the same work, so time decides).  The candidates are timed in kRounds interleaved rounds of
 // `trial_steps` steps (after one untimed warm-up step each time), and each is
 // scored by its median round.  Returns the best candidate by bytes per second,
 // except that `preferred` (the untuned choice) is kept unless another beats

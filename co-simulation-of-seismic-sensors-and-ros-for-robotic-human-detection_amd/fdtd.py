@@ -110,8 +110,9 @@ class GfBatchStats(C.Structure):
         ("tile_rows", C.c_int32),
         ("models", C.c_int32),
         ("waves", C.c_int32),
-        ("reserved", C.c_int32),
+        ("steps_per_launch", C.c_int32),
         ("budget_mb", C.c_double),
+        ("pass_bytes", C.c_double),
     ]
 
 
@@ -269,6 +270,8 @@ SIGNATURES = {
     "gf_sim_step_bytes": (_I, [C.c_void_p, _PD]),
     "gf_sim_autotune": (_I, [C.c_void_p, _I32, _PI32]),
     "gf_sim_set_tile_rows": (_I, [C.c_void_p, _I32]),
+    "gf_sim_set_launch": (_I, [C.c_void_p, _I32, _I32]),
+    "gf_sim_launch_shape": (_I, [C.c_void_p, _PI32, _PI32, _PI32]),
     "gf_bank_write": (_I, [C.c_char_p, C.POINTER(GfBankHeader), C.POINTER(C.c_float)]),
     "gf_bank_header_json": (_I, [C.POINTER(GfBankHeader), C.c_char_p, _I32]),
     "gf_bank_generate": (_I, [C.POINTER(GfShot), _I32, C.POINTER(GfBankSpec), C.POINTER(C.c_float), _PD,
@@ -588,6 +591,16 @@ class Sim:
     def SetTileRows(self, rows: int) -> None:
         _check(load_library().gf_sim_set_tile_rows(self._h, rows))
 
+    def SetLaunch(self, steps_per_launch: int, rows: int) -> None:
+        """Time steps per kernel launch (1, or 2: the temporally blocked
+        kernel) and its tile height."""
+        _check(load_library().gf_sim_set_launch(self._h, steps_per_launch, rows))
+
+    def LaunchShape(self) -> dict:
+        sp, th, w = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load_library().gf_sim_launch_shape(self._h, C.byref(sp), C.byref(th), C.byref(w)))
+        return {"steps_per_launch": sp.value, "tile_rows": th.value, "waves": w.value}
+
     def StepBytes(self) -> float:
         b = C.c_double()
         _check(load_library().gf_sim_step_bytes(self._h, C.byref(b)))
@@ -638,7 +651,8 @@ def BatchStats() -> dict:
     s = GfBatchStats()
     _check(load_library().gf_batch_stats(C.byref(s)))
     return {"kernel_ms": s.kernel_ms, "step_launches": s.step_launches, "algorithmic_bytes": s.algorithmic_bytes,
-            "tile_rows": s.tile_rows, "models": s.models, "waves": s.waves, "budget_mb": s.budget_mb}
+            "tile_rows": s.tile_rows, "models": s.models, "waves": s.waves, "steps_per_launch": s.steps_per_launch,
+            "budget_mb": s.budget_mb, "pass_bytes": s.pass_bytes}
 
 
 def _shot_c(sh: Shot):

@@ -188,9 +188,13 @@ int gf_run_batch(const gf_shot *shots, int32_t n_shots, gf_result *results,
 /* Timing of the calling thread's last gf_run_batch, gf_bank_generate or
  * gf_batch_advance (since the batch's last reset): the device time of its
  * step loop (HIP events around it on the solver's stream), the step kernels
- * launched, the algorithmic HBM bytes of every step of every model, the
- * launch shape (tile height, tiles per workgroup), the models run and the
- * chunk budget in MB (0: every model advanced together). */
+ * launched, the algorithmic HBM bytes of every step of every model (each
+ * state array read and written once per step), the launch shape (tile
+ * height, tiles per workgroup, time steps per launch), the models run, the
+ * chunk budget in MB (0: every model advanced together), and the bytes of
+ * one read and one write of every running model's state per launch (equal
+ * to algorithmic_bytes with one step per launch, about half of it with
+ * two). */
 typedef struct gf_batch_stats_t {
   double kernel_ms;
   int64_t step_launches;
@@ -198,8 +202,9 @@ typedef struct gf_batch_stats_t {
   int32_t tile_rows;
   int32_t models;
   int32_t waves;
-  int32_t reserved;
+  int32_t steps_per_launch;
   double budget_mb;
+  double pass_bytes;
 } gf_batch_stats_t;
 
 int gf_batch_stats(gf_batch_stats_t *out);
@@ -226,6 +231,14 @@ int gf_sim_read_traces(gf_sim *s, double *out, int32_t *steps_recorded);
  * other values a row-marching kernel).  No reference counterpart. */
 int gf_sim_autotune(gf_sim *s, int32_t trial_steps, int32_t *tile_rows);
 int gf_sim_set_tile_rows(gf_sim *s, int32_t tile_rows);
+/* Time steps per kernel launch (1: one step per launch; 2: the temporally
+ * blocked kernel, which reads and writes the state once per two steps) with
+ * its tile height, and the shape in use (also after gf_sim_autotune, which
+ * tries both).  Results do not depend on the shape beyond FP32 rounding of
+ * contracted products.  No reference counterpart. */
+int gf_sim_set_launch(gf_sim *s, int32_t steps_per_launch, int32_t tile_rows);
+int gf_sim_launch_shape(const gf_sim *s, int32_t *steps_per_launch, int32_t *tile_rows,
+                        int32_t *waves);
 /* Algorithmic HBM bytes one step moves (each state array read and written
  * once, plus the C-PML strips): the roofline numerator. */
 int gf_sim_step_bytes(const gf_sim *s, double *bytes);

@@ -19,8 +19,9 @@ def lib(asan=False):
     return L
 
 
-def run(model, force, cols, steps, tile_rows=8, asan=False):
-    """(traces [ncols, steps] float32, probe bits) through the emulated kernel."""
+def run(model, force, cols, steps, tile_rows=8, asan=False, span=1):
+    """(traces [ncols, steps] float32, probe bits) through the emulated kernel
+    of `span` time steps per launch."""
     L = lib(asan)
     cm, keep = model.to_c()
     f = np.ascontiguousarray(force, dtype=np.float64)
@@ -29,7 +30,7 @@ def run(model, force, cols, steps, tile_rows=8, asan=False):
     mb = np.zeros(steps // 256 + 2, dtype=np.uint32)
     rc = L.emu_run(
         C.byref(cm), f.ctypes.data_as(C.c_void_p), C.c_int64(len(f)), cl.ctypes.data_as(C.c_void_p),
-        C.c_int32(len(cl)), C.c_int32(steps), C.c_int32(tile_rows), tr.ctypes.data_as(C.c_void_p),
+        C.c_int32(len(cl)), C.c_int32(steps), C.c_int32(tile_rows), C.c_int32(span), tr.ctypes.data_as(C.c_void_p),
         mb.ctypes.data_as(C.c_void_p),
     )
     if rc:

@@ -99,19 +99,22 @@ long g_seq = 0;
 // A step launch as the kernel would see it.
 struct Launch {
   int kind = 0;  // 0 step, 1 set_base
-  int nmech = 0, th = 0, waves = 4;
+  int nmech = 0, th = 0, waves = 4, span = 1, count = 1;
   bool scalar = false;
   const gfk::GridDesc *d = nullptr;
   const uint16_t *tm = nullptr;
   int64_t tiles = 0;
-  int32_t n_off = 0;
+  int32_t n_off = 0, rb_off = 0, mb_off = 0;
   const int32_t *nbase = nullptr;
   int32_t probe = 0;
   int32_t *base_ptr = nullptr;
-  int32_t base_val = 0;
+  int32_t base_val = 0, base_buf = 0, base_mbuf = 0;
 };
 
 std::vector<ihipStream_t *> g_streams;
+// per model (keyed by its first field plane, and by its first stress psi
+// strip): the buffer its wavefields and its memory variables are in
+std::map<uintptr_t, int> g_state_buf;
 ihipStream_t g_null;  // the null stream (synchronous calls)
 
 void retire_all() {
@@ -151,10 +154,16 @@ void check_span(const void *p, size_t floats, const char *what, std::set<uintptr
   rw.insert(need(p, floats * sizeof(float), what));
 }
 
-void check_desc(const gfk::GridDesc &g, int th, int64_t step, bool probe, std::set<uintptr_t> &reads,
-                std::set<uintptr_t> &writes) {
+void check_desc(const gfk::GridDesc &g, int span, int th, int count, int64_t step, bool probe,
+                std::set<uintptr_t> &reads, std::set<uintptr_t> &writes) {
   if (g.th != th) fail("descriptor planned for %d tile rows launched with %d", g.th, th);
   if (g.tiles_z != (g.nz + g.th - 1) / g.th) fail("descriptor tiles_z %d for nz %d at %d rows", g.tiles_z, g.nz, g.th);
+  if (g.tiles_r != (g.nr + gfl::tile_cols(span) - 1) / gfl::tile_cols(span))
+    fail("descriptor tiles_r %d for nr %d launched with the %d-step kernel", g.tiles_r, g.nr, span);
+  // every column a halo lane reads stays inside its row and past nr
+  const int64_t widest = std::max<int64_t>((int64_t)((g.nr + gfk::kOwn - 1) / gfk::kOwn) * gfk::kOwn,
+                                           (int64_t)((g.nr + gfk::kOwn2 - 1) / gfk::kOwn2) * gfk::kOwn2 + 1);
+  if (widest > g.pitch - 1 || g.nr > g.pitch - 2) fail("pitch %d too narrow for nr %d", g.pitch, g.nr);
   const size_t pitch = (size_t)g.pitch, nz = (size_t)g.nz;
   const size_t origin = gfk::kGuardCols + gfk::kGuardRows * pitch;
   const size_t field = (size_t)g.field_stride;
@@ -170,12 +179,14 @@ void check_desc(const gfk::GridDesc &g, int th, int64_t step, bool probe, std::s
       check_span(g.vpsi_r[b][k], rs, "velocity psi_r strip", writes);
       check_span(g.vpsi_z[b][k], zs, "velocity psi_z strip", writes);
     }
-  for (int k = 0; k < 2; k++) {
-    check_span(g.spsi_r[k], rs, "stress psi_r strip", writes);
-    check_span(g.spsi_z[k], zs, "stress psi_z strip", writes);
-  }
-  if (g.lossy) check_span(g.mem, (size_t)4 * g.nmech * nz * pitch, "memory variables", writes);
-  check_span(g.rowc, (size_t)gfk::kRowN * nz, "row table", reads);
+  for (int b = 0; b < 2; b++)
+    for (int k = 0; k < 2; k++) {
+      check_span(g.spsi_r[b][k], rs, "stress psi_r strip", writes);
+      check_span(g.spsi_z[b][k], zs, "stress psi_z strip", writes);
+    }
+  if (g.lossy)
+    for (int b = 0; b < 2; b++) check_span(g.mem[b], (size_t)4 * g.nmech * nz * pitch, "memory variables", writes);
+  check_span(g.rowc, (size_t)gfk::kRowW * nz, "row table", reads);
   check_span(g.colc, (size_t)gfk::kColN * pitch, "column table", reads);
   // A model whose run has ended may still be in a launch (a graph block
   // captured for a longer running set): its tiles return at once.
@@ -185,14 +196,18 @@ void check_desc(const gfk::GridDesc &g, int th, int64_t step, bool probe, std::s
       check_span(g.src_tzz, g.nsrc, "source tzz", reads);
       check_span(g.src_dtdt, g.nsrc, "source rate", reads);
     }
-    if (g.nrec > 0) {
-      check_span(g.rec_col, g.nrec, "receiver columns", reads);
-      const int64_t slot = step - g.rec_base;
-      if (slot < 0 || slot >= g.trace_cap)
-        fail("step %lld writes trace slot %lld of %lld", (long long)step, (long long)slot, (long long)g.trace_cap);
-      check_span(g.traces, (size_t)g.nrec * g.trace_cap, "traces", writes);
+    // the steps this launch takes for this model
+    const int64_t last = std::min<int64_t>(step + count, g.steps);
+    for (int64_t n = step; n < last; n++) {
+      if (g.nrec > 0) {
+        check_span(g.rec_col, g.nrec, "receiver columns", reads);
+        const int64_t slot = n - g.rec_base;
+        if (slot < 0 || slot >= g.trace_cap)
+          fail("step %lld writes trace slot %lld of %lld", (long long)n, (long long)slot, (long long)g.trace_cap);
+        check_span(g.traces, (size_t)g.nrec * g.trace_cap, "traces", writes);
+      }
+      if (probe && n % 256 == 0) check_span(g.maxbits + n / 256, 1, "probe slot", writes);
     }
-    if (probe && step % 256 == 0) check_span(g.maxbits + step / 256, 1, "probe slot", writes);
   }
 }
 
@@ -200,19 +215,26 @@ void run_launch(ihipStream_t *s, const Launch &L) {
   g_stats.launches++;
   std::set<uintptr_t> reads, writes;
   if (L.kind == 1) {
-    writes.insert(need(L.base_ptr, 4, "graph base"));
+    writes.insert(need(L.base_ptr, 12, "graph base"));
     enqueue(s, reads, writes, "k_set_base");
-    *L.base_ptr = L.base_val;
+    L.base_ptr[0] = L.base_val;
+    L.base_ptr[1] = L.base_buf;
+    L.base_ptr[2] = L.base_mbuf;
     return;
   }
   const uintptr_t dd = need(L.d, sizeof(gfk::GridDesc), "descriptor array");
   reads.insert(dd);
   const size_t ndesc = (g_live[dd] - ((uintptr_t)L.d - dd)) / sizeof(gfk::GridDesc);
-  int32_t base = 0;
+  int32_t base = 0, bbuf = 0, mbuf = 0;
   if (L.nbase) {
-    reads.insert(need(L.nbase, 4, "graph base"));
-    base = *L.nbase;
+    reads.insert(need(L.nbase, 12, "graph base"));
+    base = L.nbase[0];
+    bbuf = L.nbase[1];
+    mbuf = L.nbase[2];
+    if (bbuf != 0 && bbuf != 1) fail("graph base buffer %d", bbuf);
+    if (mbuf != 0 && mbuf != 1) fail("graph base memory buffer %d", mbuf);
   }
+  const int rb = (L.rb_off ^ bbuf) & 1, mb = (L.mb_off ^ mbuf) & 1;
   const int64_t padded = (L.tiles + L.waves - 1) / L.waves * L.waves;
   std::vector<int64_t> seen(ndesc, 0);
   if (L.tm) {
@@ -232,7 +254,24 @@ void run_launch(ihipStream_t *s, const Launch &L) {
     seen[0] = L.tiles;
   }
   for (size_t m = 0; m < ndesc; m++)
-    if (seen[m]) check_desc(L.d[m], L.th, (int64_t)base + L.n_off, L.probe != 0, reads, writes);
+    if (seen[m]) {
+      const gfk::GridDesc &g = L.d[m];
+      check_desc(g, L.span, L.th, L.count, (int64_t)base + L.n_off, L.probe != 0, reads, writes);
+      // the launch must read the buffer the model's last launch wrote (any,
+      // after its state was cleared)
+      if ((int64_t)base + L.n_off < g.steps) {
+        auto it = g_state_buf.find((uintptr_t)g.f[0][0]);
+        if (it != g_state_buf.end() && it->second != rb)
+          fail("step %lld reads buffer %d but the model's wavefields are in buffer %d", (long long)(base + L.n_off), rb,
+               it->second);
+        g_state_buf[(uintptr_t)g.f[0][0]] = rb ^ 1;
+        it = g_state_buf.find((uintptr_t)g.spsi_r[0][0]);
+        if (it != g_state_buf.end() && it->second != mb)
+          fail("step %lld reads memory buffer %d but the model's memory variables are in buffer %d",
+               (long long)(base + L.n_off), mb, it->second);
+        g_state_buf[(uintptr_t)g.spsi_r[0][0]] = L.span == 2 ? mb ^ 1 : mb;
+      }
+    }
   enqueue(s, reads, writes, "k_step");
 }
 
@@ -308,6 +347,8 @@ hipError_t hipFree(void *p) {
   auto it = g_live.find((uintptr_t)p);
   if (it == g_live.end()) fail("hipFree(%p): not the base of a live allocation", p);
   retire_all();  // hipFree waits for the device
+  for (auto jt = g_state_buf.begin(); jt != g_state_buf.end();)
+    jt = (jt->first >= it->first && jt->first < it->first + it->second) ? g_state_buf.erase(jt) : std::next(jt);
   g_live.erase(it);
   g_stats.frees++;
   free(p);
@@ -490,17 +531,24 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st) {
 hipError_t launch_fill_f32(float *p, int64_t n, float v, hipStream_t st) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   no_capture(st, "fill kernel");
+  // a cleared state may be read from either buffer
+  for (auto it = g_state_buf.begin(); it != g_state_buf.end();)
+    it = (it->first >= (uintptr_t)p && it->first < (uintptr_t)(p + n)) ? g_state_buf.erase(it) : std::next(it);
   enqueue(st, {}, {need(p, sizeof(float) * n, "fill_f32")}, "k_fill_f32");
   std::fill(p, p + n, v);
   return hipSuccess;
 }
 
-hipError_t launch_set_base(int32_t *p, int32_t v, hipStream_t st) {
+hipError_t launch_set_base(int32_t *p, int32_t v, int32_t buf, int32_t mbuf, hipStream_t st) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (buf != 0 && buf != 1) fail("set_base with buffer %d", buf);
+  if (mbuf != 0 && mbuf != 1) fail("set_base with memory buffer %d", mbuf);
   Launch L;
   L.kind = 1;
   L.base_ptr = p;
   L.base_val = v;
+  L.base_buf = buf;
+  L.base_mbuf = mbuf;
   submit(st, L);
   return hipSuccess;
 }
@@ -510,28 +558,36 @@ hipError_t launch_build_tables(const TableParams &P, const LayerParams *d_layers
   std::lock_guard<std::recursive_mutex> lk(g_mu);
   no_capture(st, "table kernel");
   std::set<uintptr_t> r{need(d_layers, sizeof(LayerParams) * P.nlayers, "layer table")};
-  std::set<uintptr_t> w{need(rowc, sizeof(float) * (kRowN + 2 * P.nmech_table) * P.nz, "row table"),
+  if (kRowN + 2 * P.nmech_table > kRowW) fail("row table of %d mechanisms", P.nmech_table);
+  std::set<uintptr_t> w{need(rowc, sizeof(float) * kRowW * P.nz, "row table"),
                         need(colc, sizeof(float) * kColN * P.pitch, "column table")};
   enqueue(st, r, w, "k_build_tables");
   return hipSuccess;
 }
 
-hipError_t launch_step(int nmech, bool scalar, int th, int waves, const GridDesc *d, const uint16_t *tile_model,
-                       int64_t tiles, int32_t n_off, const int32_t *nbase, int32_t probe, hipStream_t st) {
+hipError_t launch_step(const StepLaunch &S, hipStream_t st) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  if (tiles <= 0) return hipSuccess;
-  if (waves != 1 && waves != 2 && waves != 4) fail("launch with %d tiles per workgroup", waves);
+  if (S.tiles <= 0) return hipSuccess;
+  if (S.waves != 1 && S.waves != 2 && S.waves != 4) fail("launch with %d tiles per workgroup", S.waves);
+  if (S.span != 1 && S.span != 2) fail("launch of the %d-step kernel", S.span);
+  if (S.count < 1 || S.count > S.span) fail("launch of %d steps with the %d-step kernel", S.count, S.span);
+  if (S.rb_off != 0 && S.rb_off != 1) fail("launch reading buffer %d", S.rb_off);
+  if (S.mb_off != 0 && S.mb_off != 1) fail("launch reading memory buffer %d", S.mb_off);
   Launch L;
-  L.nmech = nmech;
-  L.scalar = scalar;
-  L.th = th;
-  L.waves = waves;
-  L.d = d;
-  L.tm = tile_model;
-  L.tiles = tiles;
-  L.n_off = n_off;
-  L.nbase = nbase;
-  L.probe = probe;
+  L.nmech = S.nmech;
+  L.scalar = S.scalar;
+  L.span = S.span;
+  L.count = S.count;
+  L.th = S.th;
+  L.waves = S.waves;
+  L.d = S.d;
+  L.tm = S.tile_model;
+  L.tiles = S.tiles;
+  L.n_off = S.n_off;
+  L.rb_off = S.rb_off;
+  L.mb_off = S.mb_off;
+  L.nbase = S.nbase;
+  L.probe = S.probe;
   submit(st, L);
   return hipSuccess;
 }

@@ -1,10 +1,12 @@
-"""The step kernel's arithmetic and indexing, checked on the CPU.
+"""The step kernels' arithmetic and indexing, checked on the CPU.
 
-tests/emu runs the per-lane phases of csrc/stencil.h and the coefficient
-builders of csrc/coef.h exactly as the gfx950 kernel strings them together,
-64 lanes at a time, with every device array a separate allocation of the
-size the device arena gives it.  Its traces must match the FP64 oracle to the
-FP32 bound, for every tile height; the AddressSanitizer build must run clean.
+tests/emu runs the per-lane phases of csrc/stencil.h and csrc/march2.h and
+the coefficient builders of csrc/coef.h exactly as the gfx950 kernels string
+them together, 64 lanes at a time, with every device array a separate
+allocation of the size the device arena gives it.  Its traces must match the
+FP64 oracle to the FP32 bound, for every tile height; the two-step
+(temporally blocked) kernel must give the one-step kernel's bits; the
+AddressSanitizer build must run clean.
 """
 
 import os
@@ -50,6 +52,31 @@ def test_emulated_kernel_matches_oracle(fdtd, oracle, name, th):
     assert probes[0] > 0 and probes[1] > 0
 
 
+@pytest.mark.parametrize("name", ["elastic", "sls", "gsls-layered", "sls-layered"])
+def test_two_step_kernel_gives_the_one_step_bits(fdtd, oracle, name):
+    """Two steps per launch (march2.h) with every tile height, over an odd
+    number of steps (the last launch takes one), equal one step per launch
+    bit for bit -- traces and divergence probes -- since both run the same
+    per-cell arithmetic; and the traces match the oracle."""
+    import emu
+
+    m = cases(fdtd)[name]
+    o = oracle.OracleSim(m)
+    dt = o.Dt()
+    steps = 301
+    force = oracle.ricker(1000, 35, dt, steps)
+    ranges = [0.5, 1.0, 2.5, 4.0]
+    cols = [o.Column(r)[0] for r in ranges]
+    want = oracle.run(m, force, ranges, steps)
+    one, p1 = emu.run(m, force, cols, steps, 3, span=1)
+    for th in (1, 2, 5, 16):
+        two, p2 = emu.run(m, force, cols, steps, th, span=2)
+        np.testing.assert_array_equal(two, one, err_msg=f"{name} th={th}")
+        np.testing.assert_array_equal(p2, p1)
+    for k in range(len(ranges)):
+        assert rel_l2(one[k], want.traces[k]) <= 1e-5, (name, k)
+
+
 def test_emulated_kernel_runs_clean_under_asan(tmp_path):
     """The same emulation built with -fsanitize=address, in a child process
     with the sanitizer runtime preloaded."""
@@ -67,8 +94,9 @@ def test_emulated_kernel_runs_clean_under_asan(tmp_path):
         "for m in [fd.Model(Stack=two, GslsMechanisms=3, MaxRange=7.3, Depth=5.1, Spacing=0.1, PMLCells=12),\n"
         "          fd.Model(Stack=two, Relax=fd.Fit(40, 30), MaxRange=3.9, Depth=2.3, Spacing=0.1, PMLCells=5)]:\n"
         "    dt = O.OracleSim(m).Dt()\n"
-        "    for th in (1, 2, 5, 64):\n"
-        "        emu.run(m, O.ricker(1000, 35, dt, 40), [0, 3, 17], 40, th, asan=True)\n"
+        "    for span, ths in ((1, (1, 2, 5, 64)), (2, (1, 2, 3, 7, 64))):\n"
+        "        for th in ths:\n"
+        "            emu.run(m, O.ricker(1000, 35, dt, 41), [0, 3, 17], 41, th, asan=True, span=span)\n"
         "print('asan ok')\n"
     )
     env = dict(os.environ, LD_PRELOAD=libasan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")

@@ -92,8 +92,16 @@ def check(name, got, z, info, grows=False):
     assert np.all(np.abs(late_got - late_want) <= 1e-3 * 1.0 + 1e-2 * late_want)
 
 
+@pytest.fixture(params=[1, 2], ids=["one-step", "two-step"])
+def span(request, monkeypatch):
+    """Time steps per kernel launch: the one-step kernel, or the temporally
+    blocked two-step kernel."""
+    monkeypatch.setenv("GEOFDTD_STEPS_PER_LAUNCH", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("variant", ["R", "X"])
-def test_c2_whole_shot(fdtd, variant):
+def test_c2_whole_shot(fdtd, variant, span):
     """Config 2, all 80,000 steps, ten receivers from 0.5 to 50 m: the bound
     of WINDOWS before the reference scheme's growth, 1e-2 over it."""
     import cosim_amd.workloads as W
@@ -111,7 +119,7 @@ def test_c2_whole_shot(fdtd, variant):
     check(name, [t.Vertical for t in got.Traces], z, info, grows=True)
 
 
-def test_c3_smallest_soils_whole_second(fdtd):
+def test_c3_smallest_soils_whole_second(fdtd, span):
     """Config-3 soils 204 and 153 (seed 0), batched together, over their
     whole second (6,934 and 7,025 steps), ten bank ranges each."""
     import cosim_amd.workloads as W
@@ -123,6 +131,7 @@ def test_c3_smallest_soils_whole_second(fdtd):
     for w, (z, _) in zip(ws, golds):
         assert w.steps == int(z["steps"])
     got = fdtd.RunBatch(shots)
+    assert fdtd.BatchStats()["steps_per_launch"] == span
     for name, g, (z, info) in zip(names, got, golds):
         assert g.Dt == float(z["dt"])
         check(name, [t.Vertical for t in g.Traces], z, info)

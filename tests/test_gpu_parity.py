@@ -317,6 +317,81 @@ def test_batch_equals_single_runs(fdtd, tile_rows):
         assert b.Peak == single.Peak
 
 
+@pytest.mark.parametrize("gsls", [0, 1, 3])
+def test_two_step_kernel_equals_one_step(fdtd, oracle, monkeypatch, gsls):
+    """The temporally blocked kernel (two steps per launch, halo recomputed
+    over three rows and two columns) gives the one-step kernel's traces and
+    peak: bit-identical across its own tile heights, within 1e-6 of the peak
+    against the one-step kernel (the two are compiled separately, so a product
+    may be contracted into an FMA in one and not the other), within 1e-4 of
+    the FP64 oracle, over an odd number of steps (the last launch takes one)
+    and in a batch of uneven models."""
+    kw = _small(fdtd)
+    if gsls == 3:
+        m = fdtd.Model(**kw, GslsMechanisms=3)
+        m.Stack[0].Qp = 50
+    elif gsls == 1:
+        m = fdtd.Model(**kw, Relax=fdtd.Fit(25, 30))
+    else:
+        m = fdtd.Model(**kw)
+    dt = m.Grid()[0]
+    steps = 701
+    force = fdtd.Ricker(1000, 30, dt, steps)
+    ranges = [0.5, 1.0, 3.0, 6.0]
+    other = fdtd.Model(Stack=[fdtd.Layer(1.5, 400, 180, 1650, 20), fdtd.Layer(0, 600, 260, 1900, 60)],
+                       MaxRange=7.0, Depth=9.0, Spacing=0.12, DominantFreq=30, GslsMechanisms=3 if gsls == 3 else 0)
+    if gsls == 1:
+        other.Relax = fdtd.Fit(30, 30)
+    dt2 = other.Grid()[0]
+    shots = [fdtd.Shot(m, force, ranges, steps), fdtd.Shot(other, fdtd.Ricker(900, 35, dt2, 450), [1.0, 2.5], 450)]
+    out, peaks = {}, {}
+    for span, th in [(1, 3), (2, 8), (2, 5), (2, 16), (2, 200)]:
+        monkeypatch.setenv("GEOFDTD_STEPS_PER_LAUNCH", str(span))
+        monkeypatch.setenv("GEOFDTD_TILE_ROWS", str(th))
+        single = fdtd.Run(shots[0])
+        batch = fdtd.RunBatch(shots)
+        assert fdtd.BatchStats()["steps_per_launch"] == span
+        out[span, th] = [np.array([t.Vertical for t in r.Traces]) for r in (single, batch[0], batch[1])]
+        peaks[span, th] = [single.Peak, batch[0].Peak, batch[1].Peak]
+        np.testing.assert_array_equal(out[span, th][0], out[span, th][1])  # batch = single
+    ref = out[2, 8]
+    for th in [5, 16, 200]:
+        for a, b in zip(out[2, th], ref):
+            np.testing.assert_array_equal(a, b)
+        assert peaks[2, th] == peaks[2, 8]
+    for a, b in zip(out[1, 3], ref):
+        assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(a))
+    for p1, p2 in zip(peaks[1, 3], peaks[2, 8]):
+        assert abs(p1 - p2) <= 1e-6 * p1
+    want = oracle.run(m, force, ranges, steps)
+    compare(f"two-step, {gsls} mechanisms", ref[0], want.traces)
+
+
+def test_two_step_sim_api_matches_oracle(fdtd, oracle):
+    """The step API with two steps per launch: odd step counts between
+    receiver reads (a launch of one step, then pairs again from an odd
+    step), a reset, and the graph-replayed advance."""
+    m = fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
+    s = fdtd.New(m)
+    s.SetLaunch(2, 8)
+    assert s.LaunchShape()["steps_per_launch"] == 2
+    dt = s.Dt()
+    n = 643
+    force = fdtd.Ricker(1000, 30, dt, n)
+    s.Drive(force)
+    cols = [s.Column(1.0)[0], s.Column(3.0)[0]]
+    got = []
+    for chunk in (1, 2, 3, 137, 500):
+        got.append(s.StepN(chunk, cols))
+    g = np.concatenate(got, axis=1)
+    want = oracle.run(m, force, [1.0, 3.0], n)
+    compare("two-step step API", g, want.traces)
+    s.Reset()
+    again = np.concatenate([s.StepN(k, cols) for k in (200, 443)], axis=1)
+    np.testing.assert_array_equal(again, g)
+    s.close()
+
+
 def test_batch_reports_each_shots_error(fdtd):
     good = fdtd.Model(Stack=loam(), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30)
     dt = good.Grid()[0]

@@ -9,8 +9,10 @@ stream's unfinished operations and fails on a synchronous copy or another
 stream's operation that touches memory one of them still uses, on anything
 but kernels recorded into a captured graph, and on a step launch (direct or
 replayed) whose descriptors, tile-to-model table, field planes, C-PML strips,
-memory variables, tables, trace slot or probe slot are not live allocations
-of the sizes the device layout gives them.
+memory variables, tables, trace slots or probe slots are not live
+allocations of the sizes the device layout gives them, whose tiling is not
+the launched kernel's, or which reads another buffer than the one the
+model's previous launch wrote.
 
 The scenarios drive the batched driver as the config-3 bank generator runs
 it: cache-sized chunks with the tile-height tuner re-planning the group
@@ -18,7 +20,8 @@ between trial launches (the path a profiled run once ended in a host fault
 on), the chunk budget tuned over trial chunks re-planned between trial
 launches, all models advanced together, mixed kernel variants with failing shots
 and a bank generation, the step API, and the batch as a handle advanced in
-pieces, reset half-way and after its end, and run again.
+pieces, reset half-way and after its end, and run again; each with one and
+with two time steps per launch.
 """
 
 import os
@@ -37,9 +40,10 @@ def driver():
     return os.path.join(HERE, "lib", "batch_driver")
 
 
+@pytest.mark.parametrize("span", [1, 2])
 @pytest.mark.parametrize("scenario", ["chunked", "together", "mixed", "sim", "handle", "tuned"])
-def test_host_path_is_clean_under_asan(driver, scenario):
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0")
+def test_host_path_is_clean_under_asan(driver, scenario, span):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", GEOFDTD_STEPS_PER_LAUNCH=str(span))
     env.pop("GEOFDTD_TILE_ROWS", None)
     p = subprocess.run([driver, scenario], capture_output=True, text=True, timeout=300, env=env)
     print(p.stdout + p.stderr[-3000:])
