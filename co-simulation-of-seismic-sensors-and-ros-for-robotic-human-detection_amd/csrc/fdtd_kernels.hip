@@ -89,35 +89,12 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // The step is n_off (+ nbase[0] when replayed from a graph); the wavefields
 // are read from buffer rb_off (^ nbase[1]) and written to the other, the
 // memory variables updated in place in buffer mb_off (^ nbase[2]).
-template <int NM, bool SC, int TH>
-__global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
-                                              int32_t n_off, int32_t rb_off, int32_t mb_off,
-                                              const int32_t *__restrict__ nbase, int32_t probe) {
-  const int lane = threadIdx.x & (kLanes - 1);
-  // Wave-uniform by construction; readfirstlane lets the compiler keep the
-  // descriptor reads scalar.
-  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
-
-  // A batch's tiles are numbered model by model; the host-built table gives
-  // each tile's model in one load (a single model needs none).
-  const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
-  const GridDesc &g = gds[m];
-  const int n = n_off + (nbase ? nbase[0] : 0);
-  if (n >= g.steps) return;
-  // MaxSpeed every 256 steps (run.go:118-124), folded into the step.
-  const bool do_max = probe && (n & 255) == 0;
-  const int lt = tile - g.tile_base;
-  if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
-
-  const int th = TH > 0 ? TH : g.th;
-  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
-  const int i0 = tr * kOwn, j0 = tzi * th;
-  const int j1 = min(j0 + th, g.nz);
+template <int NM, bool SC, int TH, bool IN>
+__device__ __forceinline__ void step_tile(const GridDesc &g, int n, int rb, int mb, bool do_max, int th, int i0,
+                                          int j0, int j1, int lane) {
+  const int wb = rb ^ 1;
   const int c = i0 - 1 + lane;
-  const bool own = lane >= 1 && lane <= kOwn && c < g.nr;
-  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1, wb = rb ^ 1;
-  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1;
+  const bool own = lane >= 1 && lane <= kOwn && (IN || c < g.nr);
   const ColConst cc = load_col(g, c);
   GF_GLOBAL float *const *A = g.f[rb];
   GF_GLOBAL float *const *B = g.f[wb];
@@ -136,15 +113,15 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
     const float trr1 = lane_next(cur.trr), tqq1 = lane_next(cur.tqq), trzm = lane_prev(cur.trz);
     const bool row_own = t >= j0 && t < j1;
     float vrn, vzn;
-    velocity(g, rb, t, c, row_own && own, cc, cur.trr, trr1, cur.tqq, tqq1, cur.trz, trzm, trz_p, tzz_c,
-             cur.tzz_dn, cur.vr, cur.vz, vrn, vzn);
+    velocity<IN>(g, rb, t, c, row_own && own, cc, cur.trr, trr1, cur.tqq, tqq1, cur.trz, trzm, trz_p, tzz_c,
+                 cur.tzz_dn, cur.vr, cur.vz, vrn, vzn);
     if (row_own && own) {
       st_stream(at(B[kVr], g, t, c), vrn);
       st_stream(at(B[kVz], g, t, c), vzn);
       if (do_max) mbits = max(mbits, max(abs_bits(vrn), abs_bits(vzn)));
     }
     const float vrn_m = lane_prev(vrn);
-    if (t == 0 && g.nrec > 0) {
+    if (!IN && t == 0 && g.nrec > 0) {
       // Receivers sit on the surface row; the tile holding row 0 records them.
       const float sv = surface_velocity(g, c, cc, vrn, vrn_m, vzn);
       const int64_t slot = (int64_t)n - g.rec_base;
@@ -161,7 +138,7 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
       const float vzn_p1 = lane_next(vzn_p);  // new vz(s, c+1)
       if (own) {
         float nrr = trr_p, nqq = tqq_p, nzz = tzz_p, nrz = trz_p;
-        stress<NM, SC>(g, mb, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
+        stress<NM, SC, IN>(g, mb, n, s, c, cc, vrn_p, vrn_pm, vrn, vzn_p, vzn_p1, vzn_pp, nrr, nqq, nzz, nrz, mem_s);
         st_stream(at(B[kTrr], g, s, c), nrr);
         st_stream(at(B[kTqq], g, s, c), nqq);
         st_stream(at(B[kTzz], g, s, c), nzz);
@@ -220,46 +197,60 @@ __global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, 
   }
 }
 
-// Two time steps per launch (march2.h): the tile is kOwn2 columns by g.th
-// rows; stage 1 (step n) marches rows j0-3 .. j1+2 from buffers rb (fields)
-// and mb (memory variables, stress psi), stage 2 (step n+1) follows two rows
-// behind and stores rows [j0, j1) to buffers rb ^ 1 and mb ^ 1.  With count == 1, or when step n is a model's last, stage 1's
-// owned rows are stored instead (one step).  Loads run one march row ahead;
-// every per-row array is indexed by the march row modulo 3 at compile time,
-// so the rows in flight stay in registers without copies.
-template <int NM, bool SC>
-__global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
-                                               int32_t n_off, int32_t rb_off, int32_t mb_off,
-                                               const int32_t *__restrict__ nbase, int32_t probe, int32_t count) {
+template <int NM, bool SC, int TH>
+__global__ __launch_bounds__(256) void k_step(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
+                                              int32_t n_off, int32_t rb_off, int32_t mb_off,
+                                              const int32_t *__restrict__ nbase, int32_t probe) {
   const int lane = threadIdx.x & (kLanes - 1);
+  // Wave-uniform by construction; readfirstlane lets the compiler keep the
+  // descriptor reads scalar.
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+
+  // A batch's tiles are numbered model by model; the host-built table gives
+  // each tile's model in one load (a single model needs none).
   const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
   const GridDesc &g = gds[m];
   const int n = n_off + (nbase ? nbase[0] : 0);
   if (n >= g.steps) return;
+  // MaxSpeed every 256 steps (run.go:118-124), folded into the step.
+  const bool do_max = probe && (n & 255) == 0;
   const int lt = tile - g.tile_base;
   if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
-  // A model whose run ends after step n takes one step.
-  const bool two = count >= 2 && n + 1 < g.steps;
-  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1, wb = rb ^ 1;
-  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1, mw = mb ^ 1;
 
-  const int th = g.th;
+  const int th = TH > 0 ? TH : g.th;
   const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
-  const int i0 = tr * kOwn2, j0 = tzi * th;
+  const int i0 = tr * kOwn, j0 = tzi * th;
   const int j1 = min(j0 + th, g.nz);
+  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1;
+  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1;
+  // Velocities are formed at columns i0-1 .. i0+62 and rows j0-1 .. j1.
+  if (interior(g, i0 - 1, i0 + kOwn, j0 - 1, j1))
+    step_tile<NM, SC, TH, true>(g, n, rb, mb, do_max, th, i0, j0, j1, lane);
+  else
+    step_tile<NM, SC, TH, false>(g, n, rb, mb, do_max, th, i0, j0, j1, lane);
+}
+
+// Two time steps per launch (march2.h): the tile is kOwn2 columns by g.th
+// rows; stage 1 (step n) marches rows j0-3 .. j1+2 from buffers rb (fields)
+// and mb (memory variables, stress psi), stage 2 (step n+1) follows two rows
+// behind and stores rows [j0, j1) to buffers rb ^ 1 and mb ^ 1.  With
+// count == 1, or when step n is a model's last, stage 1's owned rows are
+// stored instead (one step).  Loads run one march row ahead;
+// every per-row array is indexed by the march row modulo 3 at compile time,
+// so the rows in flight stay in registers without copies.
+template <int NM, bool SC, bool IN>
+__device__ __forceinline__ void step2_tile(const GridDesc &g, int n, bool two, int rb, int mb, bool max1, bool max2,
+                                           int i0, int j0, int j1, int lane) {
+  const int wb = rb ^ 1, mw = mb ^ 1;
   const int c = i0 - 2 + lane;
-  const bool own = lane >= 2 && lane < 2 + kOwn2 && c < g.nr;
+  const bool own = lane >= 2 && lane < 2 + kOwn2 && (IN || c < g.nr);
   const ColConst cc = load_col(g, c);
   GF_GLOBAL float *const *A = g.f[rb];
   GF_GLOBAL float *const *B = g.f[wb];
-  // MaxSpeed every 256 steps; at most one of the two steps is a multiple.
-  const bool max1 = probe && (n & 255) == 0;
-  const bool max2 = two && probe && ((n + 1) & 255) == 0;
   uint32_t mbits = 0;
   // Receivers sit on row 0; the tile that owns it records them.
-  const bool rec = j0 == 0 && g.nrec > 0;
+  const bool rec = !IN && j0 == 0 && g.nrec > 0;
   auto record = [&](int64_t step, float sv) {
     const int64_t slot = step - g.rec_base;
     if (slot >= 0 && slot < g.trace_cap) {
@@ -274,7 +265,7 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
   Stage S1 = stage_start(ld(A[kTrz], g, j0 - 4, c), ld_tzz(A[kTzz], g, n, j0 - 3, c));
   Stage S2 = stage_start(0.f, 0.f);
   // stage 1's inputs (loaded a row ahead) and outputs, by march row mod 3
-  RowLoad IN[3];
+  RowLoad ROW[3];
   VPsi IVP[3];
   SPsi ISP[3];
   MemVars<NM> IMEM[3];
@@ -283,15 +274,19 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
 
   auto fetch = [&](auto P, int t) {
     constexpr int p = decltype(P)::value;
-    IN[p] = load_row(A, g, n, t, c);
+    ROW[p] = load_row(A, g, n, t, c);
     IVP[p] = VPsi{0.f, 0.f, 0.f, 0.f};
-    if (vel_cell(g, t, c)) IVP[p] = load_vpsi(g, rb, t, c);
+    if (!IN && vel_cell(g, t, c)) IVP[p] = load_vpsi(g, rb, t, c);
     const int s = t - 1;
     ISP[p] = SPsi{0.f, 0.f, 0.f, 0.f};
-    IMEM[p] = MemVars<NM>{};
-    if (s >= j0 - 2 && grid_cell(g, s, c)) {
-      ISP[p] = load_spsi(g, mb, s, c);
-      load_mem<NM>(g, mb, s, c, IMEM[p]);
+    if (IN) {
+      if (s >= j0 - 2) load_mem<NM>(g, mb, s, c, IMEM[p]);
+    } else {
+      IMEM[p] = MemVars<NM>{};
+      if (s >= j0 - 2 && grid_cell(g, s, c)) {
+        ISP[p] = load_spsi(g, mb, s, c);
+        load_mem<NM>(g, mb, s, c, IMEM[p]);
+      }
     }
   };
 
@@ -299,19 +294,19 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
     constexpr int p = decltype(P)::value;
     constexpr int p1 = (p + 1) % 3, p2 = (p + 2) % 3;
     if (t + 1 <= t_end) fetch(std::integral_constant<int, p1>(), t + 1);
-    const RowLoad &in = IN[p];
+    const RowLoad &in = ROW[p];
 
     // stage 1: velocities of row t
     VelOut &v1 = QV[p];
     v1.vp = IVP[p];
-    stage_velocity(g, t, c, cc, S1, in, lane_next(in.trr), lane_next(in.tqq), lane_prev(in.trz), v1.vp, v1.vr,
-                   v1.vz);
+    stage_velocity<IN>(g, t, c, cc, S1, in, lane_next(in.trr), lane_next(in.tqq), lane_prev(in.trz), v1.vp, v1.vr,
+                       v1.vz);
     const bool vown = own && t >= j0 && t < j1;
     if (vown) {
       if (!two) {
         st_stream(at(B[kVr], g, t, c), v1.vr);
         st_stream(at(B[kVz], g, t, c), v1.vz);
-        if (vel_cell(g, t, c)) store_vpsi(g, wb, t, c, v1.vp);
+        if (!IN && vel_cell(g, t, c)) store_vpsi(g, wb, t, c, v1.vp);
       }
       if (max1) mbits = max(mbits, max(abs_bits(v1.vr), abs_bits(v1.vz)));
     }
@@ -324,13 +319,13 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
       StrOut<NM> &o1 = QS[p];
       o1.sp = ISP[p];
       o1.mem = IMEM[p];
-      stage_stress<NM, SC>(g, n, s, c, cc, S1, v1.vr, vrn_pm, vzn_p1, o1.st, o1.sp, o1.mem);
+      stage_stress<NM, SC, IN>(g, n, s, c, cc, S1, v1.vr, vrn_pm, vzn_p1, o1.st, o1.sp, o1.mem);
       if (!two && own && s >= j0 && s < j1) {
         st_stream(at(B[kTrr], g, s, c), o1.st[0]);
         st_stream(at(B[kTqq], g, s, c), o1.st[1]);
         st_stream(at(B[kTzz], g, s, c), o1.st[2]);
         st_stream(at(B[kTrz], g, s, c), o1.st[3]);
-        store_spsi(g, mw, s, c, o1.sp);
+        if (!IN) store_spsi(g, mw, s, c, o1.sp);
         store_mem<NM>(g, mw, s, c, o1.mem);
       }
     }
@@ -344,21 +339,22 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
     if (u < j0 - 1) return;
     const VelOut &vu = QV[p1];
     const StrOut<NM> &su = QS[p2];
-    if (u == j0 - 1) S2 = stage_start(QS[p1].st[3], tzz_for_step(g, n + 1, u, c, su.st[2]));
+    if (u == j0 - 1) S2 = stage_start(QS[p1].st[3], IN ? su.st[2] : tzz_for_step(g, n + 1, u, c, su.st[2]));
     RowLoad in2;
     in2.trr = su.st[0];
     in2.tqq = su.st[1];
     in2.trz = su.st[3];
-    in2.tzz_dn = tzz_for_step(g, n + 1, u + 1, c, QS[p].st[2]);
+    in2.tzz_dn = IN ? QS[p].st[2] : tzz_for_step(g, n + 1, u + 1, c, QS[p].st[2]);
     in2.vr = vu.vr;
     in2.vz = vu.vz;
     VPsi vp2 = vu.vp;
     float vr2, vz2;
-    stage_velocity(g, u, c, cc, S2, in2, lane_next(in2.trr), lane_next(in2.tqq), lane_prev(in2.trz), vp2, vr2, vz2);
+    stage_velocity<IN>(g, u, c, cc, S2, in2, lane_next(in2.trr), lane_next(in2.tqq), lane_prev(in2.trz), vp2, vr2,
+                       vz2);
     if (own && u >= j0 && u < j1) {
       st_stream(at(B[kVr], g, u, c), vr2);
       st_stream(at(B[kVz], g, u, c), vz2);
-      if (vel_cell(g, u, c)) store_vpsi(g, wb, u, c, vp2);
+      if (!IN && vel_cell(g, u, c)) store_vpsi(g, wb, u, c, vp2);
       if (max2) mbits = max(mbits, max(abs_bits(vr2), abs_bits(vz2)));
     }
     if (rec && u == 0) record((int64_t)n + 1, surface_velocity(g, c, cc, vr2, lane_prev(vr2), vz2));
@@ -368,13 +364,13 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
       SPsi sp2 = QS[p1].sp;
       MemVars<NM> mem2 = QS[p1].mem;
       float o2[4];
-      stage_stress<NM, SC>(g, n + 1, s2, c, cc, S2, vr2, vrn_pm, vzn_p1, o2, sp2, mem2);
+      stage_stress<NM, SC, IN>(g, n + 1, s2, c, cc, S2, vr2, vrn_pm, vzn_p1, o2, sp2, mem2);
       if (own) {
         st_stream(at(B[kTrr], g, s2, c), o2[0]);
         st_stream(at(B[kTqq], g, s2, c), o2[1]);
         st_stream(at(B[kTzz], g, s2, c), o2[2]);
         st_stream(at(B[kTrz], g, s2, c), o2[3]);
-        store_spsi(g, mw, s2, c, sp2);
+        if (!IN) store_spsi(g, mw, s2, c, sp2);
         store_mem<NM>(g, mw, s2, c, mem2);
       }
     }
@@ -394,6 +390,38 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
     for (int off = 32; off >= 1; off >>= 1) mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, off, kLanes));
     if (lane == 0) atomicMax(&g.maxbits[(max1 ? n : n + 1) >> 8], mbits);
   }
+}
+
+template <int NM, bool SC>
+__global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
+                                               int32_t n_off, int32_t rb_off, int32_t mb_off,
+                                               const int32_t *__restrict__ nbase, int32_t probe, int32_t count) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
+  const GridDesc &g = gds[m];
+  const int n = n_off + (nbase ? nbase[0] : 0);
+  if (n >= g.steps) return;
+  const int lt = tile - g.tile_base;
+  if (lt < 0 || lt >= g.tiles_r * g.tiles_z) return;
+  // A model whose run ends after step n takes one step.
+  const bool two = count >= 2 && n + 1 < g.steps;
+  const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1;
+  const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1;
+
+  const int th = g.th;
+  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
+  const int i0 = tr * kOwn2, j0 = tzi * th;
+  const int j1 = min(j0 + th, g.nz);
+  // MaxSpeed every 256 steps; at most one of the two steps is a multiple.
+  const bool max1 = probe && (n & 255) == 0;
+  const bool max2 = two && probe && ((n + 1) & 255) == 0;
+  // Stage 1 forms columns i0-2 .. i0+61, velocity rows j0-3 .. j1+2.
+  if (interior(g, i0 - 2, i0 + kOwn2 + 1, j0 - 3, j1 + 2))
+    step2_tile<NM, SC, true>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
+  else
+    step2_tile<NM, SC, false>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
 }
 
 // Sim.SurfaceVelocity for one column of the buffer that holds step `cur`.

@@ -23,7 +23,10 @@
 // results are those of two one-step launches.
 //
 // Nothing a launch reads is written by it (the halo of one tile is the owned
-// region of another), which is why every state array is double-buffered.
+// region of another): the wavefields and velocity psi are double-buffered
+// for every launch, and a two-step launch also reads the memory variables
+// and stress psi from one buffer and writes them to the other (a one-step
+// launch updates those in place).
 //
 // Plain C++ so the CPU emulation runs the same phases lane by lane.
 #pragma once
@@ -49,25 +52,26 @@ GF_HD Stage stage_start(float trz_above, float tzz_row) {
 // New velocities of row t from the stage's input row `in` (its stresses and
 // velocities at t, tzz at t+1) and the neighbour lanes' trr(c+1), tqq(c+1),
 // trz(c-1).
+template <bool IN = false>
 GF_HD void stage_velocity(const GridDesc &g, int t, int c, const ColConst &cc, const Stage &S, const RowLoad &in,
                           float trr1, float tqq1, float trzm, VPsi &vp, float &vrn, float &vzn) {
-  velocity_core(g, t, c, cc, in.trr, trr1, in.tqq, tqq1, in.trz, trzm, S.trz_p, S.tzz_c, in.tzz_dn, in.vr, in.vz, vp,
-                vrn, vzn);
+  velocity_core<IN>(g, t, c, cc, in.trr, trr1, in.tqq, tqq1, in.trz, trzm, S.trz_p, S.tzz_c, in.tzz_dn, in.vr, in.vz,
+                    vp, vrn, vzn);
 }
 
 // New stresses of row s = t - 1 of time step n (out: trr, tqq, tzz, trz),
 // given the new velocities of row t (vrn) and the neighbour lanes' vr(s, c-1)
 // and vz(s, c+1).  Cells outside the grid keep their input values.
-template <int NM, bool SC>
+template <int NM, bool SC, bool IN = false>
 GF_HD void stage_stress(const GridDesc &g, int n, int s, int c, const ColConst &cc, const Stage &S, float vrn,
                         float vrn_pm, float vzn_p1, float out[4], SPsi &sp, MemVars<NM> &mem) {
   out[0] = S.trr_p;
   out[1] = S.tqq_p;
   out[2] = S.tzz_p;
   out[3] = S.trz_p;
-  if (grid_cell(g, s, c))
-    stress_core<NM, SC>(g, n, s, c, cc, S.vrn_p, vrn_pm, vrn, S.vzn_p, vzn_p1, S.vzn_pp, out[0], out[1], out[2],
-                        out[3], sp, mem);
+  if (IN || grid_cell(g, s, c))
+    stress_core<NM, SC, IN>(g, n, s, c, cc, S.vrn_p, vrn_pm, vrn, S.vzn_p, vzn_p1, S.vzn_pp, out[0], out[1], out[2],
+                            out[3], sp, mem);
 }
 
 // Advance the carried values past row t.

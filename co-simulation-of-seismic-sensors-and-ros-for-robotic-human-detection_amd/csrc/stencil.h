@@ -223,6 +223,15 @@ struct SPsi {
 GF_HD bool vel_cell(const GridDesc &g, int t, int c) { return t >= 0 && t <= g.nz - 2 && c >= 0 && c <= g.nr - 2; }
 GF_HD bool grid_cell(const GridDesc &g, int s, int c) { return s >= 0 && s < g.nz && c >= 0 && c < g.nr; }
 
+// Whether a tile's every computed cell is an updated cell of the interior:
+// columns [clo, chi] and rows [tlo, thi] off the axis and the surface row,
+// inside the updated range and clear of every C-PML strip.  Such a tile runs
+// the case-free form of the per-cell arithmetic (IN = true), which computes
+// the same values.
+GF_HD bool interior(const GridDesc &g, int clo, int chi, int tlo, int thi) {
+  return clo >= 1 && chi <= g.nr - 2 && chi < g.r0 && tlo >= 1 && thi <= g.nz - 2 && thi < g.z0;
+}
+
 // Byte offsets into an r strip (every row, columns from r0) and a z strip
 // (rows from z0, every column), for cells inside the strip; 32-bit, like
 // at()'s, so an access is a scalar base plus a vector offset.
@@ -286,22 +295,24 @@ GF_HD void store_spsi(const GridDesc &g, int b, int s, int c, const SPsi &p) {
 // trz_up = trz(t-1, c), *1 = column c+1, trzm = trz(t, c-1)); psi carries the
 // cell's velocity auxiliaries in and their advanced values out.  Outside the
 // updated cells the velocities are returned unchanged.
+template <bool IN = false>
 GF_HD void velocity_core(const GridDesc &g, int t, int c, const ColConst &cc, float trr, float trr1, float tqq,
                          float tqq1, float trz, float trzm, float trz_up, float tzz, float tzz_dn, float vr_old,
                          float vz_old, VPsi &psi, float &vr_new, float &vz_new) {
+#pragma clang fp contract(on)
   vr_new = vr_old;
   vz_new = vz_old;
-  if (!vel_cell(g, t, c)) return;
+  if (!IN && !vel_cell(g, t, c)) return;
   const float invh = g.invh;
   GF_GLOBAL const float *rc = row_coef(g, t);
 
   float dtrr = (trr1 - trr) * invh;
-  if (vp_trr_r(g, t, c)) {
+  if (!IN && vp_trr_r(g, t, c)) {
     psi.trr_r = cc.brh * psi.trr_r + cc.arh * dtrr;
     dtrr += psi.trr_r;
   }
-  float dtrz = t == 0 ? 2.f * trz * invh : (trz - trz_up) * invh;
-  if (vp_trz_z(g, t, c)) {
+  float dtrz = !IN && t == 0 ? 2.f * trz * invh : (trz - trz_up) * invh;
+  if (!IN && vp_trz_z(g, t, c)) {
     psi.trz_z = rc[kRowBzi] * psi.trz_z + rc[kRowAzi] * dtrz;
     dtrz += psi.trz_z;
   }
@@ -309,16 +320,16 @@ GF_HD void velocity_core(const GridDesc &g, int t, int c, const ColConst &cc, fl
   vr_new = vr_old + rc[kRowCvr] * (dtrr + dtrz + hoop);
 
   float dtzz = (tzz_dn - tzz) * invh;
-  if (vp_tzz_z(g, t, c)) {
+  if (!IN && vp_tzz_z(g, t, c)) {
     psi.tzz_z = rc[kRowBzh] * psi.tzz_z + rc[kRowAzh] * dtzz;
     dtzz += psi.tzz_z;
   }
   float radial;
-  if (c == 0) {
+  if (!IN && c == 0) {
     radial = 4.f * trz * invh;
   } else {
     float d = (trz - trzm) * invh;
-    if (vp_trz_r(g, t, c)) {
+    if (!IN && vp_trz_r(g, t, c)) {
       psi.trz_r = cc.bri * psi.trz_r + cc.ari * d;
       d += psi.trz_r;
     }
@@ -328,20 +339,24 @@ GF_HD void velocity_core(const GridDesc &g, int t, int c, const ColConst &cc, fl
 }
 
 // One-step form: the auxiliaries are read from buffer rb and, when `write`,
-// the advanced ones stored to buffer rb ^ 1.
+// the advanced ones stored to buffer rb ^ 1.  IN: the caller knows the cell
+// is an updated cell outside every C-PML strip, off the axis and off the
+// surface row (a tile of the interior), so no case is tested.
+template <bool IN = false>
 GF_HD void velocity(const GridDesc &g, int rb, int t, int c, bool write, const ColConst &cc, float trr,
                     float trr1, float tqq, float tqq1, float trz, float trzm, float trz_up, float tzz,
                     float tzz_dn, float vr_old, float vz_old, float &vr_new, float &vz_new) {
   VPsi psi = {0.f, 0.f, 0.f, 0.f};
-  const bool cell = vel_cell(g, t, c);
+  const bool cell = !IN && vel_cell(g, t, c);
   if (cell) psi = load_vpsi(g, rb, t, c);
-  velocity_core(g, t, c, cc, trr, trr1, tqq, tqq1, trz, trzm, trz_up, tzz, tzz_dn, vr_old, vz_old, psi, vr_new,
-                vz_new);
+  velocity_core<IN>(g, t, c, cc, trr, trr1, tqq, tqq1, trz, trzm, trz_up, tzz, tzz_dn, vr_old, vz_old, psi, vr_new,
+                    vz_new);
   if (cell && write) store_vpsi(g, rb ^ 1, t, c, psi);
 }
 
 // Crank-Nicolson memory variable (fdtd.go:315-320), drive already formed.
 GF_HD float relax_cn(float decay, float &r, float drive_term) {
+#pragma clang fp contract(on)
   float next = decay * r - drive_term;
   float avg = 0.5f * (r + next);
   r = next;
@@ -382,34 +397,35 @@ GF_HD void store_mem(const GridDesc &g, int b, int s, int c, const MemVars<NM> &
 // auxiliaries and memory variables.  NM = 0 is elastic; SC selects the
 // reference's scalar relaxation (NM = 1, one drive for P and S alike),
 // otherwise separate P and S drives per mechanism.
-template <int NM, bool SC>
+template <int NM, bool SC, bool IN = false>
 GF_HD void stress_core(const GridDesc &g, int n, int s, int c, const ColConst &cc, float vr, float vrm, float vr_dn,
                        float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz, SPsi &psi,
                        MemVars<NM> &mem) {
+#pragma clang fp contract(on)
   const float invh = g.invh, dt = g.dt;
   GF_GLOBAL const float *rc = row_coef(g, s);
   const float lam = rc[kRowLam], l2m = rc[kRowL2m];
 
   float dvr, vrr;
-  if (c == 0) {
+  if (!IN && c == 0) {
     dvr = 2.f * vr * invh;
     vrr = dvr;
   } else {
     dvr = (vr - vrm) * invh;
     vrr = 0.5f * (vr + vrm) * cc.iri;
   }
-  if (sp_vr_r(g, s, c)) {
+  if (!IN && sp_vr_r(g, s, c)) {
     psi.vr_r = cc.bri * psi.vr_r + cc.ari * dvr;
     dvr += psi.vr_r;
   }
   float dvz;
-  if (s == 0) {
+  if (!IN && s == 0) {
     // F6: the applied rate enters unscaled by -1/A, exactly as fdtd.go:251-256.
     float applied = (c == 0 && n < g.nsrc) ? g.src_dtdt[n] : 0.f;
     dvz = (applied - lam * (dvr + vrr)) * rc[kRowIl2m];
   } else {
     dvz = (vz - vz_up) * invh;
-    if (sp_vz_z(g, s, c)) {
+    if (!IN && sp_vz_z(g, s, c)) {
       psi.vz_z = rc[kRowBzi] * psi.vz_z + rc[kRowAzi] * dvz;
       dvz += psi.vz_z;
     }
@@ -437,7 +453,7 @@ GF_HD void stress_core(const GridDesc &g, int n, int s, int c, const ColConst &c
     tqq += dt * (eqq + sq);
   }
 
-  if (s > 0) {
+  if (IN || s > 0) {
     const float ezz = lam * (dvr + vrr) + l2m * dvz;
     if (NM == 0) {
       tzz += dt * ezz;
@@ -458,14 +474,14 @@ GF_HD void stress_core(const GridDesc &g, int n, int s, int c, const ColConst &c
     tzz = n < g.nsrc ? g.src_tzz[n] : 0.f;
   }
 
-  if (s <= g.nz - 2 && c <= g.nr - 2) {
+  if (IN || (s <= g.nz - 2 && c <= g.nr - 2)) {
     float dvzdr = (vz1 - vz) * invh;
-    if (sp_vz_r(g, s, c)) {
+    if (!IN && sp_vz_r(g, s, c)) {
       psi.vz_r = cc.brh * psi.vz_r + cc.arh * dvzdr;
       dvzdr += psi.vz_r;
     }
     float dvrdz = (vr_dn - vr) * invh;
-    if (sp_vr_z(g, s, c)) {
+    if (!IN && sp_vr_z(g, s, c)) {
       psi.vr_z = rc[kRowBzh] * psi.vr_z + rc[kRowAzh] * dvrdz;
       dvrdz += psi.vr_z;
     }
@@ -486,17 +502,19 @@ GF_HD void stress_core(const GridDesc &g, int n, int s, int c, const ColConst &c
 // One-step form: auxiliaries updated in place in buffer mb; the memory
 // variables come in (loaded ahead by the caller) and go out advanced, for
 // the caller to store.
-template <int NM, bool SC>
+template <int NM, bool SC, bool IN = false>
 GF_HD void stress(const GridDesc &g, int mb, int n, int s, int c, const ColConst &cc, float vr, float vrm,
                   float vr_dn, float vz, float vz1, float vz_up, float &trr, float &tqq, float &tzz, float &trz,
                   MemVars<NM> &mem) {
-  SPsi psi = load_spsi(g, mb, s, c);
-  stress_core<NM, SC>(g, n, s, c, cc, vr, vrm, vr_dn, vz, vz1, vz_up, trr, tqq, tzz, trz, psi, mem);
-  store_spsi(g, mb, s, c, psi);
+  SPsi psi = {0.f, 0.f, 0.f, 0.f};
+  if (!IN) psi = load_spsi(g, mb, s, c);
+  stress_core<NM, SC, IN>(g, n, s, c, cc, vr, vrm, vr_dn, vz, vz1, vz_up, trr, tqq, tzz, trz, psi, mem);
+  if (!IN) store_spsi(g, mb, s, c, psi);
 }
 
 // Sim.SurfaceVelocity (fdtd.go:330-342) from this step's row-0 velocities.
 GF_HD float surface_velocity(const GridDesc &g, int c, const ColConst &cc, float vr, float vrm, float vz) {
+#pragma clang fp contract(on)
   float dvr, vrr;
   if (c == 0) {
     dvr = 2.f * vr * g.invh;

@@ -42,7 +42,7 @@ struct Arrays {
   }
 };
 
-template <int NM, bool SC>
+template <int NM, bool SC, bool IN>
 void tile_step(const GridDesc &g, int n, int rb, int mb, int lt, bool probe) {
   const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
   const int i0 = tr * kOwn, j0 = tzi * g.th;
@@ -62,7 +62,7 @@ void tile_step(const GridDesc &g, int n, int rb, int mb, int lt, bool probe) {
   uint32_t mbits[kLanes];
   for (int L = 0; L < kLanes; L++) {
     c[L] = i0 - 1 + L;
-    own[L] = L >= 1 && L <= kOwn && c[L] < g.nr;
+    own[L] = L >= 1 && L <= kOwn && (IN || c[L] < g.nr);
     cc[L] = load_col(g, c[L]);
     trr_p[L] = tqq_p[L] = tzz_p[L] = 0.f;
     trz_p[L] = ld(A_trz, g, j0 - 2, c[L]);
@@ -86,7 +86,7 @@ void tile_step(const GridDesc &g, int n, int rb, int mb, int lt, bool probe) {
     }
     const bool row_own = t >= j0 && t < j1;
     for (int L = 0; L < kLanes; L++) {
-      velocity(g, rb, t, c[L], row_own && own[L], cc[L], trr[L], dn(trr, L), tqq[L], dn(tqq, L), trz[L],
+      velocity<IN>(g, rb, t, c[L], row_own && own[L], cc[L], trr[L], dn(trr, L), tqq[L], dn(tqq, L), trz[L],
                up(trz, L), trz_p[L], tzz_c[L], tzz_dn[L], vr_old[L], vz_old[L], vrn[L], vzn[L]);
       if (row_own && own[L]) {
         const size_t k = (size_t)t * g.pitch + c[L];
@@ -111,7 +111,7 @@ void tile_step(const GridDesc &g, int n, int rb, int mb, int lt, bool probe) {
         float nrr = trr_p[L], nqq = tqq_p[L], nzz = tzz_p[L], nrz = trz_p[L];
         MemVars<NM> mem;
         load_mem<NM>(g, mb, s, c[L], mem);
-        stress<NM, SC>(g, mb, n, s, c[L], cc[L], vrn_p[L], up(vrn_p, L), vrn[L], vzn_p[L], dn(vzn_p, L), vzn_pp[L],
+        stress<NM, SC, IN>(g, mb, n, s, c[L], cc[L], vrn_p[L], up(vrn_p, L), vrn[L], vzn_p[L], dn(vzn_p, L), vzn_pp[L],
                        nrr, nqq, nzz, nrz, mem);
         store_mem<NM>(g, mb, s, c[L], mem);
         const size_t k = (size_t)s * g.pitch + c[L];
@@ -140,7 +140,7 @@ void tile_step(const GridDesc &g, int n, int rb, int mb, int lt, bool probe) {
 }
 
 // k_step2 for one tile, lane by lane: the same march, row slots and stores.
-template <int NM, bool SC>
+template <int NM, bool SC, bool IN>
 void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, bool probe) {
   const bool two = count >= 2 && n + 1 < g.steps;
   const int wb = rb ^ 1, mw = mb ^ 1;
@@ -160,7 +160,7 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
   uint32_t mbits[kLanes];
   for (int L = 0; L < kLanes; L++) {
     c[L] = i0 - 2 + L;
-    own[L] = L >= 2 && L < 2 + kOwn2 && c[L] < g.nr;
+    own[L] = L >= 2 && L < 2 + kOwn2 && (IN || c[L] < g.nr);
     cc[L] = load_col(g, c[L]);
     S1[L] = stage_start(ld(A[kTrz], g, j0 - 4, c[L]), ld_tzz(A[kTzz], g, n, j0 - 3, c[L]));
     S2[L] = stage_start(0.f, 0.f);
@@ -178,13 +178,13 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
   };
 
   const int t0 = j0 - 3, t_end = j1 + 2;
-  static thread_local std::vector<RowLoad> IN;
+  static thread_local std::vector<RowLoad> ROW;
   static thread_local std::vector<VPsi> IVP;
   static thread_local std::vector<SPsi> ISP;
   static thread_local std::vector<MemVars<NM>> IMEM;
   static thread_local std::vector<VelOut> QV;
   static thread_local std::vector<StrOut<NM>> QS;
-  IN.assign(3 * kLanes, RowLoad{});
+  ROW.assign(3 * kLanes, RowLoad{});
   IVP.assign(3 * kLanes, VPsi{});
   ISP.assign(3 * kLanes, SPsi{});
   IMEM.assign(3 * kLanes, MemVars<NM>{});
@@ -195,14 +195,14 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
   auto fetch = [&](int p, int t) {
     for (int L = 0; L < kLanes; L++) {
       const int k = slot(p, L);
-      IN[k] = load_row(A, g, n, t, c[L]);
+      ROW[k] = load_row(A, g, n, t, c[L]);
       IVP[k] = VPsi{0.f, 0.f, 0.f, 0.f};
-      if (vel_cell(g, t, c[L])) IVP[k] = load_vpsi(g, rb, t, c[L]);
+      if (!IN && vel_cell(g, t, c[L])) IVP[k] = load_vpsi(g, rb, t, c[L]);
       const int s = t - 1;
       ISP[k] = SPsi{0.f, 0.f, 0.f, 0.f};
       IMEM[k] = MemVars<NM>{};
-      if (s >= j0 - 2 && grid_cell(g, s, c[L])) {
-        ISP[k] = load_spsi(g, mb, s, c[L]);
+      if (s >= j0 - 2 && (IN || grid_cell(g, s, c[L]))) {
+        if (!IN) ISP[k] = load_spsi(g, mb, s, c[L]);
         load_mem<NM>(g, mb, s, c[L], IMEM[k]);
       }
     }
@@ -213,14 +213,14 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
     if (t + 1 <= t_end) fetch(p1, t + 1);
     float a_trr[kLanes], a_tqq[kLanes], a_trz[kLanes], vr1[kLanes], vz1[kLanes], sv[kLanes];
     for (int L = 0; L < kLanes; L++) {
-      a_trr[L] = IN[slot(p, L)].trr;
-      a_tqq[L] = IN[slot(p, L)].tqq;
-      a_trz[L] = IN[slot(p, L)].trz;
+      a_trr[L] = ROW[slot(p, L)].trr;
+      a_tqq[L] = ROW[slot(p, L)].tqq;
+      a_trz[L] = ROW[slot(p, L)].trz;
     }
     for (int L = 0; L < kLanes; L++) {
       VelOut &v1 = QV[slot(p, L)];
       v1.vp = IVP[slot(p, L)];
-      stage_velocity(g, t, c[L], cc[L], S1[L], IN[slot(p, L)], next(a_trr, L), next(a_tqq, L), prev(a_trz, L), v1.vp,
+      stage_velocity<IN>(g, t, c[L], cc[L], S1[L], ROW[slot(p, L)], next(a_trr, L), next(a_tqq, L), prev(a_trz, L), v1.vp,
                      v1.vr, v1.vz);
       vr1[L] = v1.vr;
       vz1[L] = v1.vz;
@@ -228,7 +228,7 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
         if (!two) {
           *at(B[kVr], g, t, c[L]) = v1.vr;
           *at(B[kVz], g, t, c[L]) = v1.vz;
-          if (vel_cell(g, t, c[L])) store_vpsi(g, wb, t, c[L], v1.vp);
+          if (!IN && vel_cell(g, t, c[L])) store_vpsi(g, wb, t, c[L], v1.vp);
         }
         if (max1) mbits[L] = std::max(mbits[L], std::max(abs_bits(v1.vr), abs_bits(v1.vz)));
       }
@@ -248,18 +248,18 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
         StrOut<NM> &o1 = QS[slot(p, L)];
         o1.sp = ISP[slot(p, L)];
         o1.mem = IMEM[slot(p, L)];
-        stage_stress<NM, SC>(g, n, s, c[L], cc[L], S1[L], vr1[L], prev(vrp, L), next(vzp, L), o1.st, o1.sp, o1.mem);
+        stage_stress<NM, SC, IN>(g, n, s, c[L], cc[L], S1[L], vr1[L], prev(vrp, L), next(vzp, L), o1.st, o1.sp, o1.mem);
         if (!two && own[L] && s >= j0 && s < j1) {
           *at(B[kTrr], g, s, c[L]) = o1.st[0];
           *at(B[kTqq], g, s, c[L]) = o1.st[1];
           *at(B[kTzz], g, s, c[L]) = o1.st[2];
           *at(B[kTrz], g, s, c[L]) = o1.st[3];
-          store_spsi(g, mw, s, c[L], o1.sp);
+          if (!IN) store_spsi(g, mw, s, c[L], o1.sp);
           store_mem<NM>(g, mw, s, c[L], o1.mem);
         }
       }
     }
-    for (int L = 0; L < kLanes; L++) stage_carry(S1[L], IN[slot(p, L)], vr1[L], vz1[L]);
+    for (int L = 0; L < kLanes; L++) stage_carry(S1[L], ROW[slot(p, L)], vr1[L], vz1[L]);
     if (!two) return;
 
     const int u = t - 2;
@@ -283,12 +283,12 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
       b_trz[L] = in2[L].trz;
     }
     for (int L = 0; L < kLanes; L++) {
-      stage_velocity(g, u, c[L], cc[L], S2[L], in2[L], next(b_trr, L), next(b_tqq, L), prev(b_trz, L), vp2[L],
+      stage_velocity<IN>(g, u, c[L], cc[L], S2[L], in2[L], next(b_trr, L), next(b_tqq, L), prev(b_trz, L), vp2[L],
                      vr2[L], vz2[L]);
       if (own[L] && u >= j0 && u < j1) {
         *at(B[kVr], g, u, c[L]) = vr2[L];
         *at(B[kVz], g, u, c[L]) = vz2[L];
-        if (vel_cell(g, u, c[L])) store_vpsi(g, wb, u, c[L], vp2[L]);
+        if (!IN && vel_cell(g, u, c[L])) store_vpsi(g, wb, u, c[L], vp2[L]);
         if (max2) mbits[L] = std::max(mbits[L], std::max(abs_bits(vr2[L]), abs_bits(vz2[L])));
       }
     }
@@ -307,13 +307,13 @@ void tile_step2(const GridDesc &g, int n, int rb, int mb, int count, int lt, boo
         SPsi sp2 = QS[slot(p1, L)].sp;
         MemVars<NM> mem2 = QS[slot(p1, L)].mem;
         float o2[4];
-        stage_stress<NM, SC>(g, n + 1, s2, c[L], cc[L], S2[L], vr2[L], prev(vrp, L), next(vzp, L), o2, sp2, mem2);
+        stage_stress<NM, SC, IN>(g, n + 1, s2, c[L], cc[L], S2[L], vr2[L], prev(vrp, L), next(vzp, L), o2, sp2, mem2);
         if (own[L]) {
           *at(B[kTrr], g, s2, c[L]) = o2[0];
           *at(B[kTqq], g, s2, c[L]) = o2[1];
           *at(B[kTzz], g, s2, c[L]) = o2[2];
           *at(B[kTrz], g, s2, c[L]) = o2[3];
-          store_spsi(g, mw, s2, c[L], sp2);
+          if (!IN) store_spsi(g, mw, s2, c[L], sp2);
           store_mem<NM>(g, mw, s2, c[L], mem2);
         }
       }
@@ -342,8 +342,19 @@ void run_steps(const GridDesc &g, int steps, int span) {
   for (int n = 0; n < steps;) {
     const int count = std::min(span, steps - n);
     for (int lt = 0; lt < tiles; lt++) {
-      if (span == 2) tile_step2<NM, SC>(g, n, rb, mb, count, lt, true);
-      else tile_step<NM, SC>(g, n, rb, mb, lt, true);
+      // the kernels' interior test: a tile whose every computed cell is an
+      // updated interior cell takes the case-free arithmetic
+      const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
+      const int j0 = tzi * g.th, j1 = std::min(j0 + g.th, g.nz);
+      if (span == 2) {
+        const int i0 = tr * kOwn2;
+        if (interior(g, i0 - 2, i0 + kOwn2 + 1, j0 - 3, j1 + 2)) tile_step2<NM, SC, true>(g, n, rb, mb, count, lt, true);
+        else tile_step2<NM, SC, false>(g, n, rb, mb, count, lt, true);
+      } else {
+        const int i0 = tr * kOwn;
+        if (interior(g, i0 - 1, i0 + kOwn, j0 - 1, j1)) tile_step<NM, SC, true>(g, n, rb, mb, lt, true);
+        else tile_step<NM, SC, false>(g, n, rb, mb, lt, true);
+      }
     }
     rb ^= 1;
     if (span == 2) mb ^= 1;

@@ -29,6 +29,10 @@ def cases(fdtd):
         "sls": fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=6, Depth=6, Spacing=0.15, DominantFreq=30),
         "gsls-layered": fdtd.Model(Stack=two, GslsMechanisms=3, MaxRange=7.3, Depth=5.1, Spacing=0.1, PMLCells=12),
         "sls-layered": fdtd.Model(Stack=two, Relax=fdtd.Fit(40, 30), MaxRange=9.9, Depth=4.3, Spacing=0.1),
+        # 221 x 121 cells: tiles clear of the axis, the surface and the
+        # C-PML strips take the kernels' case-free interior path
+        "sls-interior": fdtd.Model(Stack=loam(), Relax=fdtd.Fit(25, 30), MaxRange=24, Depth=12, Spacing=0.12,
+                                   DominantFreq=30),
     }
 
 
@@ -52,7 +56,7 @@ def test_emulated_kernel_matches_oracle(fdtd, oracle, name, th):
     assert probes[0] > 0 and probes[1] > 0
 
 
-@pytest.mark.parametrize("name", ["elastic", "sls", "gsls-layered", "sls-layered"])
+@pytest.mark.parametrize("name", ["elastic", "sls", "gsls-layered", "sls-layered", "sls-interior"])
 def test_two_step_kernel_gives_the_one_step_bits(fdtd, oracle, name):
     """Two steps per launch (march2.h) with every tile height, over an odd
     number of steps (the last launch takes one), equal one step per launch
@@ -66,6 +70,9 @@ def test_two_step_kernel_gives_the_one_step_bits(fdtd, oracle, name):
     steps = 301
     force = oracle.ricker(1000, 35, dt, steps)
     ranges = [0.5, 1.0, 2.5, 4.0]
+    if name == "sls-interior":
+        ranges, steps = [0.5, 1.0, 2.0, 3.0], 151
+        force = force[:steps]
     cols = [o.Column(r)[0] for r in ranges]
     want = oracle.run(m, force, ranges, steps)
     one, p1 = emu.run(m, force, cols, steps, 3, span=1)
@@ -92,7 +99,8 @@ def test_emulated_kernel_runs_clean_under_asan(tmp_path):
         "fd = conftest.load_package().fdtd\n"
         "two = [fd.Layer(1.3, 420, 190, 1700, 20, 35), fd.Layer(0, 700, 320, 2000, 60, 100)]\n"
         "for m in [fd.Model(Stack=two, GslsMechanisms=3, MaxRange=7.3, Depth=5.1, Spacing=0.1, PMLCells=12),\n"
-        "          fd.Model(Stack=two, Relax=fd.Fit(40, 30), MaxRange=3.9, Depth=2.3, Spacing=0.1, PMLCells=5)]:\n"
+        "          fd.Model(Stack=two, Relax=fd.Fit(40, 30), MaxRange=3.9, Depth=2.3, Spacing=0.1, PMLCells=5),\n"
+        "          fd.Model(Stack=two, GslsMechanisms=3, MaxRange=16, Depth=9, Spacing=0.1, PMLCells=12)]:\n"
         "    dt = O.OracleSim(m).Dt()\n"
         "    for span, ths in ((1, (1, 2, 5, 64)), (2, (1, 2, 3, 7, 64))):\n"
         "        for th in ths:\n"
