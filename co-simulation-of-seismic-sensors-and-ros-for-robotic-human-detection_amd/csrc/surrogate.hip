@@ -21,6 +21,8 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include "surrogate_kernels.h"
 
 namespace gfsk {
@@ -117,15 +119,23 @@ __device__ __forceinline__ void store_tile(const TileRegs<ROWS, BK> &t, uint16_t
   }
 }
 
+// Four wavefronts: WM x WN of them tile the block's output, and the
+// KW = 4 / (WM WN) groups of those take turns over a stage's 16-deep k-steps
+// (k-step s goes to group s % KW); the groups' accumulators are added in LDS
+// before the epilogue, in group order.  KW > 1 is for short, skinny outputs
+// with a long K (weight gradients of the small layers).
 template <int BM, int BN, int BK, int WM, int WN, bool AT, bool BT, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   constexpr int TM = BM / WM / 32, TN = BN / WN / 32, kLDK = BK + 8;
-  static_assert(WM * WN == 4 && TM >= 1 && TN >= 1, "4 wavefronts, 32x32 MFMA tiles");
-  static_assert(BK % 16 == 0, "whole MFMA steps per stage");
+  constexpr int KW = 4 / (WM * WN);
+  static_assert(4 % (WM * WN) == 0 && TM >= 1 && TN >= 1, "4 wavefronts, 32x32 MFMA tiles");
+  static_assert(BK % (16 * KW) == 0, "whole MFMA steps per stage and k group");
+  static_assert(KW == 1 || (KW - 1) * WM * WN * TM * TN * 16 * 64 * 4 <= BM * kLDK * 2, "k-group sums fit in As");
   __shared__ __attribute__((aligned(16))) uint16_t As[BM * kLDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * kLDK];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave / WN, wn = wave % WN;
+  const int wmn = wave % (WM * WN), wk = wave / (WM * WN);
+  const int wm = wmn / WN, wn = wmn % WN;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int kbeg = blockIdx.z * g.k_split;
   const int kend = min(g.K, kbeg + g.k_split);
@@ -156,6 +166,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
 #pragma unroll
     for (int s = 0; s < BK / 16; s++) {
+      if (KW > 1 && s % KW != wk) continue;
       bf16x8 fa[TM], fb[TN];
       const int kk = s * 16 + (lane >> 5) * 8;
 #pragma unroll
@@ -172,6 +183,34 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     __syncthreads();
   }
 
+  if (KW > 1) {
+    // the loop ended on a barrier, so As is free: groups 1.. park their sums,
+    // one slot per (group, output position wmn, tile, element, lane)
+    float *red = (float *)As;
+    if (wk > 0)
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+#pragma unroll
+          for (int i = 0; i < 16; i++)
+            red[((((wk - 1) * WM * WN + wmn) * TM * TN + a * TN + b) * 16 + i) * 64 + lane] = acc[a][b][i];
+    __syncthreads();
+    if (wk > 0) return;
+#pragma unroll
+    for (int w = 0; w < KW - 1; w++)
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+#pragma unroll
+          for (int i = 0; i < 16; i++)
+            acc[a][b][i] += red[(((w * WM * WN + wmn) * TM * TN + a * TN + b) * 16 + i) * 64 + lane];
+  }
+
+  float csum[TN];
+#pragma unroll
+  for (int b = 0; b < TN; b++) csum[b] = 0.0f;
 #pragma unroll
   for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -194,11 +233,34 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
           g.cb[(int64_t)row * g.ldcb + col] = to_bf16(gelu(z));
         } else {  // kEpiGeluBwd
           const float d = v * gelu_grad(g.z[(int64_t)row * g.ldz + col]);
-          g.cf[(int64_t)row * g.ldc + col] = d;
+          if (g.cf) g.cf[(int64_t)row * g.ldc + col] = d;
           g.cb[(int64_t)row * g.ldcb + col] = to_bf16(d);
+          csum[b] += d;
         }
       }
     }
+
+  if constexpr (EPI == kEpiGeluBwd) {
+    // column sums of d over the block's rows (the next layer's bias
+    // gradient, in parts): lanes l and l + 32 hold the same column, then the
+    // WM wavefronts sharing a column add in LDS in wavefront order
+    static_assert(KW == 1 && WM * BN * 4 <= BM * kLDK * 2, "column sums fit in As");
+    if (g.colpart) {
+      float *red = (float *)As;  // free: the main loop ended on a barrier
+#pragma unroll
+      for (int b = 0; b < TN; b++) {
+        const float t = csum[b] + __shfl_xor(csum[b], 32);
+        if (lane < 32) red[wm * BN + wn * TN * 32 + b * 32 + lane] = t;
+      }
+      __syncthreads();
+      for (int c = threadIdx.x; c < BN; c += 256) {
+        if (n0 + c >= g.N) continue;
+        float t = 0.0f;
+        for (int w = 0; w < WM; w++) t += red[w * BN + c];
+        g.colpart[(int64_t)blockIdx.y * g.N + n0 + c] = t;
+      }
+    }
+  }
 }
 
 // Tile shapes (M x N x K-stage, wavefronts along M x N).  The default picks by
@@ -219,23 +281,59 @@ inline int forced_shape() {
   return v;
 }
 
-template <bool AT, bool BT, int EPI>
-hipError_t launch_shape(const GemmArgs &g, int splits, hipStream_t st) {
-  if (g.N <= 32) return launch_tiles<128, 32, 64, 4, 1, AT, BT, EPI>(g, splits, st);
+// The tile shape for a product: one id, used by the launcher and by
+// gemm_row_tiles so that both always agree.
+enum Shape { kSkinny32, kSkinny64, kNarrow, kS128x128, kS128x64, kS128x128k128, kS128x64k128, kS64x64, kS64x64k128 };
+
+static Shape pick_shape(const GemmArgs &g, bool at, bool bt, int epi) {
+  // weight gradients of the small layers: an output of at most 64 x 32 over
+  // a long K, so the wavefronts share the K range in 256-deep stages
+  if (at && !bt && epi == kEpiF32 && g.N <= 32 && g.M <= 32) return kSkinny32;
+  if (at && !bt && epi == kEpiF32 && g.N <= 32 && g.M <= 64) return kSkinny64;
+  if (g.N <= 32) return kNarrow;
   switch (forced_shape()) {
-    case 0: return launch_tiles<128, 128, 64, 2, 2, AT, BT, EPI>(g, splits, st);
-    case 1: return launch_tiles<128, 64, 64, 4, 1, AT, BT, EPI>(g, splits, st);
-    case 2: return launch_tiles<128, 128, 128, 2, 2, AT, BT, EPI>(g, splits, st);
-    case 3: return launch_tiles<128, 64, 128, 4, 1, AT, BT, EPI>(g, splits, st);
-    case 4: return launch_tiles<64, 64, 64, 2, 2, AT, BT, EPI>(g, splits, st);
-    case 5: return launch_tiles<64, 64, 128, 2, 2, AT, BT, EPI>(g, splits, st);
+    case 0: return kS128x128;
+    case 1: return kS128x64;
+    case 2: return kS128x128k128;
+    case 3: return kS128x64k128;
+    case 4: return kS64x64;
+    case 5: return kS64x64k128;
     default: break;
   }
   // 64 x 64 tiles, four wavefronts of 32 x 32: the network's GEMMs are short
   // in K (16-2048) and latency-bound, and small tiles let a CU interleave up
   // to eight workgroups (measured best of the shapes above for inference at
   // batch 8192 and for training at 4096)
-  return launch_tiles<64, 64, 64, 2, 2, AT, BT, EPI>(g, splits, st);
+  return kS64x64;
+}
+
+static int shape_rows(Shape s) {
+  switch (s) {
+    case kSkinny32: return 32;
+    case kSkinny64: case kS64x64: case kS64x64k128: return 64;
+    default: return 128;
+  }
+}
+
+template <bool AT, bool BT, int EPI>
+hipError_t launch_shape(const GemmArgs &g, int splits, hipStream_t st) {
+  switch (pick_shape(g, AT, BT, EPI)) {
+    case kSkinny32:
+    case kSkinny64:
+      if constexpr (AT && !BT && EPI == kEpiF32) {
+        if (g.M <= 32) return launch_tiles<32, 32, 256, 1, 1, AT, BT, EPI>(g, splits, st);
+        return launch_tiles<64, 32, 256, 2, 1, AT, BT, EPI>(g, splits, st);
+      }
+      return hipErrorInvalidValue;
+    case kNarrow: return launch_tiles<128, 32, 64, 4, 1, AT, BT, EPI>(g, splits, st);
+    case kS128x128: return launch_tiles<128, 128, 64, 2, 2, AT, BT, EPI>(g, splits, st);
+    case kS128x64: return launch_tiles<128, 64, 64, 4, 1, AT, BT, EPI>(g, splits, st);
+    case kS128x128k128: return launch_tiles<128, 128, 128, 2, 2, AT, BT, EPI>(g, splits, st);
+    case kS128x64k128: return launch_tiles<128, 64, 128, 4, 1, AT, BT, EPI>(g, splits, st);
+    case kS64x64: return launch_tiles<64, 64, 64, 2, 2, AT, BT, EPI>(g, splits, st);
+    case kS64x64k128: return launch_tiles<64, 64, 128, 2, 2, AT, BT, EPI>(g, splits, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int EPI>
@@ -244,6 +342,11 @@ hipError_t launch_layouts(const GemmArgs &g, bool at, bool bt, int splits, hipSt
   if (!at && !bt) return launch_shape<false, false, EPI>(g, splits, st);
   if (at && !bt) return launch_shape<true, false, EPI>(g, splits, st);
   return launch_shape<true, true, EPI>(g, splits, st);
+}
+
+int32_t gemm_row_tiles(const GemmArgs &g, bool at, bool bt, int epi) {
+  const int r = shape_rows(pick_shape(g, at, bt, epi));
+  return (g.M + r - 1) / r;
 }
 
 hipError_t launch_gemm(const GemmArgs &g, bool at, bool bt, int epi, int splits, hipStream_t st) {
@@ -331,39 +434,48 @@ __global__ __launch_bounds__(256) void k_rel_l2(const float *__restrict__ o, con
   }
 }
 
-// Column sums of x[rows][cols] over chunks of `chunk` rows, one workgroup
-// per chunk: part[chunk][col].  For cols <= 256 (a divisor of 256) the 256
-// lanes take (column, row phase) pairs so that reads stay coalesced, then
-// fold the phases in LDS; wider rows loop over columns.  The chunk length
-// keeps about 256 chunks in flight, between 16 and 1024 rows each.
-__host__ __device__ inline int64_t colsum_chunk_rows(int64_t rows) {
-  int64_t c = 16;
-  while (c < 1024 && c * 256 < rows) c *= 2;
+// Column sums of x[rows][cols] over chunks of rows: part[chunk][col].  One
+// workgroup per (256-column stripe, chunk): its threads read 16-byte vectors,
+// tpr = min(cols, 256) / 4 of them across a row and 256 / tpr rows at once,
+// so every load is a coalesced full row segment; the row phases are then
+// added in an LDS tree.  cols is a multiple of 4 that divides 256, or a
+// multiple of 256.  The chunk length keeps about 1024 workgroups in flight.
+__host__ __device__ inline int64_t colsum_chunk_rows(int64_t rows, int32_t cols) {
+  const int64_t stripes = cols > 256 ? cols / 256 : 1;
+  int64_t c = 8;
+  while (c < 4096 && (rows + c - 1) / c * stripes > 1024) c *= 2;
   return c;
 }
 
 __global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ x, int64_t rows, int32_t cols,
                                                      int64_t chunk, float *__restrict__ part) {
-  const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(rows, r0 + chunk);
-  if (cols <= 256) {
-    const int phases = 256 / cols, c = threadIdx.x % cols, ph = threadIdx.x / cols;
-    float s = 0;
-    if (ph < phases)
-      for (int64_t r = r0 + ph; r < r1; r += phases) s += x[r * cols + c];
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (ph == 0) {
-      for (int p = 1; p < phases; p++) s += red[p * cols + c];
-      part[(int64_t)blockIdx.x * cols + c] = s;
-    }
-  } else {
-    for (int c = threadIdx.x; c < cols; c += 256) {
-      float s = 0;
-      for (int64_t r = r0; r < r1; r++) s += x[r * cols + c];
-      part[(int64_t)blockIdx.x * cols + c] = s;
-    }
+  const int cw = cols < 256 ? cols : 256, tpr = cw / 4, phases = 256 / tpr;
+  const int t = threadIdx.x, c4 = t % tpr, ph = t / tpr;
+  const int col = blockIdx.x * 256 + c4 * 4;
+  const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t r = r0 + ph; r < r1; r += phases) {
+    const float4 v = *(const float4 *)(x + r * cols + col);
+    s.x += v.x;
+    s.y += v.y;
+    s.z += v.z;
+    s.w += v.w;
   }
+  __shared__ float4 red[256];
+  red[t] = s;
+  __syncthreads();
+  for (int h = phases / 2; h > 0; h /= 2) {
+    if (ph < h) {
+      const float4 o = red[t + h * tpr];
+      s.x += o.x;
+      s.y += o.y;
+      s.z += o.z;
+      s.w += o.w;
+      red[t] = s;
+    }
+    __syncthreads();
+  }
+  if (ph == 0) *(float4 *)(part + (int64_t)blockIdx.y * cols + col) = s;
 }
 
 // out[j] = sum over chunks and over columns c with c % mod == j of part: one
@@ -385,11 +497,48 @@ __global__ __launch_bounds__(256) void k_colsum_fold(const float *__restrict__ p
   if (threadIdx.x == 0) out[j] = red[0];
 }
 
-// out[i] = sum_s part[s][i]: the K splits of a weight gradient.
-__global__ void k_sum_splits(const float *__restrict__ part, int32_t splits, int64_t n, float *__restrict__ out) {
+// The K splits of a weight gradient, out[i] = sum_s part[s][i], in two
+// passes.  Pass 1: the splits fall into groups of gs consecutive slices;
+// workgroup (x, y) adds group y for outputs 64 x .. 64 x + 63 (its four
+// wavefronts take every fourth slice, then add in LDS) and writes the group's
+// sum over the group's first slice, which no other workgroup reads -- or to
+// out when there is one group.  Pass 2 adds the groups' sums.
+__global__ __launch_bounds__(256) void k_sum_groups(float *__restrict__ part, int32_t splits, int32_t gs, int64_t n,
+                                                    float *__restrict__ out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const int s0 = blockIdx.y * gs, s1 = min(splits, s0 + gs);
+  // four running sums per lane (slices k, k + 4, k + 8, k + 12 of the
+  // wavefront's stride-4 sequence), so that four loads are in flight
+  float a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  if (i < n) {
+    int k = s0 + wave;
+    for (; k + 12 < s1; k += 16) {
+      a0 += part[(int64_t)k * n + i];
+      a1 += part[(int64_t)(k + 4) * n + i];
+      a2 += part[(int64_t)(k + 8) * n + i];
+      a3 += part[(int64_t)(k + 12) * n + i];
+    }
+    for (; k < s1; k += 4) a0 += part[(int64_t)k * n + i];
+  }
+  const float s = (a0 + a1) + (a2 + a3);
+  __shared__ float red[4][64];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && i < n) {
+    const float v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (gridDim.y == 1)
+      out[i] = v;
+    else
+      part[(int64_t)s0 * n + i] = v;
+  }
+}
+
+__global__ void k_sum_strided(const float *__restrict__ part, int32_t terms, int64_t stride, int64_t n,
+                              float *__restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float s = 0;
-    for (int k = 0; k < splits; k++) s += part[(int64_t)k * n + i];
+    for (int k = 0; k < terms; k++) s += part[(int64_t)k * stride + i];
     out[i] = s;
   }
 }
@@ -416,6 +565,29 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 // T = o_0 + i o_1 (cartesian) or e^{o_0 + i o_1} (polar: log-amplitude and
 // phase), s_b the amplitude scale and w_b the delay the targets were divided
 // by; zero outside bins [k_lo, k_hi].
+// One bin's de-normalization; products are not contracted, so every kernel
+// that calls it gives the same bits.
+__device__ __forceinline__ void denorm_bin(float o0, float o1, double scale, double delay, double df, int k, int k_lo,
+                                           int k_hi, int polar, float &re_out, float &im_out) {
+#pragma clang fp contract(off)
+  if (k < k_lo || k > k_hi) {
+    re_out = 0.0f;
+    im_out = 0.0f;
+    return;
+  }
+  double ph = -2.0 * 3.141592653589793 * (double)k * df * delay;
+  double re = o0, im = o1;
+  if (polar) {
+    ph += im;
+    re = exp(re);
+    im = 0.0;
+  }
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  re_out = (float)(scale * (re * cs - im * sn));
+  im_out = (float)(scale * (re * sn + im * cs));
+}
+
 __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const double *__restrict__ scale,
                               const double *__restrict__ delay, double df, int32_t k_lo, int32_t k_hi,
                               int32_t polar, float *__restrict__ out, int32_t rows) {
@@ -423,34 +595,27 @@ __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const d
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = i / bins;
     const int k = (int)(i % bins);
-    if (k < k_lo || k > k_hi) {
-      out[2 * i] = 0.0f;
-      out[2 * i + 1] = 0.0f;
-      continue;
-    }
-    double ph = -2.0 * 3.141592653589793 * (double)k * df * delay[b];
-    double re = o[2 * i], im = o[2 * i + 1];
-    if (polar) {
-      ph += im;
-      re = exp(re);
-      im = 0.0;
-    }
-    double sn, cs;
-    sincos(ph, &sn, &cs);
-    out[2 * i] = (float)(scale[b] * (re * cs - im * sn));
-    out[2 * i + 1] = (float)(scale[b] * (re * sn + im * cs));
+    denorm_bin(o[2 * i], o[2 * i + 1], scale[b], delay[b], df, k, k_lo, k_hi, polar, out[2 * i], out[2 * i + 1]);
   }
 }
 
 // out[k][n] = in[n][k] for one layer's BF16 weights (16x16 tiles via LDS).
-__global__ __launch_bounds__(256) void k_transpose_bf16(const uint16_t *__restrict__ in, int32_t n_rows,
-                                                        int32_t k_cols, uint16_t *__restrict__ out) {
+// Every layer's [n][k] weight block transposed to [k][n] at the same offset,
+// in one launch: 16 x 16 tiles numbered layer by layer.
+__global__ __launch_bounds__(256) void k_transpose_layers(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                                                          TransposeTable tt) {
   __shared__ uint16_t t[16][17];
-  const int n0 = blockIdx.y * 16, k0 = blockIdx.x * 16;
+  int l = 0;
+  while (l + 1 < tt.layers && (int)blockIdx.x >= tt.first_tile[l + 1]) l++;
+  const int tile = blockIdx.x - tt.first_tile[l], tiles_k = (tt.k[l] + 15) / 16;
+  const int n_rows = tt.n[l], k_cols = tt.k[l];
+  const uint16_t *src = in + tt.off[l];
+  uint16_t *dst = out + tt.off[l];
+  const int n0 = tile / tiles_k * 16, k0 = tile % tiles_k * 16;
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  if (n0 + ty < n_rows && k0 + tx < k_cols) t[ty][tx] = in[(int64_t)(n0 + ty) * k_cols + k0 + tx];
+  if (n0 + ty < n_rows && k0 + tx < k_cols) t[ty][tx] = src[(int64_t)(n0 + ty) * k_cols + k0 + tx];
   __syncthreads();
-  if (k0 + ty < k_cols && n0 + tx < n_rows) out[(int64_t)(k0 + ty) * n_rows + n0 + tx] = t[tx][ty];
+  if (k0 + ty < k_cols && n0 + tx < n_rows) dst[(int64_t)(k0 + ty) * n_rows + n0 + tx] = t[tx][ty];
 }
 
 namespace {
@@ -487,22 +652,41 @@ hipError_t launch_rel_l2(const float *o, const float *t, const float *w, int32_t
   return hipGetLastError();
 }
 
-int32_t bias_grad_chunks(int64_t rows) {
-  const int64_t c = colsum_chunk_rows(rows);
-  return (int32_t)((rows + c - 1) / c);
+int64_t bias_grad_part_floats(int64_t max_rows, int32_t cols) {
+  // at most 1024 chunks while the chunk length is below its cap, and
+  // ceil(rows / cap) beyond it
+  return std::max<int64_t>(1024, (max_rows + 4095) / 4096) * cols;
 }
 
 hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, float *out,
                             hipStream_t st) {
-  if (cols <= 256 && 256 % cols != 0) return hipErrorInvalidValue;
-  const int32_t chunks = bias_grad_chunks(rows);
-  hipLaunchKernelGGL(k_colsum_part, dim3(chunks), dim3(256), 0, st, x, rows, cols, colsum_chunk_rows(rows), part);
-  hipLaunchKernelGGL(k_colsum_fold, dim3(mod), dim3(256), 0, st, part, chunks, cols, mod, out);
+  if (cols % 4 || (cols <= 256 && 256 % cols) || (cols > 256 && cols % 256) || mod < 1 || cols % mod)
+    return hipErrorInvalidValue;
+  if (rows <= 0) return hipSuccess;
+  const int64_t c = colsum_chunk_rows(rows, cols);
+  const int64_t chunks = (rows + c - 1) / c;
+  if (chunks > 65535) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_colsum_part, dim3(cols > 256 ? cols / 256 : 1, (unsigned)chunks), dim3(256), 0, st, x, rows,
+                     cols, c, part);
+  hipLaunchKernelGGL(k_colsum_fold, dim3(mod), dim3(256), 0, st, part, (int32_t)chunks, cols, mod, out);
   return hipGetLastError();
 }
 
-hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_splits, grid_for(n), dim3(256), 0, st, part, splits, n, out);
+hipError_t launch_sum_splits(float *part, int32_t splits, int64_t n, float *out, hipStream_t st) {
+  if (n <= 0 || splits < 1) return hipSuccess;
+  const int64_t colblocks = (n + 63) / 64;
+  if (colblocks > 0x7fffffff) return hipErrorInvalidConfiguration;
+  // one pass for up to 64 slices; else about 1024 workgroups in pass 1, at
+  // most 16 groups (pass 2's terms), and at least 8 slices a group
+  int64_t groups = std::max<int64_t>(1, 1024 / colblocks);
+  groups = std::min<int64_t>({groups, 16, std::max<int64_t>(1, splits / 8)});
+  if (splits <= 64) groups = 1;
+  const int32_t gs = (int32_t)((splits + groups - 1) / groups);
+  groups = (splits + gs - 1) / gs;
+  hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)colblocks, (unsigned)groups), dim3(256), 0, st, part, splits, gs, n,
+                     out);
+  if (groups > 1)
+    hipLaunchKernelGGL(k_sum_strided, grid_for(n), dim3(256), 0, st, part, (int32_t)groups, (int64_t)gs * n, n, out);
   return hipGetLastError();
 }
 
@@ -512,10 +696,11 @@ hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *p
   return hipGetLastError();
 }
 
-hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_cols, uint16_t *out,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_transpose_bf16, dim3((k_cols + 15) / 16, (n_rows + 15) / 16), dim3(256), 0, st, in, n_rows,
-                     k_cols, out);
+hipError_t launch_transpose_layers(const uint16_t *in, uint16_t *out, const TransposeTable &t, hipStream_t st) {
+  if (t.layers < 1 || t.layers > kMaxTransposeLayers) return hipErrorInvalidValue;
+  const int tiles = t.first_tile[t.layers];
+  if (tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_transpose_layers, dim3(tiles), dim3(256), 0, st, in, out, t);
   return hipGetLastError();
 }
 
@@ -662,10 +847,108 @@ __device__ __forceinline__ void pipe_fetch(uint4 (&buf)[2][8], const FusedArgs &
     for (int s = 0; s < 8; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * 32 + P::KC * 8 + s) * 64 + lane];
 }
 
+// LDS row pitch (BF16) of the decoder tail's 32 x 32 tiles.
+constexpr int kTailPitch = 32 + 8;
+constexpr int kTailTile = kFusedRows * kTailPitch;
+
+// Orders a wavefront's LDS writes before its following LDS reads (the tail's
+// tiles are private to the wavefront, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The decoder's transposed convolutions for dense-layer position p of the
+// block's 32 queries, from that position's accumulators (acc, bias bv): the
+// dense layer's GELU output, then 32 -> 2 x 32 channels (bias, GELU; rows
+// (query, position 2p + j) of the next input), then 32 -> 4 taps x (re, im)
+// (bias), written to zout as the per-layer GEMMs would write it: row
+// (query, 2p + j), column tap * 2 + component -- or, de-normalized, to spec.
+// The last product is formed transposed (W8 as the A operand), so each lane
+// holds four consecutive outputs of one query and stores them as one vector.
+// s1 and s2 are the wavefront's own LDS tiles (one, and two, of 32 x
+// kTailPitch).
+__device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &acc, float bv, int p, int m0,
+                                             int lane, uint16_t *s1, uint16_t *s2) {
+  const int c = lane & 31, hi = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int row = (i / 4) * 8 + hi * 4 + (i % 4);
+    s1[row * kTailPitch + c] = to_bf16(gelu(acc[i] + bv));
+  }
+  wave_lds_sync();
+  // first convolution: [32 rows][32 channels] x W7^T, two 32-column tiles
+  const uint4 *__restrict__ w7 = (const uint4 *)a.wp[7];
+  f32x16 c1[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < 16; i++) c1[j][i] = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    const bf16x8 fa = *(const bf16x8 *)(s1 + c * kTailPitch + ks * 16 + hi * 8);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const uint4 v = w7[(j * 2 + ks) * 64 + lane];
+      bf16x8 fb;
+      __builtin_memcpy(&fb, &v, 16);
+      c1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, c1[j], 0, 0, 0);
+    }
+  }
+  const float b1 = a.bias[7][c];
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int row = (i / 4) * 8 + hi * 4 + (i % 4);
+      s2[j * kTailTile + row * kTailPitch + c] = to_bf16(gelu(c1[j][i] + b1));
+    }
+  wave_lds_sync();
+  // second convolution, transposed: W8 [8 outputs (rows 8-31 zero, as the
+  // per-layer GEMM pads them)][32 channels] x the tile^T.  Lane (query c,
+  // half hi) gets outputs 4 hi .. 4 hi + 3 of that query in c2[0..3].
+  const int r = m0 + c;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    f32x16 c2;
+#pragma unroll
+    for (int i = 0; i < 16; i++) c2[i] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (c < 8) v = *(const uint4 *)(a.w8 + c * 32 + ks * 16 + hi * 8);
+      bf16x8 fw;
+      __builtin_memcpy(&fw, &v, 16);
+      const bf16x8 fx = *(const bf16x8 *)(s2 + j * kTailTile + c * kTailPitch + ks * 16 + hi * 8);
+      c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fw, fx, c2, 0, 0, 0);
+    }
+    const int q = 2 * p + j, col = q * 8 + hi * 4;
+    float4 o;
+    o.x = c2[0] + a.bias[8][0];
+    o.y = c2[1] + a.bias[8][1];
+    o.z = c2[2] + a.bias[8][0];
+    o.w = c2[3] + a.bias[8][1];
+    if (a.spec) {
+      if (r < a.spec_rows) {
+        const double sc = a.scale[r], dl = a.delay[r];
+        const int k = 4 * q + 2 * hi;  // bins k and k + 1 (taps 2 hi, 2 hi + 1)
+        float4 y;
+        denorm_bin(o.x, o.y, sc, dl, a.df, k, a.k_lo, a.k_hi, a.polar, y.x, y.y);
+        denorm_bin(o.z, o.w, sc, dl, a.df, k + 1, a.k_lo, a.k_hi, a.polar, y.z, y.w);
+        *(float4 *)(a.spec + (int64_t)r * 1024 + col) = y;
+      }
+    } else if (r < a.M) {
+      *(float4 *)(a.zout + (int64_t)r * 1024 + col) = o;
+    }
+  }
+  wave_lds_sync();  // the next position's tiles reuse s1 and s2
+}
+
 template <int C>
 __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f32x16 (&acc)[2], float (&bias)[2],
-                                         uint16_t (*act)[kFusedRows * kFusedPitch], const FusedArgs &a, int m0,
-                                         int wave, int lane) {
+                                         uint16_t (*act)[kFusedRows * kFusedPitch], uint16_t *tail,
+                                         const FusedArgs &a, int m0, int wave, int lane) {
   using P = PipeChunk<C>;
   if constexpr (C + 1 < kPipeChunks) {
     if constexpr ((C + 1) & 1) pipe_fetch<C + 1>(b1, a, wave, lane);
@@ -689,28 +972,36 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f
   else mfma_chunk<8, 2>(acc, b0, act[P::L & 1], P::KC, lane);
   if constexpr (P::kLastOfPair) {
     const int nt0 = wave * P::TPW + P::G * 2;
+    if (P::L == 6 && (a.zout || a.spec)) {
+      // each 32-column tile of the dense layer is one position (32 channels)
+      uint16_t *s1 = tail + wave * 3 * kTailTile;
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int col = (nt0 + t) * 32 + (lane & 31);
-      const float bv = bias[t];
+      for (int t = 0; t < 2; t++) decoder_tail(a, acc[t], bias[t], nt0 + t, m0, lane, s1, s1 + kTailTile);
+    } else {
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int row = (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
-        const uint16_t y = to_bf16(gelu(acc[t][i] + bv));
-        if constexpr (P::L == 6) {
-          if (m0 + row < a.M) a.out[(int64_t)(m0 + row) * P::N + col] = y;
-        } else {
-          act[(P::L + 1) & 1][row * kFusedPitch + col] = y;
+      for (int t = 0; t < 2; t++) {
+        const int col = (nt0 + t) * 32 + (lane & 31);
+        const float bv = bias[t];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int row = (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
+          const uint16_t y = to_bf16(gelu(acc[t][i] + bv));
+          if constexpr (P::L == 6) {
+            if (m0 + row < a.M) a.out[(int64_t)(m0 + row) * P::N + col] = y;
+          } else {
+            act[(P::L + 1) & 1][row * kFusedPitch + col] = y;
+          }
         }
       }
     }
     if constexpr (P::kLastOfLayer && P::L < 6) __syncthreads();
   }
-  if constexpr (C + 1 < kPipeChunks) pipe_run<C + 1>(b0, b1, acc, bias, act, a, m0, wave, lane);
+  if constexpr (C + 1 < kPipeChunks) pipe_run<C + 1>(b0, b1, acc, bias, act, tail, a, m0, wave, lane);
 }
 
 __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFusedRows * kFusedPitch];
+  __shared__ __attribute__((aligned(16))) uint16_t tail[kFusedWaves * 3 * kTailTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * kFusedRows;
   // the first chunk of layer 1 is in flight while layer 0 runs
@@ -728,7 +1019,7 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
   __syncthreads();
   f32x16 acc[2];
   float bias[2];
-  pipe_run<0>(b0, b1, acc, bias, act, a, m0, wave, lane);
+  pipe_run<0>(b0, b1, acc, bias, act, tail, a, m0, wave, lane);
 }
 
 hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st) {

@@ -111,11 +111,14 @@ struct gf_sur {
   int polar = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // the backward pass's second stream and its ordering events
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_ready = nullptr, ev_free[2] = {nullptr, nullptr};
   // parameters and optimizer state
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;
   uint16_t *pbt = nullptr;  // each layer's BF16 weights transposed, [K][N]
-  // the first kFusedLayers layers' BF16 weights in MFMA fragment order, for
+  // the first kFusedLayers + 1 layers' BF16 weights in MFMA fragment order, for
   // the fused inference kernel; rebuilt after the weights change
   uint16_t *wpk = nullptr;
   int64_t wpk_off[8] = {};
@@ -130,6 +133,9 @@ struct gf_sur {
   float *split = nullptr;
   int64_t split_floats = 0;
   float *colpart = nullptr;
+  // row-tile partial column sums written by the input-gradient GEMMs (the
+  // next layer's bias gradient), one buffer per gradient pair
+  float *bpart[2] = {nullptr, nullptr};
   float *feat = nullptr, *tgt = nullptr, *wgt = nullptr, *loss = nullptr;
   double *scale = nullptr, *delay = nullptr;
   float *outp = nullptr;
@@ -137,8 +143,9 @@ struct gf_sur {
 
   ~gf_sur() {
     hipStreamSynchronize(st);
+    if (st2) hipStreamSynchronize(st2);
     for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)wpk, (void *)dF[0], (void *)dF[1],
-                    (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)feat, (void *)tgt, (void *)wgt,
+                    (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)bpart[0], (void *)bpart[1], (void *)feat, (void *)tgt, (void *)wgt,
                     (void *)loss, (void *)scale, (void *)delay, (void *)outp})
       if (q) hipFree(q);
     for (auto q : X)
@@ -147,6 +154,9 @@ struct gf_sur {
       if (q) hipFree(q);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
+    for (hipEvent_t e : {ev_ready, ev_free[0], ev_free[1]})
+      if (e) hipEventDestroy(e);
+    if (st2) hipStreamDestroy(st2);
     if (st) hipStreamDestroy(st);
   }
 
@@ -160,6 +170,9 @@ struct gf_sur {
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    for (hipEvent_t &e : ev_free) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(dalloc(&p, P));
     HIPCHK(dalloc(&g, P));
     HIPCHK(dalloc(&m, P));
@@ -167,7 +180,7 @@ struct gf_sur {
     HIPCHK(dalloc(&pb, P));
     HIPCHK(dalloc(&pbt, P));
     int64_t wo = 0;
-    for (int l = 0; l < kFusedLayers; l++) {
+    for (int l = 0; l <= kFusedLayers; l++) {
       wpk_off[l] = wo;
       wo += (int64_t)L[l].n * L[l].k;
     }
@@ -187,13 +200,14 @@ struct gf_sur {
       HIPCHK(dalloc(&dB[i], B * widest));
     }
     HIPCHK(dalloc(&split, split_floats));
-    // chunk counts are at most max(256, rows / 1024) for any row count up to
-    // the largest batch (the chunk length grows with the rows)
     int64_t colpart_floats = 1;
     for (const Layer &y : L)
-      colpart_floats = std::max<int64_t>(colpart_floats,
-                                         std::max<int64_t>(256, (B * y.rows + 1023) / 1024) * (int64_t)y.n);
+      colpart_floats = std::max<int64_t>(colpart_floats, gfsk::bias_grad_part_floats(B * y.rows, y.n));
     HIPCHK(dalloc(&colpart, colpart_floats));
+    int64_t bpart_floats = 1;
+    for (size_t l = 1; l < L.size(); l++)
+      if (bias_from_epilogue(l)) bpart_floats = std::max<int64_t>(bpart_floats, (int64_t)row_tiles_dx(l, B) * L[l].k);
+    for (float *&b : bpart) HIPCHK(dalloc(&b, bpart_floats));
     HIPCHK(dalloc(&feat, B * kFeatures));
     HIPCHK(dalloc(&tgt, B * kOut));
     HIPCHK(dalloc(&wgt, B * kOut));
@@ -243,35 +257,55 @@ struct gf_sur {
 
   // pbt[layer] = pb[layer]^T, the B operand of the input-gradient GEMMs
   int transpose_weights() {
-    for (const Layer &y : L) HIPCHK(gfsk::launch_transpose_bf16(pb + y.w_off, y.n, y.k, pbt + y.w_off, st));
+    gfsk::TransposeTable t;
+    if (L.size() > (size_t)gfsk::kMaxTransposeLayers) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: more layers than the transpose table holds");
+    t.layers = (int32_t)L.size();
+    for (size_t l = 0; l < L.size(); l++) {
+      t.off[l] = L[l].w_off;
+      t.n[l] = L[l].n;
+      t.k[l] = L[l].k;
+      t.first_tile[l + 1] = t.first_tile[l] + ((L[l].n + 15) / 16) * ((L[l].k + 15) / 16);
+    }
+    HIPCHK(gfsk::launch_transpose_layers(pb, pbt, t, st));
     packed = false;
     return GF_OK;
   }
 
-  // Inference: the encoder and the decoder's dense layer in one launch
-  // (gfsk::launch_encoder_fused), then the two transposed convolutions.
+  // Inference: the whole network in one launch (gfsk::launch_encoder_fused,
+  // the transposed convolutions in the dense layer's epilogue).
   // GEOFDTD_SUR_UNFUSED=1 runs every layer as its own GEMM instead.
   static bool fused_enabled() {
     const char *e = getenv("GEOFDTD_SUR_UNFUSED");
     return !(e && atoi(e) != 0);
   }
 
-  int forward_fused(int B) {
+  // spec != null: de-normalize into spec (spec_rows rows) in the same launch.
+  int forward_fused(int B, float *spec = nullptr, int spec_rows = 0, double df = 0, int k_lo = 0, int k_hi = 0) {
     if (!packed) {
-      for (int l = 0; l < kFusedLayers; l++)
+      for (int l = 0; l <= kFusedLayers; l++)
         HIPCHK(gfsk::launch_pack_fragments(pb + L[l].w_off, L[l].n, L[l].k, wpk + wpk_off[l], st));
       packed = true;
     }
     gfsk::FusedArgs a;
     a.x0 = X[0];
     a.M = B;
-    for (int l = 0; l < kFusedLayers; l++) {
-      a.wp[l] = wpk + wpk_off[l];
-      a.bias[l] = p + L[l].b_off;
+    for (int l = 0; l <= kFusedLayers; l++) a.wp[l] = wpk + wpk_off[l];
+    for (size_t l = 0; l < L.size(); l++) a.bias[l] = p + L[l].b_off;
+    a.w8 = pb + L[8].w_off;
+    if (spec) {
+      a.spec = spec;
+      a.spec_rows = spec_rows;
+      a.scale = scale;
+      a.delay = delay;
+      a.df = df;
+      a.k_lo = k_lo;
+      a.k_hi = k_hi;
+      a.polar = polar;
+    } else {
+      a.zout = Z.back();
     }
-    a.out = X[kFusedLayers];
     HIPCHK(gfsk::launch_encoder_fused(a, st));
-    return forward_from(kFusedLayers, B, false);
+    return GF_OK;
   }
 
   // Forward over B queries whose padded BF16 features are in X[0].  keep:
@@ -312,10 +346,35 @@ struct gf_sur {
     return GF_OK;
   }
 
+  // Layer l's input gradient (into layer l - 1) is formed in layer l's row
+  // view, k = L[l].k columns.  When those columns are layer l - 1's bias
+  // channels, the GEMM's epilogue also writes the bias gradient's column sums
+  // by row tile, and no FP32 copy of the gradient is needed.
+  bool bias_from_epilogue(size_t l) const { return L[l].k == L[l - 1].bias_mod; }
+
+  gfsk::GemmArgs dx_args(size_t l, int64_t B) const {
+    gfsk::GemmArgs x;
+    x.M = (int32_t)(B * L[l].rows);
+    x.N = L[l].k;
+    x.K = L[l].n;
+    return x;
+  }
+
+  int32_t row_tiles_dx(size_t l, int64_t B) const {
+    return gfsk::gemm_row_tiles(dx_args(l, B), false, true, gfsk::kEpiGeluBwd);
+  }
+
   // Backward from the output gradient in dF[0]/dB[0] ([B][1024]); weight and
-  // bias gradients into g.
+  // bias gradients into g.  Layer l's weight and bias gradients run on st2
+  // beside its input gradient on st: both read the gradient buffers `cur`,
+  // and the input gradient writes the other pair, which layer l + 1's
+  // gradients on st2 (event ev_free[1 - cur]) must have finished reading.
+  // On return st has waited for all of st2's work.
   int backward(int B) {
     int cur = 0;
+    bool pending[2] = {false, false};
+    int32_t part_rows[2] = {0, 0};  // row tiles of bias partials in bpart[i]; 0: use dF[i]
+    HIPCHK(hipEventRecord(ev_ready, st));
     for (int l = (int)L.size() - 1; l >= 0; l--) {
       const Layer &y = L[l];
       const int64_t rows = (int64_t)B * y.rows;
@@ -334,28 +393,40 @@ struct gf_sur {
       w.k_split = (int32_t)(((rows + s - 1) / s + 31) / 32 * 32);
       const int splits = (int)((rows + w.k_split - 1) / w.k_split);
       w.split_stride = (int64_t)y.n * y.k;
-      HIPCHK(gfsk::launch_gemm(w, true, false, gfsk::kEpiF32, splits, st));
-      HIPCHK(gfsk::launch_sum_splits(split, splits, (int64_t)y.n * y.k, g + y.w_off, st));
-      HIPCHK(gfsk::launch_bias_grad(dF[cur], rows, y.n, y.bias_mod, colpart, g + y.b_off, st));
+      HIPCHK(hipStreamWaitEvent(st2, ev_ready, 0));
+      HIPCHK(gfsk::launch_gemm(w, true, false, gfsk::kEpiF32, splits, st2));
+      HIPCHK(gfsk::launch_sum_splits(split, splits, (int64_t)y.n * y.k, g + y.w_off, st2));
+      if (part_rows[cur] > 0)
+        HIPCHK(gfsk::launch_sum_splits(bpart[cur], part_rows[cur], y.bias_mod, g + y.b_off, st2));
+      else
+        HIPCHK(gfsk::launch_bias_grad(dF[cur], rows, y.n, y.bias_mod, colpart, g + y.b_off, st2));
+      HIPCHK(hipEventRecord(ev_free[cur], st2));
+      pending[cur] = true;
       if (l == 0) break;
+      if (pending[1 - cur]) HIPCHK(hipStreamWaitEvent(st, ev_free[1 - cur], 0));
       // dX = dZ W, times GELU'(Z[l-1]) of the layer below
-      gfsk::GemmArgs x;
-      x.M = (int32_t)rows;
-      x.N = y.k;
-      x.K = y.n;
+      gfsk::GemmArgs x = dx_args(l, B);
       x.A = dB[cur];
       x.lda = y.n;
       x.B = pbt + y.w_off;  // B(k=n_out, n=k_in) = W[n_out][k_in] = W^T[k_in][n_out]: k contiguous
       x.ldb = y.n;
       x.z = Z[l - 1];
       x.ldz = y.k;
-      x.cf = dF[1 - cur];
       x.ldc = y.k;
       x.cb = dB[1 - cur];
       x.ldcb = y.k;
+      if (bias_from_epilogue(l)) {
+        x.colpart = bpart[1 - cur];
+        part_rows[1 - cur] = gfsk::gemm_row_tiles(x, false, true, gfsk::kEpiGeluBwd);
+      } else {
+        x.cf = dF[1 - cur];
+        part_rows[1 - cur] = 0;
+      }
       HIPCHK(gfsk::launch_gemm(x, false, true, gfsk::kEpiGeluBwd, 1, st));
+      HIPCHK(hipEventRecord(ev_ready, st));
       cur = 1 - cur;
     }
+    HIPCHK(hipStreamWaitEvent(st, ev_free[cur], 0));
     return GF_OK;
   }
 
@@ -478,14 +549,20 @@ int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double 
     HIPCHK(hipMemcpyAsync(h->scale, scale + q0, sizeof(double) * B, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipMemcpyAsync(h->delay, delay + q0, sizeof(double) * B, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipEventRecord(h->e0, h->st));
-    if ((rc = h->stage_features(h->feat, Bp)) || (rc = h->forward(Bp, false))) return rc;
     int k_lo = 0, k_hi = kOut / 2 - 1;
     if (h->band_hi > 0 && bin_hz > 0) {
       k_lo = (int)ceil(h->band_lo / bin_hz - 1e-9);
       k_hi = (int)floor(h->band_hi / bin_hz + 1e-9);
     }
-    HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->polar,
-                                    h->outp, B, h->st));
+    if ((rc = h->stage_features(h->feat, Bp))) return rc;
+    if (gf_sur::fused_enabled()) {
+      // the whole network and the de-normalization in one launch
+      if ((rc = h->forward_fused(Bp, h->outp, B, bin_hz, k_lo, k_hi))) return rc;
+    } else {
+      if ((rc = h->forward(Bp, false))) return rc;
+      HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->polar,
+                                      h->outp, B, h->st));
+    }
     HIPCHK(hipEventRecord(h->e1, h->st));
     HIPCHK(hipMemcpyAsync(out + (int64_t)q0 * kOut, h->outp, sizeof(float) * B * kOut, hipMemcpyDeviceToHost,
                           h->st));

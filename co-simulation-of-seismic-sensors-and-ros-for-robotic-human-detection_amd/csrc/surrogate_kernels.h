@@ -10,7 +10,8 @@ enum Epilogue {
   kEpiF32 = 0,       // cf[split][m][n] = acc                       (weight gradients)
   kEpiBias = 1,      // cf[m][n] = acc + bias[n % bias_mod]         (output layer)
   kEpiBiasGelu = 2,  // z[m][n] = acc + bias (if z); cb = bf16(gelu(z))   (hidden layers)
-  kEpiGeluBwd = 3,   // d = acc * gelu'(z[m][n]); cf = d; cb = bf16(d)    (gradient into a hidden layer)
+  kEpiGeluBwd = 3,   // d = acc * gelu'(z[m][n]); cf = d (if cf); cb = bf16(d);
+                     // colpart[row tile][n] = sum of d over the tile's rows (if colpart)
 };
 
 // C[M][N] = sum_k A(m,k) B(k,n) over k in this split's range.  A(m,k) is
@@ -35,9 +36,12 @@ struct GemmArgs {
   int64_t ldz = 0;
   int32_t k_split = 1 << 30;
   int64_t split_stride = 0;
+  float *colpart = nullptr;  // [gemm_row_tiles(...)][N]
 };
 
 hipError_t launch_gemm(const GemmArgs &g, bool at, bool bt, int epi, int splits, hipStream_t st);
+// The number of row tiles (grid rows) launch_gemm uses for these arguments.
+int32_t gemm_row_tiles(const GemmArgs &g, bool at, bool bt, int epi);
 hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t st);
 hipError_t launch_pad_rows_bf16(const float *in, int32_t cols_in, uint16_t *out, int32_t cols_out, int64_t rows,
                                 hipStream_t st);
@@ -46,15 +50,27 @@ hipError_t launch_gather_rows(const float *src, const int32_t *idx, int32_t cols
 hipError_t launch_rel_l2(const float *o, const float *t, const float *w, int32_t rows, int32_t len, float scale,
                          float *loss, float *dF, uint16_t *dB, hipStream_t st);
 // Bias gradient: column sums of x[rows][cols] folded onto mod channels.
-// part needs bias_grad_chunks(rows) * cols floats; cols <= 256 must divide 256.
-int32_t bias_grad_chunks(int64_t rows);
+// part needs bias_grad_part_floats(rows, cols) floats for any row count up
+// to rows; cols is a multiple of 4 that divides 256, or a multiple of 256.
+int64_t bias_grad_part_floats(int64_t max_rows, int32_t cols);
 hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t mod, float *part, float *out,
                             hipStream_t st);
-hipError_t launch_sum_splits(const float *part, int32_t splits, int64_t n, float *out, hipStream_t st);
+// out[i] = sum over the splits of part[s][i]; part is overwritten (it holds
+// partial sums afterwards).
+hipError_t launch_sum_splits(float *part, int32_t splits, int64_t n, float *out, hipStream_t st);
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t st);
-// out[k][n] = in[n][k] (BF16).
-hipError_t launch_transpose_bf16(const uint16_t *in, int32_t n_rows, int32_t k_cols, uint16_t *out, hipStream_t st);
+// Per-layer BF16 transposes in one launch: layer i's in[off + n*k_i + k]
+// ([n_i][k_i]) goes to out[off + k*n_i + n].  first_tile[i] counts the 16 x 16
+// tiles of the layers before i (first_tile[layers] = all of them).
+constexpr int kMaxTransposeLayers = 16;
+struct TransposeTable {
+  int32_t layers = 0;
+  int64_t off[kMaxTransposeLayers] = {};
+  int32_t n[kMaxTransposeLayers] = {}, k[kMaxTransposeLayers] = {};
+  int32_t first_tile[kMaxTransposeLayers + 1] = {};
+};
+hipError_t launch_transpose_layers(const uint16_t *in, uint16_t *out, const TransposeTable &t, hipStream_t st);
 hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale, const double *delay, double df,
                               int32_t k_lo, int32_t k_hi, int32_t polar, float *out, int32_t rows, hipStream_t st);
 
@@ -65,18 +81,32 @@ hipError_t launch_denormalize(const float *o, int32_t bins, const double *scale,
 // one 16-deep k-step is one contiguous 1 KB load.
 hipError_t launch_pack_fragments(const uint16_t *w, int32_t n, int32_t k, uint16_t *out, hipStream_t st);
 
-// The encoder and the decoder's dense layer in one kernel: for each block of
-// 32 queries, x0 [M][16] (BF16 features) -> GELU(x W0^T + b0) -> five
-// 512-wide GELU layers -> GELU(h W6^T + b6) [M][2048] into out (BF16).
-// The block's activations stay in LDS between layers; the packed weights
-// stream from L2.  Same arithmetic as the per-layer GEMMs (k-steps in order,
-// bias and GELU in FP32, one BF16 rounding per output).
+// The whole network in one kernel: for each block of 32 queries, x0 [M][16]
+// (BF16 features) -> GELU(x W0^T + b0) -> five 512-wide GELU layers ->
+// GELU(h W6^T + b6), read as 64 positions x 32 channels -> the two
+// transposed convolutions (32 -> 2 x 32 channels with GELU, then 32 -> 4 x 2
+// outputs) -> zout [M][1024] (FP32).  The block's activations stay in LDS
+// between layers; the packed weights stream from L2; each wavefront runs the
+// convolutions on every dense-layer position it forms, in its epilogue.
+// With spec set, the output is de-normalized in the same epilogue (as
+// launch_denormalize would: spec [spec_rows][512] complex) instead of
+// written to zout.  With both null the kernel stops after the dense layer
+// and writes its output to out [M][2048] (BF16).  Same arithmetic as the
+// per-layer GEMMs (k-steps in order, bias and GELU in FP32, one BF16
+// rounding per output).
 struct FusedArgs {
   const uint16_t *x0 = nullptr;
   int32_t M = 0;
-  const uint16_t *wp[7] = {};
-  const float *bias[7] = {};
+  const uint16_t *wp[8] = {};  // layers 0-7 packed in fragment order
+  const float *bias[9] = {};
+  const uint16_t *w8 = nullptr;  // layer 8's BF16 weights [8][32], as stored
   uint16_t *out = nullptr;
+  float *zout = nullptr;
+  float *spec = nullptr;
+  int32_t spec_rows = 0;
+  const double *scale = nullptr, *delay = nullptr;
+  double df = 0;
+  int32_t k_lo = 0, k_hi = 0, polar = 0;
 };
 hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st);
 

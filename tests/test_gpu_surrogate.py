@@ -36,6 +36,9 @@ def bf16_bits(x):
     (64, 8, 16, 0, 1, 1),         # the output layer's shape
     (512, 512, 1000, 1, 0, 4),    # a weight gradient: A stored [K][M], split K
     (32, 16, 4096, 1, 0, 7),
+    (8, 32, 8000, 1, 0, 5),       # the 8-output convolution's weight gradient (K split over wavefronts)
+    (64, 32, 6000, 1, 0, 3),      # the 64-output convolution's weight gradient
+    (48, 24, 520, 1, 0, 1),
     (136, 40, 24, 1, 1, 2),
 ])
 def test_gemm_matches_fp64_within_fp32_accumulation(fdtd, M, N, K, at, bnk, splits):
@@ -144,9 +147,9 @@ def test_weighted_gradients_match_the_oracle(net):
                              w=w.astype(np.float64))
     assert loss == pytest.approx(l_bf, rel=2e-3)
     for i, l in enumerate(L):
-        a, b = l.w_off, l.w_off + l.n * l.k
-        r = np.linalg.norm(g[a:b] - g_bf[a:b]) / np.linalg.norm(g_bf[a:b])
-        assert r <= 5e-3, (i, r)
+        for name, a, b in (("W", l.w_off, l.w_off + l.n * l.k), ("b", l.b_off, l.b_off + l.bias_mod)):
+            r = np.linalg.norm(g[a:b] - g_bf[a:b]) / np.linalg.norm(g_bf[a:b])
+            assert r <= 5e-3, (i, name, r)
 
 
 def test_polar_predict_exponentiates(S):
@@ -315,4 +318,28 @@ def test_fused_inference_equals_the_per_layer_gemms(S, monkeypatch):
     finally:
         net.close()
     print(f"fused vs per-layer: max |diff| {np.max(np.abs(fused - per_layer)):.3e}", flush=True)
+    np.testing.assert_array_equal(fused, per_layer)
+
+
+@pytest.mark.parametrize("polar", [False, True])
+def test_fused_predict_equals_the_per_layer_path(S, monkeypatch, polar):
+    """Predict with the de-normalization fused into the network's launch gives
+    the bits of the per-layer GEMMs followed by the separate de-normalization
+    kernel (both call one device function for a bin, products not
+    contracted).  8,203 queries: a full batch and a ragged one of 11."""
+    rng = np.random.default_rng(22)
+    net = S.Network(max_batch=8192, seed=9, polar=polar)
+    try:
+        x = rng.uniform(-1, 1, (8203, S.FEATURES)).astype(np.float32)
+        scale = rng.uniform(1e-9, 1e-8, 8203)
+        delay = rng.uniform(0.0, 0.4, 8203)
+        fused = net.Predict(x, scale, delay)
+        monkeypatch.setenv("GEOFDTD_SUR_UNFUSED", "1")
+        per_layer = net.Predict(x, scale, delay)
+    finally:
+        net.close()
+    m = S.band_mask()
+    assert np.all(fused[:, ~m] == 0) and np.any(fused[:, m] != 0)
+    print(f"fused vs per-layer predict (polar={polar}): max |diff| {np.max(np.abs(fused - per_layer)):.3e}",
+          flush=True)
     np.testing.assert_array_equal(fused, per_layer)
