@@ -23,6 +23,7 @@
 
 #include <algorithm>
 
+#include "sur_math.h"
 #include "surrogate_kernels.h"
 
 namespace gfsk {
@@ -33,35 +34,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // K per LDS stage is a template parameter BK (a multiple of 16: BK/16 MFMA
 // steps per stage); LDS rows are padded to BK + 8 elements.
 
-__device__ __forceinline__ uint16_t to_bf16(float x) {
-  // round to nearest even, NaN stays NaN: gfx950's v_cvt_pk_bf16_f32
-  const __bf16 h = (__bf16)x;
-  uint16_t r;
-  __builtin_memcpy(&r, &h, 2);
-  return r;
-}
-
-__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-// GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + a z^3), c = sqrt(2/pi),
-// a = 0.044715.  With r = 1 / (1 + e^{2u}), tanh(u) = 1 - 2r, so
-// GELU = z (1 - r) and GELU' = (1 - r) + 2 z r (1 - r) c (1 + 3 a z^2).
-// e^{2u} and the reciprocal are the hardware's (v_exp_f32, v_rcp_f32, about
-// 1 ulp); r runs to 0 and 1 exactly when e^{2u} overflows or vanishes.
-constexpr float kGc = 0.7978845608028654f, kGa = 0.044715f;
-
-__device__ __forceinline__ float gelu_r(float z) {
-  const float p = z * (1.0f + kGa * z * z);
-  const float e = __builtin_amdgcn_exp2f(p * (2.0f * kGc * 1.4426950408889634f));
-  return __builtin_amdgcn_rcpf(1.0f + e);
-}
-
-__device__ __forceinline__ float gelu(float z) { return z * (1.0f - gelu_r(z)); }
-
-__device__ __forceinline__ float gelu_grad(float z) {
-  const float r = gelu_r(z), q = 1.0f - r;
-  return q + 2.0f * z * r * q * kGc * (1.0f + 3.0f * kGa * z * z);
-}
 
 // One ROWS x kBK tile, moved in two halves so that the global loads of the
 // next stage are in flight while the MFMAs of this one run: fetch_tile reads
@@ -568,7 +540,10 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 // phase), s_b the amplitude scale and w_b the delay the targets were divided
 // by; zero outside bins [k_lo, k_hi].
 // One bin's de-normalization; products are not contracted, so every kernel
-// that calls it gives the same bits.
+// that calls it gives the same bits.  The phase -2 pi k df delay (+ the
+// network's phase, polar) is formed and reduced to [-pi, pi] in FP64 (it
+// reaches ~10^3 rad); the sine, cosine, exponential and products are FP32
+// (about 1e-7 relative).
 __device__ __forceinline__ void denorm_bin(float o0, float o1, double scale, double delay, double df, int k, int k_lo,
                                            int k_hi, int polar, float &re_out, float &im_out) {
 #pragma clang fp contract(off)
@@ -577,17 +552,20 @@ __device__ __forceinline__ void denorm_bin(float o0, float o1, double scale, dou
     im_out = 0.0f;
     return;
   }
-  double ph = -2.0 * 3.141592653589793 * (double)k * df * delay;
-  double re = o0, im = o1;
+  constexpr double kTwoPi = 6.283185307179586;
+  double ph = -kTwoPi * (double)k * df * delay;
+  float re = o0, im = o1;
   if (polar) {
-    ph += im;
-    re = exp(re);
-    im = 0.0;
+    ph += (double)o1;
+    re = expf(o0);
+    im = 0.0f;
   }
-  double sn, cs;
-  sincos(ph, &sn, &cs);
-  re_out = (float)(scale * (re * cs - im * sn));
-  im_out = (float)(scale * (re * sn + im * cs));
+  const float phr = (float)(ph - kTwoPi * rint(ph * (1.0 / kTwoPi)));
+  float sn, cs;
+  sincosf(phr, &sn, &cs);
+  const float sc = (float)scale;
+  re_out = sc * (re * cs - im * sn);
+  im_out = sc * (re * sn + im * cs);
 }
 
 __global__ void k_denormalize(const float *__restrict__ o, int32_t bins, const double *__restrict__ scale,
