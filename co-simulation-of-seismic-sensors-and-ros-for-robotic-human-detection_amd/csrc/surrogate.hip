@@ -830,6 +830,10 @@ __device__ __forceinline__ void pipe_fetch(uint4 (&buf)[2][8], const FusedArgs &
 // LDS row pitch (BF16) of the decoder tail's 32 x 32 tiles.
 constexpr int kTailPitch = 32 + 8;
 constexpr int kTailTile = kFusedRows * kTailPitch;
+// The tail's constants, staged in LDS once per workgroup (every position uses
+// them): layer 7's weight fragments, layer 8's weights [8][32], layer 7's 32
+// biases and layer 8's two (offsets in BF16 elements).
+constexpr int kTwW7 = 0, kTwW8 = 4 * 64 * 8, kTwB7 = kTwW8 + 8 * 32, kTwB8 = kTwB7 + 64, kTwSize = kTwB8 + 8;
 
 // Orders a wavefront's LDS writes before its following LDS reads (the tail's
 // tiles are private to the wavefront, so no workgroup barrier is needed).
@@ -850,7 +854,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // s1 and s2 are the wavefront's own LDS tiles (one, and two, of 32 x
 // kTailPitch).
 __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &acc, float bv, int p, int m0,
-                                             int lane, uint16_t *s1, uint16_t *s2) {
+                                             int lane, uint16_t *s1, uint16_t *s2, const uint16_t *tw) {
   const int c = lane & 31, hi = lane >> 5;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
@@ -859,7 +863,7 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
   }
   wave_lds_sync();
   // first convolution: [32 rows][32 channels] x W7^T, two 32-column tiles
-  const uint4 *__restrict__ w7 = (const uint4 *)a.wp[7];
+  const uint4 *w7 = (const uint4 *)(tw + kTwW7);
   f32x16 c1[2];
 #pragma unroll
   for (int j = 0; j < 2; j++)
@@ -876,7 +880,7 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
       c1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, c1[j], 0, 0, 0);
     }
   }
-  const float b1 = a.bias[7][c];
+  const float b1 = ((const float *)(tw + kTwB7))[c];
 #pragma unroll
   for (int j = 0; j < 2; j++)
 #pragma unroll
@@ -897,18 +901,19 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
 #pragma unroll
     for (int ks = 0; ks < 2; ks++) {
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (c < 8) v = *(const uint4 *)(a.w8 + c * 32 + ks * 16 + hi * 8);
+      if (c < 8) v = *(const uint4 *)(tw + kTwW8 + c * 32 + ks * 16 + hi * 8);
       bf16x8 fw;
       __builtin_memcpy(&fw, &v, 16);
       const bf16x8 fx = *(const bf16x8 *)(s2 + j * kTailTile + c * kTailPitch + ks * 16 + hi * 8);
       c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fw, fx, c2, 0, 0, 0);
     }
     const int q = 2 * p + j, col = q * 8 + hi * 4;
+    const float *b8 = (const float *)(tw + kTwB8);
     float4 o;
-    o.x = c2[0] + a.bias[8][0];
-    o.y = c2[1] + a.bias[8][1];
-    o.z = c2[2] + a.bias[8][0];
-    o.w = c2[3] + a.bias[8][1];
+    o.x = c2[0] + b8[0];
+    o.y = c2[1] + b8[1];
+    o.z = c2[2] + b8[0];
+    o.w = c2[3] + b8[1];
     if (a.spec) {
       if (r < a.spec_rows) {
         const double sc = a.scale[r], dl = a.delay[r];
@@ -956,7 +961,8 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f
       // each 32-column tile of the dense layer is one position (32 channels)
       uint16_t *s1 = tail + wave * 3 * kTailTile;
 #pragma unroll
-      for (int t = 0; t < 2; t++) decoder_tail(a, acc[t], bias[t], nt0 + t, m0, lane, s1, s1 + kTailTile);
+      for (int t = 0; t < 2; t++)
+        decoder_tail(a, acc[t], bias[t], nt0 + t, m0, lane, s1, s1 + kTailTile, tail + kFusedWaves * 3 * kTailTile);
     } else {
 #pragma unroll
       for (int t = 0; t < 2; t++) {
@@ -981,9 +987,19 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f
 
 __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFusedRows * kFusedPitch];
-  __shared__ __attribute__((aligned(16))) uint16_t tail[kFusedWaves * 3 * kTailTile];
+  __shared__ __attribute__((aligned(16))) uint16_t tail[kFusedWaves * 3 * kTailTile + kTwSize];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * kFusedRows;
+  if (a.zout || a.spec) {
+    // the tail's constants (read after the barrier below)
+    uint16_t *tw = tail + kFusedWaves * 3 * kTailTile;
+    for (int v = threadIdx.x; v < 4 * 64; v += blockDim.x)
+      ((uint4 *)(tw + kTwW7))[v] = ((const uint4 *)a.wp[7])[v];
+    for (int v = threadIdx.x; v < 8 * 32 / 8; v += blockDim.x)
+      ((uint4 *)(tw + kTwW8))[v] = ((const uint4 *)a.w8)[v];
+    for (int v = threadIdx.x; v < 32; v += blockDim.x) ((float *)(tw + kTwB7))[v] = a.bias[7][v];
+    for (int v = threadIdx.x; v < 2; v += blockDim.x) ((float *)(tw + kTwB8))[v] = a.bias[8][v];
+  }
   // the first chunk of layer 1 is in flight while layer 0 runs
   uint4 b0[2][8], b1[2][8];
   pipe_fetch<0>(b0, a, wave, lane);
