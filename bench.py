@@ -830,7 +830,7 @@ def bench_served(args, fd) -> dict:
         "inference_traces_per_s": inf_rate,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "surrogate forward (BF16 MFMA GEMMs, FP32 accumulation)",
+                     "kernel": "k_encoder_fused: the whole network and the de-normalization in one launch per batch (BF16 MFMA, FP32 accumulation)",
                      "flops_per_query": inf["flops_per_query"]},
         "gate": {"sampled_soils": len(rep.sampled_soils), "failed_soils": len(rep.failed_soils),
                  "escalated": rep.escalated, "fdtd_soils": len(rep.fdtd_soils),
