@@ -737,7 +737,7 @@ def bench_surrogate(args, fd) -> int:
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-            "kernel": "surrogate forward (9 BF16 MFMA GEMMs + output de-normalization)",
+            "kernel": "k_encoder_fused: the whole network and the de-normalization in one launch per batch (BF16 MFMA, FP32 accumulation)",
             "flops_per_query": tr["flops_per_query"],
         },
         "surrogate": {
