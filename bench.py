@@ -239,7 +239,7 @@ def bench_c3(args, fd) -> dict:
     per_launch = st["algorithmic_bytes"] / max(st["step_launches"], 1)
     pass_per_launch = st["pass_bytes"] / max(st["step_launches"], 1)
     achieved = per_launch / launch_s / 1e9
-    lockstep = float(os.environ.get("GEOFDTD_CACHE_MB", "160")) <= 0
+    lockstep = st["budget_mb"] <= 0  # the library's plan: every model advanced together
     ratio = traffic_ratio_from_profiles("c3" if lockstep else "c3_chunked", span)
     kern = kernel_name(span)
     what = "one step per launch" if span == 1 else "two steps per launch, the state read and written once per launch"

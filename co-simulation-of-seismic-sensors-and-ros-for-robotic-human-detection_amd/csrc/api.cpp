@@ -72,14 +72,14 @@ int check_device(std::string &why) {
   return GF_OK;
 }
 
-// Time steps per launch when nothing better is known: one, unless
-// GEOFDTD_STEPS_PER_LAUNCH asks for the two-step kernel.
-int choose_span() {
+// Time steps per launch when nothing better is known: `fallback`, unless
+// GEOFDTD_STEPS_PER_LAUNCH asks for one or two.
+int choose_span(int fallback) {
   if (const char *e = getenv("GEOFDTD_STEPS_PER_LAUNCH")) {
     const int v = atoi(e);
     if (v == 1 || v == 2) return v;
   }
-  return 1;
+  return fallback;
 }
 
 // Tile height when nothing better is known.  One-step kernel: the tallest of

@@ -246,13 +246,17 @@ void Batch::setup(const gf_shot *shots, int32_t n, const std::vector<std::vector
 // double buffers is written): the figure that decides what stays in cache.
 double state_bytes(const Sim &s) { return gfs::step_bytes(s.R) / 2; }
 
-// Models of one kernel variant are advanced in chunks whose combined state fits
-// the GPU's last-level cache (256 MB Infinity Cache): a chunk runs its steps
-// while its arrays are served from cache step after step instead of streaming
-// every model's state from HBM on every step.  Soils are independent, so the
-// order changes nothing in any result.  The budget per chunk is tuned on long
-// runs (candidates 64-224 MB); GEOFDTD_CACHE_MB fixes it, 0 advancing every
-// model together.
+// Models of one kernel variant are advanced either all together or in chunks
+// whose combined state fits the GPU's last-level cache (256 MB Infinity
+// Cache), a chunk running its steps while its arrays are served from cache
+// step after step.  Soils are independent, so the order changes nothing in any
+// result.  A variant whose state exceeds kLockstepStateBytes is advanced all
+// together by the two-step kernel, which moves the state once per two steps:
+// on whole runs of 64 and 256 config-3 soils (6.5 and 26 GB) that beat the
+// best cache-sized chunks with either kernel by 7-15 %.  Smaller variants are
+// chunked, the budget tuned (candidates 96-224 MB).  GEOFDTD_CACHE_MB fixes
+// the budget, 0 advancing every model together.
+constexpr double kLockstepStateBytes = 1e9;
 
 Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
@@ -315,6 +319,9 @@ int Batch::plan() {
     }
     const bool want_tune = tune_env ? atoi(tune_env) != 0 : cell_steps >= 5000000000LL;
     const int64_t bucket = (int64_t)log2((double)std::max<int64_t>(1, cells));
+    double total_state = 0;
+    for (int i : V.order) total_state += state_bytes(*sims[i]);
+    const bool lockstep = budget_env ? atof(budget_env) <= 0 : total_state > kLockstepStateBytes;
     // the models of a chunk under a budget (all of them for 0), cut from
     // position `from` of the longest-first order
     auto prefix_for = [&](double mb, size_t from = 0) {
@@ -329,12 +336,12 @@ int Batch::plan() {
       }
       return out;
     };
-    double mb = budget_env ? atof(budget_env) : 160.0;
+    double mb = budget_env ? atof(budget_env) : lockstep ? 0.0 : 160.0;
     Group T;  // trial group: the first chunk
     T.nmech = V.nmech;
     T.scalar = V.scalar;
     T.order = prefix_for(mb);
-    const int span0 = gfs::choose_span();
+    const int span0 = gfs::choose_span(lockstep ? 2 : 1);
     int64_t cols_rows = 0;
     for (int i : T.order)
       cols_rows += (int64_t)((sims[i]->g.nr + gfl::tile_cols(span0) - 1) / gfl::tile_cols(span0)) * sims[i]->g.nz;
@@ -368,7 +375,13 @@ int Batch::plan() {
       return bytes * span;
     };
     if (want_tune && !rows_env) {
-      gft::Key key{0, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
+      // all together: the two-step kernel's shapes only (the one-step kernel
+      // streams the whole state every step)
+      gft::Key key{lockstep ? 2 : 0, V.nmech, V.scalar ? 1 : 0, (int)V.order.size(), bucket};
+      std::vector<int> cands = gft::shape_candidates();
+      if (lockstep)
+        cands.erase(std::remove_if(cands.begin(), cands.end(), [](int c) { return gft::Shape::of(c).span != 2; }),
+                    cands.end());
       auto prepare = [&](int code) {
         const gft::Shape sh = gft::Shape::of(code);
         T.waves = sh.waves;
@@ -377,11 +390,11 @@ int Batch::plan() {
       auto launch = [&](int, int k) { return trial_launch(k); };
       auto work = [&](int code) { return trial_bytes(T.order, gft::Shape::of(code).span); };
       int code = gft::cached(key);
-      if (!code) code = gft::tune(key, stream, 8, gft::shape_candidates(), shape.code(), prepare, launch, work, nullptr);
+      if (!code) code = gft::tune(key, stream, 8, cands, shape.code(), prepare, launch, work, nullptr);
       if (code) shape = gft::Shape::of(code);
       touched = T.order;
     }
-    if (want_tune && !budget_env) {
+    if (want_tune && !budget_env && !lockstep) {
       // chunk budget: the chunk each candidate cuts from the median model on
       // (the first chunk often holds a single large model whatever the
       // budget), timed at the chosen shape; the rate is its algorithmic

@@ -22,7 +22,7 @@ int check_device(std::string &why);
 // Launch shape when nothing better is known: time steps per launch (1 or 2,
 // GEOFDTD_STEPS_PER_LAUNCH overrides) and the tile height for it
 // (GEOFDTD_TILE_ROWS overrides), from the grid columns and rows to cover.
-int choose_span();
+int choose_span(int fallback = 1);
 int choose_tile_rows(int span, int nr, int nz);
 int choose_tile_rows_cells(int span, int64_t tiles_r_times_nz);
 

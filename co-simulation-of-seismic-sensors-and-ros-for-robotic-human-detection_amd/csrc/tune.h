@@ -37,7 +37,7 @@ std::vector<int> shape_candidates();
 constexpr int kBudgetsMB[] = {96, 128, 160, 192, 224};
 
 struct Key {
-  int what;  // 0: launch shape, 1: chunk budget
+  int what;  // 0: launch shape, 1: chunk budget, 2: launch shape of an all-together batch
   int nmech, scalar, models;
   int64_t cells_bucket;  // log2 of the total cell count
   bool operator<(const Key &o) const;

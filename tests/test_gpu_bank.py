@@ -175,3 +175,47 @@ def test_batch_handle_spectra_equal_bank_generation(fdtd):
     assert all(isinstance(r, fdtd.Result) for r in results)
     np.testing.assert_array_equal(snapped, want_snapped)
     np.testing.assert_array_equal(got, want)
+
+
+def test_large_batch_runs_all_together_with_the_two_step_kernel(fdtd, monkeypatch):
+    """A batch whose state passes 1 GB is planned as one group advanced by the
+    two-step kernel (no environment settings); its traces equal a forced
+    cache-sized-chunk plan on the one-step kernel and single runs, bit for
+    bit.  The first config-3 soils until 1.1 GB of state (72 B per cell), a
+    few hundred steps each, odd counts so that launches end mid-pair."""
+    import cosim_amd.workloads as W
+
+    ws, cells = [], 0
+    for w in W.c3(n_models=40):
+        ws.append(w)
+        _, nr, nz = w.model.Grid()
+        cells += nr * nz
+        if cells * 72 > 1.1e9:
+            break
+    assert cells * 72 > 1.1e9
+    ranges = [1.0, 5.0, 20.0]
+    shots = [fdtd.Shot(w.model, w.force, ranges, 201 + 38 * (i % 5)) for i, w in enumerate(ws)]
+    for var in ("GEOFDTD_CACHE_MB", "GEOFDTD_STEPS_PER_LAUNCH", "GEOFDTD_TILE_ROWS", "GEOFDTD_TUNE"):
+        monkeypatch.delenv(var, raising=False)
+    b = fdtd.Batch(shots)
+    try:
+        b.Advance(1, 1)
+        st = b.Stats()
+        together = b.Results()
+    finally:
+        b.close()
+    print(f"{len(ws)} soils, {cells / 1e6:.1f} M cells: steps per launch {st['steps_per_launch']}, "
+          f"tile rows {st['tile_rows']}", flush=True)
+    assert st["steps_per_launch"] == 2
+    monkeypatch.setenv("GEOFDTD_CACHE_MB", "160")
+    monkeypatch.setenv("GEOFDTD_STEPS_PER_LAUNCH", "1")
+    chunked = fdtd.RunBatch(shots)
+    monkeypatch.delenv("GEOFDTD_CACHE_MB")
+    for i in (0, len(ws) - 1):
+        single = fdtd.Run(shots[i])
+        for a, c, s in zip(together[i].Traces, chunked[i].Traces, single.Traces):
+            np.testing.assert_array_equal(a.Vertical, c.Vertical)
+            np.testing.assert_array_equal(a.Vertical, s.Vertical)
+    for a, c in zip(together, chunked):
+        for ta, tc in zip(a.Traces, c.Traces):
+            np.testing.assert_array_equal(ta.Vertical, tc.Vertical)
