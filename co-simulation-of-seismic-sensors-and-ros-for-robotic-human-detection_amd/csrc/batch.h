@@ -25,6 +25,12 @@ struct Group {
   int64_t descs_cap = 0;
   uint16_t *d_tile_model = nullptr;  // model of each tile
   int64_t tile_model_cap = 0;
+  // two-step groups: the tile indices split by gfk::step2_interior, each kind
+  // model by model ([interior tiles][the others]), so that the interior
+  // tiles run in the kernel instance without the general path
+  uint32_t *d_tile_list = nullptr;
+  int64_t tile_list_cap = 0;
+  std::vector<int64_t> in_prefix, bd_prefix;  // entries of each kind of the first k models
   int32_t *d_nbase = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

@@ -392,13 +392,24 @@ __device__ __forceinline__ void step2_tile(const GridDesc &g, int n, bool two, i
   }
 }
 
-template <int NM, bool SC>
+// PART 0: the launch's tiles are 0 .. (the grid's tiles), each taking the
+// interior or the general path as it finds itself.  PART 1 and 2: the tiles
+// are the n_list entries of tile_list, all interior (1) or all not (2), so
+// that the interior kernel carries none of the general path's registers (155
+// against 199 VGPRs for three mechanisms: three wavefronts per SIMD instead
+// of two).
+template <int NM, bool SC, int PART>
 __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds, const uint16_t *__restrict__ tile_model,
                                                int32_t n_off, int32_t rb_off, int32_t mb_off,
-                                               const int32_t *__restrict__ nbase, int32_t probe, int32_t count) {
+                                               const int32_t *__restrict__ nbase, int32_t probe, int32_t count,
+                                               const uint32_t *__restrict__ tile_list, int32_t n_list) {
   const int lane = threadIdx.x & (kLanes - 1);
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  int tile = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  if constexpr (PART != 0) {
+    if (tile >= n_list) return;
+    tile = __builtin_amdgcn_readfirstlane((int)tile_list[tile]);
+  }
   const int m = tile_model ? __builtin_amdgcn_readfirstlane((int)tile_model[tile]) : 0;
   const GridDesc &g = gds[m];
   const int n = n_off + (nbase ? nbase[0] : 0);
@@ -410,18 +421,22 @@ __global__ __launch_bounds__(256) void k_step2(const GridDesc *__restrict__ gds,
   const int rb = (rb_off ^ (nbase ? nbase[1] : 0)) & 1;
   const int mb = (mb_off ^ (nbase ? nbase[2] : 0)) & 1;
 
-  const int th = g.th;
-  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
-  const int i0 = tr * kOwn2, j0 = tzi * th;
-  const int j1 = min(j0 + th, g.nz);
+  const Tile2 tt = step2_tile_at(g, lt);
+  const int i0 = tt.i0, j0 = tt.j0, j1 = tt.j1;
   // MaxSpeed every 256 steps; at most one of the two steps is a multiple.
   const bool max1 = probe && (n & 255) == 0;
   const bool max2 = two && probe && ((n + 1) & 255) == 0;
   // Stage 1 forms columns i0-2 .. i0+61, velocity rows j0-3 .. j1+2.
-  if (interior(g, i0 - 2, i0 + kOwn2 + 1, j0 - 3, j1 + 2))
+  if constexpr (PART == 1) {
     step2_tile<NM, SC, true>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
-  else
+  } else if constexpr (PART == 2) {
     step2_tile<NM, SC, false>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
+  } else {
+    if (step2_interior(g, i0, j0, j1))
+      step2_tile<NM, SC, true>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
+    else
+      step2_tile<NM, SC, false>(g, n, two, rb, mb, max1, max2, i0, j0, j1, lane);
+  }
 }
 
 // Sim.SurfaceVelocity for one column of the buffer that holds step `cur`.
@@ -486,9 +501,24 @@ static hipError_t launch_step_t(const StepLaunch &L, hipStream_t st) {
 template <int NM, bool SC>
 static hipError_t launch_step_v(const StepLaunch &L, hipStream_t st) {
   if (L.span == 2) {
+    if (L.list_in || L.list_bd) {
+      // the interior tiles and the others as two launches; both read buffer
+      // rb and write the other, so their order does not matter
+      if (L.n_in > 0) {
+        unsigned blocks = (unsigned)((L.n_in + L.waves - 1) / L.waves);
+        hipLaunchKernelGGL((k_step2<NM, SC, 1>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
+                           L.rb_off, L.mb_off, L.nbase, L.probe, L.count, L.list_in, (int32_t)L.n_in);
+      }
+      if (L.n_bd > 0) {
+        unsigned blocks = (unsigned)((L.n_bd + L.waves - 1) / L.waves);
+        hipLaunchKernelGGL((k_step2<NM, SC, 2>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
+                           L.rb_off, L.mb_off, L.nbase, L.probe, L.count, L.list_bd, (int32_t)L.n_bd);
+      }
+      return hipGetLastError();
+    }
     unsigned blocks = (unsigned)((L.tiles + L.waves - 1) / L.waves);
-    hipLaunchKernelGGL((k_step2<NM, SC>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
-                       L.rb_off, L.mb_off, L.nbase, L.probe, L.count);
+    hipLaunchKernelGGL((k_step2<NM, SC, 0>), dim3(blocks), dim3(64 * L.waves), 0, st, L.d, L.tile_model, L.n_off,
+                       L.rb_off, L.mb_off, L.nbase, L.probe, L.count, (const uint32_t *)nullptr, 0);
     return hipGetLastError();
   }
   switch (L.th) {

@@ -39,6 +39,11 @@ struct StepLaunch {
   int32_t n_off = 0, rb_off = 0, mb_off = 0;
   const int32_t *nbase = nullptr;
   int32_t probe = 0, count = 1;
+  // span 2 only: the tiles as two lists of tile indices, those for which
+  // step2_interior holds (list_in, n_in) and the others (list_bd, n_bd), run
+  // as two launches; null: tiles 0 .. tiles as one launch
+  const uint32_t *list_in = nullptr, *list_bd = nullptr;
+  int64_t n_in = 0, n_bd = 0;
 };
 hipError_t launch_step(const StepLaunch &L, hipStream_t st);
 hipError_t launch_dft_f32(const float *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,

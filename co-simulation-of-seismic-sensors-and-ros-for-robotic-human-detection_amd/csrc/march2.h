@@ -93,6 +93,28 @@ GF_HD float tzz_for_step(const GridDesc &g, int n, int t, int c, float stored) {
   return stored;
 }
 
+// Tile lt of a model's two-step tiling: owned columns from i0 (kOwn2 of them,
+// fewer at the grid's edge), rows [j0, j1).
+struct Tile2 {
+  int i0, j0, j1;
+};
+
+GF_HD Tile2 step2_tile_at(const GridDesc &g, int lt) {
+  const int tr = lt % g.tiles_r, tzi = lt / g.tiles_r;
+  Tile2 t;
+  t.i0 = tr * kOwn2;
+  t.j0 = tzi * g.th;
+  t.j1 = t.j0 + g.th < g.nz ? t.j0 + g.th : g.nz;
+  return t;
+}
+
+// Whether a two-step tile is clear of the axis, the surface and the C-PML
+// strips: stage 1 forms columns i0-2 .. i0+kOwn2+1 and velocity rows j0-3 ..
+// j1+2.  The host sorts tiles by this test, with the kernel's own function.
+GF_HD bool step2_interior(const GridDesc &g, int i0, int j0, int j1) {
+  return interior(g, i0 - 2, i0 + kOwn2 + 1, j0 - 3, j1 + 2);
+}
+
 // Stage 1's outputs as stage 2 reads them: the velocity half of a row
 // (formed at march row t) and the stress half (formed at march row t + 1).
 struct VelOut {

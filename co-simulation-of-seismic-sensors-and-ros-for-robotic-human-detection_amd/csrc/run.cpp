@@ -26,6 +26,7 @@
 #include "model.h"
 #include "batch.h"
 #include "layout.h"
+#include "march2.h"
 #include "sim.h"
 #include "tune.h"
 
@@ -52,6 +53,11 @@ void free_group(Group &G) {
   if (G.d_tile_model) hipFree(G.d_tile_model);
   G.d_tile_model = nullptr;
   G.tile_model_cap = 0;
+  if (G.d_tile_list) hipFree(G.d_tile_list);
+  G.d_tile_list = nullptr;
+  G.tile_list_cap = 0;
+  G.in_prefix.clear();
+  G.bd_prefix.clear();
   G.descs_cap = 0;
   G.exec = nullptr;
   G.graph = nullptr;
@@ -121,6 +127,32 @@ int plan_group(Group &G, std::vector<std::unique_ptr<Sim>> &sims, int span, int 
     }
     HIPCHK(hipMemcpy(G.d_tile_model, tm.data(), sizeof(uint16_t) * tm.size(), hipMemcpyHostToDevice));
   }
+  G.in_prefix.clear();
+  G.bd_prefix.clear();
+  if (span == 2) {
+    // the two-step kernel's interior tiles and the others, as separate lists
+    std::vector<uint32_t> in, bd;
+    G.in_prefix.assign(1, 0);
+    G.bd_prefix.assign(1, 0);
+    for (size_t m = 0; m < descs.size(); m++) {
+      const gfk::GridDesc &d = descs[m];
+      for (int lt = 0; lt < d.tiles_r * d.tiles_z; lt++) {
+        const gfk::Tile2 t = gfk::step2_tile_at(d, lt);
+        (gfk::step2_interior(d, t.i0, t.j0, t.j1) ? in : bd).push_back((uint32_t)(G.prefix[m] + lt));
+      }
+      G.in_prefix.push_back((int64_t)in.size());
+      G.bd_prefix.push_back((int64_t)bd.size());
+    }
+    const int64_t n = (int64_t)(in.size() + bd.size());
+    if (n > G.tile_list_cap) {
+      if (G.d_tile_list) (void)hipFree(G.d_tile_list);
+      G.d_tile_list = nullptr;
+      HIPCHK(hipMalloc(&G.d_tile_list, sizeof(uint32_t) * (size_t)n));
+      G.tile_list_cap = n;
+    }
+    in.insert(in.end(), bd.begin(), bd.end());
+    HIPCHK(hipMemcpy(G.d_tile_list, in.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice));
+  }
   int rc = drop_graph(G, st);
   if (rc) return rc;
   G.graph_models = -1;
@@ -139,6 +171,12 @@ gfk::StepLaunch group_launch(const Group &G, int models) {
   L.tile_model = G.d_tile_model;
   L.tiles = G.prefix[models];
   L.count = G.span;
+  if (G.span == 2 && !G.in_prefix.empty()) {
+    L.list_in = G.d_tile_list;
+    L.n_in = G.in_prefix[models];
+    L.list_bd = G.d_tile_list + G.in_prefix.back();
+    L.n_bd = G.bd_prefix[models];
+  }
   return L;
 }
 

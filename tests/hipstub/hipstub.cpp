@@ -33,6 +33,7 @@
 
 #include "kernels.h"
 #include "layout.h"
+#include "march2.h"
 
 namespace {
 
@@ -107,6 +108,8 @@ struct Launch {
   int32_t n_off = 0, rb_off = 0, mb_off = 0;
   const int32_t *nbase = nullptr;
   int32_t probe = 0;
+  const uint32_t *list_in = nullptr, *list_bd = nullptr;  // two-step tiles split by kind
+  int64_t n_in = 0, n_bd = 0;
   int32_t *base_ptr = nullptr;
   int32_t base_val = 0, base_buf = 0, base_mbuf = 0;
 };
@@ -252,6 +255,32 @@ void run_launch(ihipStream_t *s, const Launch &L) {
   } else {
     if (L.tiles > (int64_t)L.d[0].tiles_r * L.d[0].tiles_z) fail("launch of %lld tiles over one model", (long long)L.tiles);
     seen[0] = L.tiles;
+  }
+  if (L.list_in || L.list_bd) {
+    // the interior and the other tiles of a two-step launch: both lists
+    // live, every entry a tile of this launch of the right kind, together
+    // every tile exactly once
+    if (L.span != 2) fail("tile lists on a %d-step launch", L.span);
+    if (L.n_in + L.n_bd != L.tiles)
+      fail("tile lists of %lld + %lld entries for %lld tiles", (long long)L.n_in, (long long)L.n_bd, (long long)L.tiles);
+    if (!L.tm) fail("tile lists without a tile-to-model table");
+    std::vector<char> hit((size_t)L.tiles, 0);
+    for (int kind = 0; kind < 2; kind++) {
+      const uint32_t *list = kind ? L.list_bd : L.list_in;
+      const int64_t n = kind ? L.n_bd : L.n_in;
+      if (n == 0) continue;
+      reads.insert(need(list, sizeof(uint32_t) * (size_t)n, kind ? "boundary tile list" : "interior tile list"));
+      for (int64_t e = 0; e < n; e++) {
+        const int64_t t = list[e];
+        if (t >= L.tiles) fail("tile list entry %lld is tile %lld of a launch of %lld", (long long)e, (long long)t,
+                               (long long)L.tiles);
+        if (hit[(size_t)t]++) fail("tile %lld listed twice", (long long)t);
+        const gfk::GridDesc &g = L.d[L.tm[t]];
+        const gfk::Tile2 tt = gfk::step2_tile_at(g, (int)(t - g.tile_base));
+        if (gfk::step2_interior(g, tt.i0, tt.j0, tt.j1) != (kind == 0))
+          fail("tile %lld in the %s list", (long long)t, kind ? "boundary" : "interior");
+      }
+    }
   }
   for (size_t m = 0; m < ndesc; m++)
     if (seen[m]) {
@@ -588,6 +617,10 @@ hipError_t launch_step(const StepLaunch &S, hipStream_t st) {
   L.mb_off = S.mb_off;
   L.nbase = S.nbase;
   L.probe = S.probe;
+  L.list_in = S.list_in;
+  L.list_bd = S.list_bd;
+  L.n_in = S.n_in;
+  L.n_bd = S.n_bd;
   submit(st, L);
   return hipSuccess;
 }
