@@ -18,6 +18,7 @@
 #include "../../include/geofdtd.h"
 #include "kernels.h"
 #include "layout.h"
+#include "march2.h"
 #include "model.h"
 #include "sim.h"
 #include "tune.h"
@@ -82,6 +83,14 @@ int choose_span(int fallback) {
   return fallback;
 }
 
+// Two-step launches are split by tile kind when at least this many of their
+// tiles are not interior (gfk::kSplitMinBoundaryTiles);
+// GEOFDTD_SPLIT_MIN_TILES overrides it (0 always splits).
+int64_t split_min_boundary_tiles() {
+  if (const char *e = getenv("GEOFDTD_SPLIT_MIN_TILES")) return atoll(e);
+  return gfk::kSplitMinBoundaryTiles;
+}
+
 // Tile height when nothing better is known.  One-step kernel: the tallest of
 // the unrolled variants (8, 4, 2, 1 rows) that still puts about a dozen waves
 // on each of the 1024 SIMDs, since the step is bound by how many loads are
@@ -117,6 +126,7 @@ Sim::~Sim() {
   if (graph) hipGraphDestroy(graph);
   if (arena) hipFree(arena);
   if (d_desc) hipFree(d_desc);
+  if (d_tile_list) hipFree(d_tile_list);
   if (d_scratch) hipFree(d_scratch);
   if (d_src) hipFree(d_src);
   if (d_rec) hipFree(d_rec);
@@ -200,6 +210,26 @@ int Sim::reset_state() {
 int Sim::upload_desc() {
   HIPCHK(hipStreamSynchronize(stream));
   HIPCHK(hipMemcpy(d_desc, &g, sizeof g, hipMemcpyHostToDevice));
+  n_interior = n_boundary = 0;
+  if (span == 2) {
+    // the interior tiles and the others as two lists (two launches)
+    std::vector<uint32_t> in, bd;
+    for (int lt = 0; lt < g.tiles_r * g.tiles_z; lt++) {
+      const gfk::Tile2 t = gfk::step2_tile_at(g, lt);
+      (gfk::step2_interior(g, t.i0, t.j0, t.j1) ? in : bd).push_back((uint32_t)(g.tile_base + lt));
+    }
+    n_interior = (int64_t)in.size();
+    n_boundary = (int64_t)bd.size();
+    const int64_t n = n_interior + n_boundary;
+    if (n > tile_list_cap) {
+      if (d_tile_list) (void)hipFree(d_tile_list);
+      d_tile_list = nullptr;
+      HIPCHK(hipMalloc(&d_tile_list, sizeof(uint32_t) * (size_t)n));
+      tile_list_cap = n;
+    }
+    in.insert(in.end(), bd.begin(), bd.end());
+    if (n > 0) HIPCHK(hipMemcpy(d_tile_list, in.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice));
+  }
   return GF_OK;
 }
 
@@ -364,6 +394,12 @@ gfk::StepLaunch Sim::step_launch() const {
   L.waves = waves;
   L.d = d_desc;
   L.tiles = (int64_t)g.tiles_r * g.tiles_z;
+  if (span == 2 && d_tile_list && n_interior + n_boundary == L.tiles && n_boundary >= split_min_boundary_tiles()) {
+    L.list_in = d_tile_list;
+    L.n_in = n_interior;
+    L.list_bd = d_tile_list + n_interior;
+    L.n_bd = n_boundary;
+  }
   return L;
 }
 

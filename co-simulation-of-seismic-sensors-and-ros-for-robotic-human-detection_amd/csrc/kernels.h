@@ -45,6 +45,13 @@ struct StepLaunch {
   const uint32_t *list_in = nullptr, *list_bd = nullptr;
   int64_t n_in = 0, n_bd = 0;
 };
+
+// The lists pay only when the other tiles are enough to fill the GPU by
+// themselves (their launch runs after the interior one): at least four
+// wavefronts per CU.  Below that the one launch hides them among the interior
+// tiles (a 1.2 GB single grid with ~900 of them: 691 us per two steps as one
+// launch, 600 + 113 us as two).
+constexpr int64_t kSplitMinBoundaryTiles = 4096;
 hipError_t launch_step(const StepLaunch &L, hipStream_t st);
 hipError_t launch_dft_f32(const float *x, int64_t ld, int32_t rows, int32_t n, double dt, double t0,
                           const double *freqs, int32_t nbins, double *out, hipStream_t st);

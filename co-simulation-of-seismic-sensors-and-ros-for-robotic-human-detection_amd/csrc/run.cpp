@@ -171,7 +171,7 @@ gfk::StepLaunch group_launch(const Group &G, int models) {
   L.tile_model = G.d_tile_model;
   L.tiles = G.prefix[models];
   L.count = G.span;
-  if (G.span == 2 && !G.in_prefix.empty()) {
+  if (G.span == 2 && !G.in_prefix.empty() && G.bd_prefix[models] >= gfs::split_min_boundary_tiles()) {
     L.list_in = G.d_tile_list;
     L.n_in = G.in_prefix[models];
     L.list_bd = G.d_tile_list + G.in_prefix.back();

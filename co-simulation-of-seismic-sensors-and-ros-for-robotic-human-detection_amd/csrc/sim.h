@@ -23,6 +23,7 @@ int check_device(std::string &why);
 // GEOFDTD_STEPS_PER_LAUNCH overrides) and the tile height for it
 // (GEOFDTD_TILE_ROWS overrides), from the grid columns and rows to cover.
 int choose_span(int fallback = 1);
+int64_t split_min_boundary_tiles();
 int choose_tile_rows(int span, int nr, int nz);
 int choose_tile_rows_cells(int span, int64_t tiles_r_times_nz);
 
@@ -53,6 +54,9 @@ struct Sim {
   int buf = 0;      // the buffer holding the current wavefields
   int mbuf = 0;     // the buffer holding the memory variables and stress psi
   gfk::GridDesc *d_desc = nullptr;
+  // span 2: the tiles split by gfk::step2_interior ([interior][the others])
+  uint32_t *d_tile_list = nullptr;
+  int64_t tile_list_cap = 0, n_interior = 0, n_boundary = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *arena = nullptr;

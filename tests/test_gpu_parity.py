@@ -380,12 +380,16 @@ def test_interior_path_gives_the_same_bits(fdtd, oracle, monkeypatch, gsls):
     force = fdtd.Ricker(1000, 30, dt, steps)
     ranges = [0.5, 2.0, 5.0, 9.0]
     out = {}
-    for span, th in [(1, 1), (1, 2), (1, 8), (1, 3), (2, 4), (2, 16), (2, 37)]:
+    # split: two-step launches as an interior-tile launch and another one
+    # (forced; a grid this small stays one launch by default)
+    for span, th, split in [(1, 1, 0), (1, 2, 0), (1, 8, 0), (1, 3, 0), (2, 4, 0), (2, 16, 0), (2, 37, 0), (2, 4, 1),
+                            (2, 16, 1)]:
         monkeypatch.setenv("GEOFDTD_STEPS_PER_LAUNCH", str(span))
         monkeypatch.setenv("GEOFDTD_TILE_ROWS", str(th))
+        monkeypatch.setenv("GEOFDTD_SPLIT_MIN_TILES", "0" if split else "1000000000")
         r = fdtd.Run(fdtd.Shot(m, force, ranges, steps))
-        out[span, th] = (np.array([t.Vertical for t in r.Traces]), r.Peak)
-    ref, peak = out[1, 1]
+        out[span, th, split] = (np.array([t.Vertical for t in r.Traces]), r.Peak)
+    ref, peak = out[1, 1, 0]
     for key, (tr, pk) in out.items():
         np.testing.assert_array_equal(tr, ref, err_msg=str(key))
         assert pk == peak
@@ -493,3 +497,28 @@ def test_linearity_in_the_force(fdtd):
     for ta, tb, tz in zip(a.Traces, b.Traces, z.Traces):
         assert np.max(np.abs(2 * ta.Vertical - tb.Vertical)) <= 2e-6 * np.max(np.abs(tb.Vertical))
         assert not np.any(tz.Vertical)
+
+
+def test_split_two_step_batch_launches_give_the_same_bits(fdtd, monkeypatch):
+    """Two-step batch launches split into a launch of the interior tiles and
+    one of the others (forced here; by default a batch splits only when its
+    other tiles fill the GPU) give the one launch's bits, advanced together
+    and in chunks, with odd step counts."""
+    import cosim_amd.workloads as W
+
+    ws = W.c3(n_models=3, first=3)
+    ranges = [1.0, 5.0, 20.0]
+    shots = [fdtd.Shot(w.model, w.force, ranges, 301 + 40 * i) for i, w in enumerate(ws)]
+    monkeypatch.setenv("GEOFDTD_STEPS_PER_LAUNCH", "2")
+    monkeypatch.delenv("GEOFDTD_TILE_ROWS", raising=False)
+    out = {}
+    for budget in ("0", "160"):
+        monkeypatch.setenv("GEOFDTD_CACHE_MB", budget)
+        for split in ("0", "1000000000"):
+            monkeypatch.setenv("GEOFDTD_SPLIT_MIN_TILES", split)
+            out[budget, split] = fdtd.RunBatch(shots)
+    ref = out["0", "1000000000"]
+    for key, res in out.items():
+        for a, b in zip(res, ref):
+            for ta, tb in zip(a.Traces, b.Traces):
+                np.testing.assert_array_equal(ta.Vertical, tb.Vertical, err_msg=str(key))

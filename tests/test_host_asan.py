@@ -40,11 +40,16 @@ def driver():
     return os.path.join(HERE, "lib", "batch_driver")
 
 
-@pytest.mark.parametrize("span", [1, 2])
+@pytest.mark.parametrize("span,split", [(1, False), (2, False), (2, True)], ids=["span1", "span2", "span2-split"])
 @pytest.mark.parametrize("scenario", ["chunked", "together", "mixed", "sim", "handle", "tuned"])
-def test_host_path_is_clean_under_asan(driver, scenario, span):
+def test_host_path_is_clean_under_asan(driver, scenario, span, split):
+    """split: two-step launches always split into interior and other tiles
+    (the stand-in checks both lists against the tiles and the interior test)."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", GEOFDTD_STEPS_PER_LAUNCH=str(span))
     env.pop("GEOFDTD_TILE_ROWS", None)
+    env.pop("GEOFDTD_SPLIT_MIN_TILES", None)
+    if split:
+        env["GEOFDTD_SPLIT_MIN_TILES"] = "0"
     p = subprocess.run([driver, scenario], capture_output=True, text=True, timeout=300, env=env)
     print(p.stdout + p.stderr[-3000:])
     assert p.returncode == 0, p.stderr[-3000:]
