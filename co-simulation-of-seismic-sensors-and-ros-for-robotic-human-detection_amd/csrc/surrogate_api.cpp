@@ -371,6 +371,15 @@ struct gf_sur {
   // gradients on st2 (event ev_free[1 - cur]) must have finished reading.
   // On return st has waited for all of st2's work.
   int backward(int B) {
+    // on an error return, st2's work must not outlive the call (a later call
+    // would reuse the buffers it reads)
+    struct Drain {
+      hipStream_t s;
+      bool done = false;
+      ~Drain() {
+        if (!done) (void)hipStreamSynchronize(s);
+      }
+    } drain{st2};
     int cur = 0;
     bool pending[2] = {false, false};
     int32_t part_rows[2] = {0, 0};  // row tiles of bias partials in bpart[i]; 0: use dF[i]
@@ -427,6 +436,7 @@ struct gf_sur {
       cur = 1 - cur;
     }
     HIPCHK(hipStreamWaitEvent(st, ev_free[cur], 0));
+    drain.done = true;
     return GF_OK;
   }
 
