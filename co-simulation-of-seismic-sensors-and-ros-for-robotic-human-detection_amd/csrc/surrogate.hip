@@ -714,7 +714,9 @@ hipError_t launch_pack_fragments(const uint16_t *w, int32_t n, int32_t k, uint16
 }
 
 constexpr int kFusedRows = 32;        // queries per workgroup: one MFMA row tile
-constexpr int kFusedWaves = 8;        // wavefronts per workgroup, splitting the output columns
+constexpr int kFusedWaves = 16;       // wavefronts per workgroup, splitting the output columns
+constexpr int kFusedTN = 1;           // 32-column tiles a wavefront of layers 1-6 carries at once
+static_assert(kFusedWaves * kFusedTN == 16, "one tile group per wavefront for the 512-wide layers");
 constexpr int kFusedPitch = 512 + 8;  // LDS row pitch of the activations (BF16 elements)
 
 // One layer of the fused chain: out = GELU(in W^T + bias) for the block's 32
@@ -751,7 +753,7 @@ template <int K, int N, bool GLOBAL_OUT>
 __device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *act_out, const uint4 *__restrict__ wp,
                                             const float *__restrict__ bias, uint16_t *__restrict__ gout, int m0, int M,
                                             int lane, int wave) {
-  constexpr int KS = K / 16, NT = N / 32, TPW = NT / kFusedWaves, TN = 2;
+  constexpr int KS = K / 16, NT = N / 32, TPW = NT / kFusedWaves, TN = TPW >= 2 ? 2 : 1;
   constexpr int KC = KS < 8 ? KS : 8, NC = KS / KC;
   static_assert(TPW % TN == 0 && KS % KC == 0 && (NC == 1 || NC % 2 == 0), "whole tile pairs and chunk pairs");
 #pragma unroll 1
@@ -808,21 +810,21 @@ constexpr int kPipeChunks = 5 * 4 + 16;
 template <int C>
 struct PipeChunk {
   static constexpr int L = C < 20 ? 1 + C / 4 : 6;        // layer
-  static constexpr int G = C < 20 ? 0 : (C - 20) / 4;     // tile pair within the wavefront's tiles
+  static constexpr int G = C < 20 ? 0 : (C - 20) / 4;     // tile group (kFusedTN tiles) within the wavefront's
   static constexpr int KC = C % 4;                        // chunk of 8 k-steps within K = 512
   static constexpr int N = L == 6 ? 2048 : 512;
   static constexpr int TPW = N / 32 / kFusedWaves;        // tiles per wavefront
   static constexpr bool kLastOfPair = KC == 3;
-  static constexpr bool kLastOfLayer = kLastOfPair && (L < 6 || G == TPW / 2 - 1);
+  static constexpr bool kLastOfLayer = kLastOfPair && (L < 6 || G == TPW / kFusedTN - 1);
 };
 
 template <int C>
-__device__ __forceinline__ void pipe_fetch(uint4 (&buf)[2][8], const FusedArgs &a, int wave, int lane) {
+__device__ __forceinline__ void pipe_fetch(uint4 (&buf)[kFusedTN][8], const FusedArgs &a, int wave, int lane) {
   using P = PipeChunk<C>;
   const uint4 *__restrict__ wp = (const uint4 *)a.wp[P::L];
-  const int nt0 = wave * P::TPW + P::G * 2;
+  const int nt0 = wave * P::TPW + P::G * kFusedTN;
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+  for (int t = 0; t < kFusedTN; t++)
 #pragma unroll
     for (int s = 0; s < 8; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * 32 + P::KC * 8 + s) * 64 + lane];
 }
@@ -830,6 +832,10 @@ __device__ __forceinline__ void pipe_fetch(uint4 (&buf)[2][8], const FusedArgs &
 // LDS row pitch (BF16) of the decoder tail's 32 x 32 tiles.
 constexpr int kTailPitch = 32 + 8;
 constexpr int kTailTile = kFusedRows * kTailPitch;
+// Each wavefront's tail tiles: the dense layer's GELU output shares its tile
+// with the first convolution's second output tile, written after the loads
+// that read it (a wavefront's LDS operations run in order).
+constexpr int kTailTilesPerWave = 2;
 // The tail's constants, staged in LDS once per workgroup (every position uses
 // them): layer 7's weight fragments, layer 8's weights [8][32], layer 7's 32
 // biases and layer 8's two (offsets in BF16 elements).
@@ -851,10 +857,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // (query, 2p + j), column tap * 2 + component -- or, de-normalized, to spec.
 // The last product is formed transposed (W8 as the A operand), so each lane
 // holds four consecutive outputs of one query and stores them as one vector.
-// s1 and s2 are the wavefront's own LDS tiles (one, and two, of 32 x
-// kTailPitch).
+// s1 is the first of the wavefront's own two LDS tiles of 32 x kTailPitch.
 __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &acc, float bv, int p, int m0,
-                                             int lane, uint16_t *s1, uint16_t *s2, const uint16_t *tw) {
+                                             int lane, uint16_t *s1, const uint16_t *tw) {
   const int c = lane & 31, hi = lane >> 5;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
@@ -862,32 +867,31 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
     s1[row * kTailPitch + c] = to_bf16(gelu(acc[i] + bv));
   }
   wave_lds_sync();
-  // first convolution: [32 rows][32 channels] x W7^T, two 32-column tiles
+  // first convolution: [32 rows][32 channels] x W7^T, one 32-column output
+  // tile (tap j) at a time.  Tap 0 goes to the wavefront's second tile; tap 1
+  // overwrites the input tile s1, after its own operand loads from it.
   const uint4 *w7 = (const uint4 *)(tw + kTwW7);
-  f32x16 c1[2];
+  const float b1 = ((const float *)(tw + kTwB7))[c];
+  uint16_t *const out1[2] = {s1 + kTailTile, s1};
 #pragma unroll
-  for (int j = 0; j < 2; j++)
+  for (int j = 0; j < 2; j++) {
+    f32x16 c1;
 #pragma unroll
-    for (int i = 0; i < 16; i++) c1[j][i] = 0.0f;
+    for (int i = 0; i < 16; i++) c1[i] = 0.0f;
 #pragma unroll
-  for (int ks = 0; ks < 2; ks++) {
-    const bf16x8 fa = *(const bf16x8 *)(s1 + c * kTailPitch + ks * 16 + hi * 8);
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int ks = 0; ks < 2; ks++) {
+      const bf16x8 fa = *(const bf16x8 *)(s1 + c * kTailPitch + ks * 16 + hi * 8);
       const uint4 v = w7[(j * 2 + ks) * 64 + lane];
       bf16x8 fb;
       __builtin_memcpy(&fb, &v, 16);
-      c1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, c1[j], 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, c1, 0, 0, 0);
     }
-  }
-  const float b1 = ((const float *)(tw + kTwB7))[c];
-#pragma unroll
-  for (int j = 0; j < 2; j++)
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int row = (i / 4) * 8 + hi * 4 + (i % 4);
-      s2[j * kTailTile + row * kTailPitch + c] = to_bf16(gelu(c1[j][i] + b1));
+      out1[j][row * kTailPitch + c] = to_bf16(gelu(c1[i] + b1));
     }
+  }
   wave_lds_sync();
   // second convolution, transposed: W8 [8 outputs (rows 8-31 zero, as the
   // per-layer GEMM pads them)][32 channels] x the tile^T.  Lane (query c,
@@ -904,7 +908,7 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
       if (c < 8) v = *(const uint4 *)(tw + kTwW8 + c * 32 + ks * 16 + hi * 8);
       bf16x8 fw;
       __builtin_memcpy(&fw, &v, 16);
-      const bf16x8 fx = *(const bf16x8 *)(s2 + j * kTailTile + c * kTailPitch + ks * 16 + hi * 8);
+      const bf16x8 fx = *(const bf16x8 *)(out1[j] + c * kTailPitch + ks * 16 + hi * 8);
       c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fw, fx, c2, 0, 0, 0);
     }
     const int q = 2 * p + j, col = q * 8 + hi * 4;
@@ -927,11 +931,12 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
       *(float4 *)(a.zout + (int64_t)r * 1024 + col) = o;
     }
   }
-  wave_lds_sync();  // the next position's tiles reuse s1 and s2
+  wave_lds_sync();  // the next position reuses the wavefront's tiles
 }
 
 template <int C>
-__device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f32x16 (&acc)[2], float (&bias)[2],
+__device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][8], uint4 (&b1)[kFusedTN][8], f32x16 (&acc)[kFusedTN],
+                                         float (&bias)[kFusedTN],
                                          uint16_t (*act)[kFusedRows * kFusedPitch], uint16_t *tail,
                                          const FusedArgs &a, int m0, int wave, int lane) {
   using P = PipeChunk<C>;
@@ -942,30 +947,30 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f
   if constexpr (P::KC == 0) {
     // the pair's biases, loaded now so that the epilogue's wait on them does
     // not also wait for the chunks prefetched after them
-    const int nt0 = wave * P::TPW + P::G * 2;
+    const int nt0 = wave * P::TPW + P::G * kFusedTN;
 #pragma unroll
-    for (int t = 0; t < 2; t++) bias[t] = a.bias[P::L][(nt0 + t) * 32 + (lane & 31)];
+    for (int t = 0; t < kFusedTN; t++) bias[t] = a.bias[P::L][(nt0 + t) * 32 + (lane & 31)];
   }
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (P::KC == 0) {
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < kFusedTN; t++)
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
   }
-  if constexpr (C & 1) mfma_chunk<8, 2>(acc, b1, act[P::L & 1], P::KC, lane);
-  else mfma_chunk<8, 2>(acc, b0, act[P::L & 1], P::KC, lane);
+  if constexpr (C & 1) mfma_chunk<8, kFusedTN>(acc, b1, act[P::L & 1], P::KC, lane);
+  else mfma_chunk<8, kFusedTN>(acc, b0, act[P::L & 1], P::KC, lane);
   if constexpr (P::kLastOfPair) {
-    const int nt0 = wave * P::TPW + P::G * 2;
+    const int nt0 = wave * P::TPW + P::G * kFusedTN;
     if (P::L == 6 && (a.zout || a.spec)) {
       // each 32-column tile of the dense layer is one position (32 channels)
-      uint16_t *s1 = tail + wave * 3 * kTailTile;
+      uint16_t *s1 = tail + wave * kTailTilesPerWave * kTailTile;
 #pragma unroll
-      for (int t = 0; t < 2; t++)
-        decoder_tail(a, acc[t], bias[t], nt0 + t, m0, lane, s1, s1 + kTailTile, tail + kFusedWaves * 3 * kTailTile);
+      for (int t = 0; t < kFusedTN; t++)
+        decoder_tail(a, acc[t], bias[t], nt0 + t, m0, lane, s1, tail + kFusedWaves * kTailTilesPerWave * kTailTile);
     } else {
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
+      for (int t = 0; t < kFusedTN; t++) {
         const int col = (nt0 + t) * 32 + (lane & 31);
         const float bv = bias[t];
 #pragma unroll
@@ -987,12 +992,12 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[2][8], uint4 (&b1)[2][8], f
 
 __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFusedRows * kFusedPitch];
-  __shared__ __attribute__((aligned(16))) uint16_t tail[kFusedWaves * 3 * kTailTile + kTwSize];
+  __shared__ __attribute__((aligned(16))) uint16_t tail[kFusedWaves * kTailTilesPerWave * kTailTile + kTwSize];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * kFusedRows;
   if (a.zout || a.spec) {
     // the tail's constants (read after the barrier below)
-    uint16_t *tw = tail + kFusedWaves * 3 * kTailTile;
+    uint16_t *tw = tail + kFusedWaves * kTailTilesPerWave * kTailTile;
     for (int v = threadIdx.x; v < 4 * 64; v += blockDim.x)
       ((uint4 *)(tw + kTwW7))[v] = ((const uint4 *)a.wp[7])[v];
     for (int v = threadIdx.x; v < 8 * 32 / 8; v += blockDim.x)
@@ -1001,7 +1006,7 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
     for (int v = threadIdx.x; v < 2; v += blockDim.x) ((float *)(tw + kTwB8))[v] = a.bias[8][v];
   }
   // the first chunk of layer 1 is in flight while layer 0 runs
-  uint4 b0[2][8], b1[2][8];
+  uint4 b0[kFusedTN][8], b1[kFusedTN][8];
   pipe_fetch<0>(b0, a, wave, lane);
   // the block's features (16 BF16 per row; rows past M read as zero)
   for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
@@ -1013,8 +1018,8 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
   __syncthreads();
   fused_layer<16, 512, false>(act[0], act[1], (const uint4 *)a.wp[0], a.bias[0], nullptr, m0, a.M, lane, wave);
   __syncthreads();
-  f32x16 acc[2];
-  float bias[2];
+  f32x16 acc[kFusedTN];
+  float bias[kFusedTN];
   pipe_run<0>(b0, b1, acc, bias, act, tail, a, m0, wave, lane);
 }
 
