@@ -22,18 +22,21 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 
 // GELU, tanh form: 0.5 z (1 + tanh(u)), u = c (z + a z^3), c = sqrt(2/pi),
 // a = 0.044715.  With r = 1 / (1 + e^{2u}), tanh(u) = 1 - 2r, so
-// GELU = z (1 - r) and GELU' = (1 - r) + 2 z r (1 - r) c (1 + 3 a z^2).
+// GELU = z - z r and GELU' = (1 - r) + 2 z r (1 - r) c (1 + 3 a z^2).
 // e^{2u} and the reciprocal are the hardware's (v_exp_f32, v_rcp_f32, about
 // 1 ulp); r runs to 0 and 1 exactly when e^{2u} overflows or vanishes.
 constexpr float kGc = 0.7978845608028654f, kGa = 0.044715f;
 
+// e^{2u} = 2^{z (K + K a z^2)}, K = 2 c log2(e): three VALU operations before
+// the exp2.
+constexpr float kGk = 2.0f * kGc * 1.4426950408889634f;
+
 __device__ __forceinline__ float gelu_r(float z) {
-  const float p = z * (1.0f + kGa * z * z);
-  const float e = __builtin_amdgcn_exp2f(p * (2.0f * kGc * 1.4426950408889634f));
+  const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(z * z, kGk * kGa, kGk));
   return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-__device__ __forceinline__ float gelu(float z) { return z * (1.0f - gelu_r(z)); }
+__device__ __forceinline__ float gelu(float z) { return __builtin_fmaf(-z, gelu_r(z), z); }
 
 __device__ __forceinline__ float gelu_grad(float z) {
   const float r = gelu_r(z), q = 1.0f - r;
