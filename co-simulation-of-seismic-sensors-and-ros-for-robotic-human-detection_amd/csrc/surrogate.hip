@@ -799,34 +799,37 @@ __device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *ac
   }
 }
 
-// Layers 1-6 as one software pipeline over "chunks": 8 k-steps of a pair of
-// 32-column tiles.  Layers 1-5 (512 x 512) are 4 chunks per wavefront (one
-// tile pair, K = 512); layer 6 (512 -> 2048) is 16 (four tile pairs).  The
-// weights do not depend on the activations, so the B fragments of chunk C+1
-// are loaded while chunk C's MFMAs run even across a tile pair's epilogue or
-// a layer's barrier: the load stream never drains until the last chunk.
-constexpr int kPipeChunks = 5 * 4 + 16;
+// Layers 1-6 as one software pipeline over "chunks": kChunkK k-steps of a
+// wavefront's group of kFusedTN 32-column tiles.  Layers 1-5 (512 x 512) are
+// kChunks chunks per wavefront (one tile group, K = 512); layer 6 (512 ->
+// 2048) has four tile groups.  The weights do not depend on the activations,
+// so the B fragments of chunk C+1 are loaded while chunk C's MFMAs run even
+// across a tile group's epilogue or a layer's barrier: the load stream never
+// drains until the last chunk.
+constexpr int kChunkK = 4;                 // k-steps (16 deep) per chunk
+constexpr int kChunks = 32 / kChunkK;      // chunks per tile group (K = 512)
+constexpr int kPipeChunks = 5 * kChunks + 4 * kChunks;
 
 template <int C>
 struct PipeChunk {
-  static constexpr int L = C < 20 ? 1 + C / 4 : 6;        // layer
-  static constexpr int G = C < 20 ? 0 : (C - 20) / 4;     // tile group (kFusedTN tiles) within the wavefront's
-  static constexpr int KC = C % 4;                        // chunk of 8 k-steps within K = 512
+  static constexpr int L = C < 5 * kChunks ? 1 + C / kChunks : 6;            // layer
+  static constexpr int G = C < 5 * kChunks ? 0 : (C - 5 * kChunks) / kChunks;  // tile group within the wavefront's
+  static constexpr int KC = C % kChunks;                                        // chunk within K = 512
   static constexpr int N = L == 6 ? 2048 : 512;
   static constexpr int TPW = N / 32 / kFusedWaves;        // tiles per wavefront
-  static constexpr bool kLastOfPair = KC == 3;
+  static constexpr bool kLastOfPair = KC == kChunks - 1;
   static constexpr bool kLastOfLayer = kLastOfPair && (L < 6 || G == TPW / kFusedTN - 1);
 };
 
 template <int C>
-__device__ __forceinline__ void pipe_fetch(uint4 (&buf)[kFusedTN][8], const FusedArgs &a, int wave, int lane) {
+__device__ __forceinline__ void pipe_fetch(uint4 (&buf)[kFusedTN][kChunkK], const FusedArgs &a, int wave, int lane) {
   using P = PipeChunk<C>;
   const uint4 *__restrict__ wp = (const uint4 *)a.wp[P::L];
   const int nt0 = wave * P::TPW + P::G * kFusedTN;
 #pragma unroll
   for (int t = 0; t < kFusedTN; t++)
 #pragma unroll
-    for (int s = 0; s < 8; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * 32 + P::KC * 8 + s) * 64 + lane];
+    for (int s = 0; s < kChunkK; s++) buf[t][s] = wp[((int64_t)(nt0 + t) * 32 + P::KC * kChunkK + s) * 64 + lane];
 }
 
 // LDS row pitch (BF16) of the decoder tail's 32 x 32 tiles.
@@ -935,7 +938,8 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
 }
 
 template <int C>
-__device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][8], uint4 (&b1)[kFusedTN][8], f32x16 (&acc)[kFusedTN],
+__device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][kChunkK], uint4 (&b1)[kFusedTN][kChunkK],
+                                         f32x16 (&acc)[kFusedTN],
                                          float (&bias)[kFusedTN],
                                          uint16_t (*act)[kFusedRows * kFusedPitch], uint16_t *tail,
                                          const FusedArgs &a, int m0, int wave, int lane) {
@@ -958,8 +962,8 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][8], uint4 (&b1)[k
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
   }
-  if constexpr (C & 1) mfma_chunk<8, kFusedTN>(acc, b1, act[P::L & 1], P::KC, lane);
-  else mfma_chunk<8, kFusedTN>(acc, b0, act[P::L & 1], P::KC, lane);
+  if constexpr (C & 1) mfma_chunk<kChunkK, kFusedTN>(acc, b1, act[P::L & 1], P::KC, lane);
+  else mfma_chunk<kChunkK, kFusedTN>(acc, b0, act[P::L & 1], P::KC, lane);
   if constexpr (P::kLastOfPair) {
     const int nt0 = wave * P::TPW + P::G * kFusedTN;
     if (P::L == 6 && (a.zout || a.spec)) {
@@ -1006,7 +1010,7 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
     for (int v = threadIdx.x; v < 2; v += blockDim.x) ((float *)(tw + kTwB8))[v] = a.bias[8][v];
   }
   // the first chunk of layer 1 is in flight while layer 0 runs
-  uint4 b0[kFusedTN][8], b1[kFusedTN][8];
+  uint4 b0[kFusedTN][kChunkK], b1[kFusedTN][kChunkK];
   pipe_fetch<0>(b0, a, wave, lane);
   // the block's features (16 BF16 per row; rows past M read as zero)
   for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
