@@ -391,7 +391,8 @@ __global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
 // Workgroups of k_fk_solve resident per CU, which sets its register budget:
 // 1 keeps every matrix in registers (one wavefront per SIMD); 2 halves the
 // budget and spills about 90 registers to scratch, and is faster (measured
-// on the config-2 bank: 2.06 ms against 2.65 ms), because two wavefronts hide
+// on the bank of a three-layer medium, 163 bins x 50 ranges: 2.06 ms
+// against 2.65 ms), because two wavefronts hide
 // the latency of the FP64 divide and square-root sequences.
 #ifndef GF_FK_SOLVE_GROUPS
 #define GF_FK_SOLVE_GROUPS 2

@@ -290,8 +290,9 @@ double state_bytes(const Sim &s) { return gfs::step_bytes(s.R) / 2; }
 // step after step.  Soils are independent, so the order changes nothing in any
 // result.  A variant whose state exceeds kLockstepStateBytes is advanced all
 // together by the two-step kernel, which moves the state once per two steps:
-// on whole runs of 64 and 256 config-3 soils (6.5 and 26 GB) that beat the
-// best cache-sized chunks with either kernel by 7-15 %.  Smaller variants are
+// on whole 1 s runs of 64 and 256 seeded layered soils (600-2048 x 300-1024
+// cells, GSLS; 6.5 and 26 GB) that beat the best cache-sized chunks with
+// either kernel by 7-15 %.  Smaller variants are
 // chunked, the budget tuned (candidates 96-224 MB).  GEOFDTD_CACHE_MB fixes
 // the budget, 0 advancing every model together.
 constexpr double kLockstepStateBytes = 1e9;

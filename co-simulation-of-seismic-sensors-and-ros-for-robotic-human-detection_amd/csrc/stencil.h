@@ -20,8 +20,9 @@
 // them in place in buffer mb; the two-step kernel of march2.h recomputes a
 // neighbour's stresses in its halo, so it reads them from buffer mb and
 // writes buffer mb ^ 1.  (Doubling them for every launch would grow a
-// one-step launch's working set by half and push config-2 grids out of the
-// 256 MB last-level cache.)  The caller tracks both buffer indices.
+// one-step launch's working set by half and push a 2048 x 1024 GSLS grid,
+// 305 MB of state, further out of the 256 MB last-level cache.)  The caller
+// tracks both buffer indices.
 //
 // The functions below are the per-lane phases of that march; the kernel
 // (fdtd_kernels.hip) strings them together with cross-lane shuffles.  They
