@@ -263,7 +263,6 @@ void run_launch(ihipStream_t *s, const Launch &L) {
     if (L.span != 2) fail("tile lists on a %d-step launch", L.span);
     if (L.n_in + L.n_bd != L.tiles)
       fail("tile lists of %lld + %lld entries for %lld tiles", (long long)L.n_in, (long long)L.n_bd, (long long)L.tiles);
-    if (!L.tm) fail("tile lists without a tile-to-model table");
     std::vector<char> hit((size_t)L.tiles, 0);
     for (int kind = 0; kind < 2; kind++) {
       const uint32_t *list = kind ? L.list_bd : L.list_in;
@@ -275,7 +274,7 @@ void run_launch(ihipStream_t *s, const Launch &L) {
         if (t >= L.tiles) fail("tile list entry %lld is tile %lld of a launch of %lld", (long long)e, (long long)t,
                                (long long)L.tiles);
         if (hit[(size_t)t]++) fail("tile %lld listed twice", (long long)t);
-        const gfk::GridDesc &g = L.d[L.tm[t]];
+        const gfk::GridDesc &g = L.d[L.tm ? L.tm[t] : 0];
         const gfk::Tile2 tt = gfk::step2_tile_at(g, (int)(t - g.tile_base));
         if (gfk::step2_interior(g, tt.i0, tt.j0, tt.j1) != (kind == 0))
           fail("tile %lld in the %s list", (long long)t, kind ? "boundary" : "interior");
