@@ -803,12 +803,18 @@ __device__ __forceinline__ void fused_layer(const uint16_t *act_in, uint16_t *ac
 // wavefront's group of kFusedTN 32-column tiles.  Layers 1-5 (512 x 512) are
 // kChunks chunks per wavefront (one tile group, K = 512); layer 6 (512 ->
 // 2048) has four tile groups.  The weights do not depend on the activations,
-// so the B fragments of chunk C+1 are loaded while chunk C's MFMAs run even
+// so the B fragments of chunk C+kAhead are loaded while chunk C's MFMAs run even
 // across a tile group's epilogue or a layer's barrier: the load stream never
 // drains until the last chunk.
 constexpr int kChunkK = 4;                 // k-steps (16 deep) per chunk
 constexpr int kChunks = 32 / kChunkK;      // chunks per tile group (K = 512)
 constexpr int kPipeChunks = 5 * kChunks + 4 * kChunks;
+// Chunks in flight ahead of the one whose MFMAs run: chunk C + kAhead is
+// issued before chunk C's MFMAs, into the register buffer (C + kAhead) %
+// kBufs.  Each buffer is 16 VGPRs.
+constexpr int kAhead = 2;
+constexpr int kBufs = kAhead + 1;
+using ChunkBuf = uint4[kFusedTN][kChunkK];
 
 template <int C>
 struct PipeChunk {
@@ -938,16 +944,11 @@ __device__ __forceinline__ void decoder_tail(const FusedArgs &a, const f32x16 &a
 }
 
 template <int C>
-__device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][kChunkK], uint4 (&b1)[kFusedTN][kChunkK],
-                                         f32x16 (&acc)[kFusedTN],
-                                         float (&bias)[kFusedTN],
+__device__ __forceinline__ void pipe_run(ChunkBuf (&buf)[kBufs], f32x16 (&acc)[kFusedTN], float (&bias)[kFusedTN],
                                          uint16_t (*act)[kFusedRows * kFusedPitch], uint16_t *tail,
                                          const FusedArgs &a, int m0, int wave, int lane) {
   using P = PipeChunk<C>;
-  if constexpr (C + 1 < kPipeChunks) {
-    if constexpr ((C + 1) & 1) pipe_fetch<C + 1>(b1, a, wave, lane);
-    else pipe_fetch<C + 1>(b0, a, wave, lane);
-  }
+  if constexpr (C + kAhead < kPipeChunks) pipe_fetch<C + kAhead>(buf[(C + kAhead) % kBufs], a, wave, lane);
   if constexpr (P::KC == 0) {
     // the pair's biases, loaded now so that the epilogue's wait on them does
     // not also wait for the chunks prefetched after them
@@ -962,8 +963,7 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][kChunkK], uint4 (
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
   }
-  if constexpr (C & 1) mfma_chunk<kChunkK, kFusedTN>(acc, b1, act[P::L & 1], P::KC, lane);
-  else mfma_chunk<kChunkK, kFusedTN>(acc, b0, act[P::L & 1], P::KC, lane);
+  mfma_chunk<kChunkK, kFusedTN>(acc, buf[C % kBufs], act[P::L & 1], P::KC, lane);
   if constexpr (P::kLastOfPair) {
     const int nt0 = wave * P::TPW + P::G * kFusedTN;
     if (P::L == 6 && (a.zout || a.spec)) {
@@ -991,7 +991,7 @@ __device__ __forceinline__ void pipe_run(uint4 (&b0)[kFusedTN][kChunkK], uint4 (
     }
     if constexpr (P::kLastOfLayer && P::L < 6) __syncthreads();
   }
-  if constexpr (C + 1 < kPipeChunks) pipe_run<C + 1>(b0, b1, acc, bias, act, tail, a, m0, wave, lane);
+  if constexpr (C + 1 < kPipeChunks) pipe_run<C + 1>(buf, acc, bias, act, tail, a, m0, wave, lane);
 }
 
 __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a) {
@@ -1009,22 +1009,33 @@ __global__ __launch_bounds__(kFusedWaves * 64) void k_encoder_fused(FusedArgs a)
     for (int v = threadIdx.x; v < 32; v += blockDim.x) ((float *)(tw + kTwB7))[v] = a.bias[7][v];
     for (int v = threadIdx.x; v < 2; v += blockDim.x) ((float *)(tw + kTwB8))[v] = a.bias[8][v];
   }
-  // the first chunk of layer 1 is in flight while layer 0 runs
-  uint4 b0[kFusedTN][kChunkK], b1[kFusedTN][kChunkK];
-  pipe_fetch<0>(b0, a, wave, lane);
+  // the first chunks of layer 1 are in flight while layer 0 runs
+  ChunkBuf buf[kBufs];
+  pipe_fetch<0>(buf[0], a, wave, lane);
+  if constexpr (kAhead > 1) pipe_fetch<1>(buf[1], a, wave, lane);
   // the block's features (16 BF16 per row; rows past M read as zero)
-  for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
-    const int row = v >> 1, half = v & 1;
-    uint4 val = make_uint4(0, 0, 0, 0);
-    if (m0 + row < a.M) val = *(const uint4 *)(a.x0 + (int64_t)(m0 + row) * 16 + half * 8);
-    *(uint4 *)(&act[0][row * kFusedPitch + half * 8]) = val;
+  if (a.xf) {
+    // FP32 rows of xf_cols, rounded here as k_pad_rows_bf16 rounds them
+    for (int v = threadIdx.x; v < kFusedRows * 16; v += blockDim.x) {
+      const int row = v >> 4, col = v & 15;
+      uint16_t y = 0;
+      if (m0 + row < a.M && col < a.xf_cols) y = to_bf16(a.xf[(int64_t)(m0 + row) * a.xf_cols + col]);
+      act[0][row * kFusedPitch + col] = y;
+    }
+  } else {
+    for (int v = threadIdx.x; v < kFusedRows * 2; v += blockDim.x) {
+      const int row = v >> 1, half = v & 1;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (m0 + row < a.M) val = *(const uint4 *)(a.x0 + (int64_t)(m0 + row) * 16 + half * 8);
+      *(uint4 *)(&act[0][row * kFusedPitch + half * 8]) = val;
+    }
   }
   __syncthreads();
   fused_layer<16, 512, false>(act[0], act[1], (const uint4 *)a.wp[0], a.bias[0], nullptr, m0, a.M, lane, wave);
   __syncthreads();
   f32x16 acc[kFusedTN];
   float bias[kFusedTN];
-  pipe_run<0>(b0, b1, acc, bias, act, tail, a, m0, wave, lane);
+  pipe_run<0>(buf, acc, bias, act, tail, a, m0, wave, lane);
 }
 
 hipError_t launch_encoder_fused(const FusedArgs &a, hipStream_t st) {

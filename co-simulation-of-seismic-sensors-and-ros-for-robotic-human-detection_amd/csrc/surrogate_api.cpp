@@ -279,15 +279,23 @@ struct gf_sur {
     return !(e && atoi(e) != 0);
   }
 
+  // feat != null: the kernel reads the FP32 features [B][kFeatures] there
+  // and rounds them itself (no staging launch); else the padded BF16 X[0].
   // spec != null: de-normalize into spec (spec_rows rows) in the same launch.
-  int forward_fused(int B, float *spec = nullptr, int spec_rows = 0, double df = 0, int k_lo = 0, int k_hi = 0) {
+  int forward_fused(int B, const float *feat = nullptr, float *spec = nullptr, int spec_rows = 0, double df = 0,
+                    int k_lo = 0, int k_hi = 0) {
     if (!packed) {
       for (int l = 0; l <= kFusedLayers; l++)
         HIPCHK(gfsk::launch_pack_fragments(pb + L[l].w_off, L[l].n, L[l].k, wpk + wpk_off[l], st));
       packed = true;
     }
     gfsk::FusedArgs a;
-    a.x0 = X[0];
+    if (feat) {
+      a.xf = feat;
+      a.xf_cols = kFeatures;
+    } else {
+      a.x0 = X[0];
+    }
     a.M = B;
     for (int l = 0; l <= kFusedLayers; l++) a.wp[l] = wpk + wpk_off[l];
     for (size_t l = 0; l < L.size(); l++) a.bias[l] = p + L[l].b_off;
@@ -531,7 +539,11 @@ int gf_sur_forward(gf_sur *h, const float *x, int32_t n, float *out) {
     HIPCHK(hipMemcpyAsync(h->feat, x + (int64_t)q0 * kFeatures, sizeof(float) * B * kFeatures,
                           hipMemcpyHostToDevice, h->st));
     HIPCHK(hipEventRecord(h->e0, h->st));
-    if ((rc = h->stage_features(h->feat, Bp)) || (rc = h->forward(Bp, false))) return rc;
+    if (gf_sur::fused_enabled()) {
+      if ((rc = h->forward_fused(Bp, h->feat))) return rc;
+    } else if ((rc = h->stage_features(h->feat, Bp)) || (rc = h->forward(Bp, false))) {
+      return rc;
+    }
     HIPCHK(hipEventRecord(h->e1, h->st));
     HIPCHK(hipMemcpyAsync(out + (int64_t)q0 * kOut, h->Z.back(), sizeof(float) * B * kOut, hipMemcpyDeviceToHost,
                           h->st));
@@ -564,12 +576,12 @@ int gf_sur_predict(gf_sur *h, const float *x, const double *scale, const double 
       k_lo = (int)ceil(h->band_lo / bin_hz - 1e-9);
       k_hi = (int)floor(h->band_hi / bin_hz + 1e-9);
     }
-    if ((rc = h->stage_features(h->feat, Bp))) return rc;
     if (gf_sur::fused_enabled()) {
-      // the whole network and the de-normalization in one launch
-      if ((rc = h->forward_fused(Bp, h->outp, B, bin_hz, k_lo, k_hi))) return rc;
+      // the features' rounding, the whole network and the de-normalization
+      // in one launch
+      if ((rc = h->forward_fused(Bp, h->feat, h->outp, B, bin_hz, k_lo, k_hi))) return rc;
     } else {
-      if ((rc = h->forward(Bp, false))) return rc;
+      if ((rc = h->stage_features(h->feat, Bp)) || (rc = h->forward(Bp, false))) return rc;
       HIPCHK(gfsk::launch_denormalize(h->Z.back(), kOut / 2, h->scale, h->delay, bin_hz, k_lo, k_hi, h->polar,
                                       h->outp, B, h->st));
     }

@@ -96,6 +96,10 @@ hipError_t launch_pack_fragments(const uint16_t *w, int32_t n, int32_t k, uint16
 // rounding per output).
 struct FusedArgs {
   const uint16_t *x0 = nullptr;
+  // xf set: the features as FP32 [M][xf_cols] (xf_cols < 16), rounded to
+  // BF16 and zero-padded to 16 in the kernel, instead of x0.
+  const float *xf = nullptr;
+  int32_t xf_cols = 0;
   int32_t M = 0;
   const uint16_t *wp[8] = {};  // layers 0-7 packed in fragment order
   const float *bias[9] = {};
