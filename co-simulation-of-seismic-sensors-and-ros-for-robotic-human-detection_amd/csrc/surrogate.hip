@@ -477,11 +477,11 @@ __global__ __launch_bounds__(256) void k_colsum_fold(const float *__restrict__ p
 // wavefronts take every fourth slice, then add in LDS) and writes the group's
 // sum over the group's first slice, which no other workgroup reads -- or to
 // out when there is one group.  Pass 2 adds the groups' sums.
-__global__ __launch_bounds__(256) void k_sum_groups(float *__restrict__ part, int32_t splits, int32_t gs, int64_t n,
-                                                    float *__restrict__ out) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
-  const int s0 = blockIdx.y * gs, s1 = min(splits, s0 + gs);
+// The sum of slices s0 .. s1 - 1 for outputs 64 bx .. 64 bx + 63, returned
+// by wavefront 0 (the other wavefronts' lanes return 0).
+__device__ __forceinline__ float sum_slices(const float *__restrict__ part, int s0, int s1, int64_t n, int64_t bx,
+                                            int lane, int wave, bool &valid) {
+  const int64_t i = bx * 64 + lane;
   // four running sums per lane (slices k, k + 4, k + 8, k + 12 of the
   // wavefront's stride-4 sequence), so that four loads are in flight
   float a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -499,13 +499,38 @@ __global__ __launch_bounds__(256) void k_sum_groups(float *__restrict__ part, in
   __shared__ float red[4][64];
   red[wave][lane] = s;
   __syncthreads();
-  if (wave == 0 && i < n) {
-    const float v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  valid = wave == 0 && i < n;
+  return valid ? (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]) : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_sum_groups(float *__restrict__ part, int32_t splits, int32_t gs, int64_t n,
+                                                    float *__restrict__ out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s0 = blockIdx.y * gs, s1 = min(splits, s0 + gs);
+  bool valid;
+  const float v = sum_slices(part, s0, s1, n, blockIdx.x, lane, wave, valid);
+  if (valid) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
     if (gridDim.y == 1)
       out[i] = v;
     else
       part[(int64_t)s0 * n + i] = v;
   }
+}
+
+// Two one-pass sums in one launch (the weight gradient's split-K slices and
+// the bias gradient's row-tile partials): workgroups below `blocks_a` do the
+// first, the rest the second, each as k_sum_groups with one group does it.
+__global__ __launch_bounds__(256) void k_sum_pair(const float *__restrict__ pa, int32_t sa, int64_t na,
+                                                  float *__restrict__ oa, int64_t blocks_a,
+                                                  const float *__restrict__ pb, int32_t sb, int64_t nb,
+                                                  float *__restrict__ ob) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool first = (int64_t)blockIdx.x < blocks_a;
+  const int64_t bx = first ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - blocks_a;
+  bool valid;
+  const float v = first ? sum_slices(pa, 0, sa, na, bx, lane, wave, valid) : sum_slices(pb, 0, sb, nb, bx, lane, wave, valid);
+  if (valid) (first ? oa : ob)[bx * 64 + lane] = v;
 }
 
 __global__ void k_sum_strided(const float *__restrict__ part, int32_t terms, int64_t stride, int64_t n,
@@ -660,13 +685,25 @@ hipError_t launch_sum_splits(float *part, int32_t splits, int64_t n, float *out,
   // most 16 groups (pass 2's terms), and at least 8 slices a group
   int64_t groups = std::max<int64_t>(1, 1024 / colblocks);
   groups = std::min<int64_t>({groups, 16, std::max<int64_t>(1, splits / 8)});
-  if (splits <= 64) groups = 1;
+  if (splits <= kSumOnePass) groups = 1;
   const int32_t gs = (int32_t)((splits + groups - 1) / groups);
   groups = (splits + gs - 1) / gs;
   hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)colblocks, (unsigned)groups), dim3(256), 0, st, part, splits, gs, n,
                      out);
   if (groups > 1)
     hipLaunchKernelGGL(k_sum_strided, grid_for(n), dim3(256), 0, st, part, (int32_t)groups, (int64_t)gs * n, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_splits_pair(float *pa, int32_t sa, int64_t na, float *oa, float *pb, int32_t sb, int64_t nb,
+                                  float *ob, hipStream_t st) {
+  if (sa < 1 || sb < 1 || sa > kSumOnePass || sb > kSumOnePass || na <= 0 || nb <= 0) {
+    hipError_t e = launch_sum_splits(pa, sa, na, oa, st);
+    return e != hipSuccess ? e : launch_sum_splits(pb, sb, nb, ob, st);
+  }
+  const int64_t ba = (na + 63) / 64, bb = (nb + 63) / 64;
+  if (ba + bb > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_sum_pair, dim3((unsigned)(ba + bb)), dim3(256), 0, st, pa, sa, na, oa, ba, pb, sb, nb, ob);
   return hipGetLastError();
 }
 

@@ -113,7 +113,11 @@ struct gf_sur {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // the backward pass's second stream and its ordering events
   hipStream_t st2 = nullptr;
-  hipEvent_t ev_ready = nullptr, ev_free[2] = {nullptr, nullptr};
+  // Gradient buffer sets the backward pass rotates through: the input
+  // gradient of layer l writes the set that layer l + 2's weight gradient
+  // read, so the second stream has two layers' time to finish with it.
+  static constexpr int kGradSets = 3;
+  hipEvent_t ev_ready = nullptr, ev_free[kGradSets] = {};
   // parameters and optimizer state
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;
@@ -128,14 +132,14 @@ struct gf_sur {
   // is not needed); Z[l] the FP32 pre-activation (the output for the last)
   std::vector<uint16_t *> X;
   std::vector<float *> Z;
-  float *dF[2] = {nullptr, nullptr};
-  uint16_t *dB[2] = {nullptr, nullptr};
+  float *dF[kGradSets] = {};
+  uint16_t *dB[kGradSets] = {};
   float *split = nullptr;
   int64_t split_floats = 0;
   float *colpart = nullptr;
   // row-tile partial column sums written by the input-gradient GEMMs (the
-  // next layer's bias gradient), one buffer per gradient pair
-  float *bpart[2] = {nullptr, nullptr};
+  // next layer's bias gradient), one buffer per gradient set
+  float *bpart[kGradSets] = {};
   float *feat = nullptr, *tgt = nullptr, *wgt = nullptr, *loss = nullptr;
   double *scale = nullptr, *delay = nullptr;
   float *outp = nullptr;
@@ -144,17 +148,21 @@ struct gf_sur {
   ~gf_sur() {
     hipStreamSynchronize(st);
     if (st2) hipStreamSynchronize(st2);
-    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)wpk, (void *)dF[0], (void *)dF[1],
-                    (void *)dB[0], (void *)dB[1], (void *)split, (void *)colpart, (void *)bpart[0], (void *)bpart[1], (void *)feat, (void *)tgt, (void *)wgt,
-                    (void *)loss, (void *)scale, (void *)delay, (void *)outp})
+    for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)pb, (void *)pbt, (void *)wpk, (void *)split,
+                    (void *)colpart, (void *)feat, (void *)tgt, (void *)wgt, (void *)loss, (void *)scale, (void *)delay,
+                    (void *)outp})
       if (q) hipFree(q);
+    for (int i = 0; i < kGradSets; i++)
+      for (void *q : {(void *)dF[i], (void *)dB[i], (void *)bpart[i]})
+        if (q) hipFree(q);
     for (auto q : X)
       if (q) hipFree(q);
     for (auto q : Z)
       if (q) hipFree(q);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    for (hipEvent_t e : {ev_ready, ev_free[0], ev_free[1]})
+    if (ev_ready) hipEventDestroy(ev_ready);
+    for (hipEvent_t e : ev_free)
       if (e) hipEventDestroy(e);
     if (st2) hipStreamDestroy(st2);
     if (st) hipStreamDestroy(st);
@@ -195,7 +203,7 @@ struct gf_sur {
       widest = std::max<int64_t>(widest, (int64_t)L[l].rows * std::max(L[l].n, L[l].k));
       split_floats = std::max<int64_t>(split_floats, (int64_t)splits_for(l, B) * L[l].n * L[l].k);
     }
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < kGradSets; i++) {
       HIPCHK(dalloc(&dF[i], B * widest));
       HIPCHK(dalloc(&dB[i], B * widest));
     }
@@ -374,10 +382,10 @@ struct gf_sur {
 
   // Backward from the output gradient in dF[0]/dB[0] ([B][1024]); weight and
   // bias gradients into g.  Layer l's weight and bias gradients run on st2
-  // beside its input gradient on st: both read the gradient buffers `cur`,
-  // and the input gradient writes the other pair, which layer l + 1's
-  // gradients on st2 (event ev_free[1 - cur]) must have finished reading.
-  // On return st has waited for all of st2's work.
+  // beside its input gradient on st: both read the gradient set `cur`, and
+  // the input gradient writes the next set, which layer l + 2's gradients on
+  // st2 (event ev_free[nxt]) must have finished reading.  On return st has
+  // waited for all of st2's work.
   int backward(int B) {
     // on an error return, st2's work must not outlive the call (a later call
     // would reuse the buffers it reads)
@@ -389,12 +397,41 @@ struct gf_sur {
       }
     } drain{st2};
     int cur = 0;
-    bool pending[2] = {false, false};
-    int32_t part_rows[2] = {0, 0};  // row tiles of bias partials in bpart[i]; 0: use dF[i]
+    bool pending[kGradSets] = {};
+    int32_t part_rows[kGradSets] = {};  // row tiles of bias partials in bpart[i]; 0: use dF[i]
     HIPCHK(hipEventRecord(ev_ready, st));
     for (int l = (int)L.size() - 1; l >= 0; l--) {
       const Layer &y = L[l];
       const int64_t rows = (int64_t)B * y.rows;
+      // st2 waits for the gradient set cur (ev_ready as recorded now, after
+      // the input gradient that wrote it), before ev_ready is recorded again
+      HIPCHK(hipStreamWaitEvent(st2, ev_ready, 0));
+      const int nxt = (cur + 1) % kGradSets;
+      if (l > 0) {
+        // the input gradient first: it is the critical path, and enqueued
+        // ahead of the weight-gradient launches it starts as soon as it can
+        if (pending[nxt]) HIPCHK(hipStreamWaitEvent(st, ev_free[nxt], 0));
+        // dX = dZ W, times GELU'(Z[l-1]) of the layer below
+        gfsk::GemmArgs x = dx_args(l, B);
+        x.A = dB[cur];
+        x.lda = y.n;
+        x.B = pbt + y.w_off;  // B(k=n_out, n=k_in) = W[n_out][k_in] = W^T[k_in][n_out]: k contiguous
+        x.ldb = y.n;
+        x.z = Z[l - 1];
+        x.ldz = y.k;
+        x.ldc = y.k;
+        x.cb = dB[nxt];
+        x.ldcb = y.k;
+        if (bias_from_epilogue(l)) {
+          x.colpart = bpart[nxt];
+          part_rows[nxt] = gfsk::gemm_row_tiles(x, false, true, gfsk::kEpiGeluBwd);
+        } else {
+          x.cf = dF[nxt];
+          part_rows[nxt] = 0;
+        }
+        HIPCHK(gfsk::launch_gemm(x, false, true, gfsk::kEpiGeluBwd, 1, st));
+        HIPCHK(hipEventRecord(ev_ready, st));
+      }
       // dW[n][k] = sum_rows dZ[row][n] X[row][k], split over rows
       const int s = splits_for(l, B);
       gfsk::GemmArgs w;
@@ -410,38 +447,20 @@ struct gf_sur {
       w.k_split = (int32_t)(((rows + s - 1) / s + 31) / 32 * 32);
       const int splits = (int)((rows + w.k_split - 1) / w.k_split);
       w.split_stride = (int64_t)y.n * y.k;
-      HIPCHK(hipStreamWaitEvent(st2, ev_ready, 0));
       HIPCHK(gfsk::launch_gemm(w, true, false, gfsk::kEpiF32, splits, st2));
-      HIPCHK(gfsk::launch_sum_splits(split, splits, (int64_t)y.n * y.k, g + y.w_off, st2));
-      if (part_rows[cur] > 0)
-        HIPCHK(gfsk::launch_sum_splits(bpart[cur], part_rows[cur], y.bias_mod, g + y.b_off, st2));
-      else
+      if (part_rows[cur] > 0) {
+        // the weight gradient's split sums and the bias gradient's partial
+        // sums, in one launch when both take one pass
+        HIPCHK(gfsk::launch_sum_splits_pair(split, splits, (int64_t)y.n * y.k, g + y.w_off, bpart[cur],
+                                            part_rows[cur], y.bias_mod, g + y.b_off, st2));
+      } else {
+        HIPCHK(gfsk::launch_sum_splits(split, splits, (int64_t)y.n * y.k, g + y.w_off, st2));
         HIPCHK(gfsk::launch_bias_grad(dF[cur], rows, y.n, y.bias_mod, colpart, g + y.b_off, st2));
+      }
       HIPCHK(hipEventRecord(ev_free[cur], st2));
       pending[cur] = true;
       if (l == 0) break;
-      if (pending[1 - cur]) HIPCHK(hipStreamWaitEvent(st, ev_free[1 - cur], 0));
-      // dX = dZ W, times GELU'(Z[l-1]) of the layer below
-      gfsk::GemmArgs x = dx_args(l, B);
-      x.A = dB[cur];
-      x.lda = y.n;
-      x.B = pbt + y.w_off;  // B(k=n_out, n=k_in) = W[n_out][k_in] = W^T[k_in][n_out]: k contiguous
-      x.ldb = y.n;
-      x.z = Z[l - 1];
-      x.ldz = y.k;
-      x.ldc = y.k;
-      x.cb = dB[1 - cur];
-      x.ldcb = y.k;
-      if (bias_from_epilogue(l)) {
-        x.colpart = bpart[1 - cur];
-        part_rows[1 - cur] = gfsk::gemm_row_tiles(x, false, true, gfsk::kEpiGeluBwd);
-      } else {
-        x.cf = dF[1 - cur];
-        part_rows[1 - cur] = 0;
-      }
-      HIPCHK(gfsk::launch_gemm(x, false, true, gfsk::kEpiGeluBwd, 1, st));
-      HIPCHK(hipEventRecord(ev_ready, st));
-      cur = 1 - cur;
+      cur = nxt;
     }
     HIPCHK(hipStreamWaitEvent(st, ev_free[cur], 0));
     drain.done = true;

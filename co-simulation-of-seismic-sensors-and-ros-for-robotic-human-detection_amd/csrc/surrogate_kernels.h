@@ -58,6 +58,12 @@ hipError_t launch_bias_grad(const float *x, int64_t rows, int32_t cols, int32_t 
 // out[i] = sum over the splits of part[s][i]; part is overwritten (it holds
 // partial sums afterwards).
 hipError_t launch_sum_splits(float *part, int32_t splits, int64_t n, float *out, hipStream_t st);
+// Slices summed in one pass by launch_sum_splits (more take two passes).
+constexpr int32_t kSumOnePass = 64;
+// Two launch_sum_splits jobs; in one launch when both take one pass (the
+// same sums, in the same order, as two calls).
+hipError_t launch_sum_splits_pair(float *pa, int32_t sa, int64_t na, float *oa, float *pb, int32_t sb, int64_t nb,
+                                  float *ob, hipStream_t st);
 hipError_t launch_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int64_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t st);
 // Per-layer BF16 transposes in one launch: layer i's in[off + n*k_i + k]
