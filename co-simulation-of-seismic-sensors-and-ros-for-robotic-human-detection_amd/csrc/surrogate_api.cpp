@@ -650,15 +650,22 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
     return gfs::api_set_err(GF_ERR_ARGUMENT,
                             "geofdtd: train batch must be a multiple of 8, at most max_batch and the sample count");
   float *dx = nullptr, *dt = nullptr, *dw = nullptr, *dls = nullptr;
-  int32_t *didx = nullptr, *hidx = nullptr;
+  // each epoch's permutation, uploaded once into one of two buffers; the
+  // event marks the last gather that read a buffer, before it is refilled two
+  // epochs later
+  int32_t *dperm[2] = {}, *hperm[2] = {};
+  hipEvent_t ev_perm[2] = {};
   auto cleanup = [&]() {
     hipStreamSynchronize(h->st);
     if (dx) hipFree(dx);
     if (dt) hipFree(dt);
     if (dw) hipFree(dw);
     if (dls) hipFree(dls);
-    if (didx) hipFree(didx);
-    if (hidx) hipHostFree(hidx);
+    for (int i = 0; i < 2; i++) {
+      if (dperm[i]) hipFree(dperm[i]);
+      if (hperm[i]) hipHostFree(hperm[i]);
+      if (ev_perm[i]) hipEventDestroy(ev_perm[i]);
+    }
   };
   auto fail = [&](hipError_t e, const char *w) {
     cleanup();
@@ -667,9 +674,13 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
   hipError_t e;
   if ((e = dalloc(&dx, (size_t)n * kFeatures)) || (e = dalloc(&dt, (size_t)n * kOut)) ||
       (w && (e = dalloc(&dw, (size_t)n * kOut))) ||
-      (e = dalloc(&dls, (size_t)std::max(o->steps, 1) * B)) || (e = dalloc(&didx, (size_t)B)) ||
-      (e = hipHostMalloc((void **)&hidx, sizeof(int32_t) * B, hipHostMallocDefault)))
+      (e = dalloc(&dls, (size_t)std::max(o->steps, 1) * B)))
     return fail(e, "train allocation");
+  for (int i = 0; i < 2; i++)
+    if ((e = dalloc(&dperm[i], (size_t)n)) ||
+        (e = hipHostMalloc((void **)&hperm[i], sizeof(int32_t) * n, hipHostMallocDefault)) ||
+        (e = hipEventCreateWithFlags(&ev_perm[i], hipEventDisableTiming)))
+      return fail(e, "train allocation");
   if ((e = hipMemcpyAsync(dx, x, sizeof(float) * n * kFeatures, hipMemcpyHostToDevice, h->st)) ||
       (e = hipMemcpyAsync(dt, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st)) ||
       (w && (e = hipMemcpyAsync(dw, w, sizeof(float) * n * kOut, hipMemcpyHostToDevice, h->st))))
@@ -682,21 +693,29 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
   float ms_total = 0;
   const double b1 = o->beta1 > 0 ? o->beta1 : 0.9, b2 = o->beta2 > 0 ? o->beta2 : 0.999;
   const double eps = o->eps > 0 ? o->eps : 1e-8;
+  // The steps are enqueued without waiting for one another (the stream
+  // orders them), so the host's launches run ahead of the device; the time
+  // is taken over the whole run.
+  int epoch = -1;
+  if ((e = hipEventRecord(h->e0, h->st))) return fail(e, "event");
   for (int32_t s = 0; s < o->steps; s++) {
     if (pos + B > n) {
       for (int64_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[(int64_t)(r.uniform() * (double)(i + 1))]);
       pos = 0;
+      epoch++;
+      const int b = epoch & 1;
+      if (epoch >= 2 && (e = hipEventSynchronize(ev_perm[b]))) return fail(e, "train sync");
+      memcpy(hperm[b], perm.data(), sizeof(int32_t) * n);
+      if ((e = hipMemcpyAsync(dperm[b], hperm[b], sizeof(int32_t) * n, hipMemcpyHostToDevice, h->st)))
+        return fail(e, "train indices");
     }
-    if ((e = hipStreamSynchronize(h->st))) return fail(e, "train sync");
-    memcpy(hidx, perm.data() + pos, sizeof(int32_t) * B);
+    const int32_t *didx = dperm[epoch & 1] + pos;
     pos += B;
-    if ((e = hipMemcpyAsync(didx, hidx, sizeof(int32_t) * B, hipMemcpyHostToDevice, h->st)))
-      return fail(e, "train indices");
-    if ((e = hipEventRecord(h->e0, h->st))) return fail(e, "event");
     if ((e = gfsk::launch_gather_rows(dx, didx, kFeatures, h->feat, B, h->st)) ||
         (e = gfsk::launch_gather_rows(dt, didx, kOut, h->tgt, B, h->st)) ||
         (w && (e = gfsk::launch_gather_rows(dw, didx, kOut, h->wgt, B, h->st))))
       return fail(e, "gather");
+    if (pos + B > n && (e = hipEventRecord(ev_perm[epoch & 1], h->st))) return fail(e, "event");
     if ((rc = h->stage_features(h->feat, B)) || (rc = h->forward(B, true))) {
       cleanup();
       return rc;
@@ -720,12 +739,10 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
       cleanup();
       return rc;
     }
-    if ((e = hipEventRecord(h->e1, h->st))) return fail(e, "event");
-    if ((e = hipEventSynchronize(h->e1))) return fail(e, "train step");
-    float ms = 0;
-    hipEventElapsedTime(&ms, h->e0, h->e1);
-    ms_total += ms;
   }
+  if ((e = hipEventRecord(h->e1, h->st))) return fail(e, "event");
+  if ((e = hipEventSynchronize(h->e1))) return fail(e, "train run");
+  hipEventElapsedTime(&ms_total, h->e0, h->e1);
   if (losses && o->steps > 0) {
     std::vector<float> all((size_t)o->steps * B);
     if ((e = hipMemcpy(all.data(), dls, sizeof(float) * all.size(), hipMemcpyDeviceToHost))) return fail(e, "losses");
