@@ -118,6 +118,10 @@ struct gf_sur {
   // read, so the second stream has two layers' time to finish with it.
   static constexpr int kGradSets = 3;
   hipEvent_t ev_ready = nullptr, ev_free[kGradSets] = {};
+  // a training step's weight transposes run on st2 beside the next forward
+  // pass; the next input gradient (the only reader of pbt) waits for ev_pbt
+  hipEvent_t ev_adam = nullptr, ev_pbt = nullptr;
+  bool pbt_pending = false;
   // parameters and optimizer state
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;
@@ -161,7 +165,8 @@ struct gf_sur {
       if (q) hipFree(q);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    if (ev_ready) hipEventDestroy(ev_ready);
+    for (hipEvent_t e : {ev_ready, ev_adam, ev_pbt})
+      if (e) hipEventDestroy(e);
     for (hipEvent_t e : ev_free)
       if (e) hipEventDestroy(e);
     if (st2) hipStreamDestroy(st2);
@@ -180,6 +185,8 @@ struct gf_sur {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_adam, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_pbt, hipEventDisableTiming));
     for (hipEvent_t &e : ev_free) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(dalloc(&p, P));
     HIPCHK(dalloc(&g, P));
@@ -263,8 +270,10 @@ struct gf_sur {
     return GF_OK;
   }
 
-  // pbt[layer] = pb[layer]^T, the B operand of the input-gradient GEMMs
-  int transpose_weights() {
+  // pbt[layer] = pb[layer]^T, the B operand of the input-gradient GEMMs.
+  // beside = true (after an Adam step): on st2, after the update on st,
+  // leaving ev_pbt for the next backward pass (or wait_pbt) to wait on.
+  int transpose_weights(bool beside = false) {
     gfsk::TransposeTable t;
     if (L.size() > (size_t)gfsk::kMaxTransposeLayers) return gfs::api_set_err(GF_ERR_ARGUMENT, "geofdtd: more layers than the transpose table holds");
     t.layers = (int32_t)L.size();
@@ -274,9 +283,25 @@ struct gf_sur {
       t.k[l] = L[l].k;
       t.first_tile[l + 1] = t.first_tile[l] + ((L[l].n + 15) / 16) * ((L[l].k + 15) / 16);
     }
-    HIPCHK(gfsk::launch_transpose_layers(pb, pbt, t, st));
+    if (beside) {
+      HIPCHK(hipEventRecord(ev_adam, st));
+      HIPCHK(hipStreamWaitEvent(st2, ev_adam, 0));
+      HIPCHK(gfsk::launch_transpose_layers(pb, pbt, t, st2));
+      HIPCHK(hipEventRecord(ev_pbt, st2));
+      pbt_pending = true;
+    } else {
+      HIPCHK(wait_pbt());
+      HIPCHK(gfsk::launch_transpose_layers(pb, pbt, t, st));
+    }
     packed = false;
     return GF_OK;
+  }
+
+  // st waits for transposes left running on st2.
+  hipError_t wait_pbt() {
+    if (!pbt_pending) return hipSuccess;
+    pbt_pending = false;
+    return hipStreamWaitEvent(st, ev_pbt, 0);
   }
 
   // Inference: the whole network in one launch (gfsk::launch_encoder_fused,
@@ -396,6 +421,7 @@ struct gf_sur {
         if (!done) (void)hipStreamSynchronize(s);
       }
     } drain{st2};
+    HIPCHK(wait_pbt());
     int cur = 0;
     bool pending[kGradSets] = {};
     int32_t part_rows[kGradSets] = {};  // row tiles of bias partials in bpart[i]; 0: use dF[i]
@@ -735,11 +761,12 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
     if ((e = gfsk::launch_adam(h->p, h->g, h->m, h->v, h->pb, h->P, (float)lr, (float)b1, (float)b2, (float)eps,
                                (float)c1, (float)c2, h->st)))
       return fail(e, "adam");
-    if ((rc = h->transpose_weights())) {
+    if ((rc = h->transpose_weights(true))) {
       cleanup();
       return rc;
     }
   }
+  if ((e = h->wait_pbt())) return fail(e, "transpose");
   if ((e = hipEventRecord(h->e1, h->st))) return fail(e, "event");
   if ((e = hipEventSynchronize(h->e1))) return fail(e, "train run");
   hipEventElapsedTime(&ms_total, h->e0, h->e1);

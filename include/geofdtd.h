@@ -584,7 +584,9 @@ typedef struct gf_sur_train_opts {
 } gf_sur_train_opts;
 
 /* Device time of the calling handle's last forward/predict (HIP events
- * around its kernels, inputs already in HBM) and last training run. */
+ * around its kernels, inputs already in HBM) and of its last training run
+ * (HIP events from its first step's kernels to its last, the training set
+ * already in HBM). */
 typedef struct gf_sur_stats_t {
   double forward_ms;
   int64_t forward_queries;
