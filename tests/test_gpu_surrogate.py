@@ -305,9 +305,11 @@ def test_served_request_within_the_bound_keeps_the_predictions(S, held_out):
 
 def test_fused_inference_equals_the_per_layer_gemms(S, monkeypatch):
     """The fused encoder kernel (seven layers in one launch, activations in
-    LDS, fragment-packed weights) gives the per-layer GEMMs' bits: the same
-    MFMA k-step order, the same bias + GELU epilogue and BF16 rounding.
-    8,200 queries: a full batch of 8,192 and a ragged one of 8."""
+    LDS, fragment-packed weights, the FP32 features rounded to BF16 inside
+    it) gives the per-layer GEMMs' bits, whose features a separate launch
+    rounds: the same MFMA k-step order, the same bias + GELU epilogue and
+    BF16 rounding.  8,200 queries: a full batch of 8,192 and a ragged one of
+    8."""
     rng = np.random.default_rng(21)
     net = S.Network(max_batch=8192, seed=7)
     try:
