@@ -683,6 +683,10 @@ int gf_sur_train(gf_sur *h, const float *x, const float *t, const float *w, int6
   hipEvent_t ev_perm[2] = {};
   auto cleanup = [&]() {
     hipStreamSynchronize(h->st);
+    // transposes a failed step left on st2 read pb, which a later call may
+    // overwrite
+    hipStreamSynchronize(h->st2);
+    h->pbt_pending = false;
     if (dx) hipFree(dx);
     if (dt) hipFree(dt);
     if (dw) hipFree(dw);
