@@ -212,6 +212,33 @@ def test_training_reduces_the_loss(S):
         n.close()
 
 
+def test_training_is_deterministic(S):
+    """Training runs on two streams (each layer's weight gradient beside the
+    input gradient, three gradient sets, the weight transposes beside the
+    next forward pass) with the steps enqueued back to back.  Every
+    reduction has a fixed order, so two runs from the same seeds give the
+    same losses and parameters bit for bit; a missing wait between the
+    streams would show here as a difference.  Several epochs of a small set,
+    so the double-buffered batch order is reused."""
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-1, 1, (5000, S.FEATURES)).astype(np.float32)
+    w = rng.normal(0, 1, (S.FEATURES, 2 * S.BINS)).astype(np.float32)
+    t = np.tanh(x @ w).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        n = S.Network(max_batch=4096, seed=5)
+        try:
+            losses = n.Train(x, t, steps=300, batch=1024, lr=2e-3, lr_min=4e-5, seed=6)
+            runs.append((losses, n.Params()))
+        finally:
+            n.close()
+    (l0, p0), (l1, p1) = runs
+    print(f"deterministic training: loss {l0[:10].mean():.4f} -> {l0[-10:].mean():.5f}", flush=True)
+    assert np.all(np.isfinite(l0)) and l0[-10:].mean() < 0.5 * l0[:10].mean()
+    np.testing.assert_array_equal(l0, l1)
+    np.testing.assert_array_equal(p0, p1)
+
+
 def test_trained_network_gains_over_fdtd_only(S):
     """On FDTD-generated spectra of the layered family, against the method
     without the network -- the analytic prior alone (the Boussinesq amplitude
